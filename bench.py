@@ -1,0 +1,208 @@
+"""Benchmark of the conflict-graph build on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload linear_large]
+
+A step is one full build of the conflict graph of one candidate set whose
+symbols are already resident in HBM: pack -> all-pairs edit-distance kernel ->
+CSR (degree, scan, scatter, row sort), ending with the CSR in HBM.  The default
+workload is the single large set: n=13, editDist=4, 200,000 candidates (GF(4)
+[13,8] code words + uniform filtered padding), 2.0e10 pairs per step.
+
+Prints one JSON line: Gpairs/s (value), per-step latency p50/p99, the hot
+kernel's roofline (VALU-bound integer work, HIP-event timing on the stream the
+kernel runs on) and a CPU baseline: the IndexGen reference's own FillAdjList
+(compiled from its sources into oracle/_ref) timed on a bounded sample of the
+same workload with the host's threads.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# MI355X: 256 CUs x 4 SIMD x 16 lanes per clock x 2.4 GHz = int32 VALU lane-ops/s
+PEAK_VALU_TOPS = 256 * 64 * 2.4e9 / 1e12
+PEAK_HBM_GBS = 8000.0
+
+# VALU instructions per pair of the pair kernel's inner loop (columns-per-lane
+# 4), counted from the gfx950 assembly of k_pairs<K,4,true>: the
+# Landau-Vishkin recurrence for Levenshtein <= K plus the ballot.  Each wave64
+# instruction is 64 lane-ops over 64 pairs, so this is int ops per pair.
+LV_OPS_PER_PAIR = {0: 6, 1: 8, 2: 24, 3: 38, 4: 65, 5: 89, 6: 126, 7: 160, 8: 207, 9: 251, 10: 308,
+                   11: 362, 12: 429, 13: 493, 14: 570, 15: 644, 16: 6.25}
+
+
+def workload(name: str):
+    from indexgen_amd import workloads as W
+    if name == "linear_large":
+        return W.linear_large(seed=2), 13, 4, "n=13 editDist=4 N=200000 GF(4)[13,8] codewords + uniform filtered padding"
+    if name == "plain_small":
+        return W.plain_small(seed=1), 11, 3, "n=11 editDist=3 20000 uniform strings, dedup, maxRun<=3, GC in [0.3,0.7]"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def host_threads() -> int:
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail))
+
+
+def cpu_baseline(syms: np.ndarray, min_ed: int, target_s: float = 12.0):
+    """Reference FillAdjList (oracle/_ref/ref_driver fill) on the first M
+    candidates, M chosen so the run takes about target_s seconds."""
+    from oracle import oracle as O
+    T = host_threads()
+    driver = O.REF_DRIVER
+    tmp = tempfile.mkdtemp(prefix="igc_cpu_")
+
+    def run_ref(M):
+        path = os.path.join(tmp, "c.txt")
+        with open(path, "w") as f:
+            f.write("\n".join("".join(chr(48 + c) for c in r) for r in syms[:M].tolist()) + "\n")
+        t0 = time.perf_counter()
+        out = subprocess.run([driver, "fill", str(T), str(min_ed), path], capture_output=True, text=True,
+                             check=True).stdout.split()
+        wall = time.perf_counter() - t0
+        return float(out[3]), int(out[1]), wall
+
+    def run_port(M):
+        t0 = time.perf_counter()
+        e = O.edge_count(syms[:M], min_ed, workers=T)
+        return time.perf_counter() - t0, 2 * e, None
+
+    if os.path.exists(driver):
+        kind, fn = "reference", run_ref
+    else:
+        kind, fn = "port", run_port
+    M = min(syms.shape[0], 6000)
+    secs, ones, _ = fn(M)
+    rate = (M * (M - 1) / 2) / max(secs, 1e-6)
+    M2 = int(min(syms.shape[0], max(M, np.sqrt(2 * rate * target_s))))
+    if M2 > M:
+        M = M2
+        secs, ones, _ = fn(M)
+    pairs = M * (M - 1) / 2
+    return {
+        "value": pairs / secs / 1e9,
+        "unit": "Gpairs/s",
+        "cores": T,
+        "kind": kind,
+        "sample": f"first {M} candidates of the same set ({pairs:.3e} pairs, {ones} ones), "
+                  f"{'reference FillAdjList (src/SparseMat.cpp:567-615) built from source' if kind == 'reference' else 'oracle restatement'}"
+                  f" with {T} threads, {secs:.2f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="linear_large")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the conflict-graph build here runs on a single MI355X "
+              "(multi-GPU sharding is described in DESIGN.md, not run). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import indexgen_amd as igc
+
+    syms, n, min_ed, desc = workload(args.workload)
+    N = syms.shape[0]
+    pairs = N * (N - 1) // 2
+    print(f"[bench] workload {args.workload}: {desc}; pairs/step={pairs:.3e}", file=sys.stderr, flush=True)
+
+    d_syms = torch.from_numpy(np.ascontiguousarray(syms)).to("cuda:0")
+    torch.cuda.synchronize()
+    ctx = igc.Context(0)
+    ptr = d_syms.data_ptr()
+
+    for _ in range(args.warmup):
+        ctx.graph_build_device(ptr, N, n, min_ed)
+    torch.cuda.synchronize()
+
+    step_ms, pairs_ms, csr_ms = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        dcsr = ctx.graph_build_device(ptr, N, n, min_ed)
+        step_ms.append((time.perf_counter() - s0) * 1e3)
+        t = ctx.last_timing()
+        pairs_ms.append(t.pairs_ms)
+        csr_ms.append(t.csr_ms)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = pairs / (elapsed / args.steps) / 1e9
+
+    # outputs: greedy codebook of the last build (host), reported separately
+    g0 = time.perf_counter()
+    adj = ctx.device_csr_to_host(dcsr)
+    d2h_ms = (time.perf_counter() - g0) * 1e3
+    g1 = time.perf_counter()
+    order = igc.greedy_order(adj)
+    greedy_ms = (time.perf_counter() - g1) * 1e3
+
+    K = min(min_ed - 1, 16) if min_ed - 1 < n else 16
+    kern_ms = float(np.mean(pairs_ms))
+    ops = LV_OPS_PER_PAIR[K]
+    achieved = pairs * ops / (kern_ms / 1e3) / 1e12
+    myers_ops = 17 * n
+    edge_bytes = dcsr.n_edges * 8  # the kernel's compacted (i,j) writes
+    roofline = {
+        "bound": "valu",
+        "achieved": round(achieved, 3),
+        "peak": round(PEAK_VALU_TOPS, 3),
+        "unit": "Tops/s int32 VALU",
+        "frac": round(achieved / PEAK_VALU_TOPS, 4),
+        "traffic": None,
+        "kernel": f"k_pairs<K={K},CPL=4>",
+        "kernel_ms": round(kern_ms, 4),
+        "ops_per_pair": ops,
+        "pairs_per_launch": pairs,
+        "edge_write_GBs": round(edge_bytes / (kern_ms / 1e3) / 1e9, 3),
+        "myers_equivalent_frac": round(pairs * myers_ops / (kern_ms / 1e3) / 1e12 / PEAK_VALU_TOPS, 4),
+    }
+    result = {
+        "metric": "Gpairs/s edit-distance conflict-graph build + p50/p99 job latency, bit-exact",
+        "value": round(value, 3),
+        "unit": "Gpairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "n": n, "edit_dist": min_ed, "candidates": N, "pairs": pairs,
+                   "edges": dcsr.n_edges, "matrix_ones": 2 * dcsr.n_edges},
+        "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 4),
+                       "p99": round(float(np.percentile(step_ms, 99)), 4)},
+        "breakdown_ms": {"pairs_kernel": round(kern_ms, 4), "csr": round(float(np.mean(csr_ms)), 4),
+                         "d2h_csr": round(d2h_ms, 2), "greedy_host": round(greedy_ms, 2),
+                         "codewords": int(order.size)},
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(syms, min_ed, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

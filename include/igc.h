@@ -1,0 +1,185 @@
+/*
+ * igc.h -- C ABI of the MI355X conflict-graph engine (libigc.so).
+ *
+ * The engine builds IndexGen's edit-distance conflict graph on a gfx950 GPU:
+ * for equal-length candidate strings over {A,C,G,T} (symbol codes 0..3) it
+ * adds edge (i,j) iff Levenshtein(cand[i], cand[j]) < edit_dist, exactly the
+ * predicate of the reference's EditDistanceBandedAtLeast
+ * (include/EditDistance.hpp:368-372, used at src/SparseMat.cpp:544-554).
+ *
+ * It replaces the reference's GPU hand-off, which is a process and file
+ * boundary:
+ *   FillAdjListGPU(AdjList&, const vector<string>& candidates, int minED,
+ *                  long long& matrixOnesNum, const string& inputFilename,
+ *                  double maxGPUMemoryGB, bool silent, bool isBinary)
+ *   (src/SparseMat.cpp:454-534) -> system("python3 src/gpu_graph_generator.py
+ *   cands.txt edges.bin minED maxGB") -> LE int32 pair file -> FromBinaryFile
+ *   (src/SparseMat.cpp:338-390).
+ * Here the same inputs (candidate list, n, editDist) go in by pointer and the
+ * same output (the symmetric adjacency, matrixOnesNum = 2*|E|) comes back as a
+ * CSR, with neighbour lists in a declared canonical order.
+ *
+ * Conventions
+ *   - Every call returns an int status (IGC_OK = 0).  The library never calls
+ *     exit() or abort(); igc_last_error() gives a message (thread-local).
+ *   - Inputs are caller-owned and only read during the call.
+ *   - Host CSR outputs are library-allocated and released with igc_csr_free().
+ *   - A context (igc_ctx) owns one device, one HIP stream and cached device
+ *     buffers; calls on one context are serialised by an internal mutex, calls
+ *     on different contexts may run concurrently from different host threads.
+ *   - Neighbour order: ascending by default, which is the order the reference
+ *     CPU path produces with one thread (src/SparseMat.cpp:536-615, T = 1).
+ *     IGC_ORDER_THREADS(T) reproduces the order of its T-thread merge
+ *     (key (min(x,y) mod T, min(x,y), max(x,y)) within row x).  The greedy
+ *     codebook depends on this order (src/SparseMat.cpp:122-127, 207-251).
+ */
+#ifndef IGC_H_
+#define IGC_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IGC_API __attribute__((visibility("default")))
+
+/* ---- status codes ---- */
+#define IGC_OK 0
+#define IGC_ERR_INVALID 1   /* bad argument (size, symbol > 3, n out of range ...) */
+#define IGC_ERR_OOM 2       /* device or host allocation failed */
+#define IGC_ERR_DEVICE 3    /* HIP runtime error or no usable gfx950 device */
+#define IGC_ERR_ADMISSION 4 /* the job does not fit the configured HBM budget */
+#define IGC_ERR_IO 5        /* file could not be written/read */
+#define IGC_ERR_BUSY 6      /* result not ready yet (server poll) */
+
+/* longest candidate the kernels accept (two 16-bit symbol planes per word) */
+#define IGC_MAX_LEN 16
+
+/* ---- flags for igc_graph_build ---- */
+#define IGC_ORDER_ASCENDING 0u
+/* neighbour order of the reference's FillAdjList with T threads, 1 <= T <= 65535 */
+#define IGC_ORDER_THREADS(T) (((uint32_t)(T)&0xFFFFu) << 16)
+
+/* where a CSR's arrays live */
+#define IGC_ON_HOST 0u
+#define IGC_ON_DEVICE 1u
+
+typedef struct igc_ctx igc_ctx;
+
+/* Symmetric adjacency of the conflict graph.  n_edges counts unordered pairs;
+ * the reference's matrixOnesNum ("Number of Ones in Matrix") is 2*n_edges. */
+typedef struct igc_csr {
+    uint32_t n_nodes;
+    uint32_t location; /* IGC_ON_HOST or IGC_ON_DEVICE */
+    uint64_t n_edges;
+    uint64_t *row_ptr; /* n_nodes + 1 entries */
+    uint32_t *col;     /* 2 * n_edges entries */
+} igc_csr;
+
+/* Timings of the last build on a context, from HIP events on its stream. */
+typedef struct igc_timing {
+    float pack_ms;      /* symbol packing kernel */
+    float pairs_ms;     /* all-pairs edit-distance kernel (the hot kernel) */
+    float csr_ms;       /* degree + scan + scatter + row sort */
+    float total_ms;     /* first event to last event of the build */
+    uint64_t pairs;     /* N(N-1)/2 pairs tested */
+    uint64_t n_edges;
+    uint32_t launches;  /* kernels launched by the last build */
+    uint32_t retries;   /* edge-buffer regrowths in the last build */
+} igc_timing;
+
+/* ---- context ---- */
+
+/* Creates a context on HIP device `device`.  Fails with IGC_ERR_DEVICE when no
+ * gfx950 device is present (no CPU fallback exists). */
+IGC_API int igc_ctx_create(int device, igc_ctx **out);
+IGC_API int igc_ctx_destroy(igc_ctx *ctx);
+/* Use an existing hipStream_t (as void*) instead of the context's own; NULL
+ * restores the context's own stream. */
+IGC_API int igc_ctx_set_stream(igc_ctx *ctx, void *hip_stream);
+IGC_API void *igc_ctx_stream(igc_ctx *ctx);
+IGC_API int igc_last_timing(igc_ctx *ctx, igc_timing *out);
+
+/* ---- single candidate set ---- */
+
+/* Builds the conflict graph of N candidates of length n (1 <= n <= 16) given
+ * as an N x n row-major array of symbol codes 0..3 in host memory.  edit_dist
+ * is the reference's minED: edge iff Levenshtein < edit_dist.  The CSR comes
+ * back in host memory.
+ * Replaces FillAdjListGPU (src/SparseMat.cpp:454-534) and the Python solver it
+ * runs (src/gpu_graph_generator.py:17-80, src/cuda_edit_distance.py:278-447). */
+IGC_API int igc_graph_build(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint32_t n, uint32_t edit_dist,
+                            uint32_t flags, igc_csr *out);
+
+/* Same, with d_syms already in device memory and the CSR left in device
+ * memory owned by the context (valid until the next build on it; do not free).
+ * Ascending order only.  Runs asynchronously on the context's stream except
+ * for one final read of the edge count. */
+IGC_API int igc_graph_build_device(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n,
+                                   uint32_t edit_dist, igc_csr *out);
+
+/* Counts the edges without materialising them: the verify mode of the
+ * reference (VerifyDist, src/Utils.cpp:203-309: zero edges == SUCCESS). */
+IGC_API int igc_graph_count(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint32_t n, uint32_t edit_dist,
+                            uint64_t *n_edges);
+
+/* Copies a device CSR (from igc_graph_build_device) to a new host CSR. */
+IGC_API int igc_csr_to_host(igc_ctx *ctx, const igc_csr *dev, uint32_t flags, igc_csr *host);
+
+/* Reorders the neighbour lists of an ascending host CSR in place to the order
+ * `flags` names (IGC_ORDER_THREADS(T): the reference's T-thread merge order,
+ * src/SparseMat.cpp:599-609).  IGC_ORDER_ASCENDING leaves it unchanged. */
+IGC_API int igc_csr_reorder(igc_csr *csr, uint32_t flags);
+
+/* Releases a host CSR's arrays (no-op for device CSRs owned by a context). */
+IGC_API int igc_csr_free(igc_csr *csr);
+
+/* ---- greedy independent set and outputs (host) ---- */
+
+/* Min-degree greedy independent set of a host CSR, with the reference's exact
+ * bucket semantics (AdjList::RowsBySum/MinSumRow/DelBall/DelRowCol/DecreaseSum/
+ * DeleteRow, src/SparseMat.cpp:81-251; Codebook loop :698-729).  Writes the
+ * selected node ids in selection order to `order` (capacity n_nodes). */
+IGC_API int igc_greedy(const igc_csr *csr, uint32_t *order, uint32_t *count);
+
+/* Writes the graph as the reference's edge file: little-endian int32 pairs
+ * (u, v), u < v, one per edge, ascending (format of src/gpu_graph_generator.py:57-77
+ * read by AdjList::FromBinaryFile, src/SparseMat.cpp:338-390). */
+IGC_API int igc_write_pair_file(const igc_csr *csr, const char *path);
+
+/* Header values of the reference's codebook file (src/Utils.cpp:311-379). */
+typedef struct igc_codebook_info {
+    int code_len;
+    int min_ed;
+    int max_run;
+    double min_gc;
+    double max_gc;
+    const char *method_line;  /* e.g. "Using Generation Method: Random (seed=1)"; may be NULL */
+    int64_t candidate_num;
+    int64_t matrix_ones;      /* 2*|E| */
+    int threads;
+    double candidates_s;
+    double fill_s;
+    double process_s;
+    double overall_s;
+} igc_codebook_info;
+
+/* Writes dir/CodeSize-%07d_CodeLen-%02d_MinED-%d.txt (src/Utils.cpp:381-418):
+ * the parameter header, the '=' separator line, then one codeword per line as
+ * digits '0'..'3' in selection order.  The file name is returned in name_out
+ * (capacity name_cap) when name_out is not NULL. */
+IGC_API int igc_write_codebook(const char *dir, const uint8_t *syms, uint32_t n, const uint32_t *order,
+                               uint32_t count, const igc_codebook_info *info, char *name_out, size_t name_cap);
+
+/* ---- diagnostics ---- */
+IGC_API const char *igc_last_error(void);
+/* "hip:gfx950" when a device context can be created, else "none". */
+IGC_API const char *igc_backend(void);
+IGC_API const char *igc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IGC_H_ */

@@ -1,0 +1,406 @@
+"""indexgen_amd -- MI355X conflict-graph engine for IndexGen codebook generation.
+
+Python host mirror of the reference's hot path.  The names follow the
+reference's C++ host (src/SparseMat.cpp) so a caller of the reference finds the
+same steps:
+
+    FillAdjListGPU(adj, candidates, minED, ones&)   -> fill_adj_list_gpu(candidates, min_ed)
+    Codebook(adj, codebook&, candidates, ...)       -> codebook(adj, candidates)
+    SolveIndependentSet(candidates, minED, ...)     -> solve_independent_set(candidates, min_ed)
+    VerifyDist(codebook, minED, ...)                -> verify_dist(codebook, min_ed)
+    ToFile(codebook, params, ...)                   -> write_codebook(...)
+
+All graph work runs in libigc.so (hand-written HIP kernels for gfx950, behind
+the C ABI of include/igc.h).  There is no CPU fallback: without a gfx950 device
+every graph call raises IGCError.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import weakref
+from dataclasses import dataclass
+from typing import Iterable, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libigc.so")
+
+IGC_OK = 0
+IGC_ERR_INVALID = 1
+IGC_ERR_OOM = 2
+IGC_ERR_DEVICE = 3
+IGC_ERR_ADMISSION = 4
+IGC_ERR_IO = 5
+IGC_ERR_BUSY = 6
+IGC_MAX_LEN = 16
+IGC_ON_HOST = 0
+IGC_ON_DEVICE = 1
+
+
+def order_threads(T: int) -> int:
+    """Flag reproducing the neighbour order of the reference's T-thread FillAdjList."""
+    return (int(T) & 0xFFFF) << 16
+
+
+class IGCError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"igc error {code}: {msg}")
+        self.code = code
+
+
+class _Csr(ctypes.Structure):
+    _fields_ = [
+        ("n_nodes", ctypes.c_uint32),
+        ("location", ctypes.c_uint32),
+        ("n_edges", ctypes.c_uint64),
+        ("row_ptr", ctypes.POINTER(ctypes.c_uint64)),
+        ("col", ctypes.POINTER(ctypes.c_uint32)),
+    ]
+
+
+class _Timing(ctypes.Structure):
+    _fields_ = [
+        ("pack_ms", ctypes.c_float),
+        ("pairs_ms", ctypes.c_float),
+        ("csr_ms", ctypes.c_float),
+        ("total_ms", ctypes.c_float),
+        ("pairs", ctypes.c_uint64),
+        ("n_edges", ctypes.c_uint64),
+        ("launches", ctypes.c_uint32),
+        ("retries", ctypes.c_uint32),
+    ]
+
+
+class _CodebookInfo(ctypes.Structure):
+    _fields_ = [
+        ("code_len", ctypes.c_int),
+        ("min_ed", ctypes.c_int),
+        ("max_run", ctypes.c_int),
+        ("min_gc", ctypes.c_double),
+        ("max_gc", ctypes.c_double),
+        ("method_line", ctypes.c_char_p),
+        ("candidate_num", ctypes.c_int64),
+        ("matrix_ones", ctypes.c_int64),
+        ("threads", ctypes.c_int),
+        ("candidates_s", ctypes.c_double),
+        ("fill_s", ctypes.c_double),
+        ("process_s", ctypes.c_double),
+        ("overall_s", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Loads the in-tree libigc.so (built by indexgen_amd.build / __graft_entry__.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise IGCError(IGC_ERR_DEVICE, f"{LIB_PATH} is missing: run `python -m indexgen_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u8p = ctypes.c_void_p, ctypes.c_void_p
+    u32, u64, i = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+    sig = {
+        "igc_ctx_create": (i, [i, ctypes.POINTER(vp)]),
+        "igc_ctx_destroy": (i, [vp]),
+        "igc_ctx_set_stream": (i, [vp, vp]),
+        "igc_ctx_stream": (vp, [vp]),
+        "igc_last_timing": (i, [vp, ctypes.POINTER(_Timing)]),
+        "igc_graph_build": (i, [vp, u8p, u32, u32, u32, u32, ctypes.POINTER(_Csr)]),
+        "igc_graph_build_device": (i, [vp, u8p, u32, u32, u32, ctypes.POINTER(_Csr)]),
+        "igc_graph_count": (i, [vp, u8p, u32, u32, u32, ctypes.POINTER(u64)]),
+        "igc_csr_to_host": (i, [vp, ctypes.POINTER(_Csr), u32, ctypes.POINTER(_Csr)]),
+        "igc_csr_free": (i, [ctypes.POINTER(_Csr)]),
+        "igc_csr_reorder": (i, [ctypes.POINTER(_Csr), u32]),
+        "igc_greedy": (i, [ctypes.POINTER(_Csr), vp, ctypes.POINTER(u32)]),
+        "igc_write_pair_file": (i, [ctypes.POINTER(_Csr), ctypes.c_char_p]),
+        "igc_write_codebook": (i, [ctypes.c_char_p, u8p, u32, vp, u32, ctypes.POINTER(_CodebookInfo),
+                                   ctypes.c_char_p, ctypes.c_size_t]),
+        "igc_last_error": (ctypes.c_char_p, []),
+        "igc_backend": (ctypes.c_char_p, []),
+        "igc_version": (ctypes.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != IGC_OK:
+        raise IGCError(rc, lib().igc_last_error().decode(errors="replace"))
+
+
+def backend() -> str:
+    return lib().igc_backend().decode()
+
+
+# ---------------------------------------------------------------------------
+# candidate representation (VecToStr, src/Utils.cpp:601-609: symbols 0..3 as '0'..'3')
+
+_ACGT = {ord("A"): 0, ord("C"): 1, ord("G"): 2, ord("T"): 3}
+
+
+def to_symbols(candidates) -> np.ndarray:
+    """N x n uint8 array of symbol codes 0..3 from strings of '0'..'3' or 'ACGT',
+    or from an existing integer array."""
+    if isinstance(candidates, np.ndarray):
+        a = np.ascontiguousarray(candidates, dtype=np.uint8)
+        if a.ndim != 2:
+            raise ValueError("symbol array must be 2-D (N, n)")
+        return a
+    cands = list(candidates)
+    if not cands:
+        return np.zeros((0, 1), dtype=np.uint8)
+    n = len(cands[0])
+    if any(len(c) != n for c in cands):
+        raise ValueError("candidates must all have the same length")
+    raw = np.frombuffer("".join(cands).encode("ascii"), dtype=np.uint8).reshape(len(cands), n)
+    out = raw - ord("0")
+    if (out > 3).any():
+        lut = np.full(256, 255, dtype=np.uint8)
+        for k, v in _ACGT.items():
+            lut[k] = v
+        for k in range(4):
+            lut[ord("0") + k] = k
+        out = lut[raw]
+    if (out > 3).any():
+        raise ValueError("candidates must use symbols 0-3 or A,C,G,T")
+    return np.ascontiguousarray(out, dtype=np.uint8)
+
+
+def to_strings(syms: np.ndarray) -> list:
+    s = np.ascontiguousarray(syms, dtype=np.uint8) + ord("0")
+    n = s.shape[1]
+    b = s.tobytes().decode("ascii")
+    return [b[k * n:(k + 1) * n] for k in range(s.shape[0])]
+
+
+# ---------------------------------------------------------------------------
+# graph objects
+
+
+@dataclass
+class AdjList:
+    """Symmetric conflict graph in CSR form (the reference's AdjList,
+    include/SparseMat.hpp:42-194, as flat arrays)."""
+
+    row_ptr: np.ndarray  # uint64 [N+1]
+    col: np.ndarray      # uint32 [2|E|]
+
+    @property
+    def n_nodes(self) -> int:
+        return int(self.row_ptr.size - 1)
+
+    @property
+    def n_edges(self) -> int:
+        return int(self.col.size // 2)
+
+    @property
+    def matrix_ones(self) -> int:
+        """The reference's matrixOnesNum / "Number of Ones in Matrix" = 2|E|."""
+        return int(self.col.size)
+
+    def neighbors(self, i: int) -> np.ndarray:
+        return self.col[int(self.row_ptr[i]):int(self.row_ptr[i + 1])]
+
+    def _as_c(self) -> _Csr:
+        c = _Csr()
+        c.n_nodes = self.n_nodes
+        c.location = IGC_ON_HOST
+        c.n_edges = self.n_edges
+        self._rp = np.ascontiguousarray(self.row_ptr, dtype=np.uint64)
+        self._col = np.ascontiguousarray(self.col, dtype=np.uint32)
+        if self._col.size == 0:
+            self._col = np.zeros(1, dtype=np.uint32)
+        c.row_ptr = self._rp.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+        c.col = self._col.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        return c
+
+
+def _adj_from_c(c: _Csr) -> AdjList:
+    N = int(c.n_nodes)
+    E2 = 2 * int(c.n_edges)
+    rp = np.ctypeslib.as_array(c.row_ptr, shape=(N + 1,)).copy()
+    col = np.ctypeslib.as_array(c.col, shape=(max(E2, 1),))[:E2].copy()
+    lib().igc_csr_free(ctypes.byref(c))
+    return AdjList(rp, col)
+
+
+@dataclass
+class Timing:
+    pack_ms: float
+    pairs_ms: float
+    csr_ms: float
+    total_ms: float
+    pairs: int
+    n_edges: int
+    launches: int
+    retries: int
+
+
+@dataclass
+class DeviceCSR:
+    """CSR left in device memory by Context.graph_build_device (owned by the
+    context; valid until its next build)."""
+
+    n_nodes: int
+    n_edges: int
+    row_ptr: int  # device address
+    col: int      # device address
+    _raw: _Csr = None
+
+
+class Context:
+    """One GPU, one HIP stream, cached device buffers (an igc_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib().igc_ctx_create(int(device), ctypes.byref(h)))
+        self._h = h
+        self._fin = weakref.finalize(self, lib().igc_ctx_destroy, h)
+
+    def close(self) -> None:
+        self._fin()
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._h
+
+    def set_stream(self, stream_ptr: int | None) -> None:
+        _check(lib().igc_ctx_set_stream(self._h, ctypes.c_void_p(stream_ptr or 0)))
+
+    @property
+    def stream(self) -> int:
+        return int(lib().igc_ctx_stream(self._h) or 0)
+
+    def graph_build(self, syms, min_ed: int, flags: int = 0) -> AdjList:
+        s = to_symbols(syms)
+        N, n = s.shape
+        c = _Csr()
+        _check(lib().igc_graph_build(self._h, s.ctypes.data, N, n, int(min_ed), int(flags), ctypes.byref(c)))
+        return _adj_from_c(c)
+
+    def graph_build_device(self, d_syms_ptr: int, N: int, n: int, min_ed: int) -> DeviceCSR:
+        c = _Csr()
+        _check(lib().igc_graph_build_device(self._h, ctypes.c_void_p(d_syms_ptr), int(N), int(n), int(min_ed),
+                                            ctypes.byref(c)))
+        return DeviceCSR(int(c.n_nodes), int(c.n_edges), ctypes.cast(c.row_ptr, ctypes.c_void_p).value or 0,
+                         ctypes.cast(c.col, ctypes.c_void_p).value or 0, c)
+
+    def device_csr_to_host(self, dcsr: DeviceCSR, flags: int = 0) -> AdjList:
+        c = _Csr()
+        _check(lib().igc_csr_to_host(self._h, ctypes.byref(dcsr._raw), int(flags), ctypes.byref(c)))
+        return _adj_from_c(c)
+
+    def graph_count(self, syms, min_ed: int) -> int:
+        s = to_symbols(syms)
+        N, n = s.shape
+        out = ctypes.c_uint64()
+        _check(lib().igc_graph_count(self._h, s.ctypes.data, N, n, int(min_ed), ctypes.byref(out)))
+        return int(out.value)
+
+    def last_timing(self) -> Timing:
+        t = _Timing()
+        _check(lib().igc_last_timing(self._h, ctypes.byref(t)))
+        return Timing(t.pack_ms, t.pairs_ms, t.csr_ms, t.total_ms, int(t.pairs), int(t.n_edges), int(t.launches),
+                      int(t.retries))
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+# ---------------------------------------------------------------------------
+# reference-shaped host API
+
+
+def fill_adj_list_gpu(candidates, min_ed: int, ctx: Context | None = None, threads_order: int = 1):
+    """FillAdjListGPU (src/SparseMat.cpp:454-534): the conflict graph of the
+    candidates (edge iff edit distance < min_ed).  Returns (AdjList, matrix_ones)
+    with matrix_ones = 2|E| as the reference reports it.  threads_order=T
+    reproduces the neighbour order of the reference's T-thread CPU builder."""
+    ctx = ctx or default_context()
+    flags = order_threads(threads_order) if threads_order > 1 else 0
+    adj = ctx.graph_build(candidates, min_ed, flags)
+    return adj, adj.matrix_ones
+
+
+def reorder(adj: AdjList, flags: int) -> AdjList:
+    """Copy of an ascending AdjList with neighbour lists in the order `flags` names."""
+    out = AdjList(adj.row_ptr.copy(), adj.col.copy())
+    c = out._as_c()
+    _check(lib().igc_csr_reorder(ctypes.byref(c), int(flags)))
+    out.col = out._col[: out.col.size].copy()
+    return out
+
+
+def greedy_order(adj: AdjList) -> np.ndarray:
+    """Node ids of the min-degree greedy independent set, in selection order
+    (AdjList::FindMinDel loop of Codebook, src/SparseMat.cpp:698-729)."""
+    c = adj._as_c()
+    order = np.zeros(max(adj.n_nodes, 1), dtype=np.uint32)
+    cnt = ctypes.c_uint32()
+    _check(lib().igc_greedy(ctypes.byref(c), order.ctypes.data, ctypes.byref(cnt)))
+    return order[: cnt.value].copy()
+
+
+def codebook(adj: AdjList, candidates) -> list:
+    """Codebook (src/SparseMat.cpp:679-731): codewords in selection order."""
+    syms = to_symbols(candidates)
+    return to_strings(syms[greedy_order(adj)])
+
+
+def solve_independent_set(candidates, min_ed: int, ctx: Context | None = None) -> list:
+    """SolveIndependentSet (src/SparseMat.cpp:784-851)."""
+    syms = to_symbols(candidates)
+    if syms.shape[0] == 0:
+        return []
+    if syms.shape[0] == 1:
+        return to_strings(syms)
+    adj, _ = fill_adj_list_gpu(syms, min_ed, ctx)
+    return to_strings(syms[greedy_order(adj)])
+
+
+def verify_dist(codewords, min_ed: int, ctx: Context | None = None) -> bool:
+    """VerifyDist (src/Utils.cpp:203-309): True iff no pair is closer than min_ed."""
+    syms = to_symbols(codewords)
+    if syms.shape[0] < 2:
+        return True
+    ctx = ctx or default_context()
+    return ctx.graph_count(syms, min_ed) == 0
+
+
+def write_pair_file(adj: AdjList, path: str) -> None:
+    """The reference's LE int32 (u, v) u<v edge file (src/gpu_graph_generator.py:57-77)."""
+    c = adj._as_c()
+    _check(lib().igc_write_pair_file(ctypes.byref(c), path.encode()))
+
+
+def write_codebook(directory: str, candidates, order: Sequence[int], *, code_len: int, min_ed: int,
+                   max_run: int = 0, min_gc: float = 0.0, max_gc: float = 0.0, method_line: str | None = None,
+                   candidate_num: int = 0, matrix_ones: int = 0, threads: int = 1, candidates_s: float = 0.0,
+                   fill_s: float = 0.0, process_s: float = 0.0, overall_s: float = 0.0) -> str:
+    """ToFile (src/Utils.cpp:381-418): writes CodeSize-..._CodeLen-.._MinED-..txt and
+    returns its path."""
+    syms = to_symbols(candidates)
+    order = np.ascontiguousarray(order, dtype=np.uint32)
+    info = _CodebookInfo(code_len, min_ed, max_run, min_gc, max_gc,
+                         method_line.encode() if method_line else None, candidate_num, matrix_ones, threads,
+                         candidates_s, fill_s, process_s, overall_s)
+    name = ctypes.create_string_buffer(128)
+    _check(lib().igc_write_codebook(directory.encode(), syms.ctypes.data, syms.shape[1], order.ctypes.data,
+                                    order.size, ctypes.byref(info), name, 128))
+    return os.path.join(directory, name.value.decode())

@@ -1,0 +1,233 @@
+// igc_host.cpp -- host side of libigc.so: the greedy independent set, neighbour
+// order policies and the reference's output formats.  Plain C++17, no device code.
+#include "igc_internal.hpp"
+
+#include <algorithm>
+#include <cerrno>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace igc {
+
+void reorder_threads(igc_csr &csr, uint32_t T)
+{
+    const uint32_t N = csr.n_nodes;
+    for (uint32_t x = 0; x < N; ++x) {
+        uint32_t *b = csr.col + csr.row_ptr[x];
+        uint32_t *e = csr.col + csr.row_ptr[x + 1];
+        std::stable_sort(b, e, [x, T](uint32_t y1, uint32_t y2) {
+            const uint32_t m1 = std::min(x, y1), m2 = std::min(x, y2);
+            const uint32_t r1 = m1 % T, r2 = m2 % T;
+            if (r1 != r2) return r1 < r2;
+            if (m1 != m2) return m1 < m2;
+            return std::max(x, y1) < std::max(x, y2);
+        });
+    }
+}
+
+// Min-degree greedy over a CSR, with the bucket mechanics of the reference's
+// AdjList (src/SparseMat.cpp:81-251):
+//   - bucket d lists the active nodes of current degree d; built in node order
+//     (RowsBySum, :92-120);
+//   - the pick is the LAST node of the lowest non-empty bucket (MinSumRow, :122-127);
+//   - DelBall (:229-251) deletes the pick's active neighbours in list order and
+//     then the pick; each deletion (DelRowCol, :207-227) moves every active
+//     neighbour u from bucket deg(u) to deg(u)-1 (DecreaseSum, :156-167): a
+//     swap-remove that fills the hole with the bucket's last node (DeleteRow,
+//     :139-154) and an append to the lower bucket.
+// Buckets live in one flat array: bucket d never holds more nodes than there
+// are nodes of initial degree >= d, because degrees only fall.
+class Greedy {
+  public:
+    Greedy(uint32_t N, const uint64_t *rp, const uint32_t *col) : N_(N), rp_(rp), col_(col) {}
+
+    int run(uint32_t *order, uint32_t *count)
+    {
+        try {
+            degree_.resize(N_);
+            pos_.resize(N_);
+            deleted_.assign(N_, 0);
+            uint32_t maxd = 0;
+            for (uint32_t v = 0; v < N_; ++v) {
+                degree_[v] = (uint32_t)(rp_[v + 1] - rp_[v]);
+                maxd = std::max(maxd, degree_[v]);
+            }
+            nb_ = maxd + 1;
+            std::vector<uint64_t> ge(nb_ + 1, 0);
+            for (uint32_t v = 0; v < N_; ++v) ge[degree_[v]]++;
+            for (int64_t d = (int64_t)nb_ - 1; d >= 0; --d) ge[d] += ge[d + 1];
+            off_.resize(nb_ + 1);
+            off_[0] = 0;
+            for (uint32_t d = 0; d < nb_; ++d) off_[d + 1] = off_[d] + ge[d];
+            store_.resize(off_[nb_]);
+            len_.assign(nb_, 0);
+        } catch (const std::bad_alloc &) {
+            return set_error(IGC_ERR_OOM, "greedy: host allocation failed");
+        }
+        tracker_ = nb_;
+        for (uint32_t v = 0; v < N_; ++v) {
+            push(degree_[v], v);
+            if (degree_[v] < tracker_) tracker_ = degree_[v];
+        }
+        uint32_t active = N_, picked = 0;
+        std::vector<uint32_t> todel;
+        while (active > 0) {
+            const uint32_t v = store_[off_[tracker_] + len_[tracker_] - 1];
+            order[picked++] = v;
+            todel.clear();
+            for (uint64_t q = rp_[v]; q < rp_[v + 1]; ++q)
+                if (!deleted_[col_[q]]) todel.push_back(col_[q]);
+            todel.push_back(v);
+            for (uint32_t x : todel) {
+                if (deleted_[x]) continue;
+                for (uint64_t q = rp_[x]; q < rp_[x + 1]; ++q) {
+                    const uint32_t u = col_[q];
+                    if (deleted_[u]) continue;
+                    const uint32_t du = degree_[u];
+                    if (du > 0) {
+                        remove(du, u);
+                        degree_[u] = du - 1;
+                        push(du - 1, u);
+                        if (du - 1 < tracker_) tracker_ = du - 1;
+                    }
+                }
+                remove(degree_[x], x);
+                deleted_[x] = 1;
+                --active;
+            }
+        }
+        *count = picked;
+        return IGC_OK;
+    }
+
+  private:
+    void push(uint32_t d, uint32_t v)
+    {
+        pos_[v] = len_[d];
+        store_[off_[d] + len_[d]++] = v;
+    }
+    void remove(uint32_t d, uint32_t v)
+    {
+        const uint32_t p = pos_[v];
+        const uint32_t last = store_[off_[d] + len_[d] - 1];
+        store_[off_[d] + p] = last;
+        pos_[last] = p;
+        --len_[d];
+        if (d == tracker_ && len_[d] == 0)
+            while (tracker_ < nb_ && len_[tracker_] == 0) ++tracker_;
+    }
+
+    uint32_t N_;
+    const uint64_t *rp_;
+    const uint32_t *col_;
+    uint32_t nb_ = 0, tracker_ = 0;
+    std::vector<uint32_t> degree_, pos_, len_, store_;
+    std::vector<uint64_t> off_;
+    std::vector<uint8_t> deleted_;
+};
+
+} // namespace igc
+
+using namespace igc;
+
+extern "C" {
+
+IGC_API int igc_greedy(const igc_csr *csr, uint32_t *order, uint32_t *count)
+{
+    if (!csr || !count || (!order && csr->n_nodes)) return set_error(IGC_ERR_INVALID, "null argument");
+    if (csr->location != IGC_ON_HOST) return set_error(IGC_ERR_INVALID, "igc_greedy needs a host CSR");
+    *count = 0;
+    if (csr->n_nodes == 0) return IGC_OK;
+    if (!csr->row_ptr || (!csr->col && csr->n_edges)) return set_error(IGC_ERR_INVALID, "CSR has null arrays");
+    Greedy g(csr->n_nodes, csr->row_ptr, csr->col);
+    return g.run(order, count);
+}
+
+IGC_API int igc_csr_reorder(igc_csr *csr, uint32_t flags)
+{
+    if (!csr) return set_error(IGC_ERR_INVALID, "null argument");
+    if (csr->location != IGC_ON_HOST) return set_error(IGC_ERR_INVALID, "igc_csr_reorder needs a host CSR");
+    const uint32_t T = (flags >> 16) & 0xFFFFu;
+    if (T > 1) reorder_threads(*csr, T);
+    return IGC_OK;
+}
+
+IGC_API int igc_write_pair_file(const igc_csr *csr, const char *path)
+{
+    if (!csr || !path) return set_error(IGC_ERR_INVALID, "null argument");
+    if (csr->location != IGC_ON_HOST) return set_error(IGC_ERR_INVALID, "igc_write_pair_file needs a host CSR");
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return set_error(IGC_ERR_IO, "cannot open %s: %s", path, std::strerror(errno));
+    std::vector<int32_t> buf;
+    buf.reserve(1 << 16);
+    bool ok = true;
+    for (uint32_t u = 0; u < csr->n_nodes && ok; ++u) {
+        for (uint64_t q = csr->row_ptr[u]; q < csr->row_ptr[u + 1]; ++q) {
+            const uint32_t v = csr->col[q];
+            if (v <= u) continue;
+            buf.push_back((int32_t)u);
+            buf.push_back((int32_t)v);
+            if (buf.size() >= (1u << 16)) {
+                ok = std::fwrite(buf.data(), sizeof(int32_t), buf.size(), f) == buf.size();
+                buf.clear();
+            }
+        }
+    }
+    if (ok && !buf.empty()) ok = std::fwrite(buf.data(), sizeof(int32_t), buf.size(), f) == buf.size();
+    if (std::fclose(f) != 0) ok = false;
+    if (!ok) return set_error(IGC_ERR_IO, "write to %s failed", path);
+    return IGC_OK;
+}
+
+IGC_API int igc_write_codebook(const char *dir, const uint8_t *syms, uint32_t n, const uint32_t *order, uint32_t count,
+                               const igc_codebook_info *info, char *name_out, size_t name_cap)
+{
+    if (!dir || !info || (count && (!syms || !order))) return set_error(IGC_ERR_INVALID, "null argument");
+    if (n < 1 || n > 99) return set_error(IGC_ERR_INVALID, "bad code length %u", n);
+    if (count > 9999999u) return set_error(IGC_ERR_INVALID, "codebook too large for the file name format");
+    char name[96];
+    std::snprintf(name, sizeof name, "CodeSize-%07u_CodeLen-%02d_MinED-%d.txt", count, info->code_len, info->min_ed);
+    std::string path = std::string(dir) + "/" + name;
+    FILE *f = std::fopen(path.c_str(), "w");
+    if (!f) return set_error(IGC_ERR_IO, "cannot open %s: %s", path.c_str(), std::strerror(errno));
+    // header fields and spacing follow PrintParamsToFile (src/Utils.cpp:311-379)
+    std::fprintf(f, "--- Global Parameters ---\n");
+    std::fprintf(f, "Code Length:\t\t\t%d\n", info->code_len);
+    std::fprintf(f, "Min Codebook Edit Distance:\t%d\n\n", info->min_ed);
+    std::fprintf(f, "Max Run:\t\t\t%d\n", info->max_run);
+    std::fprintf(f, "Min GC Content:\t\t\t%g\n", info->min_gc);
+    std::fprintf(f, "Max GC Content:\t\t\t%g\n\n", info->max_gc);
+    std::fprintf(f, "%s\n\n", info->method_line ? info->method_line : "Using Generation Method: External candidates");
+    std::fprintf(f, "--- Results Summary ---\n");
+    std::fprintf(f, "Number of Candidates:\t\t%lld\n", (long long)info->candidate_num);
+    std::fprintf(f, "Number of Ones in Matrix:\t%lld\n", (long long)info->matrix_ones);
+    std::fprintf(f, "Number of Code Words:\t\t%u\n\n", count);
+    std::fprintf(f, "--- Performance Metrics ---\n");
+    std::fprintf(f, "Number of Threads:\t\t%d\n", info->threads);
+    std::fprintf(f, "Candidate Generation Time:\t%.2f\tseconds\n", info->candidates_s);
+    std::fprintf(f, "Fill Adjacency List Time:\t%.2f\tseconds\n", info->fill_s);
+    std::fprintf(f, "Process Matrix Time:\t\t%.2f\tseconds\n", info->process_s);
+    std::fprintf(f, "Overall Execution Time:\t\t%.2f\tseconds\n", info->overall_s);
+    std::fprintf(f, "=========================================== \n");
+    std::vector<char> line(n + 1);
+    line[n] = '\n';
+    bool ok = true;
+    for (uint32_t k = 0; k < count && ok; ++k) {
+        const uint8_t *s = syms + (size_t)order[k] * n;
+        for (uint32_t p = 0; p < n; ++p) line[p] = (char)('0' + s[p]);
+        ok = std::fwrite(line.data(), 1, n + 1, f) == n + 1;
+    }
+    if (std::fclose(f) != 0) ok = false;
+    if (!ok) return set_error(IGC_ERR_IO, "write to %s failed", path.c_str());
+    if (name_out && name_cap) {
+        std::strncpy(name_out, name, name_cap - 1);
+        name_out[name_cap - 1] = '\0';
+    }
+    return IGC_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,18 @@
+// igc_internal.hpp -- declarations shared by the translation units of libigc.so.
+#pragma once
+#include "../../include/igc.h"
+
+#include <cstdint>
+
+namespace igc {
+
+// Records a message for igc_last_error() and returns `code`.
+int set_error(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+const char *last_error();
+
+// Reorders every row of a host CSR from ascending to the neighbour order of
+// the reference's T-thread FillAdjList merge (src/SparseMat.cpp:599-609):
+// within row x, neighbour y is keyed by (min(x,y) mod T, min(x,y), max(x,y)).
+void reorder_threads(igc_csr &csr, uint32_t T);
+
+} // namespace igc

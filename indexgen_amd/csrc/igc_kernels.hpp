@@ -1,0 +1,456 @@
+// igc_kernels.hpp -- gfx950 kernels of the conflict-graph engine.
+//
+// Data layout in HBM
+//   packed[i]  one 32-bit word per candidate: bit p = sym[p] & 1 and
+//              bit 16+p = sym[p] >> 1 for p < n <= 16, all other bits 0
+//              (two bit planes, so a shift by d moves whole symbols).
+//   edges[e]   (i << 32) | j for every conflicting pair i < j, in no order.
+//   row_ptr    uint64 [N+1];  col  uint32 [2|E|]  -- the symmetric CSR.
+//
+// The pair test.  Edge (i,j) iff Levenshtein(A,B) <= K where K = editDist-1
+// (the reference predicate EditDistanceBandedAtLeast, include/EditDistance.hpp
+// :368-372, equals min(Levenshtein, K+1) for equal lengths).  Instead of the
+// reference's per-column Myers/Hyyro recurrence we run Landau-Vishkin
+// furthest-reaching diagonals over mismatch bitmasks:
+//   mism_d bit j = (A[j+d] != B[j])   for the 2*floor(K/2)+1 diagonals |d|<=K/2
+//   FR[e][d] = slide_d(max(FR[e-1][d]+1, FR[e-1][d-1], FR[e-1][d+1]+1))
+//   slide_d(L) = L + ctz(mism_d >> L)
+// restricted to |d| <= min(e, K-e), and edge iff FR[K][0] >= n.  Each cell is a
+// max3, a shift, a find-first-bit and an add on one 32-bit register, so a pair
+// costs O(K^2) VALU ops independent of n (about 40 at K=3) instead of the
+// 17*n of one Myers column step per symbol.
+// Bits 16..31 of every mask are forced to 1: a slide can never run off the
+// word, and any furthest-reaching value past a diagonal's end only means
+// "end reached", which propagates correctly because neighbouring DP cells
+// differ by at most one.  For diagonal 0 the bits n..15 are 0 (both strings
+// are zero-padded), so FR[K][0] >= n  <=>  (q0 >> L) == 0 with q0 the 16-bit
+// mask and L the last level's start.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "igc_lv.hpp"
+
+namespace igc {
+
+constexpr int kWaves = 4;          // waves per workgroup of the pair kernel
+constexpr int kStage = 256;        // edges staged per wave in LDS before a flush
+
+// ---------------------------------------------------------------------------
+// symbol packing
+__global__ void k_pack(const uint8_t *__restrict__ syms, uint32_t N, uint32_t n, uint32_t *__restrict__ packed,
+                       uint32_t *__restrict__ bad)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const uint8_t *s = syms + (size_t)i * n;
+    uint32_t w = 0, or_all = 0;
+    for (uint32_t p = 0; p < n; ++p) {
+        const uint32_t c = s[p];
+        or_all |= c;
+        w |= (c & 1u) << p;
+        w |= ((c >> 1) & 1u) << (16 + p);
+    }
+    packed[i] = w;
+    if (or_all > 3u) atomicOr(bad, 1u);
+}
+
+__global__ void k_fill_u32(uint32_t *__restrict__ p, uint64_t count, uint32_t v)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// pair predicate
+
+// ---------------------------------------------------------------------------
+// all-pairs tiling of the upper triangle
+
+// Row blocks of R rows, column blocks of W = q*R columns.  Group g holds the q
+// row blocks g*q .. g*q+q-1, each owning column blocks g .. NBc-1.
+struct TileGeom {
+    uint32_t N;
+    uint32_t R;
+    uint32_t W;
+    uint32_t q;
+    uint32_t NBr;
+    uint32_t NBc;
+};
+
+__device__ __forceinline__ uint64_t tiles_before_group(uint64_t g, const TileGeom &G)
+{
+    return (uint64_t)G.q * (g * (uint64_t)G.NBc - (g * (g - (g > 0 ? 1 : 0))) / 2);
+}
+
+__device__ __forceinline__ void decode_tile(uint64_t t, const TileGeom &G, uint32_t &rb, uint32_t &cb)
+{
+    const double nb = (double)G.NBc + 0.5;
+    const double disc = nb * nb - 2.0 * (double)t / (double)G.q;
+    int64_t g = (int64_t)(nb - sqrt(disc > 0.0 ? disc : 0.0));
+    if (g < 0) g = 0;
+    if (g > (int64_t)G.NBc - 1) g = (int64_t)G.NBc - 1;
+    while (g > 0 && tiles_before_group((uint64_t)g, G) > t) --g;
+    while (g + 1 < (int64_t)G.NBc && tiles_before_group((uint64_t)g + 1, G) <= t) ++g;
+    const uint64_t tp = t - tiles_before_group((uint64_t)g, G);
+    const uint32_t per = G.NBc - (uint32_t)g;
+    rb = (uint32_t)g * G.q + (uint32_t)(tp / per);
+    cb = (uint32_t)g + (uint32_t)(tp % per);
+}
+
+__device__ __forceinline__ uint32_t lane_id()
+{
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+__device__ __forceinline__ uint32_t rank_in_mask(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint64_t uniform64(uint64_t v)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Moves a wave's staged edges to the global edge list with one atomic.
+__device__ __forceinline__ void flush_stage(const uint64_t *st, uint32_t staged, uint32_t lane,
+                                            uint64_t *__restrict__ edges, uint64_t cap,
+                                            unsigned long long *__restrict__ counter)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(counter, (unsigned long long)staged);
+    base = uniform64(base);
+    for (uint32_t q = lane; q < staged; q += 64) {
+        const uint64_t idx = base + q;
+        if (idx < cap) edges[idx] = st[q];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// One workgroup = one tile: R rows x W columns, each of the 4 waves owning
+// 64*CPL consecutive columns (lane l, slot c -> column colw + 64c + l).  Rows
+// are walked in order with the row word uniform (scalar registers), so each
+// lane tests CPL pairs per row.  Nonzero ballots (rare) are compacted through
+// a per-wave LDS stage.  EMIT=false only counts.
+template <int K, int CPL, bool EMIT>
+__global__ __launch_bounds__(256) void k_pairs(const uint32_t *__restrict__ packed, TileGeom G,
+                                               uint64_t *__restrict__ edges, uint64_t cap,
+                                               unsigned long long *__restrict__ counter)
+{
+    constexpr int WD = LvShape<K>::WD;
+    __shared__ uint64_t stage[kWaves][kStage];
+
+    const uint32_t lane = lane_id();
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    uint32_t rb, cb;
+    decode_tile(blockIdx.x, G, rb, cb);
+    const uint32_t N = G.N;
+    const uint32_t row0 = rb * G.R;
+    const uint32_t colw = cb * G.W + wave * 64u * CPL;
+    if (row0 >= N || colw >= N) return;
+    const uint32_t colend = min(colw + 64u * CPL, N); // exclusive
+    // rows i with some column j > i in this wave: i < colend - 1
+    if (colend - 1u <= row0) return;
+    const uint32_t rows = min(min(G.R, N - row0), colend - 1u - row0);
+
+    uint32_t b[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+        const uint32_t j = colw + 64u * c + lane;
+        b[c] = j < N ? packed[j] : 0u;
+    }
+    uint64_t *st = stage[wave];
+    uint32_t staged = 0;
+    uint64_t counted = 0;
+
+    for (uint32_t r = 0; r < rows; ++r) {
+        const uint32_t i = row0 + r;
+        const uint32_t a = packed[i];
+        uint32_t sa[WD];
+        row_shifts<K>(a, sa);
+        uint64_t mk[CPL];
+        uint64_t any = 0;
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            mk[c] = __ballot(lv_edge<K>(sa, b[c]));
+            any |= mk[c];
+        }
+        if (any) {
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                if (!mk[c]) continue;
+                const uint32_t j = colw + 64u * c + lane;
+                const bool mine = ((mk[c] >> lane) & 1ull) && j > i && j < N;
+                const uint64_t m2 = __ballot(mine);
+                if (!m2) continue;
+                const uint32_t cnt = (uint32_t)__popcll(m2);
+                if constexpr (EMIT) {
+                    if (staged + cnt > (uint32_t)kStage) {
+                        flush_stage(st, staged, lane, edges, cap, counter);
+                        staged = 0;
+                    }
+                    if (mine) st[staged + rank_in_mask(m2)] = ((uint64_t)i << 32) | j;
+                    staged += cnt;
+                } else {
+                    counted += cnt;
+                }
+            }
+        }
+    }
+    if constexpr (EMIT) {
+        if (staged) flush_stage(st, staged, lane, edges, cap, counter);
+    } else {
+        if (counted && lane == 0) atomicAdd(counter, (unsigned long long)counted);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CSR assembly
+
+__device__ __forceinline__ uint64_t edge_total(const unsigned long long *counter, uint64_t cap)
+{
+    const uint64_t e = (uint64_t)*counter;
+    return e < cap ? e : cap;
+}
+
+__global__ void k_degree(const uint64_t *__restrict__ edges, const unsigned long long *__restrict__ counter,
+                         uint64_t cap, uint32_t *__restrict__ deg)
+{
+    const uint64_t E = edge_total(counter, cap);
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t v = edges[e];
+        atomicAdd(&deg[(uint32_t)(v >> 32)], 1u);
+        atomicAdd(&deg[(uint32_t)v], 1u);
+    }
+}
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+// Exclusive block-wide scan of one uint64 per thread; returns the block total.
+__device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t v, uint64_t *sh /* [kScanThreads/64 + 1] */,
+                                                         uint64_t &excl)
+{
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t y = __shfl_up(incl, o, 64);
+        if (lane >= (uint32_t)o) incl += y;
+    }
+    if (lane == 63) sh[wave] = incl;
+    __syncthreads();
+    uint64_t wave_off = 0, total = 0;
+    for (uint32_t w = 0; w < kScanThreads / 64; ++w) {
+        if (w < wave) wave_off += sh[w];
+        total += sh[w];
+    }
+    excl = wave_off + incl - v;
+    __syncthreads();
+    return total;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *__restrict__ deg, uint32_t N,
+                                                               uint64_t *__restrict__ bsum)
+{
+    __shared__ uint64_t sh[kScanThreads / 64 + 1];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < N) s += deg[base + k];
+    uint64_t excl;
+    const uint64_t tot = block_exclusive_scan(s, sh, excl);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_blocks(uint64_t *__restrict__ bsum, uint32_t nb)
+{
+    __shared__ uint64_t sh[kScanThreads / 64 + 1];
+    uint64_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nb; b0 += kScanThreads) {
+        const uint32_t b = b0 + threadIdx.x;
+        const uint64_t v = b < nb ? bsum[b] : 0;
+        uint64_t excl;
+        const uint64_t tot = block_exclusive_scan(v, sh, excl);
+        if (b < nb) bsum[b] = carry + excl;
+        carry += tot;
+    }
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t *__restrict__ deg, uint32_t N,
+                                                              const uint64_t *__restrict__ boff,
+                                                              uint64_t *__restrict__ row_ptr)
+{
+    __shared__ uint64_t sh[kScanThreads / 64 + 1];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (base + k < N) ? deg[base + k] : 0u;
+        s += v[k];
+    }
+    uint64_t excl;
+    block_exclusive_scan(s, sh, excl);
+    uint64_t run = boff[blockIdx.x] + excl;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < N) {
+            row_ptr[base + k] = run;
+            run += v[k];
+            if (base + k == (uint64_t)N - 1) row_ptr[N] = run;
+        }
+    }
+}
+
+__global__ void k_scatter(const uint64_t *__restrict__ edges, const unsigned long long *__restrict__ counter,
+                          uint64_t cap, const uint64_t *__restrict__ row_ptr, uint32_t *__restrict__ cursor,
+                          uint32_t *__restrict__ col)
+{
+    const uint64_t E = edge_total(counter, cap);
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t v = edges[e];
+        const uint32_t i = (uint32_t)(v >> 32), j = (uint32_t)v;
+        const uint32_t p = atomicAdd(&cursor[i], 1u);
+        col[row_ptr[i] + p] = j;
+        const uint32_t q = atomicAdd(&cursor[j], 1u);
+        col[row_ptr[j] + q] = i;
+    }
+}
+
+// Row sort, short rows: one wave per row, bitonic network across lanes for
+// rows of <= 64 entries; longer rows are queued for the block kernels.
+constexpr uint32_t kMidMax = 4096;
+
+__global__ __launch_bounds__(256) void k_sort_short(const uint64_t *__restrict__ row_ptr, uint32_t N,
+                                                     uint32_t *__restrict__ col, uint32_t *__restrict__ mid_list,
+                                                     uint32_t *__restrict__ big_list, uint32_t *__restrict__ counts)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave_global = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t x = wave_global; x < N; x += waves) {
+        const uint64_t s = row_ptr[x], len = row_ptr[x + 1] - s;
+        if (len <= 1) continue;
+        if (len > 64) {
+            if (lane == 0) {
+                if (len <= kMidMax) mid_list[atomicAdd(&counts[0], 1u)] = x;
+                else big_list[atomicAdd(&counts[1], 1u)] = x;
+            }
+            continue;
+        }
+        uint32_t v = lane < len ? col[s + lane] : 0xFFFFFFFFu;
+        for (uint32_t k = 2; k <= 64; k <<= 1) {
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                const uint32_t o = __shfl_xor(v, (int)j, 64);
+                const bool up = (lane & k) == 0;
+                const bool low = (lane & j) == 0;
+                const uint32_t mn = v < o ? v : o, mx = v < o ? o : v;
+                v = (low == up) ? mn : mx;
+            }
+        }
+        if (lane < len) col[s + lane] = v;
+    }
+}
+
+// Rows of 65..4096 entries: one workgroup per row, bitonic sort in LDS.
+__global__ __launch_bounds__(256) void k_sort_mid(const uint64_t *__restrict__ row_ptr, uint32_t *__restrict__ col,
+                                                   const uint32_t *__restrict__ mid_list,
+                                                   const uint32_t *__restrict__ counts)
+{
+    __shared__ uint32_t sh[kMidMax];
+    const uint32_t nrows = counts[0];
+    for (uint32_t w = blockIdx.x; w < nrows; w += gridDim.x) {
+        const uint32_t x = mid_list[w];
+        const uint64_t s = row_ptr[x];
+        const uint32_t len = (uint32_t)(row_ptr[x + 1] - s);
+        uint32_t P = 64;
+        while (P < len) P <<= 1;
+        for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) sh[t] = t < len ? col[s + t] : 0xFFFFFFFFu;
+        __syncthreads();
+        for (uint32_t k = 2; k <= P; k <<= 1) {
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) {
+                    const uint32_t u = t ^ j;
+                    if (u > t) {
+                        const uint32_t a = sh[t], b = sh[u];
+                        const bool up = (t & k) == 0;
+                        if ((a > b) == up) {
+                            sh[t] = b;
+                            sh[u] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) col[s + t] = sh[t];
+        __syncthreads();
+    }
+}
+
+// Rows over 4096 entries: values are distinct node ids < N, so a bitmap over
+// value chunks of 2^19 ids in LDS, read back in order, sorts the row.
+constexpr uint32_t kBigChunkWords = 16384; // 64 KiB of LDS = 2^19 ids
+
+__global__ __launch_bounds__(256) void k_sort_big(const uint64_t *__restrict__ row_ptr, uint32_t N,
+                                                   uint32_t *__restrict__ col, uint32_t *__restrict__ tmp,
+                                                   const uint32_t *__restrict__ big_list,
+                                                   const uint32_t *__restrict__ counts)
+{
+    __shared__ uint32_t bm[kBigChunkWords];
+    __shared__ uint32_t wsum[256];
+    const uint32_t nrows = counts[1];
+    constexpr uint32_t per = kBigChunkWords / 256; // words owned by each thread in the read-back
+    for (uint32_t w = blockIdx.x; w < nrows; w += gridDim.x) {
+        const uint32_t x = big_list[w];
+        const uint64_t s = row_ptr[x];
+        const uint32_t len = (uint32_t)(row_ptr[x + 1] - s);
+        // stage the row in tmp (same offsets) so the writes below cannot race the reads
+        for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) tmp[s + t] = col[s + t];
+        __syncthreads();
+        uint32_t written = 0;
+        for (uint64_t base = 0; base < N; base += (uint64_t)kBigChunkWords * 32) {
+            for (uint32_t t = threadIdx.x; t < kBigChunkWords; t += blockDim.x) bm[t] = 0;
+            __syncthreads();
+            for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) {
+                const uint64_t v = tmp[s + t];
+                if (v >= base && v < base + (uint64_t)kBigChunkWords * 32) {
+                    const uint32_t o = (uint32_t)(v - base);
+                    atomicOr(&bm[o >> 5], 1u << (o & 31));
+                }
+            }
+            __syncthreads();
+            uint32_t mine = 0;
+            for (uint32_t k = 0; k < per; ++k) mine += __popc(bm[threadIdx.x * per + k]);
+            wsum[threadIdx.x] = mine;
+            __syncthreads();
+            uint32_t before = 0, total = 0;
+            for (uint32_t t = 0; t < 256; ++t) {
+                if (t < threadIdx.x) before += wsum[t];
+                total += wsum[t];
+            }
+            uint32_t out = written + before;
+            for (uint32_t k = 0; k < per; ++k) {
+                uint32_t word = bm[threadIdx.x * per + k];
+                while (word) {
+                    const uint32_t bit = (uint32_t)__builtin_ctz(word);
+                    word &= word - 1;
+                    col[s + out++] = (uint32_t)(base + (uint64_t)(threadIdx.x * per + k) * 32 + bit);
+                }
+            }
+            written += total;
+            __syncthreads();
+        }
+    }
+}
+
+} // namespace igc

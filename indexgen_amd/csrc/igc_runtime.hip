@@ -1,0 +1,538 @@
+// igc_runtime.hip -- device orchestration and C ABI of libigc.so (gfx950).
+//
+// One build = pack -> pairs -> degree -> scan -> scatter -> row sort, all on the
+// context's stream with no host synchronisation until the final read of the
+// edge count.  If the edge buffer overflowed (the count is exact even then),
+// the buffer is regrown to the exact count and the build is replayed.
+#include "igc_kernels.hpp"
+#include "igc_internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace igc {
+
+static thread_local std::string g_err;
+
+int set_error(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+const char *last_error() { return g_err.c_str(); }
+
+#define HIP_TRY(expr)                                                                                         \
+    do {                                                                                                      \
+        hipError_t e_ = (expr);                                                                               \
+        if (e_ != hipSuccess)                                                                                 \
+            return set_error(e_ == hipErrorOutOfMemory ? IGC_ERR_OOM : IGC_ERR_DEVICE, "%s failed: %s (%s:%d)", \
+                             #expr, hipGetErrorString(e_), __FILE__, __LINE__);                               \
+    } while (0)
+
+// grow-only device buffer
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t want)
+    {
+        if (want <= bytes && p) return IGC_OK;
+        if (p) {
+            hipError_t e = hipFree(p);
+            (void)e;
+            p = nullptr;
+            bytes = 0;
+        }
+        size_t sz = std::max<size_t>(want, 256);
+        hipError_t e = hipMalloc(&p, sz);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return set_error(IGC_ERR_OOM, "hipMalloc(%zu) failed: %s", sz, hipGetErrorString(e));
+        }
+        bytes = sz;
+        return IGC_OK;
+    }
+    void release()
+    {
+        if (p) {
+            hipError_t e = hipFree(p);
+            (void)e;
+        }
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+} // namespace igc
+
+using namespace igc;
+
+struct igc_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    DevBuf syms, packed, edges, misc, deg, cursor, row_ptr, col, mid, big, bsum;
+    hipEvent_t ev[5] = {};
+    igc_timing last{};
+    int cu_count = 256;
+};
+
+namespace igc {
+
+// misc buffer layout (uint32 words): [0..1] edge counter (u64), [2] bad symbol
+// flag, [4] mid-row count, [5] big-row count
+constexpr size_t kMiscWords = 8;
+
+// P(Levenshtein <= K) for uniformly random filtered 4-ary strings, measured
+// offline; used only to size the first edge buffer (the build is exact either way).
+static double density_prior(uint32_t n, int K)
+{
+    static const double tab[7][6] = {
+        // K=0      1        2         3         4        5
+        {2e-6, 3.6e-5, 6.2e-4, 6.7e-3, 4.6e-2, 0.19},   // n=10
+        {5e-7, 9e-6, 2.1e-4, 2.6e-3, 2.0e-2, 0.10},     // n=11
+        {2e-7, 2e-6, 6.6e-5, 8.5e-4, 7.6e-3, 4.5e-2},   // n=12
+        {1e-7, 5e-7, 1.7e-5, 2.8e-4, 2.8e-3, 2.0e-2},   // n=13
+        {1e-7, 2e-7, 3.5e-6, 9.8e-5, 1.1e-3, 8.3e-3},   // n=14
+        {1e-7, 1e-7, 1.5e-6, 2.3e-5, 3.6e-4, 3.3e-3},   // n=15
+        {1e-7, 1e-7, 1.5e-6, 7.5e-6, 1.2e-4, 1.3e-3},   // n=16
+    };
+    if (n >= 10 && n <= 16 && K >= 0 && K <= 5) return tab[n - 10][K];
+    return 0.2;
+}
+
+uint64_t initial_edge_cap(uint32_t N, uint32_t n, int K)
+{
+    const uint64_t pairs = (uint64_t)N * (N - 1) / 2;
+    if (pairs <= (1ull << 22)) return std::max<uint64_t>(pairs, 1);
+    double est = (double)pairs * density_prior(n, K) * 2.0 + (double)(1u << 20);
+    if (est > (double)pairs) est = (double)pairs;
+    return (uint64_t)est;
+}
+
+TileGeom choose_geometry(uint32_t N, int cpl, uint32_t q)
+{
+    TileGeom G;
+    G.N = N;
+    G.W = 256u * (uint32_t)cpl;
+    G.q = q;
+    G.R = G.W / q;
+    G.NBr = (N + G.R - 1) / G.R;
+    G.NBc = (N + G.W - 1) / G.W;
+    return G;
+}
+
+uint64_t tile_count(const TileGeom &G)
+{
+    uint64_t t = 0;
+    for (uint32_t g = 0; g < G.NBc; ++g) {
+        const uint32_t rb0 = g * G.q;
+        if (rb0 >= G.NBr) break;
+        const uint32_t nrb = std::min(G.q, G.NBr - rb0);
+        t += (uint64_t)nrb * (G.NBc - g);
+    }
+    return t;
+}
+
+// Tile shape: enough workgroups to fill 256 CUs several times over.
+void default_shape(uint32_t N, int &cpl, uint32_t &q)
+{
+    if (N >= 65536) {
+        cpl = 4;
+        q = 1;
+    } else if (N >= 16384) {
+        cpl = 2;
+        q = N >= 32768 ? 2 : 4;
+    } else {
+        cpl = 1;
+        q = N >= 4096 ? 4 : 8;
+    }
+}
+
+typedef void (*pairs_fn)(const uint32_t *, TileGeom, uint64_t *, uint64_t, unsigned long long *);
+
+template <int K, int CPL, bool EMIT>
+static void launch_pairs_t(hipStream_t s, uint64_t tiles, const uint32_t *packed, const TileGeom &G, uint64_t *edges,
+                           uint64_t cap, unsigned long long *counter)
+{
+    hipLaunchKernelGGL((k_pairs<K, CPL, EMIT>), dim3((uint32_t)tiles), dim3(256), 0, s, packed, G, edges, cap,
+                       counter);
+}
+
+template <int CPL, bool EMIT>
+static int launch_pairs_k(int K, hipStream_t s, uint64_t tiles, const uint32_t *packed, const TileGeom &G,
+                          uint64_t *edges, uint64_t cap, unsigned long long *counter)
+{
+    switch (K) {
+#define IGC_CASE(k)                                                                                           \
+    case k:                                                                                                   \
+        launch_pairs_t<k, CPL, EMIT>(s, tiles, packed, G, edges, cap, counter);                               \
+        return IGC_OK;
+        IGC_CASE(0) IGC_CASE(1) IGC_CASE(2) IGC_CASE(3) IGC_CASE(4) IGC_CASE(5) IGC_CASE(6) IGC_CASE(7)
+        IGC_CASE(8) IGC_CASE(9) IGC_CASE(10) IGC_CASE(11) IGC_CASE(12) IGC_CASE(13) IGC_CASE(14) IGC_CASE(15)
+        IGC_CASE(16)
+#undef IGC_CASE
+    default:
+        return set_error(IGC_ERR_INVALID, "unsupported K=%d", K);
+    }
+}
+
+int launch_pairs(int K, int cpl, bool emit, hipStream_t s, uint64_t tiles, const uint32_t *packed,
+                 const TileGeom &G, uint64_t *edges, uint64_t cap, unsigned long long *counter)
+{
+    if (tiles == 0) return IGC_OK;
+    if (tiles > 0x7FFFFFFFull) return set_error(IGC_ERR_INVALID, "too many tiles (%llu)", (unsigned long long)tiles);
+    int rc;
+    if (cpl == 1) rc = emit ? launch_pairs_k<1, true>(K, s, tiles, packed, G, edges, cap, counter)
+                            : launch_pairs_k<1, false>(K, s, tiles, packed, G, edges, cap, counter);
+    else if (cpl == 2) rc = emit ? launch_pairs_k<2, true>(K, s, tiles, packed, G, edges, cap, counter)
+                                 : launch_pairs_k<2, false>(K, s, tiles, packed, G, edges, cap, counter);
+    else if (cpl == 4) rc = emit ? launch_pairs_k<4, true>(K, s, tiles, packed, G, edges, cap, counter)
+                                 : launch_pairs_k<4, false>(K, s, tiles, packed, G, edges, cap, counter);
+    else return set_error(IGC_ERR_INVALID, "unsupported columns-per-lane %d", cpl);
+    if (rc) return rc;
+    HIP_TRY(hipGetLastError());
+    return IGC_OK;
+}
+
+static uint32_t grid_for(uint64_t work, uint32_t per_block, uint32_t cap_blocks)
+{
+    uint64_t b = (work + per_block - 1) / per_block;
+    if (b < 1) b = 1;
+    if (b > cap_blocks) b = cap_blocks;
+    return (uint32_t)b;
+}
+
+// Effective K for the kernels: edges iff ED <= K; K >= n means every pair.
+int effective_k(uint32_t n, uint32_t edit_dist)
+{
+    const int K = (int)edit_dist - 1;
+    if (K >= (int)n) return kAllPairs;
+    return K;
+}
+
+static int check_args(igc_ctx *ctx, uint32_t N, uint32_t n)
+{
+    if (!ctx) return set_error(IGC_ERR_INVALID, "null context");
+    if (n < 1 || n > IGC_MAX_LEN) return set_error(IGC_ERR_INVALID, "string length n=%u outside 1..%d", n, IGC_MAX_LEN);
+    if (N > 0x7FFFFFFFu) return set_error(IGC_ERR_INVALID, "N=%u too large", N);
+    return IGC_OK;
+}
+
+// Full device build on ctx->stream.  Leaves the CSR in ctx->row_ptr/col.
+static int build_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n, uint32_t edit_dist,
+                               uint64_t *n_edges_out, bool count_only)
+{
+    hipStream_t s = ctx->stream;
+    igc_timing tm{};
+    tm.pairs = (uint64_t)N * (N > 0 ? N - 1 : 0) / 2;
+    *n_edges_out = 0;
+    if (edit_dist == 0 || N < 2) {
+        // no pair can conflict; still produce a valid empty CSR
+        int rc = ctx->row_ptr.ensure(sizeof(uint64_t) * ((size_t)N + 1));
+        if (rc) return rc;
+        if (!count_only) {
+            hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(2 * ((uint64_t)N + 1), 256, 4096)), dim3(256), 0, s,
+                               ctx->row_ptr.as<uint32_t>(), 2 * ((uint64_t)N + 1), 0u);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipStreamSynchronize(s));
+        }
+        ctx->last = tm;
+        return IGC_OK;
+    }
+    const int K = effective_k(n, edit_dist);
+    int cpl;
+    uint32_t q;
+    default_shape(N, cpl, q);
+    const TileGeom G = choose_geometry(N, cpl, q);
+    const uint64_t tiles = tile_count(G);
+
+    int rc;
+    if ((rc = ctx->packed.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
+    if ((rc = ctx->misc.ensure(sizeof(uint32_t) * kMiscWords))) return rc;
+    uint32_t *misc = ctx->misc.as<uint32_t>();
+    unsigned long long *counter = reinterpret_cast<unsigned long long *>(misc);
+
+    uint64_t cap = count_only ? 0 : initial_edge_cap(N, n, K);
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        if (!count_only) {
+            if ((rc = ctx->edges.ensure(sizeof(uint64_t) * (size_t)cap))) return rc;
+        }
+        HIP_TRY(hipEventRecord(ctx->ev[0], s));
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, misc, (uint64_t)kMiscWords, 0u);
+        hipLaunchKernelGGL(k_pack, dim3((N + 255) / 256), dim3(256), 0, s, d_syms, N, n,
+                           ctx->packed.as<uint32_t>(), misc + 2);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(ctx->ev[1], s));
+        rc = launch_pairs(K, cpl, !count_only, s, tiles, ctx->packed.as<uint32_t>(), G,
+                          count_only ? nullptr : ctx->edges.as<uint64_t>(), cap, counter);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(ctx->ev[2], s));
+        uint32_t launches = 3;
+        if (!count_only) {
+            // CSR: degree -> scan -> scatter -> sort.  Sized by N and cap, so no
+            // host round trip is needed before the final count.
+            const uint32_t nsb = (uint32_t)(((uint64_t)N + kScanTile - 1) / kScanTile);
+            if ((rc = ctx->deg.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
+            if ((rc = ctx->cursor.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
+            if ((rc = ctx->row_ptr.ensure(sizeof(uint64_t) * ((size_t)N + 1)))) return rc;
+            if ((rc = ctx->col.ensure(sizeof(uint32_t) * 2 * (size_t)cap))) return rc;
+            if ((rc = ctx->bsum.ensure(sizeof(uint64_t) * (size_t)nsb))) return rc;
+            if ((rc = ctx->mid.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
+            if ((rc = ctx->big.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
+            const uint32_t fillg = grid_for(N, 256, 8192);
+            hipLaunchKernelGGL(k_fill_u32, dim3(fillg), dim3(256), 0, s, ctx->deg.as<uint32_t>(), (uint64_t)N, 0u);
+            hipLaunchKernelGGL(k_fill_u32, dim3(fillg), dim3(256), 0, s, ctx->cursor.as<uint32_t>(), (uint64_t)N, 0u);
+            const uint32_t eg = grid_for(cap, 256, (uint32_t)ctx->cu_count * 16);
+            hipLaunchKernelGGL(k_degree, dim3(eg), dim3(256), 0, s, ctx->edges.as<uint64_t>(), counter, cap,
+                               ctx->deg.as<uint32_t>());
+            hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(kScanThreads), 0, s, ctx->deg.as<uint32_t>(), N,
+                               ctx->bsum.as<uint64_t>());
+            hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, s, ctx->bsum.as<uint64_t>(), nsb);
+            hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(kScanThreads), 0, s, ctx->deg.as<uint32_t>(), N,
+                               ctx->bsum.as<uint64_t>(), ctx->row_ptr.as<uint64_t>());
+            hipLaunchKernelGGL(k_scatter, dim3(eg), dim3(256), 0, s, ctx->edges.as<uint64_t>(), counter, cap,
+                               ctx->row_ptr.as<uint64_t>(), ctx->cursor.as<uint32_t>(), ctx->col.as<uint32_t>());
+            hipLaunchKernelGGL(k_sort_short, dim3(grid_for((uint64_t)N * 64, 256, 8192)), dim3(256), 0, s,
+                               ctx->row_ptr.as<uint64_t>(), N, ctx->col.as<uint32_t>(), ctx->mid.as<uint32_t>(),
+                               ctx->big.as<uint32_t>(), misc + 4);
+            hipLaunchKernelGGL(k_sort_mid, dim3((uint32_t)ctx->cu_count * 4), dim3(256), 0, s,
+                               ctx->row_ptr.as<uint64_t>(), ctx->col.as<uint32_t>(), ctx->mid.as<uint32_t>(),
+                               misc + 4);
+            hipLaunchKernelGGL(k_sort_big, dim3((uint32_t)ctx->cu_count * 2), dim3(256), 0, s,
+                               ctx->row_ptr.as<uint64_t>(), N, ctx->col.as<uint32_t>(),
+                               ctx->edges.as<uint32_t>() /* edge list is dead after the scatter */,
+                               ctx->big.as<uint32_t>(), misc + 4);
+            HIP_TRY(hipGetLastError());
+            launches += 11;
+        }
+        HIP_TRY(hipEventRecord(ctx->ev[3], s));
+        uint32_t host_misc[kMiscWords];
+        HIP_TRY(hipMemcpyAsync(host_misc, misc, sizeof host_misc, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (host_misc[2]) return set_error(IGC_ERR_INVALID, "symbol code above 3 in input");
+        uint64_t E = (uint64_t)host_misc[0] | ((uint64_t)host_misc[1] << 32);
+        HIP_TRY(hipEventElapsedTime(&tm.pack_ms, ctx->ev[0], ctx->ev[1]));
+        HIP_TRY(hipEventElapsedTime(&tm.pairs_ms, ctx->ev[1], ctx->ev[2]));
+        HIP_TRY(hipEventElapsedTime(&tm.csr_ms, ctx->ev[2], ctx->ev[3]));
+        HIP_TRY(hipEventElapsedTime(&tm.total_ms, ctx->ev[0], ctx->ev[3]));
+        tm.launches = launches;
+        tm.retries = (uint32_t)attempt;
+        tm.n_edges = E;
+        if (count_only || E <= cap) {
+            *n_edges_out = E;
+            ctx->last = tm;
+            return IGC_OK;
+        }
+        cap = E + E / 64 + 1024; // exact regrow, then replay
+    }
+    return set_error(IGC_ERR_DEVICE, "edge buffer kept overflowing");
+}
+
+} // namespace igc
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+IGC_API const char *igc_last_error(void) { return igc::last_error(); }
+
+IGC_API const char *igc_version(void) { return "igc 0.1 gfx950"; }
+
+IGC_API const char *igc_backend(void)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1) return "none";
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return "none";
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return "none";
+    return "hip:gfx950";
+}
+
+IGC_API int igc_ctx_create(int device, igc_ctx **out)
+{
+    if (!out) return set_error(IGC_ERR_INVALID, "null out");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
+        return set_error(IGC_ERR_DEVICE, "no HIP device available (libigc has no CPU fallback)");
+    if (device < 0 || device >= count) return set_error(IGC_ERR_INVALID, "device %d out of range", device);
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return set_error(IGC_ERR_DEVICE, "device %d is %s, kernels are built for gfx950", device, prop.gcnArchName);
+    HIP_TRY(hipSetDevice(device));
+    igc_ctx *c = new (std::nothrow) igc_ctx();
+    if (!c) return set_error(IGC_ERR_OOM, "host allocation failed");
+    c->device = device;
+    c->cu_count = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return set_error(IGC_ERR_DEVICE, "hipStreamCreate failed");
+    }
+    c->stream = c->own_stream;
+    for (auto &e : c->ev) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete c;
+            return set_error(IGC_ERR_DEVICE, "hipEventCreate failed");
+        }
+    }
+    *out = c;
+    return IGC_OK;
+}
+
+IGC_API int igc_ctx_destroy(igc_ctx *ctx)
+{
+    if (!ctx) return IGC_OK;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        for (DevBuf *b : {&ctx->syms, &ctx->packed, &ctx->edges, &ctx->misc, &ctx->deg, &ctx->cursor, &ctx->row_ptr,
+                          &ctx->col, &ctx->mid, &ctx->big, &ctx->bsum})
+            b->release();
+        for (auto &e : ctx->ev)
+            if (e) (void)hipEventDestroy(e);
+        if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    }
+    delete ctx;
+    return IGC_OK;
+}
+
+IGC_API int igc_ctx_set_stream(igc_ctx *ctx, void *hip_stream)
+{
+    if (!ctx) return set_error(IGC_ERR_INVALID, "null context");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->own_stream;
+    return IGC_OK;
+}
+
+IGC_API void *igc_ctx_stream(igc_ctx *ctx) { return ctx ? static_cast<void *>(ctx->stream) : nullptr; }
+
+IGC_API int igc_last_timing(igc_ctx *ctx, igc_timing *out)
+{
+    if (!ctx || !out) return set_error(IGC_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    *out = ctx->last;
+    return IGC_OK;
+}
+
+IGC_API int igc_graph_build_device(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n, uint32_t edit_dist,
+                                   igc_csr *out)
+{
+    int rc = check_args(ctx, N, n);
+    if (rc) return rc;
+    if (!out || (!d_syms && N)) return set_error(IGC_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    uint64_t E = 0;
+    rc = build_device_locked(ctx, d_syms, N, n, edit_dist, &E, false);
+    if (rc) return rc;
+    out->n_nodes = N;
+    out->location = IGC_ON_DEVICE;
+    out->n_edges = E;
+    out->row_ptr = ctx->row_ptr.as<uint64_t>();
+    out->col = ctx->col.as<uint32_t>();
+    return IGC_OK;
+}
+
+IGC_API int igc_csr_free(igc_csr *csr)
+{
+    if (!csr) return IGC_OK;
+    if (csr->location == IGC_ON_HOST) {
+        std::free(csr->row_ptr);
+        std::free(csr->col);
+    }
+    csr->row_ptr = nullptr;
+    csr->col = nullptr;
+    csr->n_edges = 0;
+    return IGC_OK;
+}
+
+static int csr_to_host_locked(igc_ctx *ctx, uint32_t N, uint64_t E, const uint64_t *d_row_ptr, const uint32_t *d_col,
+                              uint32_t flags, igc_csr *host)
+{
+    host->n_nodes = N;
+    host->location = IGC_ON_HOST;
+    host->n_edges = E;
+    host->row_ptr = static_cast<uint64_t *>(std::malloc(sizeof(uint64_t) * ((size_t)N + 1)));
+    host->col = static_cast<uint32_t *>(std::malloc(sizeof(uint32_t) * std::max<size_t>(2 * (size_t)E, 1)));
+    if (!host->row_ptr || !host->col) {
+        igc_csr_free(host);
+        return set_error(IGC_ERR_OOM, "host CSR allocation failed");
+    }
+    if (N == 0) {
+        host->row_ptr[0] = 0;
+        return IGC_OK;
+    }
+    HIP_TRY(hipMemcpyAsync(host->row_ptr, d_row_ptr, sizeof(uint64_t) * ((size_t)N + 1), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    if (E)
+        HIP_TRY(hipMemcpyAsync(host->col, d_col, sizeof(uint32_t) * 2 * (size_t)E, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const uint32_t T = (flags >> 16) & 0xFFFFu;
+    if (T > 1) igc::reorder_threads(*host, T);
+    return IGC_OK;
+}
+
+IGC_API int igc_csr_to_host(igc_ctx *ctx, const igc_csr *dev, uint32_t flags, igc_csr *host)
+{
+    if (!ctx || !dev || !host) return set_error(IGC_ERR_INVALID, "null argument");
+    if (dev->location != IGC_ON_DEVICE) return set_error(IGC_ERR_INVALID, "source CSR is not on the device");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    return csr_to_host_locked(ctx, dev->n_nodes, dev->n_edges, dev->row_ptr, dev->col, flags, host);
+}
+
+IGC_API int igc_graph_build(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint32_t n, uint32_t edit_dist,
+                            uint32_t flags, igc_csr *out)
+{
+    int rc = check_args(ctx, N, n);
+    if (rc) return rc;
+    if (!out || (!syms && N)) return set_error(IGC_ERR_INVALID, "null argument");
+    std::memset(out, 0, sizeof *out);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t bytes = (size_t)N * n;
+    if ((rc = ctx->syms.ensure(std::max<size_t>(bytes, 1)))) return rc;
+    if (bytes) HIP_TRY(hipMemcpyAsync(ctx->syms.p, syms, bytes, hipMemcpyHostToDevice, ctx->stream));
+    uint64_t E = 0;
+    rc = build_device_locked(ctx, ctx->syms.as<uint8_t>(), N, n, edit_dist, &E, false);
+    if (rc) return rc;
+    return csr_to_host_locked(ctx, N, E, ctx->row_ptr.as<uint64_t>(), ctx->col.as<uint32_t>(), flags, out);
+}
+
+IGC_API int igc_graph_count(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint32_t n, uint32_t edit_dist,
+                            uint64_t *n_edges)
+{
+    int rc = check_args(ctx, N, n);
+    if (rc) return rc;
+    if (!n_edges || (!syms && N)) return set_error(IGC_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t bytes = (size_t)N * n;
+    if ((rc = ctx->syms.ensure(std::max<size_t>(bytes, 1)))) return rc;
+    if (bytes) HIP_TRY(hipMemcpyAsync(ctx->syms.p, syms, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return build_device_locked(ctx, ctx->syms.as<uint8_t>(), N, n, edit_dist, n_edges, true);
+}
+
+} // extern "C"

@@ -1,0 +1,151 @@
+// ref_driver.cpp -- a small command-line front end, written for this project,
+// over the IndexGen reference's own C++ sources (compiled in place from the
+// reference tree by oracle/Makefile into oracle/_ref/).
+//
+// TEST INFRASTRUCTURE ONLY: it produces golden vectors (tests/golden) and the
+// CPU baseline's "reference" timing.  Nothing in the shipped library uses it.
+//
+// Modes
+//   pairs                      stdin: lines "k text pattern" (symbols '0'..'3')
+//                              stdout: "banded wf" per line, where banded is
+//                              EditDistanceBanded(text, MakePattern(pattern), k)
+//                              (include/EditDistance.hpp:327-330) and wf is
+//                              FastEditDistance(text, pattern) (src/Utils.cpp:15-51)
+//   graph T minED cands dir    FillAdjList with T threads (src/SparseMat.cpp:567-615),
+//                              then AdjList::ToFile -> dir/adj.txt (neighbour
+//                              lists in list order), Codebook (:679-731) run
+//                              inside dir -> dir/codebook.txt (selection order);
+//                              prints "ones <2|E|> fill_s <t> greedy_s <t>"
+//   fill T minED cands         FillAdjList only, prints "ones <n> fill_s <t>"
+//                              (used as the timed CPU baseline)
+//   linear n minHD bias rowp colp maxRun minGC maxGC
+//                              Candidates(params) for the LinearCode generator
+//                              with MANUAL vectors (src/Candidates.cpp:178-188),
+//                              one candidate per line
+#include "CandidateGenerator.hpp"
+#include "Candidates.hpp"
+#include "EditDistance.hpp"
+#include "IndexGen.hpp"
+#include "SparseMat.hpp"
+#include "Utils.hpp"
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <filesystem>
+#include <fstream>
+#include <iostream>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <unistd.h>
+#include <vector>
+
+// Defined with external linkage in src/SparseMat.cpp but not declared in its header.
+void FillAdjList(AdjList &adjList, const std::vector<std::string> &candidates, const int minED, const int threadNum,
+                 const int saveInterval, const bool resume, long long int &matrixOnesNum);
+void Codebook(AdjList &adjList, std::vector<std::string> &codebook, const std::vector<std::string> &candidates,
+              const int saveInterval, const bool resume);
+
+static std::vector<std::string> read_lines(const std::string &path)
+{
+    std::ifstream in(path);
+    std::vector<std::string> out;
+    std::string s;
+    while (std::getline(in, s)) {
+        if (!s.empty() && s.back() == '\r') s.pop_back();
+        if (!s.empty()) out.push_back(s);
+    }
+    return out;
+}
+
+static std::vector<int> parse_csv(const std::string &s)
+{
+    std::vector<int> v;
+    if (s == "-") return v;
+    std::stringstream ss(s);
+    std::string tok;
+    while (std::getline(ss, tok, ',')) v.push_back(std::stoi(tok));
+    return v;
+}
+
+int main(int argc, char **argv)
+{
+    if (argc < 2) {
+        std::cerr << "usage: ref_driver pairs|graph|fill|linear ..." << std::endl;
+        return 2;
+    }
+    std::string mode = argv[1];
+    if (mode == "pairs") {
+        int k;
+        std::string text, pat;
+        std::ios::sync_with_stdio(false);
+        while (std::cin >> k >> text >> pat) {
+            PatternHandle H = MakePattern(pat);
+            int banded = EditDistanceBanded(text, H, k);
+            int wf = FastEditDistance(text, pat);
+            std::cout << banded << ' ' << wf << '\n';
+        }
+        return 0;
+    }
+    if ((mode == "graph" && argc == 6) || (mode == "fill" && argc == 5)) {
+        int T = std::atoi(argv[2]);
+        int minED = std::atoi(argv[3]);
+        std::vector<std::string> cands = read_lines(argv[4]);
+        AdjList adj;
+        long long ones = 0;
+        auto t0 = std::chrono::steady_clock::now();
+        // FillAdjList prints nothing; it writes/deletes per-thread progress files
+        // only when saveInterval > 0.
+        FillAdjList(adj, cands, minED, T, 0, false, ones);
+        auto t1 = std::chrono::steady_clock::now();
+        double fill_s = std::chrono::duration<double>(t1 - t0).count();
+        if (mode == "fill") {
+            std::cout << "ones " << ones << " fill_s " << fill_s << std::endl;
+            return 0;
+        }
+        std::string dir = argv[5];
+        adj.RowsBySum();
+        adj.ToFile(dir + "/adj.txt");
+        // Codebook() writes and removes progress_* files in the working directory.
+        if (chdir(dir.c_str()) != 0) {
+            std::cerr << "chdir failed" << std::endl;
+            return 1;
+        }
+        std::vector<std::string> codebook;
+        std::streambuf *old = std::cout.rdbuf();
+        std::ostringstream sink;
+        std::cout.rdbuf(sink.rdbuf());
+        auto t2 = std::chrono::steady_clock::now();
+        Codebook(adj, codebook, cands, 0, false);
+        auto t3 = std::chrono::steady_clock::now();
+        std::cout.rdbuf(old);
+        std::ofstream cb("codebook.txt");
+        for (const auto &w : codebook) cb << w << '\n';
+        std::cout << "ones " << ones << " fill_s " << fill_s << " greedy_s "
+                  << std::chrono::duration<double>(t3 - t2).count() << std::endl;
+        return 0;
+    }
+    if (mode == "linear" && argc == 10) {
+        int n = std::atoi(argv[2]);
+        int minHD = std::atoi(argv[3]);
+        auto lc = std::make_unique<LinearCodeConstraints>(minHD);
+        lc->bias = parse_csv(argv[4]);
+        lc->row_perm = parse_csv(argv[5]);
+        lc->col_perm = parse_csv(argv[6]);
+        lc->bias_mode = lc->bias.empty() ? VectorMode::DEFAULT : VectorMode::MANUAL;
+        lc->row_perm_mode = lc->row_perm.empty() ? VectorMode::DEFAULT : VectorMode::MANUAL;
+        lc->col_perm_mode = lc->col_perm.empty() ? VectorMode::DEFAULT : VectorMode::MANUAL;
+        Params params(n, 3, std::atoi(argv[7]), std::atof(argv[8]), std::atof(argv[9]), 1, 0,
+                      GenerationMethod::LINEAR_CODE, std::move(lc));
+        std::streambuf *old = std::cout.rdbuf();
+        std::ostringstream sink;
+        std::cout.rdbuf(sink.rdbuf());
+        std::vector<std::string> cands = Candidates(params);
+        std::cout.rdbuf(old);
+        for (const auto &s : cands) std::cout << s << '\n';
+        return 0;
+    }
+    std::cerr << "bad arguments" << std::endl;
+    return 2;
+}

@@ -1,0 +1,157 @@
+"""GPU parity of the conflict-graph build (libigc.so through its C ABI) with the
+oracle and the reference's golden outputs.  Integer work: every comparison is
+bit-exact (edge sets, CSR arrays, neighbour order, codebooks)."""
+import os
+
+import numpy as np
+import pytest
+
+import indexgen_amd as igc
+from indexgen_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+
+
+def check_csr_wellformed(adj: igc.AdjList):
+    N = adj.n_nodes
+    rp = adj.row_ptr.astype(np.int64)
+    assert rp[0] == 0 and rp[-1] == adj.col.size and (np.diff(rp) >= 0).all()
+    rows = np.repeat(np.arange(N, dtype=np.int64), np.diff(rp))
+    col = adj.col.astype(np.int64)
+    assert (col < N).all() and (col != rows).all()
+    # ascending within rows
+    same = rows[1:] == rows[:-1]
+    assert (col[1:][same] > col[:-1][same]).all()
+    # symmetric: sorted (row, col) equals sorted (col, row)
+    a = np.sort(rows * N + col)
+    b = np.sort(col * N + rows)
+    assert np.array_equal(a, b)
+
+
+def test_golden_graphs_and_codebooks(gpu_ctx, graphs_golden):
+    for k, s in enumerate(graphs_golden):
+        banner(f"golden set {k} shape={s['syms'].shape} min_ed={s['min_ed']}")
+        for T, (rp, col, words) in s["T"].items():
+            flags = igc.order_threads(T) if T > 1 else 0
+            adj = gpu_ctx.graph_build(s["syms"], s["min_ed"], flags)
+            assert np.array_equal(adj.row_ptr, rp)
+            assert np.array_equal(adj.col, col)
+            assert np.array_equal(s["syms"][igc.greedy_order(adj)], words)
+
+
+@pytest.mark.parametrize("n", list(range(1, 17)))
+def test_random_sets_every_length_and_threshold(gpu_ctx, n):
+    rng = np.random.default_rng(1000 + n)
+    N = 700 if n > 3 else 300
+    syms = rng.integers(0, 4, size=(N, n), dtype=np.uint8)
+    # plant near copies so every threshold sees edges
+    for r in range(0, N // 4):
+        src = syms[int(rng.integers(0, N))].copy()
+        src[int(rng.integers(0, n))] = int(rng.integers(0, 4))
+        syms[r] = src
+    for ed in range(0, n + 3):
+        banner(f"n={n} min_ed={ed} N={N}")
+        adj = gpu_ctx.graph_build(syms, ed)
+        rp, col = O.graph_csr(syms, ed)
+        assert np.array_equal(adj.row_ptr, rp), (n, ed)
+        assert np.array_equal(adj.col, col), (n, ed)
+
+
+def test_edge_cases(gpu_ctx):
+    banner("edge cases")
+    # empty and single inputs
+    for N in (0, 1):
+        adj = gpu_ctx.graph_build(np.zeros((N, 5), np.uint8), 3)
+        assert adj.n_nodes == N and adj.col.size == 0
+    # two strings
+    adj = gpu_ctx.graph_build(np.array([[0, 1, 2], [0, 1, 3]], np.uint8), 2)
+    assert adj.col.tolist() == [1, 0]
+    # all strings equal: complete graph
+    syms = np.tile(np.array([[1, 2, 3, 0, 1]], np.uint8), (300, 1))
+    adj = gpu_ctx.graph_build(syms, 1)
+    assert adj.n_edges == 300 * 299 // 2
+    check_csr_wellformed(adj)
+    # min_ed 0: no edges; min_ed > n: every pair
+    rng = np.random.default_rng(7)
+    syms = W.dedup_first(rng.integers(0, 4, size=(500, 6), dtype=np.uint8))
+    assert gpu_ctx.graph_build(syms, 0).n_edges == 0
+    assert gpu_ctx.graph_build(syms, 7).n_edges == syms.shape[0] * (syms.shape[0] - 1) // 2
+    assert gpu_ctx.graph_build(syms, 40).n_edges == syms.shape[0] * (syms.shape[0] - 1) // 2
+    # ragged sizes around tile and wave boundaries
+    for N in (63, 64, 65, 255, 256, 257, 1023, 1024, 1025, 4097):
+        s = rng.integers(0, 4, size=(N, 9), dtype=np.uint8)
+        adj = gpu_ctx.graph_build(s, 4)
+        rp, col = O.graph_csr(s, 4)
+        assert np.array_equal(adj.row_ptr, rp) and np.array_equal(adj.col, col), N
+    # invalid input is refused, not computed
+    with pytest.raises(igc.IGCError) as e:
+        gpu_ctx.graph_build(np.array([[0, 4, 1]], np.uint8).repeat(3, 0), 2)
+    assert e.value.code == igc.IGC_ERR_INVALID
+    with pytest.raises(igc.IGCError):
+        gpu_ctx.graph_build(np.zeros((4, 17), np.uint8), 3)
+
+
+def test_dense_rows_use_every_sort_path(gpu_ctx):
+    # random n=12 strings at editDist 5 give rows of ~90 neighbours (wave sort
+    # <= 64 and LDS bitonic sort <= 4096); a cluster of 5000 strings sharing 10
+    # symbols gives rows above 4096 (bitmap sort)
+    rng = np.random.default_rng(8)
+    syms = rng.integers(0, 4, size=(12000, 12), dtype=np.uint8)
+    syms[:5000, :10] = syms[0, :10]
+    banner("dense rows N=12000 n=12 min_ed=5")
+    adj = gpu_ctx.graph_build(syms, 5)
+    deg = np.diff(adj.row_ptr.astype(np.int64))
+    assert deg.max() > 4096 and (deg <= 64).any() and ((deg > 64) & (deg <= 4096)).any()
+    rp, col = O.graph_csr(syms, 5)
+    assert np.array_equal(adj.row_ptr, rp) and np.array_equal(adj.col, col)
+
+
+def test_plain_small_config_full(gpu_ctx):
+    syms = W.plain_small()
+    banner(f"plain_small N={syms.shape[0]} n=11 min_ed=3")
+    adj = gpu_ctx.graph_build(syms, 3)
+    rp, col = O.graph_csr(syms, 3)
+    assert np.array_equal(adj.row_ptr, rp) and np.array_equal(adj.col, col)
+    order = igc.greedy_order(adj)
+    assert np.array_equal(order.astype(np.int32), O.greedy(rp, col))
+    assert igc.verify_dist(syms[order], 3, gpu_ctx)
+    assert gpu_ctx.graph_count(syms, 3) == adj.n_edges
+
+
+def test_published_codebook_edge_counts(gpu_ctx, golden_dir):
+    for fn in sorted(os.listdir(golden_dir)):
+        if not fn.startswith("published_"):
+            continue
+        z = np.load(os.path.join(golden_dir, fn))
+        syms, ed = z["syms"], int(z["min_ed"])
+        banner(f"published {fn} N={syms.shape[0]} min_ed={ed}")
+        adj = gpu_ctx.graph_build(syms, ed)
+        assert adj.matrix_ones == int(z["ones"])
+        check_csr_wellformed(adj)
+        # the published codebook is an independent set of this graph
+        sel = np.zeros(adj.n_nodes, bool)
+        sel[z["order"]] = True
+        rows = np.repeat(np.arange(adj.n_nodes), np.diff(adj.row_ptr.astype(np.int64)))
+        assert not (sel[rows] & sel[adj.col]).any()
+
+
+def test_linear_large_config_properties(gpu_ctx):
+    syms = W.linear_large()
+    banner(f"linear_large N={syms.shape[0]} n=13 min_ed=4")
+    adj = gpu_ctx.graph_build(syms, 4)
+    check_csr_wellformed(adj)
+    # sampled rows against the oracle's row scan
+    rng = np.random.default_rng(9)
+    for i in rng.integers(0, syms.shape[0], 40):
+        assert np.array_equal(adj.neighbors(int(i)), O.row_neighbors(syms, 4, int(i)))
+    # repeated builds are identical, and the count-only mode agrees
+    adj2 = gpu_ctx.graph_build(syms, 4)
+    assert np.array_equal(adj.col, adj2.col) and np.array_equal(adj.row_ptr, adj2.row_ptr)
+    assert gpu_ctx.graph_count(syms, 4) == adj.n_edges
+    order = igc.greedy_order(adj)
+    assert igc.verify_dist(syms[order], 4, gpu_ctx)
