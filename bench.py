@@ -112,7 +112,18 @@ def main():
     ap.add_argument("--workload", default="linear_large")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 3 device-to-device copies of MIB MiB (scales rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(args.copy_calibration << 20, dtype=torch.uint8, device="cuda:0").fill_(1)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel(), "copies": 3}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the conflict-graph build here runs on a single MI355X "
               "(multi-GPU sharding is described in DESIGN.md, not run). Use --gpus 1.", file=sys.stderr)
@@ -163,13 +174,20 @@ def main():
     achieved = pairs * ops / (kern_ms / 1e3) / 1e12
     myers_ops = 17 * n
     edge_bytes = dcsr.n_edges * 8  # the kernel's compacted (i,j) writes
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_pairs_traffic.json")
+    if os.path.exists(tpath):
+        rec = json.load(open(tpath)).get(args.workload)
+        if rec:  # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE passes
+            traffic = int(rec["fetch_size_kib"] * 1024 * rec["fetch_scale"] + rec["write_size_kib"] * 1024 * rec["write_scale"])
     roofline = {
         "bound": "valu",
         "achieved": round(achieved, 3),
         "peak": round(PEAK_VALU_TOPS, 3),
         "unit": "Tops/s int32 VALU",
         "frac": round(achieved / PEAK_VALU_TOPS, 4),
-        "traffic": None,
+        "traffic": traffic,
+        "algorithmic_bytes": int(N * 4 + edge_bytes),
         "kernel": f"k_pairs<K={K},CPL=4>",
         "kernel_ms": round(kern_ms, 4),
         "ops_per_pair": ops,
