@@ -104,6 +104,48 @@ def cpu_baseline(syms: np.ndarray, min_ed: int, target_s: float = 12.0):
     }
 
 
+def batched_block(ctx, torch, steps: int = 3, seed: int = 3, sets_n: int = 64, scale: float = 1.0,
+                  label: str = "batched"):
+    """Uneven sets: one persistent launch for all of them against the plain form
+    (one host-API build per set: copy in, launch, synchronise, copy the CSR out --
+    what a per-cluster FillAdjListGPU call does, minus the process start)."""
+    from indexgen_amd import workloads as W
+    sets = W.batched(seed=seed, sets=sets_n, scale=scale)
+    pairs = sum(x.pairs for x in sets)
+    dev = [torch.from_numpy(np.ascontiguousarray(x.syms)).to("cuda:0") for x in sets]
+    spec = [(t.data_ptr(), x.syms.shape[0], x.n, x.edit_dist) for t, x in zip(dev, sets)]
+    torch.cuda.synchronize()
+    ctx.graph_build_batch_device(spec)  # warm
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        d, off, edges = ctx.graph_build_batch_device(spec)
+        lat.append((time.perf_counter() - s0) * 1e3)
+    batched_s = (time.perf_counter() - t0) / steps
+    tb = ctx.last_timing()
+    # plain: one build per set through the host API
+    t0 = time.perf_counter()
+    plain_edges = 0
+    for x in sets:
+        plain_edges += ctx.graph_build(x.syms, x.edit_dist).n_edges
+    plain_s = time.perf_counter() - t0
+    assert plain_edges == int(edges.sum()), "batched and per-set builds disagree"
+    return {
+        "workload": f"{label}: {len(sets)} sets, n in 10..16, editDist in 3..5, sizes lognormal (seed {seed}, scale {scale})",
+        "pairs": pairs,
+        "edges": int(edges.sum()),
+        "batched_gpairs_s": round(pairs / batched_s / 1e9, 3),
+        "batched_ms": round(batched_s * 1e3, 3),
+        "batched_pairs_kernel_ms": round(tb.pairs_ms, 3),
+        "batched_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                               "p99": round(float(np.percentile(lat, 99)), 3)},
+        "plain_per_set_gpairs_s": round(pairs / plain_s / 1e9, 3),
+        "plain_per_set_ms": round(plain_s * 1e3, 3),
+        "speedup_vs_plain": round(plain_s / batched_s, 2),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +154,7 @@ def main():
     ap.add_argument("--workload", default="linear_large")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-batched", action="store_true", help="skip the uneven-sets comparison block")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (scales rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -217,6 +260,10 @@ def main():
                          "codewords": int(order.size)},
         "roofline": roofline,
     }
+    if not args.no_batched:
+        result["batched"] = batched_block(ctx, torch)
+        result["many_small_sets"] = batched_block(ctx, torch, steps=5, seed=4, sets_n=256, scale=0.08,
+                                                  label="per-cluster sets")
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(syms, min_ed, args.cpu_seconds)
     print(json.dumps(result), flush=True)

@@ -136,6 +136,33 @@ IGC_API int igc_csr_reorder(igc_csr *csr, uint32_t flags);
 /* Releases a host CSR's arrays (no-op for device CSRs owned by a context). */
 IGC_API int igc_csr_free(igc_csr *csr);
 
+/* ---- many candidate sets in one launch ---- */
+
+/* One candidate set of a batch: N strings of length n (symbol codes 0..3,
+ * row-major), edge iff Levenshtein < edit_dist; flags as for igc_graph_build. */
+typedef struct igc_set {
+    const uint8_t *syms;
+    uint32_t N;
+    uint32_t n;
+    uint32_t edit_dist;
+    uint32_t flags;
+} igc_set;
+
+/* Builds the conflict graphs of `count` independent, unevenly sized sets (each
+ * with its own n and edit_dist) in one persistent kernel launch plus one CSR
+ * assembly over their disjoint union; outs[s] is set s's host CSR, identical
+ * to what igc_graph_build returns for that set alone.  Replaces one
+ * FillAdjListGPU process per cluster in the reference's clustered loop
+ * (SolveIndependentSet, src/SparseMat.cpp:784-851, called at :985-1020). */
+IGC_API int igc_graph_build_batch(igc_ctx *ctx, const igc_set *sets, uint32_t count, igc_csr *outs);
+
+/* Same with set symbols in device memory; the union CSR (global node ids) stays
+ * in device memory owned by the context.  node_off[count+1] receives each set's
+ * first global id (set s owns rows node_off[s]..node_off[s+1]-1, ids shifted
+ * by node_off[s]); set_edges[count] its edge count. */
+IGC_API int igc_graph_build_batch_device(igc_ctx *ctx, const igc_set *sets, uint32_t count, igc_csr *union_out,
+                                         uint32_t *node_off, uint64_t *set_edges);
+
 /* ---- greedy independent set and outputs (host) ---- */
 
 /* Min-degree greedy independent set of a host CSR, with the reference's exact

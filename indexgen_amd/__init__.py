@@ -73,6 +73,16 @@ class _Timing(ctypes.Structure):
     ]
 
 
+class _Set(ctypes.Structure):
+    _fields_ = [
+        ("syms", ctypes.c_void_p),
+        ("N", ctypes.c_uint32),
+        ("n", ctypes.c_uint32),
+        ("edit_dist", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
+    ]
+
+
 class _CodebookInfo(ctypes.Structure):
     _fields_ = [
         ("code_len", ctypes.c_int),
@@ -113,6 +123,8 @@ def lib() -> ctypes.CDLL:
         "igc_graph_build": (i, [vp, u8p, u32, u32, u32, u32, ctypes.POINTER(_Csr)]),
         "igc_graph_build_device": (i, [vp, u8p, u32, u32, u32, ctypes.POINTER(_Csr)]),
         "igc_graph_count": (i, [vp, u8p, u32, u32, u32, ctypes.POINTER(u64)]),
+        "igc_graph_build_batch": (i, [vp, ctypes.POINTER(_Set), u32, ctypes.POINTER(_Csr)]),
+        "igc_graph_build_batch_device": (i, [vp, ctypes.POINTER(_Set), u32, ctypes.POINTER(_Csr), vp, vp]),
         "igc_csr_to_host": (i, [vp, ctypes.POINTER(_Csr), u32, ctypes.POINTER(_Csr)]),
         "igc_csr_free": (i, [ctypes.POINTER(_Csr)]),
         "igc_csr_reorder": (i, [ctypes.POINTER(_Csr), u32]),
@@ -298,6 +310,31 @@ class Context:
         c = _Csr()
         _check(lib().igc_csr_to_host(self._h, ctypes.byref(dcsr._raw), int(flags), ctypes.byref(c)))
         return _adj_from_c(c)
+
+    def graph_build_batch(self, sets, flags: int = 0) -> list:
+        """Conflict graphs of many independent sets [(candidates, min_ed), ...] in
+        one persistent launch; one AdjList per set, in order."""
+        arrs = [to_symbols(c) for c, _ in sets]
+        cs = (_Set * max(len(sets), 1))()
+        for k, (a, (_, ed)) in enumerate(zip(arrs, sets)):
+            cs[k] = _Set(a.ctypes.data if a.size else None, a.shape[0], a.shape[1], int(ed), int(flags))
+        outs = (_Csr * max(len(sets), 1))()
+        _check(lib().igc_graph_build_batch(self._h, cs, len(sets), outs))
+        return [_adj_from_c(outs[k]) for k in range(len(sets))]
+
+    def graph_build_batch_device(self, dev_sets):
+        """dev_sets: [(device_ptr, N, n, min_ed), ...].  Returns (union DeviceCSR,
+        node_off list, per-set edge counts); the CSR stays in HBM."""
+        m = len(dev_sets)
+        cs = (_Set * m)(*[_Set(p, N, n, ed, 0) for p, N, n, ed in dev_sets])
+        off = np.zeros(m + 1, dtype=np.uint32)
+        edges = np.zeros(m, dtype=np.uint64)
+        c = _Csr()
+        _check(lib().igc_graph_build_batch_device(self._h, cs, m, ctypes.byref(c), off.ctypes.data,
+                                                  edges.ctypes.data))
+        d = DeviceCSR(int(c.n_nodes), int(c.n_edges), ctypes.cast(c.row_ptr, ctypes.c_void_p).value or 0,
+                      ctypes.cast(c.col, ctypes.c_void_p).value or 0, c)
+        return d, off, edges
 
     def graph_count(self, syms, min_ed: int) -> int:
         s = to_symbols(syms)

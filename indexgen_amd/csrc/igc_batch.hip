@@ -1,0 +1,403 @@
+// igc_batch.hip -- many uneven candidate sets in one persistent launch.
+//
+// The sets are treated as one disjoint union of graphs on a concatenated node
+// space (set s owns global ids node_off[s] .. node_off[s]+N_s-1).  One pack
+// kernel packs every set (per-set length n), one persistent pair kernel pulls
+// tiles of all sets from a global work counter (the set of a tile is found by
+// binary search over per-set tile prefix sums; K is uniform per tile and is
+// dispatched at run time), and the unchanged CSR assembly runs once over the
+// union.  Set s's CSR is the row slice node_off[s].. with ids shifted back.
+//
+// This replaces the reference's one process + one GPU context + one launch per
+// cluster (SolveIndependentSet, src/SparseMat.cpp:784-851, called from the
+// cluster loop at :985-1020).
+#include "igc_device.hpp"
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+namespace igc {
+
+// Largest K the persistent kernel dispatches at run time; sets with a larger
+// (non-trivial) K get their own k_pairs launch into the same edge list.
+constexpr int kBatchMaxK = 8;
+
+struct SetDesc {
+    TileGeom G;          // per-set geometry (G.N = N_s)
+    uint64_t tile_begin; // first global tile of the set
+    uint64_t sym_off;    // byte offset of the set's symbols in the concatenated input
+    uint32_t node_off;   // first global node id
+    uint32_t n;          // string length
+    int32_t K;           // Levenshtein threshold (edge iff ED <= K); kAllPairs = every pair
+    uint32_t pad;
+};
+
+static __device__ __forceinline__ uint32_t find_set_by(const SetDesc *__restrict__ sets, uint32_t nsets, uint64_t key,
+                                                       bool by_node)
+{
+    uint32_t lo = 0, hi = nsets - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        const uint64_t v = by_node ? (uint64_t)sets[mid].node_off : sets[mid].tile_begin;
+        if (v <= key) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+static __global__ void k_pack_batch(const uint8_t *__restrict__ syms, const SetDesc *__restrict__ sets,
+                                    uint32_t nsets, uint32_t total, uint32_t *__restrict__ packed,
+                                    uint32_t *__restrict__ bad)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const uint32_t s = find_set_by(sets, nsets, g, true);
+    const SetDesc &d = sets[s];
+    const uint32_t n = d.n;
+    const uint8_t *p = syms + d.sym_off + (size_t)(g - d.node_off) * n;
+    uint32_t or_all = 0;
+    for (uint32_t q = 0; q < n; ++q) or_all |= p[q];
+    packed[g] = pack_word(p, n);
+    if (or_all > 3u) atomicOr(bad, 1u);
+}
+
+template <int CPL>
+__device__ __forceinline__ void batch_tile(int K, const uint32_t *__restrict__ packed, const TileGeom &G, uint32_t rb,
+                                           uint32_t cb, uint32_t node_off, uint64_t *st, uint32_t lane,
+                                           uint32_t wave, uint64_t *__restrict__ edges, uint64_t cap,
+                                           unsigned long long *__restrict__ counter)
+{
+    switch (K) {
+#define IGC_BCASE(k)                                                                                          \
+    case k:                                                                                                   \
+        wave_tile<k, CPL, true>(packed, G, rb, cb, node_off, st, lane, wave, edges, cap, counter);            \
+        break;
+        IGC_BCASE(0) IGC_BCASE(1) IGC_BCASE(2) IGC_BCASE(3) IGC_BCASE(4) IGC_BCASE(5) IGC_BCASE(6)
+        IGC_BCASE(7) IGC_BCASE(8) IGC_BCASE(16)
+#undef IGC_BCASE
+    default:
+        break;
+    }
+}
+
+// Persistent: each workgroup loops, taking the next tile from the work
+// counter until all `total_tiles` are done.  Tiles are ordered heaviest set
+// first (host side), so the tail is made of cheap tiles.
+template <int CPL>
+__global__ __launch_bounds__(256) void k_pairs_batch(const uint32_t *__restrict__ packed,
+                                                     const SetDesc *__restrict__ sets, uint32_t nsets,
+                                                     uint64_t total_tiles, unsigned long long *__restrict__ work,
+                                                     uint64_t *__restrict__ edges, uint64_t cap,
+                                                     unsigned long long *__restrict__ counter)
+{
+    __shared__ uint64_t stage[kWaves][kStage];
+    __shared__ unsigned long long next;
+    const uint32_t lane = lane_id();
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(work, 1ull);
+        __syncthreads();
+        const uint64_t t = next;
+        __syncthreads();
+        if (t >= total_tiles) break;
+        const uint32_t s = find_set_by(sets, nsets, t, false);
+        const SetDesc d = sets[s];
+        uint32_t rb, cb;
+        decode_tile(t - d.tile_begin, d.G, rb, cb);
+        batch_tile<CPL>(d.K, packed + d.node_off, d.G, rb, cb, d.node_off, stage[wave], lane, wave, edges, cap,
+                        counter);
+    }
+}
+
+// Gathers row_ptr at the set boundaries so the host can split the union CSR.
+static __global__ void k_gather_bounds(const uint64_t *__restrict__ row_ptr, const SetDesc *__restrict__ sets,
+                                       uint32_t nsets, uint32_t total, uint64_t *__restrict__ out)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nsets) out[s] = row_ptr[sets[s].node_off];
+    if (s == nsets) out[s] = row_ptr[total];
+}
+
+struct BatchPlan {
+    std::vector<SetDesc> desc;     // in processing order (heaviest first); tile_begin ascending
+    std::vector<SetDesc> by_node;  // the same descriptors in caller order; node_off ascending
+    std::vector<uint32_t> order;   // desc index -> caller's set index
+    std::vector<uint32_t> node_off_by_set;
+    std::vector<uint32_t> extra;   // desc indices launched separately (9 <= K <= 15)
+    uint64_t tiles = 0;            // tiles of the persistent kernel
+    uint64_t pairs = 0;
+    uint64_t sym_bytes = 0;
+    uint32_t total = 0;
+    uint64_t cap = 0;
+};
+
+static int plan_batch(igc_ctx *ctx, const igc_set *sets, uint32_t count, int cpl, BatchPlan &P)
+{
+    P = BatchPlan();
+    P.node_off_by_set.assign(count + 1, 0);
+    std::vector<uint32_t> idx(count);
+    std::iota(idx.begin(), idx.end(), 0u);
+    std::vector<int> K(count);
+    uint64_t total = 0, symb = 0;
+    std::vector<uint64_t> sym_off(count);
+    for (uint32_t s = 0; s < count; ++s) {
+        const igc_set &x = sets[s];
+        if (x.n < 1 || x.n > IGC_MAX_LEN)
+            return set_error(IGC_ERR_INVALID, "set %u: string length n=%u outside 1..%d", s, x.n, IGC_MAX_LEN);
+        if (!x.syms && x.N) return set_error(IGC_ERR_INVALID, "set %u: null symbols", s);
+        K[s] = x.edit_dist == 0 ? -1 : effective_k(x.n, x.edit_dist);
+        P.node_off_by_set[s] = (uint32_t)total;
+        sym_off[s] = symb;
+        total += x.N;
+        symb += (uint64_t)x.N * x.n;
+        P.pairs += (uint64_t)x.N * (x.N > 0 ? x.N - 1 : 0) / 2;
+    }
+    if (total > 0x7FFFFFFFull) return set_error(IGC_ERR_INVALID, "batch has %llu nodes", (unsigned long long)total);
+    P.node_off_by_set[count] = (uint32_t)total;
+    P.total = (uint32_t)total;
+    P.sym_bytes = symb;
+    // heaviest tiles first: larger K costs more per pair; then larger sets
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
+        const int ka = K[a] == kAllPairs ? -1 : K[a], kb = K[b] == kAllPairs ? -1 : K[b];
+        if (ka != kb) return ka > kb;
+        return sets[a].N > sets[b].N;
+    });
+    double est_edges = 0;
+    for (uint32_t s : idx) {
+        const igc_set &x = sets[s];
+        SetDesc d{};
+        const Shape sh{cpl, 1};
+        d.G = choose_geometry(x.N, sh.cpl, x.N >= 8192 ? 1u : (x.N >= 2048 ? 2u : 4u));
+        d.sym_off = sym_off[s];
+        d.node_off = P.node_off_by_set[s];
+        d.n = x.n;
+        d.K = K[s];
+        const uint64_t pairs = (uint64_t)x.N * (x.N > 0 ? x.N - 1 : 0) / 2;
+        if (K[s] >= 0 && x.N >= 2) est_edges += (double)initial_edge_cap(x.N, x.n, K[s]);
+        (void)pairs;
+        const bool runnable = K[s] >= 0 && x.N >= 2;
+        const bool in_kernel = runnable && (K[s] <= kBatchMaxK || K[s] == kAllPairs);
+        d.tile_begin = P.tiles;
+        if (in_kernel) P.tiles += tile_count(d.G);
+        if (runnable && !in_kernel) P.extra.push_back((uint32_t)P.desc.size());
+        if (!in_kernel) d.K = -1; // never dispatched by the persistent kernel (it owns no tiles)
+        P.order.push_back(s);
+        P.desc.push_back(d);
+    }
+    // true K of the separately launched sets
+    for (uint32_t e : P.extra) P.desc[e].K = K[P.order[e]];
+    P.by_node.resize(P.desc.size());
+    for (size_t k = 0; k < P.desc.size(); ++k) P.by_node[P.order[k]] = P.desc[k];
+    P.cap = std::max<uint64_t>((uint64_t)est_edges, 1);
+    (void)ctx;
+    return IGC_OK;
+}
+
+template <int CPL>
+static void launch_batch_kernel(igc_ctx *ctx, const BatchPlan &P, const SetDesc *d_sets, unsigned long long *work,
+                                unsigned long long *counter, uint64_t cap)
+{
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(P.tiles, (uint64_t)ctx->cu_count * 8);
+    hipLaunchKernelGGL((k_pairs_batch<CPL>), dim3(grid), dim3(256), 0, ctx->stream, ctx->packed.as<uint32_t>(),
+                       d_sets, (uint32_t)P.desc.size(), P.tiles, work, ctx->edges.as<uint64_t>(), cap, counter);
+}
+
+// Builds the union CSR of all sets in ctx->row_ptr / ctx->col; fills
+// bounds_out[0..count] (row_ptr at each set's first row, in caller order) and
+// the union edge count.  d_syms: concatenated symbols already on the device.
+static int batch_device_locked(igc_ctx *ctx, const uint8_t *d_syms, const BatchPlan &Pin, int cpl,
+                               std::vector<uint64_t> &bounds_out, uint64_t *n_edges)
+{
+    BatchPlan P = Pin;
+    hipStream_t s = ctx->stream;
+    igc_timing tm{};
+    tm.pairs = P.pairs;
+    const uint32_t nd = (uint32_t)P.desc.size();
+    int rc;
+    if ((rc = ctx->packed.ensure(sizeof(uint32_t) * std::max<size_t>(P.total, 1)))) return rc;
+    if ((rc = ctx->misc.ensure(sizeof(uint32_t) * kMiscWords))) return rc;
+    if ((rc = ctx->sets.ensure(2 * sizeof(SetDesc) * std::max<size_t>(nd, 1)))) return rc;
+    if ((rc = ctx->bounds.ensure(sizeof(uint64_t) * ((size_t)nd + 1)))) return rc;
+    const SetDesc *d_tile_order = ctx->sets.as<SetDesc>();
+    const SetDesc *d_node_order = ctx->sets.as<SetDesc>() + nd;
+    HIP_TRY(hipMemcpyAsync(ctx->sets.p, P.desc.data(), sizeof(SetDesc) * nd, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(ctx->sets.as<SetDesc>() + nd, P.by_node.data(), sizeof(SetDesc) * nd, hipMemcpyHostToDevice,
+                           s));
+    uint32_t *misc = ctx->misc.as<uint32_t>();
+    unsigned long long *counter = reinterpret_cast<unsigned long long *>(misc);
+    unsigned long long *work = reinterpret_cast<unsigned long long *>(misc + 6);
+    uint64_t cap = P.cap;
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        if ((rc = ctx->edges.ensure(sizeof(uint64_t) * (size_t)cap))) return rc;
+        HIP_TRY(hipEventRecord(ctx->ev[0], s));
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, misc, (uint64_t)kMiscWords, 0u);
+        if (P.total)
+            hipLaunchKernelGGL(k_pack_batch, dim3((P.total + 255) / 256), dim3(256), 0, s, d_syms, d_node_order, nd,
+                               P.total, ctx->packed.as<uint32_t>(), misc + 2);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(ctx->ev[1], s));
+        uint32_t launches = 2;
+        if (P.tiles) {
+            if (cpl == 1) launch_batch_kernel<1>(ctx, P, d_tile_order, work, counter, cap);
+            else if (cpl == 2) launch_batch_kernel<2>(ctx, P, d_tile_order, work, counter, cap);
+            else launch_batch_kernel<4>(ctx, P, d_tile_order, work, counter, cap);
+            HIP_TRY(hipGetLastError());
+            launches++;
+        }
+        for (uint32_t e : P.extra) {
+            const SetDesc &d = P.desc[e];
+            if ((rc = launch_pairs(d.K, cpl, true, s, tile_count(d.G), ctx->packed.as<uint32_t>() + d.node_off, d.G,
+                                   d.node_off, ctx->edges.as<uint64_t>(), cap, counter)))
+                return rc;
+            launches++;
+        }
+        HIP_TRY(hipEventRecord(ctx->ev[2], s));
+        if ((rc = launch_csr(ctx, P.total, cap, counter, misc, &launches))) return rc;
+        hipLaunchKernelGGL(k_gather_bounds, dim3((nd + 1 + 255) / 256), dim3(256), 0, s, ctx->row_ptr.as<uint64_t>(),
+                           d_node_order, nd, P.total, ctx->bounds.as<uint64_t>());
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(ctx->ev[3], s));
+        uint32_t host_misc[kMiscWords];
+        std::vector<uint64_t> b(nd + 1);
+        HIP_TRY(hipMemcpyAsync(host_misc, misc, sizeof host_misc, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(b.data(), ctx->bounds.p, sizeof(uint64_t) * (nd + 1), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (host_misc[2]) return set_error(IGC_ERR_INVALID, "symbol code above 3 in input");
+        const uint64_t E = (uint64_t)host_misc[0] | ((uint64_t)host_misc[1] << 32);
+        HIP_TRY(hipEventElapsedTime(&tm.pack_ms, ctx->ev[0], ctx->ev[1]));
+        HIP_TRY(hipEventElapsedTime(&tm.pairs_ms, ctx->ev[1], ctx->ev[2]));
+        HIP_TRY(hipEventElapsedTime(&tm.csr_ms, ctx->ev[2], ctx->ev[3]));
+        HIP_TRY(hipEventElapsedTime(&tm.total_ms, ctx->ev[0], ctx->ev[3]));
+        tm.launches = launches + 1;
+        tm.retries = (uint32_t)attempt;
+        tm.n_edges = E;
+        if (E <= cap) {
+            bounds_out = b; // caller order: the gather walked the node-ordered table
+            *n_edges = E;
+            ctx->last = tm;
+            return IGC_OK;
+        }
+        cap = E + E / 64 + 1024;
+    }
+    return set_error(IGC_ERR_DEVICE, "edge buffer kept overflowing");
+}
+
+// Host CSR of set s from the union CSR already copied to pinned host memory.
+static int split_set(const uint64_t *rp, const uint32_t *col, uint32_t off, uint32_t N, uint32_t flags, igc_csr *out)
+{
+    const uint64_t base = rp[off];
+    const uint64_t ents = rp[off + N] - base;
+    out->n_nodes = N;
+    out->location = IGC_ON_HOST;
+    out->n_edges = ents / 2;
+    out->row_ptr = static_cast<uint64_t *>(std::malloc(sizeof(uint64_t) * ((size_t)N + 1)));
+    out->col = static_cast<uint32_t *>(std::malloc(sizeof(uint32_t) * std::max<uint64_t>(ents, 1)));
+    if (!out->row_ptr || !out->col) {
+        igc_csr_free(out);
+        return set_error(IGC_ERR_OOM, "host CSR allocation failed");
+    }
+    for (uint32_t x = 0; x <= N; ++x) out->row_ptr[x] = rp[off + x] - base;
+    for (uint64_t q = 0; q < ents; ++q) out->col[q] = col[base + q] - off;
+    const uint32_t T = (flags >> 16) & 0xFFFFu;
+    if (T > 1) reorder_threads(*out, T);
+    return IGC_OK;
+}
+
+static int batch_shape(const igc_set *sets, uint32_t count)
+{
+    uint64_t maxN = 0;
+    for (uint32_t s = 0; s < count; ++s) maxN = std::max<uint64_t>(maxN, sets[s].N);
+    return maxN >= 32768 ? 4 : (maxN >= 4096 ? 2 : 1);
+}
+
+} // namespace igc
+
+using namespace igc;
+
+extern "C" {
+
+IGC_API int igc_graph_build_batch(igc_ctx *ctx, const igc_set *sets, uint32_t count, igc_csr *outs)
+{
+    if (!ctx || (!sets && count) || (!outs && count)) return set_error(IGC_ERR_INVALID, "null argument");
+    for (uint32_t s = 0; s < count; ++s) std::memset(&outs[s], 0, sizeof(igc_csr));
+    if (count == 0) return IGC_OK;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int cpl = batch_shape(sets, count);
+    BatchPlan P;
+    int rc = plan_batch(ctx, sets, count, cpl, P);
+    if (rc) return rc;
+    if ((rc = ctx->syms.ensure(std::max<size_t>(P.sym_bytes, 1)))) return rc;
+    {
+        uint64_t off = 0;
+        for (uint32_t s = 0; s < count; ++s) {
+            const size_t bytes = (size_t)sets[s].N * sets[s].n;
+            if (bytes)
+                HIP_TRY(hipMemcpyAsync(ctx->syms.as<uint8_t>() + off, sets[s].syms, bytes, hipMemcpyHostToDevice,
+                                       ctx->stream));
+            off += bytes;
+        }
+    }
+    std::vector<uint64_t> bounds;
+    uint64_t E = 0;
+    if ((rc = batch_device_locked(ctx, ctx->syms.as<uint8_t>(), P, cpl, bounds, &E))) return rc;
+    // union CSR -> pinned staging -> per-set host CSRs
+    const uint64_t ents = 2 * E;
+    if ((rc = ctx->host_rp.ensure(sizeof(uint64_t) * ((size_t)P.total + 1)))) return rc;
+    if ((rc = ctx->host_col.ensure(sizeof(uint32_t) * std::max<uint64_t>(ents, 1)))) return rc;
+    HIP_TRY(hipMemcpyAsync(ctx->host_rp.p, ctx->row_ptr.p, sizeof(uint64_t) * ((size_t)P.total + 1),
+                           hipMemcpyDeviceToHost, ctx->stream));
+    if (ents)
+        HIP_TRY(hipMemcpyAsync(ctx->host_col.p, ctx->col.p, sizeof(uint32_t) * ents, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (uint32_t s = 0; s < count; ++s) {
+        rc = split_set(ctx->host_rp.as<uint64_t>(), ctx->host_col.as<uint32_t>(), P.node_off_by_set[s], sets[s].N,
+                       sets[s].flags, &outs[s]);
+        if (rc) {
+            for (uint32_t t = 0; t < s; ++t) igc_csr_free(&outs[t]);
+            return rc;
+        }
+    }
+    return IGC_OK;
+}
+
+IGC_API int igc_graph_build_batch_device(igc_ctx *ctx, const igc_set *sets, uint32_t count, igc_csr *union_out,
+                                         uint32_t *node_off, uint64_t *set_edges)
+{
+    if (!ctx || !sets || !union_out || !node_off || !set_edges || count == 0)
+        return set_error(IGC_ERR_INVALID, "null argument or empty batch");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int cpl = batch_shape(sets, count);
+    BatchPlan P;
+    int rc = plan_batch(ctx, sets, count, cpl, P);
+    if (rc) return rc;
+    if ((rc = ctx->syms.ensure(std::max<size_t>(P.sym_bytes, 1)))) return rc;
+    {
+        uint64_t off = 0;
+        for (uint32_t s = 0; s < count; ++s) {
+            const size_t bytes = (size_t)sets[s].N * sets[s].n;
+            if (bytes)
+                HIP_TRY(hipMemcpyAsync(ctx->syms.as<uint8_t>() + off, sets[s].syms, bytes, hipMemcpyDeviceToDevice,
+                                       ctx->stream));
+            off += bytes;
+        }
+    }
+    std::vector<uint64_t> bounds;
+    uint64_t E = 0;
+    if ((rc = batch_device_locked(ctx, ctx->syms.as<uint8_t>(), P, cpl, bounds, &E))) return rc;
+    for (uint32_t s = 0; s <= count; ++s) node_off[s] = P.node_off_by_set[s];
+    for (uint32_t s = 0; s < count; ++s) set_edges[s] = (bounds[s + 1] - bounds[s]) / 2;
+    union_out->n_nodes = P.total;
+    union_out->location = IGC_ON_DEVICE;
+    union_out->n_edges = E;
+    union_out->row_ptr = ctx->row_ptr.as<uint64_t>();
+    union_out->col = ctx->col.as<uint32_t>();
+    return IGC_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,119 @@
+// igc_device.hpp -- device-side plumbing shared by the HIP translation units of
+// libigc.so: the context, grow-only device buffers and the launch helpers.
+#pragma once
+#include "igc_internal.hpp"
+#include "igc_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+
+#define HIP_TRY(expr)                                                                                         \
+    do {                                                                                                      \
+        hipError_t e_ = (expr);                                                                               \
+        if (e_ != hipSuccess)                                                                                 \
+            return ::igc::set_error(e_ == hipErrorOutOfMemory ? IGC_ERR_OOM : IGC_ERR_DEVICE,                 \
+                                    "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+namespace igc {
+
+// grow-only device buffer
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t want)
+    {
+        if (want <= bytes && p) return IGC_OK;
+        release();
+        const size_t sz = std::max<size_t>(want, 256);
+        const hipError_t e = hipMalloc(&p, sz);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return set_error(IGC_ERR_OOM, "hipMalloc(%zu) failed: %s", sz, hipGetErrorString(e));
+        }
+        bytes = sz;
+        return IGC_OK;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// grow-only pinned host buffer (fast device-to-host copies of the CSR)
+struct HostBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t want)
+    {
+        if (want <= bytes && p) return IGC_OK;
+        release();
+        const size_t sz = std::max<size_t>(want, 256);
+        const hipError_t e = hipHostMalloc(&p, sz, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return set_error(IGC_ERR_OOM, "hipHostMalloc(%zu) failed: %s", sz, hipGetErrorString(e));
+        }
+        bytes = sz;
+        return IGC_OK;
+    }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// misc buffer layout (uint32 words): [0..1] edge counter (u64), [2] bad symbol
+// flag, [4] mid-row count, [5] big-row count, [6..7] batch work counter (u64)
+constexpr size_t kMiscWords = 8;
+
+// Tile shape of the single-set kernel: columns per lane and rows per tile W/q.
+struct Shape {
+    int cpl;
+    uint32_t q;
+};
+
+TileGeom choose_geometry(uint32_t N, int cpl, uint32_t q);
+uint64_t tile_count(const TileGeom &G);
+Shape default_shape(uint32_t N);
+uint64_t initial_edge_cap(uint32_t N, uint32_t n, int K);
+int effective_k(uint32_t n, uint32_t edit_dist);
+uint32_t grid_for(uint64_t work, uint32_t per_block, uint32_t cap_blocks);
+
+// Launches k_pairs<K, cpl, emit> over `tiles` tiles of geometry G.
+int launch_pairs(int K, int cpl, bool emit, hipStream_t s, uint64_t tiles, const uint32_t *packed,
+                 const TileGeom &G, uint32_t node_off, uint64_t *edges, uint64_t cap, unsigned long long *counter);
+
+} // namespace igc
+
+struct igc_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    igc::DevBuf syms, packed, edges, misc, deg, cursor, row_ptr, col, mid, big, bsum, sets, bounds;
+    igc::HostBuf host_rp, host_col;
+    hipEvent_t ev[5] = {};
+    igc_timing last{};
+    int cu_count = 256;
+    // tile shapes chosen by the autotuner, keyed by (K, N bucket); 0 = default
+    igc::Shape tuned[17][8] = {};
+};
+
+namespace igc {
+// Builds the CSR of the edge list in ctx->edges (count in counter, capacity cap)
+// over N nodes into ctx->row_ptr / ctx->col; asynchronous on ctx->stream.
+int launch_csr(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *counter, uint32_t *misc,
+               uint32_t *launches);
+Shape pick_shape(igc_ctx *ctx, int K, uint32_t N);
+} // namespace igc

@@ -38,7 +38,7 @@ constexpr int kStage = 256;        // edges staged per wave in LDS before a flus
 
 // ---------------------------------------------------------------------------
 // symbol packing
-__global__ void k_pack(const uint8_t *__restrict__ syms, uint32_t N, uint32_t n, uint32_t *__restrict__ packed,
+static __global__ void k_pack(const uint8_t *__restrict__ syms, uint32_t N, uint32_t n, uint32_t *__restrict__ packed,
                        uint32_t *__restrict__ bad)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -55,7 +55,7 @@ __global__ void k_pack(const uint8_t *__restrict__ syms, uint32_t N, uint32_t n,
     if (or_all > 3u) atomicOr(bad, 1u);
 }
 
-__global__ void k_fill_u32(uint32_t *__restrict__ p, uint64_t count, uint32_t v)
+static __global__ void k_fill_u32(uint32_t *__restrict__ p, uint64_t count, uint32_t v)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;
@@ -133,23 +133,20 @@ __device__ __forceinline__ void flush_stage(const uint64_t *st, uint32_t staged,
     __builtin_amdgcn_wave_barrier();
 }
 
-// One workgroup = one tile: R rows x W columns, each of the 4 waves owning
-// 64*CPL consecutive columns (lane l, slot c -> column colw + 64c + l).  Rows
-// are walked in order with the row word uniform (scalar registers), so each
-// lane tests CPL pairs per row.  Nonzero ballots (rare) are compacted through
-// a per-wave LDS stage.  EMIT=false only counts.
+// One wave's share of tile (rb, cb) of a candidate set: rows rb*R .. rb*R+R-1,
+// columns colw .. colw+64*CPL-1 with colw = cb*W + wave*64*CPL (lane l, slot c
+// -> column colw + 64c + l).  Rows are walked in order with the row word
+// uniform (scalar registers), so each lane tests CPL pairs per row.  Nonzero
+// ballots (rare) are compacted through the wave's LDS stage `st`; edges are
+// written as global ids (node_off + i, node_off + j).  EMIT=false only counts.
+// Contains no workgroup barrier: a wave may return early.
 template <int K, int CPL, bool EMIT>
-__global__ __launch_bounds__(256) void k_pairs(const uint32_t *__restrict__ packed, TileGeom G,
-                                               uint64_t *__restrict__ edges, uint64_t cap,
-                                               unsigned long long *__restrict__ counter)
+__device__ __forceinline__ void wave_tile(const uint32_t *__restrict__ packed, const TileGeom &G, uint32_t rb,
+                                          uint32_t cb, uint32_t node_off, uint64_t *st, uint32_t lane,
+                                          uint32_t wave, uint64_t *__restrict__ edges, uint64_t cap,
+                                          unsigned long long *__restrict__ counter)
 {
     constexpr int WD = LvShape<K>::WD;
-    __shared__ uint64_t stage[kWaves][kStage];
-
-    const uint32_t lane = lane_id();
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    uint32_t rb, cb;
-    decode_tile(blockIdx.x, G, rb, cb);
     const uint32_t N = G.N;
     const uint32_t row0 = rb * G.R;
     const uint32_t colw = cb * G.W + wave * 64u * CPL;
@@ -165,7 +162,6 @@ __global__ __launch_bounds__(256) void k_pairs(const uint32_t *__restrict__ pack
         const uint32_t j = colw + 64u * c + lane;
         b[c] = j < N ? packed[j] : 0u;
     }
-    uint64_t *st = stage[wave];
     uint32_t staged = 0;
     uint64_t counted = 0;
 
@@ -195,7 +191,7 @@ __global__ __launch_bounds__(256) void k_pairs(const uint32_t *__restrict__ pack
                         flush_stage(st, staged, lane, edges, cap, counter);
                         staged = 0;
                     }
-                    if (mine) st[staged + rank_in_mask(m2)] = ((uint64_t)i << 32) | j;
+                    if (mine) st[staged + rank_in_mask(m2)] = ((uint64_t)(node_off + i) << 32) | (node_off + j);
                     staged += cnt;
                 } else {
                     counted += cnt;
@@ -210,6 +206,20 @@ __global__ __launch_bounds__(256) void k_pairs(const uint32_t *__restrict__ pack
     }
 }
 
+// Single set: one workgroup per tile of the triangle (blockIdx -> tile).
+template <int K, int CPL, bool EMIT>
+__global__ __launch_bounds__(256) void k_pairs(const uint32_t *__restrict__ packed, TileGeom G, uint32_t node_off,
+                                               uint64_t *__restrict__ edges, uint64_t cap,
+                                               unsigned long long *__restrict__ counter)
+{
+    __shared__ uint64_t stage[kWaves][kStage];
+    const uint32_t lane = lane_id();
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    uint32_t rb, cb;
+    decode_tile(blockIdx.x, G, rb, cb);
+    wave_tile<K, CPL, EMIT>(packed, G, rb, cb, node_off, stage[wave], lane, wave, edges, cap, counter);
+}
+
 // ---------------------------------------------------------------------------
 // CSR assembly
 
@@ -219,7 +229,7 @@ __device__ __forceinline__ uint64_t edge_total(const unsigned long long *counter
     return e < cap ? e : cap;
 }
 
-__global__ void k_degree(const uint64_t *__restrict__ edges, const unsigned long long *__restrict__ counter,
+static __global__ void k_degree(const uint64_t *__restrict__ edges, const unsigned long long *__restrict__ counter,
                          uint64_t cap, uint32_t *__restrict__ deg)
 {
     const uint64_t E = edge_total(counter, cap);
@@ -257,7 +267,7 @@ __device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t v, uint64_t *s
     return total;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *__restrict__ deg, uint32_t N,
+static __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *__restrict__ deg, uint32_t N,
                                                                uint64_t *__restrict__ bsum)
 {
     __shared__ uint64_t sh[kScanThreads / 64 + 1];
@@ -271,7 +281,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *__
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_blocks(uint64_t *__restrict__ bsum, uint32_t nb)
+static __global__ __launch_bounds__(kScanThreads) void k_scan_blocks(uint64_t *__restrict__ bsum, uint32_t nb)
 {
     __shared__ uint64_t sh[kScanThreads / 64 + 1];
     uint64_t carry = 0;
@@ -285,7 +295,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_blocks(uint64_t *__restri
     }
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t *__restrict__ deg, uint32_t N,
+static __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t *__restrict__ deg, uint32_t N,
                                                               const uint64_t *__restrict__ boff,
                                                               uint64_t *__restrict__ row_ptr)
 {
@@ -311,7 +321,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t *__r
     }
 }
 
-__global__ void k_scatter(const uint64_t *__restrict__ edges, const unsigned long long *__restrict__ counter,
+static __global__ void k_scatter(const uint64_t *__restrict__ edges, const unsigned long long *__restrict__ counter,
                           uint64_t cap, const uint64_t *__restrict__ row_ptr, uint32_t *__restrict__ cursor,
                           uint32_t *__restrict__ col)
 {
@@ -330,7 +340,7 @@ __global__ void k_scatter(const uint64_t *__restrict__ edges, const unsigned lon
 // rows of <= 64 entries; longer rows are queued for the block kernels.
 constexpr uint32_t kMidMax = 4096;
 
-__global__ __launch_bounds__(256) void k_sort_short(const uint64_t *__restrict__ row_ptr, uint32_t N,
+static __global__ __launch_bounds__(256) void k_sort_short(const uint64_t *__restrict__ row_ptr, uint32_t N,
                                                      uint32_t *__restrict__ col, uint32_t *__restrict__ mid_list,
                                                      uint32_t *__restrict__ big_list, uint32_t *__restrict__ counts)
 {
@@ -362,7 +372,7 @@ __global__ __launch_bounds__(256) void k_sort_short(const uint64_t *__restrict__
 }
 
 // Rows of 65..4096 entries: one workgroup per row, bitonic sort in LDS.
-__global__ __launch_bounds__(256) void k_sort_mid(const uint64_t *__restrict__ row_ptr, uint32_t *__restrict__ col,
+static __global__ __launch_bounds__(256) void k_sort_mid(const uint64_t *__restrict__ row_ptr, uint32_t *__restrict__ col,
                                                    const uint32_t *__restrict__ mid_list,
                                                    const uint32_t *__restrict__ counts)
 {
@@ -401,7 +411,7 @@ __global__ __launch_bounds__(256) void k_sort_mid(const uint64_t *__restrict__ r
 // value chunks of 2^19 ids in LDS, read back in order, sorts the row.
 constexpr uint32_t kBigChunkWords = 16384; // 64 KiB of LDS = 2^19 ids
 
-__global__ __launch_bounds__(256) void k_sort_big(const uint64_t *__restrict__ row_ptr, uint32_t N,
+static __global__ __launch_bounds__(256) void k_sort_big(const uint64_t *__restrict__ row_ptr, uint32_t N,
                                                    uint32_t *__restrict__ col, uint32_t *__restrict__ tmp,
                                                    const uint32_t *__restrict__ big_list,
                                                    const uint32_t *__restrict__ counts)

@@ -4,10 +4,7 @@
 // context's stream with no host synchronisation until the final read of the
 // edge count.  If the edge buffer overflowed (the count is exact even then),
 // the buffer is regrown to the exact count and the build is replayed.
-#include "igc_kernels.hpp"
-#include "igc_internal.hpp"
-
-#include <hip/hip_runtime.h>
+#include "igc_device.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -37,68 +34,11 @@ int set_error(int code, const char *fmt, ...)
 
 const char *last_error() { return g_err.c_str(); }
 
-#define HIP_TRY(expr)                                                                                         \
-    do {                                                                                                      \
-        hipError_t e_ = (expr);                                                                               \
-        if (e_ != hipSuccess)                                                                                 \
-            return set_error(e_ == hipErrorOutOfMemory ? IGC_ERR_OOM : IGC_ERR_DEVICE, "%s failed: %s (%s:%d)", \
-                             #expr, hipGetErrorString(e_), __FILE__, __LINE__);                               \
-    } while (0)
-
-// grow-only device buffer
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    int ensure(size_t want)
-    {
-        if (want <= bytes && p) return IGC_OK;
-        if (p) {
-            hipError_t e = hipFree(p);
-            (void)e;
-            p = nullptr;
-            bytes = 0;
-        }
-        size_t sz = std::max<size_t>(want, 256);
-        hipError_t e = hipMalloc(&p, sz);
-        if (e != hipSuccess) {
-            p = nullptr;
-            return set_error(IGC_ERR_OOM, "hipMalloc(%zu) failed: %s", sz, hipGetErrorString(e));
-        }
-        bytes = sz;
-        return IGC_OK;
-    }
-    void release()
-    {
-        if (p) {
-            hipError_t e = hipFree(p);
-            (void)e;
-        }
-        p = nullptr;
-        bytes = 0;
-    }
-    template <class T> T *as() const { return static_cast<T *>(p); }
-};
-
 } // namespace igc
 
 using namespace igc;
 
-struct igc_ctx {
-    int device = 0;
-    hipStream_t own_stream = nullptr;
-    hipStream_t stream = nullptr;
-    std::mutex mu;
-    DevBuf syms, packed, edges, misc, deg, cursor, row_ptr, col, mid, big, bsum;
-    hipEvent_t ev[5] = {};
-    igc_timing last{};
-    int cu_count = 256;
-};
-
 namespace igc {
-
-// misc buffer layout (uint32 words): [0..1] edge counter (u64), [2] bad symbol
-// flag, [4] mid-row count, [5] big-row count
-constexpr size_t kMiscWords = 8;
 
 // P(Levenshtein <= K) for uniformly random filtered 4-ary strings, measured
 // offline; used only to size the first edge buffer (the build is exact either way).
@@ -152,38 +92,42 @@ uint64_t tile_count(const TileGeom &G)
 }
 
 // Tile shape: enough workgroups to fill 256 CUs several times over.
-void default_shape(uint32_t N, int &cpl, uint32_t &q)
+Shape default_shape(uint32_t N)
 {
-    if (N >= 65536) {
-        cpl = 4;
-        q = 1;
-    } else if (N >= 16384) {
-        cpl = 2;
-        q = N >= 32768 ? 2 : 4;
-    } else {
-        cpl = 1;
-        q = N >= 4096 ? 4 : 8;
-    }
+    if (N >= 65536) return {4, 1};
+    if (N >= 16384) return {2, N >= 32768 ? 2u : 4u};
+    return {1, N >= 4096 ? 4u : 8u};
 }
 
-typedef void (*pairs_fn)(const uint32_t *, TileGeom, uint64_t *, uint64_t, unsigned long long *);
+static int size_bucket(uint32_t N)
+{
+    int b = 0;
+    for (uint32_t t = 4096; t < N && b < 7; t <<= 2) ++b;
+    return b;
+}
+
+Shape pick_shape(igc_ctx *ctx, int K, uint32_t N)
+{
+    const Shape t = ctx->tuned[K < 0 ? 0 : (K > 16 ? 16 : K)][size_bucket(N)];
+    return t.cpl ? t : default_shape(N);
+}
 
 template <int K, int CPL, bool EMIT>
-static void launch_pairs_t(hipStream_t s, uint64_t tiles, const uint32_t *packed, const TileGeom &G, uint64_t *edges,
-                           uint64_t cap, unsigned long long *counter)
+static void launch_pairs_t(hipStream_t s, uint64_t tiles, const uint32_t *packed, const TileGeom &G,
+                           uint32_t node_off, uint64_t *edges, uint64_t cap, unsigned long long *counter)
 {
-    hipLaunchKernelGGL((k_pairs<K, CPL, EMIT>), dim3((uint32_t)tiles), dim3(256), 0, s, packed, G, edges, cap,
-                       counter);
+    hipLaunchKernelGGL((k_pairs<K, CPL, EMIT>), dim3((uint32_t)tiles), dim3(256), 0, s, packed, G, node_off, edges,
+                       cap, counter);
 }
 
 template <int CPL, bool EMIT>
 static int launch_pairs_k(int K, hipStream_t s, uint64_t tiles, const uint32_t *packed, const TileGeom &G,
-                          uint64_t *edges, uint64_t cap, unsigned long long *counter)
+                          uint32_t node_off, uint64_t *edges, uint64_t cap, unsigned long long *counter)
 {
     switch (K) {
 #define IGC_CASE(k)                                                                                           \
     case k:                                                                                                   \
-        launch_pairs_t<k, CPL, EMIT>(s, tiles, packed, G, edges, cap, counter);                               \
+        launch_pairs_t<k, CPL, EMIT>(s, tiles, packed, G, node_off, edges, cap, counter);                     \
         return IGC_OK;
         IGC_CASE(0) IGC_CASE(1) IGC_CASE(2) IGC_CASE(3) IGC_CASE(4) IGC_CASE(5) IGC_CASE(6) IGC_CASE(7)
         IGC_CASE(8) IGC_CASE(9) IGC_CASE(10) IGC_CASE(11) IGC_CASE(12) IGC_CASE(13) IGC_CASE(14) IGC_CASE(15)
@@ -195,24 +139,24 @@ static int launch_pairs_k(int K, hipStream_t s, uint64_t tiles, const uint32_t *
 }
 
 int launch_pairs(int K, int cpl, bool emit, hipStream_t s, uint64_t tiles, const uint32_t *packed,
-                 const TileGeom &G, uint64_t *edges, uint64_t cap, unsigned long long *counter)
+                 const TileGeom &G, uint32_t node_off, uint64_t *edges, uint64_t cap, unsigned long long *counter)
 {
     if (tiles == 0) return IGC_OK;
     if (tiles > 0x7FFFFFFFull) return set_error(IGC_ERR_INVALID, "too many tiles (%llu)", (unsigned long long)tiles);
     int rc;
-    if (cpl == 1) rc = emit ? launch_pairs_k<1, true>(K, s, tiles, packed, G, edges, cap, counter)
-                            : launch_pairs_k<1, false>(K, s, tiles, packed, G, edges, cap, counter);
-    else if (cpl == 2) rc = emit ? launch_pairs_k<2, true>(K, s, tiles, packed, G, edges, cap, counter)
-                                 : launch_pairs_k<2, false>(K, s, tiles, packed, G, edges, cap, counter);
-    else if (cpl == 4) rc = emit ? launch_pairs_k<4, true>(K, s, tiles, packed, G, edges, cap, counter)
-                                 : launch_pairs_k<4, false>(K, s, tiles, packed, G, edges, cap, counter);
+    if (cpl == 1) rc = emit ? launch_pairs_k<1, true>(K, s, tiles, packed, G, node_off, edges, cap, counter)
+                            : launch_pairs_k<1, false>(K, s, tiles, packed, G, node_off, edges, cap, counter);
+    else if (cpl == 2) rc = emit ? launch_pairs_k<2, true>(K, s, tiles, packed, G, node_off, edges, cap, counter)
+                                 : launch_pairs_k<2, false>(K, s, tiles, packed, G, node_off, edges, cap, counter);
+    else if (cpl == 4) rc = emit ? launch_pairs_k<4, true>(K, s, tiles, packed, G, node_off, edges, cap, counter)
+                                 : launch_pairs_k<4, false>(K, s, tiles, packed, G, node_off, edges, cap, counter);
     else return set_error(IGC_ERR_INVALID, "unsupported columns-per-lane %d", cpl);
     if (rc) return rc;
     HIP_TRY(hipGetLastError());
     return IGC_OK;
 }
 
-static uint32_t grid_for(uint64_t work, uint32_t per_block, uint32_t cap_blocks)
+uint32_t grid_for(uint64_t work, uint32_t per_block, uint32_t cap_blocks)
 {
     uint64_t b = (work + per_block - 1) / per_block;
     if (b < 1) b = 1;
@@ -233,6 +177,47 @@ static int check_args(igc_ctx *ctx, uint32_t N, uint32_t n)
     if (!ctx) return set_error(IGC_ERR_INVALID, "null context");
     if (n < 1 || n > IGC_MAX_LEN) return set_error(IGC_ERR_INVALID, "string length n=%u outside 1..%d", n, IGC_MAX_LEN);
     if (N > 0x7FFFFFFFu) return set_error(IGC_ERR_INVALID, "N=%u too large", N);
+    return IGC_OK;
+}
+
+// CSR of the edge list: degree -> scan -> scatter -> sort, sized by N and cap so
+// no host round trip is needed before the final count.
+int launch_csr(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *counter, uint32_t *misc,
+               uint32_t *launches)
+{
+    hipStream_t s = ctx->stream;
+    int rc;
+    const uint32_t nsb = (uint32_t)(((uint64_t)N + kScanTile - 1) / kScanTile);
+    if ((rc = ctx->deg.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
+    if ((rc = ctx->cursor.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
+    if ((rc = ctx->row_ptr.ensure(sizeof(uint64_t) * ((size_t)N + 1)))) return rc;
+    if ((rc = ctx->col.ensure(sizeof(uint32_t) * 2 * (size_t)cap))) return rc;
+    if ((rc = ctx->bsum.ensure(sizeof(uint64_t) * (size_t)nsb))) return rc;
+    if ((rc = ctx->mid.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
+    if ((rc = ctx->big.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
+    const uint32_t fillg = grid_for(N, 256, 8192);
+    hipLaunchKernelGGL(k_fill_u32, dim3(fillg), dim3(256), 0, s, ctx->deg.as<uint32_t>(), (uint64_t)N, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(fillg), dim3(256), 0, s, ctx->cursor.as<uint32_t>(), (uint64_t)N, 0u);
+    const uint32_t eg = grid_for(cap, 256, (uint32_t)ctx->cu_count * 16);
+    hipLaunchKernelGGL(k_degree, dim3(eg), dim3(256), 0, s, ctx->edges.as<uint64_t>(), counter, cap,
+                       ctx->deg.as<uint32_t>());
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(kScanThreads), 0, s, ctx->deg.as<uint32_t>(), N,
+                       ctx->bsum.as<uint64_t>());
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, s, ctx->bsum.as<uint64_t>(), nsb);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(kScanThreads), 0, s, ctx->deg.as<uint32_t>(), N,
+                       ctx->bsum.as<uint64_t>(), ctx->row_ptr.as<uint64_t>());
+    hipLaunchKernelGGL(k_scatter, dim3(eg), dim3(256), 0, s, ctx->edges.as<uint64_t>(), counter, cap,
+                       ctx->row_ptr.as<uint64_t>(), ctx->cursor.as<uint32_t>(), ctx->col.as<uint32_t>());
+    hipLaunchKernelGGL(k_sort_short, dim3(grid_for((uint64_t)N * 64, 256, 8192)), dim3(256), 0, s,
+                       ctx->row_ptr.as<uint64_t>(), N, ctx->col.as<uint32_t>(), ctx->mid.as<uint32_t>(),
+                       ctx->big.as<uint32_t>(), misc + 4);
+    hipLaunchKernelGGL(k_sort_mid, dim3((uint32_t)ctx->cu_count * 4), dim3(256), 0, s, ctx->row_ptr.as<uint64_t>(),
+                       ctx->col.as<uint32_t>(), ctx->mid.as<uint32_t>(), misc + 4);
+    hipLaunchKernelGGL(k_sort_big, dim3((uint32_t)ctx->cu_count * 2), dim3(256), 0, s, ctx->row_ptr.as<uint64_t>(),
+                       N, ctx->col.as<uint32_t>(), ctx->edges.as<uint32_t>() /* edge list is dead after the scatter */,
+                       ctx->big.as<uint32_t>(), misc + 4);
+    HIP_TRY(hipGetLastError());
+    *launches += 11;
     return IGC_OK;
 }
 
@@ -258,10 +243,9 @@ static int build_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, 
         return IGC_OK;
     }
     const int K = effective_k(n, edit_dist);
-    int cpl;
-    uint32_t q;
-    default_shape(N, cpl, q);
-    const TileGeom G = choose_geometry(N, cpl, q);
+    const Shape sh = pick_shape(ctx, K, N);
+    const int cpl = sh.cpl;
+    const TileGeom G = choose_geometry(N, sh.cpl, sh.q);
     const uint64_t tiles = tile_count(G);
 
     int rc;
@@ -281,47 +265,13 @@ static int build_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, 
                            ctx->packed.as<uint32_t>(), misc + 2);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(ctx->ev[1], s));
-        rc = launch_pairs(K, cpl, !count_only, s, tiles, ctx->packed.as<uint32_t>(), G,
+        rc = launch_pairs(K, cpl, !count_only, s, tiles, ctx->packed.as<uint32_t>(), G, 0u,
                           count_only ? nullptr : ctx->edges.as<uint64_t>(), cap, counter);
         if (rc) return rc;
         HIP_TRY(hipEventRecord(ctx->ev[2], s));
         uint32_t launches = 3;
         if (!count_only) {
-            // CSR: degree -> scan -> scatter -> sort.  Sized by N and cap, so no
-            // host round trip is needed before the final count.
-            const uint32_t nsb = (uint32_t)(((uint64_t)N + kScanTile - 1) / kScanTile);
-            if ((rc = ctx->deg.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
-            if ((rc = ctx->cursor.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
-            if ((rc = ctx->row_ptr.ensure(sizeof(uint64_t) * ((size_t)N + 1)))) return rc;
-            if ((rc = ctx->col.ensure(sizeof(uint32_t) * 2 * (size_t)cap))) return rc;
-            if ((rc = ctx->bsum.ensure(sizeof(uint64_t) * (size_t)nsb))) return rc;
-            if ((rc = ctx->mid.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
-            if ((rc = ctx->big.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
-            const uint32_t fillg = grid_for(N, 256, 8192);
-            hipLaunchKernelGGL(k_fill_u32, dim3(fillg), dim3(256), 0, s, ctx->deg.as<uint32_t>(), (uint64_t)N, 0u);
-            hipLaunchKernelGGL(k_fill_u32, dim3(fillg), dim3(256), 0, s, ctx->cursor.as<uint32_t>(), (uint64_t)N, 0u);
-            const uint32_t eg = grid_for(cap, 256, (uint32_t)ctx->cu_count * 16);
-            hipLaunchKernelGGL(k_degree, dim3(eg), dim3(256), 0, s, ctx->edges.as<uint64_t>(), counter, cap,
-                               ctx->deg.as<uint32_t>());
-            hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(kScanThreads), 0, s, ctx->deg.as<uint32_t>(), N,
-                               ctx->bsum.as<uint64_t>());
-            hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, s, ctx->bsum.as<uint64_t>(), nsb);
-            hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(kScanThreads), 0, s, ctx->deg.as<uint32_t>(), N,
-                               ctx->bsum.as<uint64_t>(), ctx->row_ptr.as<uint64_t>());
-            hipLaunchKernelGGL(k_scatter, dim3(eg), dim3(256), 0, s, ctx->edges.as<uint64_t>(), counter, cap,
-                               ctx->row_ptr.as<uint64_t>(), ctx->cursor.as<uint32_t>(), ctx->col.as<uint32_t>());
-            hipLaunchKernelGGL(k_sort_short, dim3(grid_for((uint64_t)N * 64, 256, 8192)), dim3(256), 0, s,
-                               ctx->row_ptr.as<uint64_t>(), N, ctx->col.as<uint32_t>(), ctx->mid.as<uint32_t>(),
-                               ctx->big.as<uint32_t>(), misc + 4);
-            hipLaunchKernelGGL(k_sort_mid, dim3((uint32_t)ctx->cu_count * 4), dim3(256), 0, s,
-                               ctx->row_ptr.as<uint64_t>(), ctx->col.as<uint32_t>(), ctx->mid.as<uint32_t>(),
-                               misc + 4);
-            hipLaunchKernelGGL(k_sort_big, dim3((uint32_t)ctx->cu_count * 2), dim3(256), 0, s,
-                               ctx->row_ptr.as<uint64_t>(), N, ctx->col.as<uint32_t>(),
-                               ctx->edges.as<uint32_t>() /* edge list is dead after the scatter */,
-                               ctx->big.as<uint32_t>(), misc + 4);
-            HIP_TRY(hipGetLastError());
-            launches += 11;
+            if ((rc = launch_csr(ctx, N, cap, counter, misc, &launches))) return rc;
         }
         HIP_TRY(hipEventRecord(ctx->ev[3], s));
         uint32_t host_misc[kMiscWords];

@@ -155,3 +155,46 @@ def test_linear_large_config_properties(gpu_ctx):
     assert gpu_ctx.graph_count(syms, 4) == adj.n_edges
     order = igc.greedy_order(adj)
     assert igc.verify_dist(syms[order], 4, gpu_ctx)
+
+
+def test_batch_matches_single_and_oracle(gpu_ctx):
+    rng = np.random.default_rng(21)
+    sets = []
+    for k in range(40):
+        n = int(rng.integers(1, 17))
+        N = int(rng.choice([0, 1, 2, 5, 63, 64, 65, 300, 1000, 2500]))
+        ed = int(rng.integers(0, n + 3))
+        syms = rng.integers(0, 4, size=(N, n), dtype=np.uint8)
+        for r in range(0, N // 3):
+            src = syms[int(rng.integers(0, N))].copy()
+            src[int(rng.integers(0, n))] = int(rng.integers(0, 4))
+            syms[r] = src
+        sets.append((syms, ed))
+    # sets with 9 <= K <= 15 take the separate-launch path
+    sets.append((rng.integers(0, 4, size=(400, 16), dtype=np.uint8), 11))
+    sets.append((rng.integers(0, 4, size=(300, 14), dtype=np.uint8), 13))
+    banner(f"batch of {len(sets)} uneven sets")
+    outs = gpu_ctx.graph_build_batch(sets)
+    for (syms, ed), adj in zip(sets, outs):
+        rp, col = O.graph_csr(syms, ed) if syms.shape[0] else (np.zeros(1, np.uint64), np.zeros(0, np.uint32))
+        assert np.array_equal(adj.row_ptr, rp), (syms.shape, ed)
+        assert np.array_equal(adj.col, col), (syms.shape, ed)
+    # thread-order flag applies per set, as in the single-set call
+    outs4 = gpu_ctx.graph_build_batch(sets[:10], flags=igc.order_threads(4))
+    for (syms, ed), adj in zip(sets[:10], outs4):
+        if syms.shape[0]:
+            rp, col = O.graph_csr(syms, ed, T=4)
+            assert np.array_equal(adj.col, col)
+
+
+def test_batched_config_sets_alone_and_in_the_mix(gpu_ctx):
+    # a scaled-down batched workload: each set's graph is the same alone and in the batch
+    sets = W.batched(seed=3, sets=24, scale=0.05)
+    banner(f"batched workload {len(sets)} sets, pairs={sum(s.pairs for s in sets):.3e}")
+    outs = gpu_ctx.graph_build_batch([(s.syms, s.edit_dist) for s in sets])
+    for s, adj in zip(sets, outs):
+        alone = gpu_ctx.graph_build(s.syms, s.edit_dist)
+        assert np.array_equal(adj.row_ptr, alone.row_ptr) and np.array_equal(adj.col, alone.col)
+    for k in range(4):
+        rp, col = O.graph_csr(sets[k].syms, sets[k].edit_dist)
+        assert np.array_equal(outs[k].col, col)
