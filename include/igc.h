@@ -163,6 +163,62 @@ IGC_API int igc_graph_build_batch(igc_ctx *ctx, const igc_set *sets, uint32_t co
 IGC_API int igc_graph_build_batch_device(igc_ctx *ctx, const igc_set *sets, uint32_t count, igc_csr *union_out,
                                          uint32_t *node_off, uint64_t *set_edges);
 
+/* ---- K-means on one-hot symbol features (cluster refinement) ---- */
+
+/* run the Lloyd iterations eagerly instead of replaying a captured hipGraph */
+#define IGC_KMEANS_EAGER 1u
+
+/* FP32 K-means of N strings of length n on their one-hot 4n-dimensional
+ * features: k seeds chosen by igc_kmeans_seeds(N, k, seed), `iters` Lloyd
+ * iterations (assign by squared Euclidean distance, first index on ties; update
+ * = per-cluster mean, empty clusters keep their centroid), then a final
+ * assignment.  assign_out[N] gets the cluster of each string; centroids_out
+ * (k x 4n, may be NULL) the final centroids.  Deterministic: fixed-order sums,
+ * integer histograms.  Replaces the centre assignment of the reference's
+ * clustered loop (RandomCluster::fit, src/clustering/impl/random_cluster/
+ * RandomCluster.cpp:42-136; src/gpu_cluster.py:16-193,287).  k*4n/2 + k <= 16384. */
+IGC_API int igc_kmeans(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint32_t n, uint32_t k, uint32_t iters,
+                       uint64_t seed, uint32_t flags, int32_t *assign_out, float *centroids_out);
+
+/* The k distinct seed indices igc_kmeans starts from: a partial Fisher-Yates
+ * shuffle of 0..N-1 driven by SplitMix64(seed). */
+IGC_API int igc_kmeans_seeds(uint32_t N, uint32_t k, uint64_t seed, uint32_t *out);
+
+/* ---- cluster-based refinement ---- */
+
+#define IGC_REFINE_MAX_ROUNDS 32
+
+typedef struct igc_refine_round {
+    uint32_t candidates; /* strings entering the round */
+    uint32_t clusters;   /* non-empty clusters */
+    uint32_t codewords;  /* strings leaving the round */
+    uint32_t pad;
+    uint64_t pairs;      /* sum over clusters of N_c(N_c-1)/2 */
+    float kmeans_ms;     /* device time of the K-means (HIP events) */
+    float pairs_ms;      /* device time of the batched pair kernel */
+    double cluster_s;    /* host wall time of the cluster step */
+    double graph_s;      /* host wall time of the batched build incl. copies */
+    double greedy_s;     /* host wall time of the per-cluster greedy */
+} igc_refine_round;
+
+typedef struct igc_refine_stats {
+    uint32_t rounds;
+    uint32_t converged;
+    igc_refine_round round[IGC_REFINE_MAX_ROUNDS];
+} igc_refine_stats;
+
+/* The reference's clustered codebook loop (src/SparseMat.cpp:918-1095) on one
+ * context: per round, K-means into k clusters (k raised so that N/k <= 200,000),
+ * every cluster's conflict graph in one batched launch, the greedy independent
+ * set per cluster on `host_threads` threads, and the cluster codebooks
+ * concatenated in cluster order as the next candidates; stop when the last
+ * converge_n sizes are equal or after max_rounds.  Round r seeds the K-means
+ * with seed + 0x9E37*r.  out_idx (capacity N) receives the final codebook as
+ * indices into syms, in order. */
+IGC_API int igc_refine(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint32_t n, uint32_t edit_dist, uint32_t k,
+                       uint32_t kmeans_iters, uint32_t max_rounds, uint32_t converge_n, uint64_t seed,
+                       uint32_t host_threads, uint32_t *out_idx, uint32_t *out_count, igc_refine_stats *stats);
+
 /* ---- greedy independent set and outputs (host) ---- */
 
 /* Min-degree greedy independent set of a host CSR, with the reference's exact

@@ -83,6 +83,30 @@ class _Set(ctypes.Structure):
     ]
 
 
+IGC_REFINE_MAX_ROUNDS = 32
+IGC_KMEANS_EAGER = 1
+
+
+class _RefineRound(ctypes.Structure):
+    _fields_ = [
+        ("candidates", ctypes.c_uint32),
+        ("clusters", ctypes.c_uint32),
+        ("codewords", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
+        ("pairs", ctypes.c_uint64),
+        ("kmeans_ms", ctypes.c_float),
+        ("pairs_ms", ctypes.c_float),
+        ("cluster_s", ctypes.c_double),
+        ("graph_s", ctypes.c_double),
+        ("greedy_s", ctypes.c_double),
+    ]
+
+
+class _RefineStats(ctypes.Structure):
+    _fields_ = [("rounds", ctypes.c_uint32), ("converged", ctypes.c_uint32),
+                ("round", _RefineRound * IGC_REFINE_MAX_ROUNDS)]
+
+
 class _CodebookInfo(ctypes.Structure):
     _fields_ = [
         ("code_len", ctypes.c_int),
@@ -124,6 +148,10 @@ def lib() -> ctypes.CDLL:
         "igc_graph_build_device": (i, [vp, u8p, u32, u32, u32, ctypes.POINTER(_Csr)]),
         "igc_graph_count": (i, [vp, u8p, u32, u32, u32, ctypes.POINTER(u64)]),
         "igc_graph_build_batch": (i, [vp, ctypes.POINTER(_Set), u32, ctypes.POINTER(_Csr)]),
+        "igc_kmeans": (i, [vp, u8p, u32, u32, u32, u32, u64, u32, vp, vp]),
+        "igc_kmeans_seeds": (i, [u32, u32, u64, vp]),
+        "igc_refine": (i, [vp, u8p, u32, u32, u32, u32, u32, u32, u32, u64, u32, vp, ctypes.POINTER(u32),
+                           ctypes.POINTER(_RefineStats)]),
         "igc_graph_build_batch_device": (i, [vp, ctypes.POINTER(_Set), u32, ctypes.POINTER(_Csr), vp, vp]),
         "igc_csr_to_host": (i, [vp, ctypes.POINTER(_Csr), u32, ctypes.POINTER(_Csr)]),
         "igc_csr_free": (i, [ctypes.POINTER(_Csr)]),
@@ -336,6 +364,35 @@ class Context:
                       ctypes.cast(c.col, ctypes.c_void_p).value or 0, c)
         return d, off, edges
 
+    def kmeans(self, syms, k: int, iters: int, seed: int, eager: bool = False):
+        """FP32 K-means on one-hot features: (assign int32[N], centroids float32[k, 4n])."""
+        s = to_symbols(syms)
+        N, n = s.shape
+        assign = np.zeros(max(N, 1), dtype=np.int32)
+        C = np.zeros((k, 4 * n), dtype=np.float32)
+        _check(lib().igc_kmeans(self._h, s.ctypes.data, N, n, int(k), int(iters), int(seed),
+                                IGC_KMEANS_EAGER if eager else 0, assign.ctypes.data, C.ctypes.data))
+        return assign[:N], C
+
+    def refine(self, syms, min_ed: int, k: int, kmeans_iters: int = 10, max_rounds: int = 5, converge_n: int = 3,
+               seed: int = 1, host_threads: int = 8):
+        """Clustered refinement loop; returns (final codebook indices, stats dict)."""
+        s = to_symbols(syms)
+        N, n = s.shape
+        out = np.zeros(max(N, 1), dtype=np.uint32)
+        cnt = ctypes.c_uint32()
+        st = _RefineStats()
+        _check(lib().igc_refine(self._h, s.ctypes.data, N, n, int(min_ed), int(k), int(kmeans_iters),
+                                int(max_rounds), int(converge_n), int(seed), int(host_threads), out.ctypes.data,
+                                ctypes.byref(cnt), ctypes.byref(st)))
+        rounds = []
+        for r in range(min(st.rounds, IGC_REFINE_MAX_ROUNDS)):
+            x = st.round[r]
+            rounds.append({"candidates": x.candidates, "clusters": x.clusters, "codewords": x.codewords,
+                           "pairs": int(x.pairs), "kmeans_ms": x.kmeans_ms, "pairs_ms": x.pairs_ms,
+                           "cluster_s": x.cluster_s, "graph_s": x.graph_s, "greedy_s": x.greedy_s})
+        return out[: cnt.value].copy(), {"rounds": rounds, "converged": bool(st.converged)}
+
     def graph_count(self, syms, min_ed: int) -> int:
         s = to_symbols(syms)
         N, n = s.shape
@@ -441,3 +498,9 @@ def write_codebook(directory: str, candidates, order: Sequence[int], *, code_len
     _check(lib().igc_write_codebook(directory.encode(), syms.ctypes.data, syms.shape[1], order.ctypes.data,
                                     order.size, ctypes.byref(info), name, 128))
     return os.path.join(directory, name.value.decode())
+
+
+def kmeans_seeds(N: int, k: int, seed: int) -> np.ndarray:
+    out = np.zeros(k, dtype=np.uint32)
+    _check(lib().igc_kmeans_seeds(int(N), int(k), int(seed), out.ctypes.data))
+    return out

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libigc.so")
 CLI = os.path.join(HERE, "igc_cli")
 
-SOURCES = ["igc_runtime.hip", "igc_host.cpp", "igc_batch.hip", "igc_kmeans.hip", "igc_server.cpp"]
+SOURCES = ["igc_runtime.hip", "igc_host.cpp", "igc_batch.hip", "igc_kmeans.hip", "igc_refine.hip", "igc_server.hip"]
 
 HIPCC_FLAGS = [
     "-O3",

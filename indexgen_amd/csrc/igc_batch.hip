@@ -326,6 +326,17 @@ IGC_API int igc_graph_build_batch(igc_ctx *ctx, const igc_set *sets, uint32_t co
     if (count == 0) return IGC_OK;
     std::lock_guard<std::mutex> lk(ctx->mu);
     HIP_TRY(hipSetDevice(ctx->device));
+    return batch_build_host_locked(ctx, sets, count, outs);
+}
+
+} // extern "C"
+
+namespace igc {
+
+int batch_build_host_locked(igc_ctx *ctx, const igc_set *sets, uint32_t count, igc_csr *outs)
+{
+    for (uint32_t s = 0; s < count; ++s) std::memset(&outs[s], 0, sizeof(igc_csr));
+    if (count == 0) return IGC_OK;
     const int cpl = batch_shape(sets, count);
     BatchPlan P;
     int rc = plan_batch(ctx, sets, count, cpl, P);
@@ -364,6 +375,10 @@ IGC_API int igc_graph_build_batch(igc_ctx *ctx, const igc_set *sets, uint32_t co
     }
     return IGC_OK;
 }
+
+} // namespace igc
+
+extern "C" {
 
 IGC_API int igc_graph_build_batch_device(igc_ctx *ctx, const igc_set *sets, uint32_t count, igc_csr *union_out,
                                          uint32_t *node_off, uint64_t *set_edges)

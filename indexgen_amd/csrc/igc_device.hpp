@@ -108,6 +108,7 @@ struct igc_ctx {
     int cu_count = 256;
     // tile shapes chosen by the autotuner, keyed by (K, N bucket); 0 = default
     igc::Shape tuned[17][8] = {};
+    void *km = nullptr; // K-means buffers (igc_kmeans.hip)
 };
 
 namespace igc {
@@ -116,4 +117,13 @@ namespace igc {
 int launch_csr(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *counter, uint32_t *misc,
                uint32_t *launches);
 Shape pick_shape(igc_ctx *ctx, int K, uint32_t N);
+
+int kmeans_attach(igc_ctx *ctx);
+void kmeans_release(igc_ctx *ctx);
+int kmeans_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n, uint32_t k, uint32_t iters,
+                         uint64_t seed, bool use_graph, float *ms_out);
+int32_t *kmeans_assign_ptr(igc_ctx *ctx);
+
+// igc_graph_build_batch without taking the context lock (caller holds it)
+int batch_build_host_locked(igc_ctx *ctx, const igc_set *sets, uint32_t count, igc_csr *outs);
 } // namespace igc

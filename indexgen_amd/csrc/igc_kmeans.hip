@@ -1,0 +1,319 @@
+// igc_kmeans.hip -- FP32 K-means on one-hot symbol features, for cluster-based
+// refinement of a candidate set.
+//
+// Feature of a length-n string: the one-hot 4n-vector (dimension 4p+s is 1 iff
+// symbol p equals s).  For a centroid c, ||x - c||^2 = n - 2 sum_p C[c][4p+x_p]
+// + ||c||^2, so the assignment needs n table lookups per centroid, not a 4n-wide
+// dot product; the table (k x 4n floats, 64 KiB at k=256, n=16) sits in LDS.
+// Determinism: every sum runs in a fixed order with FP contraction off, ties go
+// to the lowest centroid index, and the update uses exact integer histograms
+// (cnt[c][4p+s] / size[c], one correctly rounded division), so the result does
+// not depend on scheduling or on which other work shares the GPU.
+//
+// Replaces the reference's random-centre assignment (RandomCluster::fit,
+// src/clustering/impl/random_cluster/RandomCluster.cpp:42-136, whose distances
+// come from src/gpu_cluster.py:16-193,287) with Lloyd iterations captured in a
+// hipGraph and replayed.
+#include "igc_device.hpp"
+
+#include <cstring>
+#include <vector>
+
+#pragma clang fp contract(off)
+
+namespace igc {
+
+constexpr int kKmThreads = 256;
+
+__device__ __forceinline__ uint32_t sym_at(uint32_t w, uint32_t p)
+{
+    return ((w >> p) & 1u) | (((w >> (16 + p)) & 1u) << 1);
+}
+
+// assign[i] = argmin_c (cn2[c] - 2 * sum_p C[c][4p + s_p]), first index on ties.
+// The centroid table goes through LDS in chunks of kc centroids (<= 32 KiB).
+static __global__ __launch_bounds__(kKmThreads) void k_km_assign(const uint32_t *__restrict__ packed, uint32_t N,
+                                                                  uint32_t n, const float *__restrict__ C,
+                                                                  const float *__restrict__ cn2, uint32_t k,
+                                                                  uint32_t kc, int32_t *__restrict__ assign)
+{
+    extern __shared__ float sh[];
+    const uint32_t D = 4 * n;
+    float *sC = sh;
+    float *sN = sh + (size_t)kc * D;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < N; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t w = i < N ? packed[i] : 0u;
+        uint32_t off[IGC_MAX_LEN];
+#pragma unroll
+        for (uint32_t p = 0; p < IGC_MAX_LEN; ++p) off[p] = 4 * p + sym_at(w, p);
+        float best = 0.0f;
+        int32_t bi = -1;
+        for (uint32_t c0 = 0; c0 < k; c0 += kc) {
+            const uint32_t m = min(kc, k - c0);
+            __syncthreads();
+            for (uint32_t t = threadIdx.x; t < m * D; t += blockDim.x) sC[t] = C[(size_t)c0 * D + t];
+            for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) sN[t] = cn2[c0 + t];
+            __syncthreads();
+            for (uint32_t c = 0; c < m; ++c) {
+                const float *row = sC + (size_t)c * D;
+                float acc = 0.0f;
+                for (uint32_t p = 0; p < n; ++p) acc = acc + row[off[p]];
+                const float twice = 2.0f * acc;
+                const float dist = sN[c] - twice;
+                if (bi < 0 || dist < best) {
+                    best = dist;
+                    bi = (int32_t)(c0 + c);
+                }
+            }
+        }
+        if (i < N) assign[i] = bi;
+    }
+}
+
+// Integer histograms per cluster: cnt[c][4p+s] and size[c].  Each workgroup
+// counts in LDS with two 16-bit counters per word (a workgroup sees at most
+// 32768 points, so no half overflows) and flushes nonzero entries with one
+// global atomic each.
+static __global__ __launch_bounds__(kKmThreads) void k_km_accum(const uint32_t *__restrict__ packed, uint32_t N,
+                                                                 uint32_t n, const int32_t *__restrict__ assign,
+                                                                 uint32_t k, uint32_t *__restrict__ cnt,
+                                                                 uint32_t *__restrict__ size)
+{
+    extern __shared__ uint32_t hs[];
+    const uint32_t D = 4 * n;
+    const uint32_t words = (k * D + 1) / 2;
+    uint32_t *hc = hs;
+    uint32_t *hz = hs + words;
+    for (uint32_t t = threadIdx.x; t < words + k; t += blockDim.x) hs[t] = 0;
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const uint32_t w = packed[i];
+        const uint32_t c = (uint32_t)assign[i];
+        for (uint32_t p = 0; p < n; ++p) {
+            const uint32_t t = c * D + 4 * p + sym_at(w, p);
+            atomicAdd(&hc[t >> 1], 1u << (16 * (t & 1)));
+        }
+        atomicAdd(&hz[c], 1u);
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < k * D; t += blockDim.x) {
+        const uint32_t v = (hc[t >> 1] >> (16 * (t & 1))) & 0xFFFFu;
+        if (v) atomicAdd(&cnt[t], v);
+    }
+    for (uint32_t t = threadIdx.x; t < k; t += blockDim.x)
+        if (hz[t]) atomicAdd(&size[t], hz[t]);
+}
+
+// C[c][d] = cnt[c][d] / size[c] (an empty cluster keeps its centroid), then
+// cn2[c] = sum_d C[c][d]^2 in dimension order.  One workgroup per centroid.
+static __global__ void k_km_update(const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ size, uint32_t D,
+                                   float *__restrict__ C, float *__restrict__ cn2)
+{
+    const uint32_t c = blockIdx.x;
+    const uint32_t z = size[c];
+    for (uint32_t d = threadIdx.x; d < D; d += blockDim.x)
+        if (z) C[(size_t)c * D + d] = (float)cnt[(size_t)c * D + d] / (float)z;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float s = 0.0f;
+        for (uint32_t d = 0; d < D; ++d) {
+            const float v = C[(size_t)c * D + d];
+            const float sq = v * v;
+            s = s + sq;
+        }
+        cn2[c] = s;
+    }
+}
+
+static __global__ void k_km_init(const uint32_t *__restrict__ packed, const uint32_t *__restrict__ seeds, uint32_t n,
+                                 uint32_t k, float *__restrict__ C, float *__restrict__ cn2)
+{
+    const uint32_t c = blockIdx.x;
+    const uint32_t D = 4 * n;
+    const uint32_t w = packed[seeds[c]];
+    for (uint32_t d = threadIdx.x; d < D; d += blockDim.x) C[(size_t)c * D + d] = (sym_at(w, d >> 2) == (d & 3)) ? 1.0f : 0.0f;
+    if (threadIdx.x == 0) cn2[c] = (float)n;
+}
+
+// k distinct seed indices: a partial Fisher-Yates shuffle of 0..N-1 driven by
+// SplitMix64(seed); seed index c is the c-th drawn.
+void kmeans_seeds(uint32_t N, uint32_t k, uint64_t seed, std::vector<uint32_t> &out)
+{
+    std::vector<uint32_t> idx(N);
+    for (uint32_t i = 0; i < N; ++i) idx[i] = i;
+    uint64_t st = seed;
+    out.resize(k);
+    for (uint32_t i = 0; i < k; ++i) {
+        st += 0x9E3779B97F4A7C15ull;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        const uint32_t j = i + (uint32_t)(z % (uint64_t)(N - i));
+        std::swap(idx[i], idx[j]);
+        out[i] = idx[i];
+    }
+}
+
+struct KmBufs {
+    DevBuf C, cn2, cnt, size, seeds, assign;
+};
+
+static KmBufs &km_bufs(igc_ctx *ctx) { return *reinterpret_cast<KmBufs *>(ctx->km); }
+
+static uint32_t km_chunk(uint32_t D) { return std::max<uint32_t>(1, std::min<uint32_t>(256, 8192 / D)); }
+
+static uint32_t km_grid(igc_ctx *ctx, uint32_t N)
+{
+    // at least ceil(N / 32768) workgroups so each sees <= 32768 points in k_km_accum
+    return std::max<uint32_t>(grid_for(N, kKmThreads, (uint32_t)ctx->cu_count * 4), (N + 32767) / 32768);
+}
+
+// Launches one Lloyd iteration on ctx->stream (capturable: kernels only).
+static void lloyd_iteration(igc_ctx *ctx, KmBufs &b, uint32_t N, uint32_t n, uint32_t k)
+{
+    hipStream_t s = ctx->stream;
+    const uint32_t D = 4 * n;
+    const uint32_t kc = km_chunk(D);
+    const size_t lds_assign = sizeof(float) * ((size_t)kc * D + kc);
+    const size_t lds_accum = sizeof(uint32_t) * (((size_t)k * D + 1) / 2 + k);
+    const uint32_t grid = km_grid(ctx, N);
+    hipLaunchKernelGGL(k_km_assign, dim3(grid), dim3(kKmThreads), lds_assign, s, ctx->packed.as<uint32_t>(), N, n,
+                       b.C.as<float>(), b.cn2.as<float>(), k, kc, b.assign.as<int32_t>());
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for((uint64_t)k * D, 256, 1024)), dim3(256), 0, s, b.cnt.as<uint32_t>(),
+                       (uint64_t)k * D, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, b.size.as<uint32_t>(), (uint64_t)k, 0u);
+    hipLaunchKernelGGL(k_km_accum, dim3(grid), dim3(kKmThreads), lds_accum, s, ctx->packed.as<uint32_t>(), N, n,
+                       b.assign.as<int32_t>(), k, b.cnt.as<uint32_t>(), b.size.as<uint32_t>());
+    hipLaunchKernelGGL(k_km_update, dim3(k), dim3(64), 0, s, b.cnt.as<uint32_t>(), b.size.as<uint32_t>(), D,
+                       b.C.as<float>(), b.cn2.as<float>());
+}
+
+// K-means on symbols already in device memory: seeds -> `iters` Lloyd
+// iterations (assign, histogram, update) -> a final assign against the last
+// centroids, left in the context's assign buffer.
+int kmeans_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n, uint32_t k, uint32_t iters,
+                         uint64_t seed, bool use_graph, float *ms_out)
+{
+    if (n < 1 || n > IGC_MAX_LEN) return set_error(IGC_ERR_INVALID, "string length n=%u outside 1..%d", n, IGC_MAX_LEN);
+    if (k < 1 || k > N) return set_error(IGC_ERR_INVALID, "k=%u must be in 1..N (N=%u)", k, N);
+    const uint32_t D = 4 * n;
+    if (((size_t)k * D + 1) / 2 * 4 + (size_t)k * 4 > 64 * 1024)
+        return set_error(IGC_ERR_INVALID, "k=%u too large for the LDS histogram (k*4n/2 + k words <= 16384)", k);
+    KmBufs &b = km_bufs(ctx);
+    int rc;
+    if ((rc = ctx->packed.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
+    if ((rc = ctx->misc.ensure(sizeof(uint32_t) * kMiscWords))) return rc;
+    if ((rc = b.C.ensure(sizeof(float) * (size_t)k * D))) return rc;
+    if ((rc = b.cn2.ensure(sizeof(float) * k))) return rc;
+    if ((rc = b.cnt.ensure(sizeof(uint32_t) * (size_t)k * D))) return rc;
+    if ((rc = b.size.ensure(sizeof(uint32_t) * k))) return rc;
+    if ((rc = b.seeds.ensure(sizeof(uint32_t) * k))) return rc;
+    if ((rc = b.assign.ensure(sizeof(int32_t) * (size_t)N))) return rc;
+    std::vector<uint32_t> seeds;
+    kmeans_seeds(N, k, seed, seeds);
+    hipStream_t s = ctx->stream;
+    uint32_t *misc = ctx->misc.as<uint32_t>();
+    HIP_TRY(hipEventRecord(ctx->ev[0], s));
+    HIP_TRY(hipMemcpyAsync(b.seeds.p, seeds.data(), sizeof(uint32_t) * k, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, misc, (uint64_t)kMiscWords, 0u);
+    hipLaunchKernelGGL(k_pack, dim3((N + 255) / 256), dim3(256), 0, s, d_syms, N, n, ctx->packed.as<uint32_t>(),
+                       misc + 2);
+    hipLaunchKernelGGL(k_km_init, dim3(k), dim3(64), 0, s, ctx->packed.as<uint32_t>(), b.seeds.as<uint32_t>(), n, k,
+                       b.C.as<float>(), b.cn2.as<float>());
+    HIP_TRY(hipGetLastError());
+    if (use_graph && iters > 0) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        lloyd_iteration(ctx, b, N, n, k);
+        hipError_t e = hipStreamEndCapture(s, &g);
+        if (e != hipSuccess) return set_error(IGC_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return set_error(IGC_ERR_DEVICE, "graph instantiate failed: %s", hipGetErrorString(e));
+        }
+        for (uint32_t t = 0; t < iters && e == hipSuccess; ++t) e = hipGraphLaunch(ge, s);
+        (void)hipGraphExecDestroy(ge);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return set_error(IGC_ERR_DEVICE, "graph launch failed: %s", hipGetErrorString(e));
+    } else {
+        for (uint32_t t = 0; t < iters; ++t) lloyd_iteration(ctx, b, N, n, k);
+        HIP_TRY(hipGetLastError());
+    }
+    // final assignment against the last centroids
+    const uint32_t kc = km_chunk(D);
+    hipLaunchKernelGGL(k_km_assign, dim3(km_grid(ctx, N)), dim3(kKmThreads), sizeof(float) * ((size_t)kc * D + kc), s,
+                       ctx->packed.as<uint32_t>(), N, n, b.C.as<float>(), b.cn2.as<float>(), k, kc,
+                       b.assign.as<int32_t>());
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ctx->ev[1], s));
+    uint32_t host_misc[kMiscWords];
+    HIP_TRY(hipMemcpyAsync(host_misc, misc, sizeof host_misc, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (host_misc[2]) return set_error(IGC_ERR_INVALID, "symbol code above 3 in input");
+    if (ms_out) HIP_TRY(hipEventElapsedTime(ms_out, ctx->ev[0], ctx->ev[1]));
+    return IGC_OK;
+}
+
+void kmeans_release(igc_ctx *ctx)
+{
+    if (!ctx->km) return;
+    KmBufs *b = reinterpret_cast<KmBufs *>(ctx->km);
+    for (DevBuf *d : {&b->C, &b->cn2, &b->cnt, &b->size, &b->seeds, &b->assign}) d->release();
+    delete b;
+    ctx->km = nullptr;
+}
+
+int kmeans_attach(igc_ctx *ctx)
+{
+    if (!ctx->km) ctx->km = new (std::nothrow) KmBufs();
+    return ctx->km ? IGC_OK : set_error(IGC_ERR_OOM, "host allocation failed");
+}
+
+int32_t *kmeans_assign_ptr(igc_ctx *ctx) { return reinterpret_cast<KmBufs *>(ctx->km)->assign.as<int32_t>(); }
+
+} // namespace igc
+
+using namespace igc;
+
+extern "C" {
+
+IGC_API int igc_kmeans(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint32_t n, uint32_t k, uint32_t iters,
+                       uint64_t seed, uint32_t flags, int32_t *assign_out, float *centroids_out)
+{
+    if (!ctx || (!syms && N) || !assign_out) return set_error(IGC_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    int rc;
+    if ((rc = kmeans_attach(ctx))) return rc;
+    const size_t bytes = (size_t)N * n;
+    if ((rc = ctx->syms.ensure(std::max<size_t>(bytes, 1)))) return rc;
+    if (bytes) HIP_TRY(hipMemcpyAsync(ctx->syms.p, syms, bytes, hipMemcpyHostToDevice, ctx->stream));
+    float ms = 0;
+    if ((rc = kmeans_device_locked(ctx, ctx->syms.as<uint8_t>(), N, n, k, iters, seed, !(flags & IGC_KMEANS_EAGER),
+                                   &ms)))
+        return rc;
+    KmBufs &b = *reinterpret_cast<KmBufs *>(ctx->km);
+    HIP_TRY(hipMemcpyAsync(assign_out, b.assign.p, sizeof(int32_t) * (size_t)N, hipMemcpyDeviceToHost, ctx->stream));
+    if (centroids_out)
+        HIP_TRY(hipMemcpyAsync(centroids_out, b.C.p, sizeof(float) * (size_t)k * 4 * n, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->last = igc_timing{};
+    ctx->last.total_ms = ms;
+    return IGC_OK;
+}
+
+IGC_API int igc_kmeans_seeds(uint32_t N, uint32_t k, uint64_t seed, uint32_t *out)
+{
+    if (!out || k > N) return set_error(IGC_ERR_INVALID, "bad arguments");
+    std::vector<uint32_t> v;
+    kmeans_seeds(N, k, seed, v);
+    std::memcpy(out, v.data(), sizeof(uint32_t) * k);
+    return IGC_OK;
+}
+
+} // extern "C"

@@ -359,6 +359,7 @@ IGC_API int igc_ctx_destroy(igc_ctx *ctx)
         for (DevBuf *b : {&ctx->syms, &ctx->packed, &ctx->edges, &ctx->misc, &ctx->deg, &ctx->cursor, &ctx->row_ptr,
                           &ctx->col, &ctx->mid, &ctx->big, &ctx->bsum})
             b->release();
+        kmeans_release(ctx);
         for (auto &e : ctx->ev)
             if (e) (void)hipEventDestroy(e);
         if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);

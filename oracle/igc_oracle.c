@@ -491,3 +491,93 @@ ORC_EXPORT void orc_conflict_pairs(const uint8_t *A, const uint8_t *B, int M, in
 {
     for (int r = 0; r < M; ++r) out[r] = (uint8_t)orc_conflict(A + (size_t)r * n, B + (size_t)r * n, n, min_ed);
 }
+
+/* ------------------------------------------------------------------ */
+/* FP32 K-means on one-hot symbol features: the CPU statement of the GPU
+ * kernels' arithmetic (no reference counterpart: the reference assigns to
+ * random centres by edit distance, src/clustering/impl/random_cluster/
+ * RandomCluster.cpp:42-136).  Seeds: partial Fisher-Yates over 0..N-1 driven
+ * by SplitMix64(seed).  Every float sum runs in the same order as on the GPU;
+ * this file is compiled with -ffp-contract=off. */
+static uint64_t splitmix_next(uint64_t *st)
+{
+    *st += 0x9E3779B97F4A7C15ULL;
+    uint64_t z = *st;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+ORC_EXPORT void orc_kmeans_seeds(int N, int k, uint64_t seed, uint32_t *out)
+{
+    uint32_t *idx = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(N > 0 ? N : 1));
+    for (int i = 0; i < N; ++i) idx[i] = (uint32_t)i;
+    uint64_t st = seed;
+    for (int i = 0; i < k; ++i) {
+        uint64_t z = splitmix_next(&st);
+        uint32_t j = (uint32_t)i + (uint32_t)(z % (uint64_t)(N - i));
+        uint32_t t = idx[i]; idx[i] = idx[j]; idx[j] = t;
+        out[i] = idx[i];
+    }
+    free(idx);
+}
+
+static void km_assign(const uint8_t *syms, int N, int n, int k, const float *C, const float *cn2, int32_t *assign)
+{
+    const int D = 4 * n;
+    for (int i = 0; i < N; ++i) {
+        const uint8_t *s = syms + (size_t)i * n;
+        float best = 0.0f;
+        int bi = -1;
+        for (int c = 0; c < k; ++c) {
+            const float *row = C + (size_t)c * D;
+            float acc = 0.0f;
+            for (int p = 0; p < n; ++p) acc = acc + row[4 * p + s[p]];
+            float twice = 2.0f * acc;
+            float dist = cn2[c] - twice;
+            if (bi < 0 || dist < best) { best = dist; bi = c; }
+        }
+        assign[i] = bi;
+    }
+}
+
+ORC_EXPORT void orc_kmeans(const uint8_t *syms, int N, int n, int k, int iters, uint64_t seed, int32_t *assign,
+                           float *C_out)
+{
+    const int D = 4 * n;
+    float *C = (float *)calloc((size_t)k * D, sizeof(float));
+    float *cn2 = (float *)calloc((size_t)k, sizeof(float));
+    uint32_t *cnt = (uint32_t *)calloc((size_t)k * D, sizeof(uint32_t));
+    uint32_t *size = (uint32_t *)calloc((size_t)k, sizeof(uint32_t));
+    uint32_t *seeds = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)k);
+    orc_kmeans_seeds(N, k, seed, seeds);
+    for (int c = 0; c < k; ++c) {
+        const uint8_t *s = syms + (size_t)seeds[c] * n;
+        for (int p = 0; p < n; ++p) C[(size_t)c * D + 4 * p + s[p]] = 1.0f;
+        cn2[c] = (float)n;
+    }
+    for (int it = 0; it < iters; ++it) {
+        km_assign(syms, N, n, k, C, cn2, assign);
+        memset(cnt, 0, sizeof(uint32_t) * (size_t)k * D);
+        memset(size, 0, sizeof(uint32_t) * (size_t)k);
+        for (int i = 0; i < N; ++i) {
+            const uint8_t *s = syms + (size_t)i * n;
+            for (int p = 0; p < n; ++p) cnt[(size_t)assign[i] * D + 4 * p + s[p]]++;
+            size[assign[i]]++;
+        }
+        for (int c = 0; c < k; ++c) {
+            if (size[c])
+                for (int d = 0; d < D; ++d) C[(size_t)c * D + d] = (float)cnt[(size_t)c * D + d] / (float)size[c];
+            float acc = 0.0f;
+            for (int d = 0; d < D; ++d) {
+                float v = C[(size_t)c * D + d];
+                float sq = v * v;
+                acc = acc + sq;
+            }
+            cn2[c] = acc;
+        }
+    }
+    km_assign(syms, N, n, k, C, cn2, assign);
+    if (C_out) memcpy(C_out, C, sizeof(float) * (size_t)k * D);
+    free(C); free(cn2); free(cnt); free(size); free(seeds);
+}

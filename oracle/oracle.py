@@ -64,6 +64,11 @@ def lib() -> ctypes.CDLL:
         L.orc_conflict_matrix.restype = None
         L.orc_conflict_pairs.argtypes = [u8p, u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
         L.orc_conflict_pairs.restype = None
+        L.orc_kmeans_seeds.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_void_p]
+        L.orc_kmeans_seeds.restype = None
+        L.orc_kmeans.argtypes = [u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                 ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_kmeans.restype = None
         _lib = L
     return _lib
 
@@ -171,3 +176,19 @@ def conflict_pairs(A: np.ndarray, B: np.ndarray, min_ed: int) -> np.ndarray:
     out = np.zeros(a.shape[0], dtype=np.uint8)
     lib().orc_conflict_pairs(pa, pb, a.shape[0], a.shape[1], int(min_ed), out.ctypes.data)
     return out
+
+
+def kmeans_seeds(N: int, k: int, seed: int) -> np.ndarray:
+    out = np.zeros(k, dtype=np.uint32)
+    lib().orc_kmeans_seeds(int(N), int(k), int(seed), out.ctypes.data)
+    return out
+
+
+def kmeans(syms: np.ndarray, k: int, iters: int, seed: int):
+    """(assign int32[N], centroids float32[k, 4n]) with the GPU kernels' arithmetic."""
+    s, ps = _u8(syms)
+    N, n = s.shape
+    assign = np.zeros(N, dtype=np.int32)
+    C = np.zeros((k, 4 * n), dtype=np.float32)
+    lib().orc_kmeans(ps, N, n, int(k), int(iters), int(seed), assign.ctypes.data, C.ctypes.data)
+    return assign, C
