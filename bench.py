@@ -146,6 +146,48 @@ def batched_block(ctx, torch, steps: int = 3, seed: int = 3, sets_n: int = 64, s
     }
 
 
+def serving_block(jobs: int = 200, pipelines: int = 2, host_threads: int = 16, seed: int = 5):
+    """The codebook-request stream (Poisson 20 jobs/s, n in 11..16, editDist in
+    3..5, sizes lognormal(ln 30000, 1.2) clipped to [500, 400000]) replayed in
+    real time: each job is submitted at its arrival time; latency = completion
+    (codebook ready on the host) - arrival."""
+    import indexgen_amd as igc
+    from indexgen_amd import workloads as W
+    specs = W.serving(seed=seed)[:jobs]
+    data = [(sp.draw(), sp.edit_dist) for sp in specs]
+    srv = igc.Server(0, pipelines=pipelines, host_threads=host_threads)
+    srv.wait(srv.submit(data[0][0][:64], data[0][1]))  # warm the pipelines
+    t0 = srv.clock()
+    ids = []
+    for sp, (syms, ed) in zip(specs, data):
+        delay = t0 + sp.arrival_s - srv.clock()
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(srv.submit(syms, ed))
+    res = [srv.wait(j) for j in ids]
+    info = srv.stats()
+    srv.close()
+    bad = [r for r in res if r["status"] != 0]
+    lat = np.array([r["done_s"] - r["submit_s"] for r in res]) * 1e3
+    gpu_lat = np.array([r["gpu_done_s"] - r["submit_s"] for r in res]) * 1e3
+    makespan = max(r["done_s"] for r in res) - min(r["submit_s"] for r in res)
+    pairs = sum(sp.pairs for sp in specs)
+    return {
+        "workload": f"first {jobs} jobs of the seeded stream (seed {seed}), real-time arrivals over "
+                    f"{specs[-1].arrival_s:.1f} s",
+        "jobs": jobs, "failed": len(bad), "pairs": pairs, "edges": int(sum(r["n_edges"] for r in res)),
+        "throughput_gpairs_s": round(pairs / makespan / 1e9, 3),
+        "offered_gpairs_s": round(pairs / specs[-1].arrival_s / 1e9, 3),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2), "p99": round(float(np.percentile(lat, 99)), 2),
+                       "max": round(float(lat.max()), 2)},
+        "graph_latency_ms": {"p50": round(float(np.percentile(gpu_lat, 50)), 2),
+                             "p99": round(float(np.percentile(gpu_lat, 99)), 2)},
+        "makespan_s": round(makespan, 3),
+        "launches": int(info["batches"]), "max_jobs_per_launch": int(info["max_batch_jobs"]),
+        "gpu_busy_s": round(info["gpu_busy_s"], 3), "pipelines": pipelines, "greedy_threads": host_threads,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +197,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-batched", action="store_true", help="skip the uneven-sets comparison block")
+    ap.add_argument("--serving-jobs", type=int, default=200, help="jobs of the serving stream to replay (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (scales rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -260,6 +303,11 @@ def main():
                          "codewords": int(order.size)},
         "roofline": roofline,
     }
+    if args.serving_jobs > 0:
+        result["serving"] = serving_block(args.serving_jobs, host_threads=host_threads())
+        result["step_latency_ms"] = result["latency_ms"]
+        result["latency_ms"] = {k: result["serving"]["latency_ms"][k] for k in ("p50", "p99")}
+        result["latency_source"] = "serving stream job latency (arrival to codebook); step_latency_ms is one build"
     if not args.no_batched:
         result["batched"] = batched_block(ctx, torch)
         result["many_small_sets"] = batched_block(ctx, torch, steps=5, seed=4, sets_n=256, scale=0.08,

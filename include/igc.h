@@ -219,6 +219,52 @@ IGC_API int igc_refine(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint32_t n
                        uint32_t kmeans_iters, uint32_t max_rounds, uint32_t converge_n, uint64_t seed,
                        uint32_t host_threads, uint32_t *out_idx, uint32_t *out_count, igc_refine_stats *stats);
 
+/* ---- continuous-batching server ---- */
+
+typedef struct igc_server igc_server;
+
+/* Result of one job: the greedy codebook (indices into the job's candidates, in
+ * selection order; malloc'd, release with igc_job_free) and its timeline in
+ * seconds on the server clock.  Latency = done_s - submit_s. */
+typedef struct igc_job_result {
+    uint64_t id;
+    int status;           /* IGC_OK, or the job's error (IGC_ERR_ADMISSION when it can never fit) */
+    uint32_t n_nodes;
+    uint32_t codewords;
+    uint32_t batch;       /* sequence number of the launch the job ran in */
+    uint32_t batch_jobs;  /* jobs sharing that launch */
+    uint32_t pad;
+    uint32_t *order;
+    uint64_t n_edges;
+    double submit_s, admit_s, gpu_done_s, done_s;
+} igc_job_result;
+
+typedef struct igc_server_stats {
+    uint64_t batches, jobs_done, pairs_done, queued;
+    uint64_t in_use_bytes, max_in_use_bytes, max_batch_jobs;
+    double gpu_busy_s;
+} igc_server_info;
+
+/* Starts a server on `device` with an HBM budget (bytes; 0 = 200 GiB) shared by
+ * the launches in flight, `pipelines` GPU pipelines (1 or 2, each with its own
+ * stream) and `host_threads` greedy workers.  Jobs submitted while others run
+ * are admitted FIFO as soon as a pipeline is free and the budget allows, and
+ * all jobs admitted together run in one persistent launch. */
+IGC_API int igc_server_create(int device, uint64_t hbm_budget_bytes, uint32_t pipelines, uint32_t host_threads,
+                              igc_server **out);
+/* Queues a codebook job (the candidates are copied); returns at once. */
+IGC_API int igc_server_submit(igc_server *srv, const uint8_t *syms, uint32_t N, uint32_t n, uint32_t edit_dist,
+                              uint64_t *job_id);
+/* Waits up to timeout_ms (< 0: forever) for a job; IGC_ERR_BUSY on timeout.
+ * On success the job is retired and its result handed over. */
+IGC_API int igc_server_wait(igc_server *srv, uint64_t job_id, int64_t timeout_ms, igc_job_result *out);
+IGC_API int igc_server_poll(igc_server *srv, uint64_t job_id, igc_job_result *out);
+IGC_API int igc_job_free(igc_job_result *res);
+IGC_API double igc_server_clock(igc_server *srv);
+IGC_API int igc_server_stats(igc_server *srv, igc_server_info *out);
+/* Drains the queue, joins the threads and releases the devices' buffers. */
+IGC_API int igc_server_destroy(igc_server *srv);
+
 /* ---- greedy independent set and outputs (host) ---- */
 
 /* Min-degree greedy independent set of a host CSR, with the reference's exact

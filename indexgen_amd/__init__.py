@@ -107,6 +107,30 @@ class _RefineStats(ctypes.Structure):
                 ("round", _RefineRound * IGC_REFINE_MAX_ROUNDS)]
 
 
+class _JobResult(ctypes.Structure):
+    _fields_ = [
+        ("id", ctypes.c_uint64),
+        ("status", ctypes.c_int),
+        ("n_nodes", ctypes.c_uint32),
+        ("codewords", ctypes.c_uint32),
+        ("batch", ctypes.c_uint32),
+        ("batch_jobs", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
+        ("order", ctypes.POINTER(ctypes.c_uint32)),
+        ("n_edges", ctypes.c_uint64),
+        ("submit_s", ctypes.c_double),
+        ("admit_s", ctypes.c_double),
+        ("gpu_done_s", ctypes.c_double),
+        ("done_s", ctypes.c_double),
+    ]
+
+
+class _ServerInfo(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in ("batches", "jobs_done", "pairs_done", "queued", "in_use_bytes",
+                                               "max_in_use_bytes", "max_batch_jobs")] + [("gpu_busy_s",
+                                                                                          ctypes.c_double)]
+
+
 class _CodebookInfo(ctypes.Structure):
     _fields_ = [
         ("code_len", ctypes.c_int),
@@ -160,6 +184,14 @@ def lib() -> ctypes.CDLL:
         "igc_write_pair_file": (i, [ctypes.POINTER(_Csr), ctypes.c_char_p]),
         "igc_write_codebook": (i, [ctypes.c_char_p, u8p, u32, vp, u32, ctypes.POINTER(_CodebookInfo),
                                    ctypes.c_char_p, ctypes.c_size_t]),
+        "igc_server_create": (i, [i, u64, u32, u32, ctypes.POINTER(vp)]),
+        "igc_server_submit": (i, [vp, u8p, u32, u32, u32, ctypes.POINTER(u64)]),
+        "igc_server_wait": (i, [vp, u64, ctypes.c_int64, ctypes.POINTER(_JobResult)]),
+        "igc_server_poll": (i, [vp, u64, ctypes.POINTER(_JobResult)]),
+        "igc_job_free": (i, [ctypes.POINTER(_JobResult)]),
+        "igc_server_clock": (ctypes.c_double, [vp]),
+        "igc_server_stats": (i, [vp, ctypes.POINTER(_ServerInfo)]),
+        "igc_server_destroy": (i, [vp]),
         "igc_last_error": (ctypes.c_char_p, []),
         "igc_backend": (ctypes.c_char_p, []),
         "igc_version": (ctypes.c_char_p, []),
@@ -405,6 +437,51 @@ class Context:
         _check(lib().igc_last_timing(self._h, ctypes.byref(t)))
         return Timing(t.pack_ms, t.pairs_ms, t.csr_ms, t.total_ms, int(t.pairs), int(t.n_edges), int(t.launches),
                       int(t.retries))
+
+
+class Server:
+    """Continuous-batching codebook server (igc_server): submit jobs from any
+    thread, collect codebooks with their timelines."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0, pipelines: int = 2, host_threads: int = 8):
+        h = ctypes.c_void_p()
+        _check(lib().igc_server_create(int(device), int(hbm_budget_bytes), int(pipelines), int(host_threads),
+                                       ctypes.byref(h)))
+        self._h = h
+        self._fin = weakref.finalize(self, lib().igc_server_destroy, h)
+
+    def close(self) -> None:
+        self._fin()
+
+    def clock(self) -> float:
+        return float(lib().igc_server_clock(self._h))
+
+    def submit(self, candidates, min_ed: int) -> int:
+        s = to_symbols(candidates)
+        jid = ctypes.c_uint64()
+        _check(lib().igc_server_submit(self._h, s.ctypes.data if s.size else None, s.shape[0], s.shape[1],
+                                       int(min_ed), ctypes.byref(jid)))
+        return int(jid.value)
+
+    def wait(self, job_id: int, timeout_ms: int = -1):
+        """dict with status, order (np.uint32), n_edges, timeline; None on timeout."""
+        r = _JobResult()
+        rc = lib().igc_server_wait(self._h, int(job_id), int(timeout_ms), ctypes.byref(r))
+        if rc == IGC_ERR_BUSY:
+            return None
+        _check(rc)
+        order = (np.ctypeslib.as_array(r.order, shape=(r.codewords,)).copy() if r.codewords
+                 else np.zeros(0, dtype=np.uint32))
+        lib().igc_job_free(ctypes.byref(r))
+        return {"id": int(r.id), "status": int(r.status), "order": order, "n_nodes": int(r.n_nodes),
+                "n_edges": int(r.n_edges), "submit_s": r.submit_s, "admit_s": r.admit_s,
+                "gpu_done_s": r.gpu_done_s, "done_s": r.done_s, "batch": int(r.batch),
+                "batch_jobs": int(r.batch_jobs)}
+
+    def stats(self) -> dict:
+        x = _ServerInfo()
+        _check(lib().igc_server_stats(self._h, ctypes.byref(x)))
+        return {k: getattr(x, k) for k, _ in _ServerInfo._fields_}
 
 
 _default_ctx = None

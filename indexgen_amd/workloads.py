@@ -72,29 +72,37 @@ def uniform_strings(g: np.random.Generator, count: int, n: int) -> np.ndarray:
     return g.integers(0, 4, size=(count, n), dtype=np.uint8)
 
 
+def _keys(syms: np.ndarray) -> np.ndarray:
+    """One integer per string (base-4 digits), equal iff the strings are equal."""
+    n = syms.shape[1]
+    w = (4 ** np.arange(n, dtype=np.uint64)).astype(np.uint64)
+    return (syms.astype(np.uint64) * w[None, :]).sum(axis=1)
+
+
 def filtered_uniform(g: np.random.Generator, count: int, n: int, exclude: np.ndarray | None = None,
                      predicate=None) -> np.ndarray:
     """`count` distinct uniform strings passing the filters (and `predicate`),
     none equal to a row of `exclude`, in draw order."""
-    have = np.zeros((0, n), dtype=np.uint8)
-    seen = set() if exclude is None else {r.tobytes() for r in exclude}
-    while have.shape[0] < count:
-        need = count - have.shape[0]
+    chunks, have = [], 0
+    seen = np.zeros(0, dtype=np.uint64) if exclude is None else np.unique(_keys(exclude))
+    while have < count:
+        need = count - have
         batch = uniform_strings(g, max(2 * need, 1024), n)
         ok = filter_mask(batch)
         if predicate is not None:
             ok &= predicate(batch)
-        keep = []
-        for r in batch[ok]:
-            k = r.tobytes()
-            if k not in seen:
-                seen.add(k)
-                keep.append(r)
-                if len(keep) == need:
-                    break
-        if keep:
-            have = np.concatenate([have, np.stack(keep)])
-    return have
+        batch = batch[ok]
+        k = _keys(batch)
+        _, first = np.unique(k, return_index=True)
+        first = np.sort(first)            # first occurrence within the draw, in draw order
+        batch, k = batch[first], k[first]
+        fresh = ~np.isin(k, seen)
+        batch, k = batch[fresh][:need], k[fresh][:need]
+        if batch.shape[0]:
+            chunks.append(batch)
+            have += batch.shape[0]
+            seen = np.union1d(seen, k)
+    return np.concatenate(chunks) if chunks else np.zeros((0, n), dtype=np.uint8)
 
 
 def plain_small(seed: int = 1, count: int = 20000, n: int = 11) -> np.ndarray:
@@ -165,15 +173,26 @@ def vt_refine(seed: int = 4, total: int = 500000, n: int = 16) -> np.ndarray:
 
 
 @dataclass
-class Job:
+class JobSpec:
     arrival_s: float
-    cset: CandidateSet
+    n: int
+    edit_dist: int
+    size: int
+    seed: int
+
+    @property
+    def pairs(self) -> int:
+        return self.size * (self.size - 1) // 2
+
+    def draw(self) -> np.ndarray:
+        """The job's candidates: `size` distinct uniform filtered strings."""
+        return filtered_uniform(rng(self.seed), self.size, self.n)
 
 
 def serving(seed: int = 5, jobs: int = 2000, rate: float = 20.0, mu: float = np.log(30000), sigma: float = 1.2,
-            lo: int = 500, hi: int = 400000, scale: float = 1.0, materialize: bool = True) -> list:
-    """Poisson arrivals; `scale` shrinks sizes.  With materialize=False the
-    strings are drawn lazily by `Job.cset` users via `draw_job`."""
+            lo: int = 500, hi: int = 400000) -> list:
+    """Poisson(rate) arrivals of codebook jobs; strings are drawn per job with
+    JobSpec.draw() so a long stream need not be held in memory."""
     g = rng(seed)
     t = 0.0
     out = []
@@ -181,8 +200,6 @@ def serving(seed: int = 5, jobs: int = 2000, rate: float = 20.0, mu: float = np.
         t += float(g.exponential(1.0 / rate))
         n = int(g.integers(11, 17))
         ed = int(g.integers(3, 6))
-        size = max(int(np.clip(np.exp(g.normal(mu, sigma)), lo, hi) * scale), 2)
-        job_seed = int(g.integers(0, 2 ** 62))
-        syms = filtered_uniform(rng(job_seed), size, n) if materialize else None
-        out.append(Job(t, CandidateSet(syms if syms is not None else np.zeros((size, n), np.uint8), n, ed)))
+        size = int(np.clip(np.exp(g.normal(mu, sigma)), lo, hi))
+        out.append(JobSpec(t, n, ed, size, int(g.integers(0, 2 ** 62))))
     return out

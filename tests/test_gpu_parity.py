@@ -261,3 +261,50 @@ def test_refine_matches_oracle(gpu_ctx):
     # the codebook of the last round is independent inside every cluster; across
     # clusters the reference's loop guarantees nothing, so only the size is checked
     assert 0 < got.size <= syms.shape[0]
+
+
+def test_server_results_same_alone_and_in_the_mix(gpu_ctx):
+    specs = W.serving(seed=5, jobs=40, mu=np.log(3000), sigma=1.0, lo=100, hi=20000)
+    data = [(sp.draw(), sp.edit_dist) for sp in specs]
+    banner(f"server: {len(data)} jobs, pairs={sum(sp.pairs for sp in specs):.3e}")
+    srv = igc.Server(0, pipelines=2, host_threads=4)
+    try:
+        ids = [srv.submit(s, ed) for s, ed in data]          # all at once: they share launches
+        mixed = [srv.wait(j) for j in ids]
+        assert max(r["batch_jobs"] for r in mixed) > 1        # work really was batched
+        alone = []
+        for s, ed in data[:12]:                               # one at a time: a launch each
+            r = srv.wait(srv.submit(s, ed))
+            assert r["batch_jobs"] == 1
+            alone.append(r)
+    finally:
+        srv.close()
+    for (s, ed), r in zip(data, mixed):
+        assert r["status"] == 0
+        adj = gpu_ctx.graph_build(s, ed)                      # the single-set path
+        assert r["n_edges"] == adj.n_edges
+        assert np.array_equal(r["order"], igc.greedy_order(adj))
+        assert r["submit_s"] <= r["admit_s"] <= r["gpu_done_s"] <= r["done_s"]
+    for r_alone, r_mix in zip(alone, mixed):
+        assert np.array_equal(r_alone["order"], r_mix["order"])
+    # one job checked all the way down to the oracle
+    s, ed = data[0]
+    rp, col = O.graph_csr(s, ed)
+    assert np.array_equal(mixed[0]["order"].astype(np.int32), O.greedy(rp, col))
+
+
+def test_server_admission_budget(gpu_ctx):
+    rng = np.random.default_rng(41)
+    big = W.filtered_uniform(rng, 20000, 12)
+    srv = igc.Server(0, hbm_budget_bytes=1 << 20, pipelines=1, host_threads=2)   # 1 MiB
+    try:
+        r = srv.wait(srv.submit(big, 4))                      # can never fit: refused
+        assert r["status"] == igc.IGC_ERR_ADMISSION
+        small = [W.filtered_uniform(rng, 300, 12) for _ in range(6)]
+        ids = [srv.submit(s, 4) for s in small]               # fit only a few at a time
+        res = [srv.wait(j) for j in ids]
+        assert all(x["status"] == 0 for x in res)
+        for s, x in zip(small, res):
+            assert np.array_equal(x["order"], igc.greedy_order(gpu_ctx.graph_build(s, 4)))
+    finally:
+        srv.close()
