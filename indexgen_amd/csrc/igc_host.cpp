@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
@@ -39,64 +40,78 @@ void reorder_threads(igc_csr &csr, uint32_t T)
 //     neighbour u from bucket deg(u) to deg(u)-1 (DecreaseSum, :156-167): a
 //     swap-remove that fills the hole with the bucket's last node (DeleteRow,
 //     :139-154) and an append to the lower bucket.
-// Buckets live in one flat array: bucket d never holds more nodes than there
-// are nodes of initial degree >= d, because degrees only fall.
+// Layout for speed, not for semantics: a node's degree and bucket position
+// share one 8-byte record, the deleted flags are a byte array (the most
+// frequent test), buckets live in one flat array (bucket d never holds more
+// nodes than there are nodes of initial degree >= d, since degrees only fall),
+// and on large graphs the records of the next neighbours are prefetched.  The
+// visiting order is exactly the reference's.
 class Greedy {
   public:
     Greedy(uint32_t N, const uint64_t *rp, const uint32_t *col) : N_(N), rp_(rp), col_(col) {}
 
     int run(uint32_t *order, uint32_t *count)
     {
+        uint32_t maxd = 0;
         try {
-            degree_.resize(N_);
-            pos_.resize(N_);
-            deleted_.assign(N_, 0);
-            uint32_t maxd = 0;
+            st_.resize(N_);
+            dead_.assign(N_, 0);
             for (uint32_t v = 0; v < N_; ++v) {
-                degree_[v] = (uint32_t)(rp_[v + 1] - rp_[v]);
-                maxd = std::max(maxd, degree_[v]);
+                st_[v].deg = (uint32_t)(rp_[v + 1] - rp_[v]);
+                maxd = std::max(maxd, st_[v].deg);
             }
             nb_ = maxd + 1;
             std::vector<uint64_t> ge(nb_ + 1, 0);
-            for (uint32_t v = 0; v < N_; ++v) ge[degree_[v]]++;
+            for (uint32_t v = 0; v < N_; ++v) ge[st_[v].deg]++;
             for (int64_t d = (int64_t)nb_ - 1; d >= 0; --d) ge[d] += ge[d + 1];
-            off_.resize(nb_ + 1);
-            off_[0] = 0;
-            for (uint32_t d = 0; d < nb_; ++d) off_[d + 1] = off_[d] + ge[d];
-            store_.resize(off_[nb_]);
-            len_.assign(nb_, 0);
+            bk_.resize(nb_);
+            uint64_t off = 0;
+            for (uint32_t d = 0; d < nb_; ++d) {
+                bk_[d].off = off;
+                bk_[d].len = 0;
+                off += ge[d];
+            }
+            // capacity 2|E| + N, but only bucket tails are ever touched: leave it
+            // uninitialised so untouched pages are never faulted in
+            store_.reset(new uint32_t[off > 0 ? off : 1]);
         } catch (const std::bad_alloc &) {
             return set_error(IGC_ERR_OOM, "greedy: host allocation failed");
         }
+        // prefetching only pays once the node records no longer fit in cache
+        const bool pf = N_ > (1u << 18);
         tracker_ = nb_;
         for (uint32_t v = 0; v < N_; ++v) {
-            push(degree_[v], v);
-            if (degree_[v] < tracker_) tracker_ = degree_[v];
+            push(st_[v].deg, v);
+            if (st_[v].deg < tracker_) tracker_ = st_[v].deg;
         }
         uint32_t active = N_, picked = 0;
         std::vector<uint32_t> todel;
+        todel.reserve(maxd + 1);
         while (active > 0) {
-            const uint32_t v = store_[off_[tracker_] + len_[tracker_] - 1];
+            const Bucket &b = bk_[tracker_];
+            const uint32_t v = store_[b.off + b.len - 1];
             order[picked++] = v;
             todel.clear();
-            for (uint64_t q = rp_[v]; q < rp_[v + 1]; ++q)
-                if (!deleted_[col_[q]]) todel.push_back(col_[q]);
+            const uint64_t v0 = rp_[v], v1 = rp_[v + 1];
+            for (uint64_t q = v0; q < v1; ++q)
+                if (!dead_[col_[q]]) todel.push_back(col_[q]);
             todel.push_back(v);
-            for (uint32_t x : todel) {
-                if (deleted_[x]) continue;
-                for (uint64_t q = rp_[x]; q < rp_[x + 1]; ++q) {
+            for (const uint32_t x : todel) {
+                if (dead_[x]) continue;
+                const uint64_t x0 = rp_[x], x1 = rp_[x + 1];
+                for (uint64_t q = x0; q < x1; ++q) {
+                    if (pf && q + kAhead < x1) __builtin_prefetch(&st_[col_[q + kAhead]]);
                     const uint32_t u = col_[q];
-                    if (deleted_[u]) continue;
-                    const uint32_t du = degree_[u];
-                    if (du > 0) {
-                        remove(du, u);
-                        degree_[u] = du - 1;
-                        push(du - 1, u);
-                        if (du - 1 < tracker_) tracker_ = du - 1;
-                    }
+                    if (dead_[u]) continue;
+                    const uint32_t du = st_[u].deg;
+                    if (du == 0) continue;
+                    remove(du, u);
+                    st_[u].deg = du - 1;
+                    push(du - 1, u);
+                    if (du - 1 < tracker_) tracker_ = du - 1;
                 }
-                remove(degree_[x], x);
-                deleted_[x] = 1;
+                remove(st_[x].deg, x);
+                dead_[x] = 1;
                 --active;
             }
         }
@@ -105,29 +120,41 @@ class Greedy {
     }
 
   private:
+    static constexpr uint64_t kAhead = 8;
+    struct Node {
+        uint32_t deg;
+        uint32_t pos;
+    };
+    struct Bucket {
+        uint64_t off;
+        uint32_t len;
+    };
     void push(uint32_t d, uint32_t v)
     {
-        pos_[v] = len_[d];
-        store_[off_[d] + len_[d]++] = v;
+        Bucket &b = bk_[d];
+        st_[v].pos = b.len;
+        store_[b.off + b.len++] = v;
     }
     void remove(uint32_t d, uint32_t v)
     {
-        const uint32_t p = pos_[v];
-        const uint32_t last = store_[off_[d] + len_[d] - 1];
-        store_[off_[d] + p] = last;
-        pos_[last] = p;
-        --len_[d];
-        if (d == tracker_ && len_[d] == 0)
-            while (tracker_ < nb_ && len_[tracker_] == 0) ++tracker_;
+        Bucket &b = bk_[d];
+        const uint32_t p = st_[v].pos;
+        const uint32_t last = store_[b.off + b.len - 1];
+        store_[b.off + p] = last;
+        st_[last].pos = p;
+        --b.len;
+        if (d == tracker_ && b.len == 0)
+            while (tracker_ < nb_ && bk_[tracker_].len == 0) ++tracker_;
     }
 
     uint32_t N_;
     const uint64_t *rp_;
     const uint32_t *col_;
     uint32_t nb_ = 0, tracker_ = 0;
-    std::vector<uint32_t> degree_, pos_, len_, store_;
-    std::vector<uint64_t> off_;
-    std::vector<uint8_t> deleted_;
+    std::vector<Node> st_;
+    std::vector<uint8_t> dead_;
+    std::vector<Bucket> bk_;
+    std::unique_ptr<uint32_t[]> store_;
 };
 
 } // namespace igc
