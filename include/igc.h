@@ -120,6 +120,25 @@ IGC_API int igc_graph_build(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint3
 IGC_API int igc_graph_build_device(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n,
                                    uint32_t edit_dist, igc_csr *out);
 
+/* One timed tile shape of the pair kernel: columns per lane `cpl` (1, 2, 4),
+ * rows per tile 256*cpl/q, best time over the repetitions. */
+typedef struct igc_tune_entry {
+    int cpl;
+    uint32_t q;
+    float ms;
+    float gpairs_s;
+} igc_tune_entry;
+
+/* Autotuner: times every tile shape of the pair kernel on N random strings of
+ * length n at edit_dist (device-generated, count-only kernel, best of `reps`
+ * after a warm-up) and makes the fastest the context's shape for that
+ * threshold and size class (sizes bucketed by powers of 4 from 4096).  Writes
+ * up to `cap` entries and the number tried to *count. */
+IGC_API int igc_autotune(igc_ctx *ctx, uint32_t n, uint32_t edit_dist, uint32_t N, uint32_t reps,
+                         igc_tune_entry *entries, uint32_t cap, uint32_t *count);
+/* Shape the context will use for a set of N strings at (n, edit_dist). */
+IGC_API int igc_tuned_shape(igc_ctx *ctx, uint32_t n, uint32_t edit_dist, uint32_t N, int *cpl, uint32_t *q);
+
 /* Counts the edges without materialising them: the verify mode of the
  * reference (VerifyDist, src/Utils.cpp:203-309: zero edges == SUCCESS). */
 IGC_API int igc_graph_count(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint32_t n, uint32_t edit_dist,
@@ -303,6 +322,12 @@ IGC_API int igc_write_codebook(const char *dir, const uint8_t *syms, uint32_t n,
                                uint32_t count, const igc_codebook_info *info, char *name_out, size_t name_cap);
 
 /* ---- diagnostics ---- */
+
+/* Issue rate of the pair kernel's int32 instruction types (xor, lshrrev, ffbl,
+ * max3, or3, add3, add), each in independent chains over the whole GPU:
+ * wave-level VALU instructions per second, 7 values.  No reference
+ * counterpart: it measures the roofline the pair kernel is compared with. */
+IGC_API int igc_valu_probe(igc_ctx *ctx, double *wave_instr_per_s);
 IGC_API const char *igc_last_error(void);
 /* "hip:gfx950" when a device context can be created, else "none". */
 IGC_API const char *igc_backend(void);

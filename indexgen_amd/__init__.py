@@ -131,6 +131,10 @@ class _ServerInfo(ctypes.Structure):
                                                                                           ctypes.c_double)]
 
 
+class _TuneEntry(ctypes.Structure):
+    _fields_ = [("cpl", ctypes.c_int), ("q", ctypes.c_uint32), ("ms", ctypes.c_float), ("gpairs_s", ctypes.c_float)]
+
+
 class _CodebookInfo(ctypes.Structure):
     _fields_ = [
         ("code_len", ctypes.c_int),
@@ -192,6 +196,9 @@ def lib() -> ctypes.CDLL:
         "igc_server_clock": (ctypes.c_double, [vp]),
         "igc_server_stats": (i, [vp, ctypes.POINTER(_ServerInfo)]),
         "igc_server_destroy": (i, [vp]),
+        "igc_autotune": (i, [vp, u32, u32, u32, u32, ctypes.POINTER(_TuneEntry), u32, ctypes.POINTER(u32)]),
+        "igc_tuned_shape": (i, [vp, u32, u32, u32, ctypes.POINTER(i), ctypes.POINTER(u32)]),
+        "igc_valu_probe": (i, [vp, vp]),
         "igc_last_error": (ctypes.c_char_p, []),
         "igc_backend": (ctypes.c_char_p, []),
         "igc_version": (ctypes.c_char_p, []),
@@ -424,6 +431,25 @@ class Context:
                            "pairs": int(x.pairs), "kmeans_ms": x.kmeans_ms, "pairs_ms": x.pairs_ms,
                            "cluster_s": x.cluster_s, "graph_s": x.graph_s, "greedy_s": x.greedy_s})
         return out[: cnt.value].copy(), {"rounds": rounds, "converged": bool(st.converged)}
+
+    def autotune(self, n: int, min_ed: int, N: int, reps: int = 3) -> list:
+        """Times every tile shape for (n, min_ed, N) and adopts the fastest; returns
+        [{cpl, q, ms, gpairs_s}, ...]."""
+        ents = (_TuneEntry * 32)()
+        cnt = ctypes.c_uint32()
+        _check(lib().igc_autotune(self._h, int(n), int(min_ed), int(N), int(reps), ents, 32, ctypes.byref(cnt)))
+        return [{"cpl": e.cpl, "q": e.q, "ms": e.ms, "gpairs_s": e.gpairs_s} for e in ents[: cnt.value]]
+
+    def tuned_shape(self, n: int, min_ed: int, N: int):
+        cpl, q = ctypes.c_int(), ctypes.c_uint32()
+        _check(lib().igc_tuned_shape(self._h, int(n), int(min_ed), int(N), ctypes.byref(cpl), ctypes.byref(q)))
+        return cpl.value, q.value
+
+    def valu_probe(self) -> dict:
+        """Wave-level VALU instructions per second for each int32 op type of the pair kernel."""
+        out = np.zeros(7, dtype=np.float64)
+        _check(lib().igc_valu_probe(self._h, out.ctypes.data))
+        return dict(zip(["xor", "lshrrev", "ffbl", "max3", "or3", "add3", "add"], out.tolist()))
 
     def graph_count(self, syms, min_ed: int) -> int:
         s = to_symbols(syms)

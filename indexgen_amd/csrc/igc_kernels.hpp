@@ -55,6 +55,20 @@ static __global__ void k_pack(const uint8_t *__restrict__ syms, uint32_t N, uint
     if (or_all > 3u) atomicOr(bad, 1u);
 }
 
+// Packed words of uniformly random strings of length n (SplitMix64 per index),
+// used to time tile shapes.
+static __global__ void k_random_packed(uint32_t *__restrict__ packed, uint32_t N, uint32_t n, uint64_t seed)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * (i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const uint32_t mask = n >= 16 ? 0xFFFFu : ((1u << n) - 1u);
+    packed[i] = ((uint32_t)z & mask) | (((uint32_t)(z >> 32) & mask) << 16);
+}
+
 static __global__ void k_fill_u32(uint32_t *__restrict__ p, uint64_t count, uint32_t v)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)

@@ -472,6 +472,67 @@ IGC_API int igc_graph_build(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint3
     return csr_to_host_locked(ctx, N, E, ctx->row_ptr.as<uint64_t>(), ctx->col.as<uint32_t>(), flags, out);
 }
 
+IGC_API int igc_autotune(igc_ctx *ctx, uint32_t n, uint32_t edit_dist, uint32_t N, uint32_t reps,
+                         igc_tune_entry *entries, uint32_t cap, uint32_t *count)
+{
+    int rc = check_args(ctx, N, n);
+    if (rc) return rc;
+    if (!count || (!entries && cap)) return set_error(IGC_ERR_INVALID, "null argument");
+    if (N < 2 || edit_dist == 0) return set_error(IGC_ERR_INVALID, "nothing to tune for N=%u editDist=%u", N, edit_dist);
+    if (reps < 1) reps = 1;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const int K = effective_k(n, edit_dist);
+    if ((rc = ctx->packed.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
+    if ((rc = ctx->misc.ensure(sizeof(uint32_t) * kMiscWords))) return rc;
+    hipLaunchKernelGGL(k_random_packed, dim3((N + 255) / 256), dim3(256), 0, s, ctx->packed.as<uint32_t>(), N, n,
+                       0x5EEDull + N);
+    HIP_TRY(hipGetLastError());
+    unsigned long long *counter = reinterpret_cast<unsigned long long *>(ctx->misc.as<uint32_t>());
+    float best = 0;
+    Shape best_shape = default_shape(N);
+    uint32_t k = 0;
+    for (int cpl : {1, 2, 4}) {
+        for (uint32_t q : {1u, 2u, 4u, 8u}) {
+            const TileGeom G = choose_geometry(N, cpl, q);
+            const uint64_t tiles = tile_count(G);
+            if (tiles == 0 || tiles > 0x7FFFFFFFull) continue;
+            float t_min = 0;
+            for (uint32_t r = 0; r <= reps; ++r) { // first run warms up
+                HIP_TRY(hipEventRecord(ctx->ev[0], s));
+                if ((rc = launch_pairs(K, cpl, false, s, tiles, ctx->packed.as<uint32_t>(), G, 0u, nullptr, 0,
+                                       counter)))
+                    return rc;
+                HIP_TRY(hipEventRecord(ctx->ev[1], s));
+                HIP_TRY(hipEventSynchronize(ctx->ev[1]));
+                float ms = 0;
+                HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+                if (r > 0 && (r == 1 || ms < t_min)) t_min = ms;
+            }
+            if (k < cap) entries[k] = igc_tune_entry{cpl, q, t_min, (float)((double)N * (N - 1) / 2 / (t_min * 1e6))};
+            ++k;
+            if (best == 0 || t_min < best) {
+                best = t_min;
+                best_shape = Shape{cpl, q};
+            }
+        }
+    }
+    ctx->tuned[K < 0 ? 0 : K][size_bucket(N)] = best_shape;
+    *count = k;
+    return IGC_OK;
+}
+
+IGC_API int igc_tuned_shape(igc_ctx *ctx, uint32_t n, uint32_t edit_dist, uint32_t N, int *cpl, uint32_t *q)
+{
+    if (!ctx || !cpl || !q) return set_error(IGC_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const Shape sh = pick_shape(ctx, effective_k(n, edit_dist), N);
+    *cpl = sh.cpl;
+    *q = sh.q;
+    return IGC_OK;
+}
+
 IGC_API int igc_graph_count(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint32_t n, uint32_t edit_dist,
                             uint64_t *n_edges)
 {

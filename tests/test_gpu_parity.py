@@ -308,3 +308,34 @@ def test_server_admission_budget(gpu_ctx):
             assert np.array_equal(x["order"], igc.greedy_order(gpu_ctx.graph_build(s, 4)))
     finally:
         srv.close()
+
+
+def test_autotuned_shapes_give_identical_graphs(gpu_ctx):
+    rng = np.random.default_rng(51)
+    syms = W.filtered_uniform(rng, 9000, 12)
+    rp, col = O.graph_csr(syms, 4)
+    ents = gpu_ctx.autotune(12, 4, syms.shape[0], reps=2)
+    assert len(ents) == 12 and all(e["ms"] > 0 for e in ents)
+    best = min(ents, key=lambda e: e["ms"])
+    assert gpu_ctx.tuned_shape(12, 4, syms.shape[0]) == (best["cpl"], best["q"])
+    adj = gpu_ctx.graph_build(syms, 4)  # built with the tuned shape
+    assert np.array_equal(adj.row_ptr, rp) and np.array_equal(adj.col, col)
+
+
+def test_cli_writes_the_reference_codebook_file(gpu_ctx, tmp_path):
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    syms = W.plain_small(seed=3, count=2500)
+    cf = tmp_path / "cands.txt"
+    cf.write_text("\n".join(igc.to_strings(syms)) + "\n")
+    out = subprocess.run([os.path.join(root, "indexgen_amd", "igc_cli"), "--cands", str(cf), "--min-ed", "3",
+                          "--out", str(tmp_path), "--verify", "--order-threads", "4"],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert "Vector distance SUCCESS" in out.stdout
+    rp, col = O.graph_csr(syms, 3, T=4)
+    want = igc.to_strings(syms[O.greedy(rp, col)])
+    files = [f for f in os.listdir(tmp_path) if f.startswith("CodeSize-")]
+    assert files == [f"CodeSize-{len(want):07d}_CodeLen-11_MinED-3.txt"]
+    body = open(tmp_path / files[0]).read().split("=========================================== \n")[1].split()
+    assert body == want
