@@ -54,8 +54,10 @@ extern "C" {
 #define IGC_ERR_IO 5        /* file could not be written/read */
 #define IGC_ERR_BUSY 6      /* result not ready yet (server poll) */
 
-/* longest candidate the kernels accept (two 16-bit symbol planes per word) */
-#define IGC_MAX_LEN 16
+/* longest candidate the kernels accept: the reference GPU solver's limit
+ * (L <= 32, src/cuda_edit_distance.py:287).  Packed as two symbol bit planes
+ * in a 32-bit word for n <= 16, in a 64-bit word above. */
+#define IGC_MAX_LEN 32
 
 /* ---- flags for igc_graph_build ---- */
 #define IGC_ORDER_ASCENDING 0u
@@ -104,7 +106,7 @@ IGC_API int igc_last_timing(igc_ctx *ctx, igc_timing *out);
 
 /* ---- single candidate set ---- */
 
-/* Builds the conflict graph of N candidates of length n (1 <= n <= 16) given
+/* Builds the conflict graph of N candidates of length n (1 <= n <= 32) given
  * as an N x n row-major array of symbol codes 0..3 in host memory.  edit_dist
  * is the reference's minED: edge iff Levenshtein < edit_dist.  The CSR comes
  * back in host memory.

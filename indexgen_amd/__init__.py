@@ -34,7 +34,7 @@ IGC_ERR_DEVICE = 3
 IGC_ERR_ADMISSION = 4
 IGC_ERR_IO = 5
 IGC_ERR_BUSY = 6
-IGC_MAX_LEN = 16
+IGC_MAX_LEN = 32
 IGC_ON_HOST = 0
 IGC_ON_DEVICE = 1
 

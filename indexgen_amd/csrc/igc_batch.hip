@@ -2,11 +2,14 @@
 //
 // The sets are treated as one disjoint union of graphs on a concatenated node
 // space (set s owns global ids node_off[s] .. node_off[s]+N_s-1).  One pack
-// kernel packs every set (per-set length n), one persistent pair kernel pulls
-// tiles of all sets from a global work counter (the set of a tile is found by
-// binary search over per-set tile prefix sums; K is uniform per tile and is
-// dispatched at run time), and the unchanged CSR assembly runs once over the
-// union.  Set s's CSR is the row slice node_off[s].. with ids shifted back.
+// kernel packs every set (per-set length n, each set's words at its own byte
+// offset: 32-bit words for n <= 16, 64-bit above), one persistent pair kernel
+// pulls tiles of all sets with n <= 16 from a global work counter (the set of a
+// tile is found by binary search over per-set tile prefix sums; the predicate
+// and K are uniform per tile and dispatched at run time), sets with n > 16 get
+// one pair launch each into the same edge list, and the unchanged CSR assembly
+// runs once over the union.  Set s's CSR is the row slice node_off[s].. with
+// ids shifted back.
 //
 // This replaces the reference's one process + one GPU context + one launch per
 // cluster (SolveIndependentSet, src/SparseMat.cpp:784-851, called from the
@@ -21,18 +24,21 @@
 
 namespace igc {
 
-// Largest K the persistent kernel dispatches at run time; sets with a larger
-// (non-trivial) K get their own k_pairs launch into the same edge list.
+// Predicates the persistent kernel dispatches at run time (sets with n <= 16):
+// Landau-Vishkin for K <= kBatchMaxK (pred_kind never picks it above 8 for
+// n <= 16), Myers, every pair.
 constexpr int kBatchMaxK = 8;
+enum BatchPred : uint32_t { kBatchLV = 0, kBatchMyers = 1, kBatchAll = 2, kBatchNone = 3 };
 
 struct SetDesc {
     TileGeom G;          // per-set geometry (G.N = N_s)
     uint64_t tile_begin; // first global tile of the set
     uint64_t sym_off;    // byte offset of the set's symbols in the concatenated input
+    uint64_t word_off;   // byte offset of the set's packed words (8-byte aligned)
     uint32_t node_off;   // first global node id
     uint32_t n;          // string length
     int32_t K;           // Levenshtein threshold (edge iff ED <= K); kAllPairs = every pair
-    uint32_t pad;
+    uint32_t pred;       // BatchPred of the persistent kernel; kBatchNone = not in it
 };
 
 static __device__ __forceinline__ uint32_t find_set_by(const SetDesc *__restrict__ sets, uint32_t nsets, uint64_t key,
@@ -49,7 +55,7 @@ static __device__ __forceinline__ uint32_t find_set_by(const SetDesc *__restrict
 }
 
 static __global__ void k_pack_batch(const uint8_t *__restrict__ syms, const SetDesc *__restrict__ sets,
-                                    uint32_t nsets, uint32_t total, uint32_t *__restrict__ packed,
+                                    uint32_t nsets, uint32_t total, uint8_t *__restrict__ packed,
                                     uint32_t *__restrict__ bad)
 {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -57,26 +63,39 @@ static __global__ void k_pack_batch(const uint8_t *__restrict__ syms, const SetD
     const uint32_t s = find_set_by(sets, nsets, g, true);
     const SetDesc &d = sets[s];
     const uint32_t n = d.n;
-    const uint8_t *p = syms + d.sym_off + (size_t)(g - d.node_off) * n;
+    const uint32_t i = g - d.node_off;
+    const uint8_t *p = syms + d.sym_off + (size_t)i * n;
     uint32_t or_all = 0;
     for (uint32_t q = 0; q < n; ++q) or_all |= p[q];
-    packed[g] = pack_word(p, n);
+    if (n > 16) reinterpret_cast<uint64_t *>(packed + d.word_off)[i] = pack_word64(p, n);
+    else reinterpret_cast<uint32_t *>(packed + d.word_off)[i] = pack_word(p, n);
     if (or_all > 3u) atomicOr(bad, 1u);
 }
 
 template <int CPL>
-__device__ __forceinline__ void batch_tile(int K, const uint32_t *__restrict__ packed, const TileGeom &G, uint32_t rb,
-                                           uint32_t cb, uint32_t node_off, uint64_t *st, uint32_t lane,
-                                           uint32_t wave, uint64_t *__restrict__ edges, uint64_t cap,
+__device__ __forceinline__ void batch_tile(const SetDesc &d, const uint32_t *__restrict__ packed, uint32_t rb,
+                                           uint32_t cb, uint64_t *st, uint32_t lane, uint32_t wave,
+                                           uint64_t *__restrict__ edges, uint64_t cap,
                                            unsigned long long *__restrict__ counter)
 {
-    switch (K) {
+    if (d.pred == kBatchMyers) {
+        wave_tile<PolMyers<uint32_t>, CPL, true>(packed, d.G, rb, cb, d.node_off, d.n, d.K, st, lane, wave, edges,
+                                                 cap, counter);
+        return;
+    }
+    if (d.pred == kBatchAll) {
+        wave_tile<PolAll<uint32_t>, CPL, true>(packed, d.G, rb, cb, d.node_off, d.n, d.K, st, lane, wave, edges,
+                                               cap, counter);
+        return;
+    }
+    switch (d.K) {
 #define IGC_BCASE(k)                                                                                          \
     case k:                                                                                                   \
-        wave_tile<k, CPL, true>(packed, G, rb, cb, node_off, st, lane, wave, edges, cap, counter);            \
+        wave_tile<PolLV<k, uint32_t>, CPL, true>(packed, d.G, rb, cb, d.node_off, d.n, d.K, st, lane, wave,     \
+                                                 edges, cap, counter);                                        \
         break;
         IGC_BCASE(0) IGC_BCASE(1) IGC_BCASE(2) IGC_BCASE(3) IGC_BCASE(4) IGC_BCASE(5) IGC_BCASE(6)
-        IGC_BCASE(7) IGC_BCASE(8) IGC_BCASE(16)
+        IGC_BCASE(7) IGC_BCASE(8)
 #undef IGC_BCASE
     default:
         break;
@@ -87,7 +106,7 @@ __device__ __forceinline__ void batch_tile(int K, const uint32_t *__restrict__ p
 // counter until all `total_tiles` are done.  Tiles are ordered heaviest set
 // first (host side), so the tail is made of cheap tiles.
 template <int CPL>
-__global__ __launch_bounds__(256) void k_pairs_batch(const uint32_t *__restrict__ packed,
+__global__ __launch_bounds__(256) void k_pairs_batch(const uint8_t *__restrict__ packed,
                                                      const SetDesc *__restrict__ sets, uint32_t nsets,
                                                      uint64_t total_tiles, unsigned long long *__restrict__ work,
                                                      uint64_t *__restrict__ edges, uint64_t cap,
@@ -107,8 +126,8 @@ __global__ __launch_bounds__(256) void k_pairs_batch(const uint32_t *__restrict_
         const SetDesc d = sets[s];
         uint32_t rb, cb;
         decode_tile(t - d.tile_begin, d.G, rb, cb);
-        batch_tile<CPL>(d.K, packed + d.node_off, d.G, rb, cb, d.node_off, stage[wave], lane, wave, edges, cap,
-                        counter);
+        batch_tile<CPL>(d, reinterpret_cast<const uint32_t *>(packed + d.word_off), rb, cb, stage[wave], lane, wave,
+                        edges, cap, counter);
     }
 }
 
@@ -126,10 +145,11 @@ struct BatchPlan {
     std::vector<SetDesc> by_node;  // the same descriptors in caller order; node_off ascending
     std::vector<uint32_t> order;   // desc index -> caller's set index
     std::vector<uint32_t> node_off_by_set;
-    std::vector<uint32_t> extra;   // desc indices launched separately (9 <= K <= 15)
+    std::vector<uint32_t> extra;   // desc indices launched separately (n > 16)
     uint64_t tiles = 0;            // tiles of the persistent kernel
     uint64_t pairs = 0;
     uint64_t sym_bytes = 0;
+    uint64_t word_bytes = 0;       // packed words of all sets
     uint32_t total = 0;
     uint64_t cap = 0;
 };
@@ -141,8 +161,8 @@ static int plan_batch(igc_ctx *ctx, const igc_set *sets, uint32_t count, int cpl
     std::vector<uint32_t> idx(count);
     std::iota(idx.begin(), idx.end(), 0u);
     std::vector<int> K(count);
-    uint64_t total = 0, symb = 0;
-    std::vector<uint64_t> sym_off(count);
+    uint64_t total = 0, symb = 0, wordb = 0;
+    std::vector<uint64_t> sym_off(count), word_off(count);
     for (uint32_t s = 0; s < count; ++s) {
         const igc_set &x = sets[s];
         if (x.n < 1 || x.n > IGC_MAX_LEN)
@@ -151,18 +171,30 @@ static int plan_batch(igc_ctx *ctx, const igc_set *sets, uint32_t count, int cpl
         K[s] = x.edit_dist == 0 ? -1 : effective_k(x.n, x.edit_dist);
         P.node_off_by_set[s] = (uint32_t)total;
         sym_off[s] = symb;
+        word_off[s] = wordb;
         total += x.N;
         symb += (uint64_t)x.N * x.n;
+        wordb += ((uint64_t)x.N * word_bytes(x.n) + 7) & ~7ull;
         P.pairs += (uint64_t)x.N * (x.N > 0 ? x.N - 1 : 0) / 2;
     }
     if (total > 0x7FFFFFFFull) return set_error(IGC_ERR_INVALID, "batch has %llu nodes", (unsigned long long)total);
     P.node_off_by_set[count] = (uint32_t)total;
     P.total = (uint32_t)total;
     P.sym_bytes = symb;
-    // heaviest tiles first: larger K costs more per pair; then larger sets
+    P.word_bytes = wordb;
+    // heaviest tiles first: costlier predicate per pair first, then larger sets
+    auto cost = [&](uint32_t s) -> uint32_t {
+        if (K[s] < 0) return 0;
+        switch (pred_kind(sets[s].n, K[s])) {
+        case PredKind::LV32: return lv_cost(K[s], false);
+        case PredKind::LV64: return lv_cost(K[s], true);
+        case PredKind::MYERS: return myers_cost(sets[s].n);
+        default: return 1;
+        }
+    };
     std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
-        const int ka = K[a] == kAllPairs ? -1 : K[a], kb = K[b] == kAllPairs ? -1 : K[b];
-        if (ka != kb) return ka > kb;
+        const uint32_t ca = cost(a), cb = cost(b);
+        if (ca != cb) return ca > cb;
         return sets[a].N > sets[b].N;
     });
     double est_edges = 0;
@@ -172,23 +204,26 @@ static int plan_batch(igc_ctx *ctx, const igc_set *sets, uint32_t count, int cpl
         const Shape sh{cpl, 1};
         d.G = choose_geometry(x.N, sh.cpl, x.N >= 8192 ? 1u : (x.N >= 2048 ? 2u : 4u));
         d.sym_off = sym_off[s];
+        d.word_off = word_off[s];
         d.node_off = P.node_off_by_set[s];
         d.n = x.n;
         d.K = K[s];
-        const uint64_t pairs = (uint64_t)x.N * (x.N > 0 ? x.N - 1 : 0) / 2;
+        d.pred = kBatchNone;
         if (K[s] >= 0 && x.N >= 2) est_edges += (double)initial_edge_cap(x.N, x.n, K[s]);
-        (void)pairs;
         const bool runnable = K[s] >= 0 && x.N >= 2;
-        const bool in_kernel = runnable && (K[s] <= kBatchMaxK || K[s] == kAllPairs);
+        if (runnable && !wide_words(x.n)) {
+            const PredKind pk = pred_kind(x.n, K[s]);
+            if (pk == PredKind::ALL) d.pred = kBatchAll;
+            else if (pk == PredKind::MYERS) d.pred = kBatchMyers;
+            else if (K[s] <= kBatchMaxK) d.pred = kBatchLV;
+        }
+        const bool in_kernel = d.pred != kBatchNone;
         d.tile_begin = P.tiles;
         if (in_kernel) P.tiles += tile_count(d.G);
         if (runnable && !in_kernel) P.extra.push_back((uint32_t)P.desc.size());
-        if (!in_kernel) d.K = -1; // never dispatched by the persistent kernel (it owns no tiles)
         P.order.push_back(s);
         P.desc.push_back(d);
     }
-    // true K of the separately launched sets
-    for (uint32_t e : P.extra) P.desc[e].K = K[P.order[e]];
     P.by_node.resize(P.desc.size());
     for (size_t k = 0; k < P.desc.size(); ++k) P.by_node[P.order[k]] = P.desc[k];
     P.cap = std::max<uint64_t>((uint64_t)est_edges, 1);
@@ -201,7 +236,7 @@ static void launch_batch_kernel(igc_ctx *ctx, const BatchPlan &P, const SetDesc 
                                 unsigned long long *counter, uint64_t cap)
 {
     const uint32_t grid = (uint32_t)std::min<uint64_t>(P.tiles, (uint64_t)ctx->cu_count * 8);
-    hipLaunchKernelGGL((k_pairs_batch<CPL>), dim3(grid), dim3(256), 0, ctx->stream, ctx->packed.as<uint32_t>(),
+    hipLaunchKernelGGL((k_pairs_batch<CPL>), dim3(grid), dim3(256), 0, ctx->stream, ctx->packed.as<uint8_t>(),
                        d_sets, (uint32_t)P.desc.size(), P.tiles, work, ctx->edges.as<uint64_t>(), cap, counter);
 }
 
@@ -217,7 +252,7 @@ static int batch_device_locked(igc_ctx *ctx, const uint8_t *d_syms, const BatchP
     tm.pairs = P.pairs;
     const uint32_t nd = (uint32_t)P.desc.size();
     int rc;
-    if ((rc = ctx->packed.ensure(sizeof(uint32_t) * std::max<size_t>(P.total, 1)))) return rc;
+    if ((rc = ctx->packed.ensure(std::max<size_t>(P.word_bytes, 8)))) return rc;
     if ((rc = ctx->misc.ensure(sizeof(uint32_t) * kMiscWords))) return rc;
     if ((rc = ctx->sets.ensure(2 * sizeof(SetDesc) * std::max<size_t>(nd, 1)))) return rc;
     if ((rc = ctx->bounds.ensure(sizeof(uint64_t) * ((size_t)nd + 1)))) return rc;
@@ -236,7 +271,7 @@ static int batch_device_locked(igc_ctx *ctx, const uint8_t *d_syms, const BatchP
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, misc, (uint64_t)kMiscWords, 0u);
         if (P.total)
             hipLaunchKernelGGL(k_pack_batch, dim3((P.total + 255) / 256), dim3(256), 0, s, d_syms, d_node_order, nd,
-                               P.total, ctx->packed.as<uint32_t>(), misc + 2);
+                               P.total, ctx->packed.as<uint8_t>(), misc + 2);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(ctx->ev[1], s));
         uint32_t launches = 2;
@@ -249,7 +284,7 @@ static int batch_device_locked(igc_ctx *ctx, const uint8_t *d_syms, const BatchP
         }
         for (uint32_t e : P.extra) {
             const SetDesc &d = P.desc[e];
-            if ((rc = launch_pairs(d.K, cpl, true, s, tile_count(d.G), ctx->packed.as<uint32_t>() + d.node_off, d.G,
+            if ((rc = launch_pairs(d.n, d.K, cpl, true, s, tile_count(d.G), ctx->packed.as<uint8_t>() + d.word_off, d.G,
                                    d.node_off, ctx->edges.as<uint64_t>(), cap, counter)))
                 return rc;
             launches++;

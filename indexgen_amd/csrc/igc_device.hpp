@@ -90,9 +90,18 @@ uint64_t initial_edge_cap(uint32_t N, uint32_t n, int K);
 int effective_k(uint32_t n, uint32_t edit_dist);
 uint32_t grid_for(uint64_t work, uint32_t per_block, uint32_t cap_blocks);
 
-// Launches k_pairs<K, cpl, emit> over `tiles` tiles of geometry G.
-int launch_pairs(int K, int cpl, bool emit, hipStream_t s, uint64_t tiles, const uint32_t *packed,
+// Packed words are 32-bit for n <= 16 and 64-bit for 17 <= n <= 32.
+inline bool wide_words(uint32_t n) { return n > 16; }
+inline size_t word_bytes(uint32_t n) { return wide_words(n) ? 8 : 4; }
+
+// Launches the pair kernel of the predicate pred_kind(n, K) picks, with cpl
+// columns per lane, over `tiles` tiles of geometry G; `packed` holds words of
+// the width of n.
+int launch_pairs(uint32_t n, int K, int cpl, bool emit, hipStream_t s, uint64_t tiles, const void *packed,
                  const TileGeom &G, uint32_t node_off, uint64_t *edges, uint64_t cap, unsigned long long *counter);
+int launch_pack(hipStream_t s, const uint8_t *d_syms, uint32_t N, uint32_t n, void *packed, uint32_t *bad);
+int pred_variant(uint32_t n, int K);
+constexpr int kPredVariants = 2 * (kLvLaunchMaxK + 1) + 4;
 
 } // namespace igc
 
@@ -106,8 +115,9 @@ struct igc_ctx {
     hipEvent_t ev[5] = {};
     igc_timing last{};
     int cu_count = 256;
-    // tile shapes chosen by the autotuner, keyed by (K, N bucket); 0 = default
-    igc::Shape tuned[17][8] = {};
+    // tile shapes chosen by the autotuner, keyed by (kernel variant, N bucket);
+    // cpl 0 = not tuned
+    igc::Shape tuned[igc::kPredVariants][8] = {};
     void *km = nullptr; // K-means buffers (igc_kmeans.hip)
 };
 
@@ -116,7 +126,7 @@ namespace igc {
 // over N nodes into ctx->row_ptr / ctx->col; asynchronous on ctx->stream.
 int launch_csr(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *counter, uint32_t *misc,
                uint32_t *launches);
-Shape pick_shape(igc_ctx *ctx, int K, uint32_t N);
+Shape pick_shape(igc_ctx *ctx, uint32_t n, int K, uint32_t N);
 
 int kmeans_attach(igc_ctx *ctx);
 void kmeans_release(igc_ctx *ctx);

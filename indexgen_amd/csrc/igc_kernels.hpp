@@ -1,9 +1,10 @@
 // igc_kernels.hpp -- gfx950 kernels of the conflict-graph engine.
 //
 // Data layout in HBM
-//   packed[i]  one 32-bit word per candidate: bit p = sym[p] & 1 and
-//              bit 16+p = sym[p] >> 1 for p < n <= 16, all other bits 0
-//              (two bit planes, so a shift by d moves whole symbols).
+//   packed[i]  one word per candidate, two symbol bit planes (igc_lv.hpp):
+//              32-bit words for n <= 16 (bit p = sym[p] & 1, bit 16+p =
+//              sym[p] >> 1), 64-bit words for 17 <= n <= 32 (planes at bits 0
+//              and 32); all other bits 0, so a shift by d moves whole symbols.
 //   edges[e]   (i << 32) | j for every conflicting pair i < j, in no order.
 //   row_ptr    uint64 [N+1];  col  uint32 [2|E|]  -- the symmetric CSR.
 //
@@ -16,14 +17,16 @@
 //   FR[e][d] = slide_d(max(FR[e-1][d]+1, FR[e-1][d-1], FR[e-1][d+1]+1))
 //   slide_d(L) = L + ctz(mism_d >> L)
 // restricted to |d| <= min(e, K-e), and edge iff FR[K][0] >= n.  Each cell is a
-// max3, a shift, a find-first-bit and an add on one 32-bit register, so a pair
-// costs O(K^2) VALU ops independent of n (about 40 at K=3) instead of the
-// 17*n of one Myers column step per symbol.
-// Bits 16..31 of every mask are forced to 1: a slide can never run off the
+// max3, a shift, a find-first-bit and an add on one register, so a pair costs
+// O(K^2) VALU ops independent of n (about 40 at K=3 on 32-bit words) instead of
+// the 17*n of one Myers column step per symbol.  Where that quadratic cost
+// exceeds the Myers recurrence (large K), the kernels run Myers instead
+// (pred_kind in igc_lv.hpp).
+// The upper half of every mask is forced to 1: a slide can never run off the
 // word, and any furthest-reaching value past a diagonal's end only means
 // "end reached", which propagates correctly because neighbouring DP cells
-// differ by at most one.  For diagonal 0 the bits n..15 are 0 (both strings
-// are zero-padded), so FR[K][0] >= n  <=>  (q0 >> L) == 0 with q0 the 16-bit
+// differ by at most one.  For diagonal 0 the bits n..half-1 are 0 (both strings
+// are zero-padded), so FR[K][0] >= n  <=>  (q0 >> L) == 0 with q0 the low-half
 // mask and L the last level's start.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -38,18 +41,21 @@ constexpr int kStage = 256;        // edges staged per wave in LDS before a flus
 
 // ---------------------------------------------------------------------------
 // symbol packing
-static __global__ void k_pack(const uint8_t *__restrict__ syms, uint32_t N, uint32_t n, uint32_t *__restrict__ packed,
+template <class W>
+__global__ void k_pack(const uint8_t *__restrict__ syms, uint32_t N, uint32_t n, W *__restrict__ packed,
                        uint32_t *__restrict__ bad)
 {
+    constexpr int H = WordTraits<W>::half;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const uint8_t *s = syms + (size_t)i * n;
-    uint32_t w = 0, or_all = 0;
+    W w = 0;
+    uint32_t or_all = 0;
     for (uint32_t p = 0; p < n; ++p) {
         const uint32_t c = s[p];
         or_all |= c;
-        w |= (c & 1u) << p;
-        w |= ((c >> 1) & 1u) << (16 + p);
+        w |= (W)(c & 1u) << p;
+        w |= (W)((c >> 1) & 1u) << (H + p);
     }
     packed[i] = w;
     if (or_all > 3u) atomicOr(bad, 1u);
@@ -57,16 +63,19 @@ static __global__ void k_pack(const uint8_t *__restrict__ syms, uint32_t N, uint
 
 // Packed words of uniformly random strings of length n (SplitMix64 per index),
 // used to time tile shapes.
-static __global__ void k_random_packed(uint32_t *__restrict__ packed, uint32_t N, uint32_t n, uint64_t seed)
+template <class W>
+__global__ void k_random_packed(W *__restrict__ packed, uint32_t N, uint32_t n, uint64_t seed)
 {
+    constexpr int H = WordTraits<W>::half;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     uint64_t z = seed + 0x9E3779B97F4A7C15ull * (i + 1);
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     z ^= z >> 31;
-    const uint32_t mask = n >= 16 ? 0xFFFFu : ((1u << n) - 1u);
-    packed[i] = ((uint32_t)z & mask) | (((uint32_t)(z >> 32) & mask) << 16);
+    const uint32_t mask = n >= 32 ? 0xFFFFFFFFu : ((1u << n) - 1u);
+    const uint32_t lo = (uint32_t)z & mask, hi = (uint32_t)(z >> 32) & mask;
+    packed[i] = (W)lo | ((W)hi << H);
 }
 
 static __global__ void k_fill_u32(uint32_t *__restrict__ p, uint64_t count, uint32_t v)
@@ -76,7 +85,40 @@ static __global__ void k_fill_u32(uint32_t *__restrict__ p, uint64_t count, uint
 }
 
 // ---------------------------------------------------------------------------
-// pair predicate
+// pair predicates as policies of the tile code: Row holds what a wave derives
+// once from the (uniform) row word, edge() tests one column word against it.
+
+template <int K, class W>
+struct PolLV {
+    using Word = W;
+    struct Row {
+        W sa[LvShape<K>::WD];
+    };
+    static __device__ __forceinline__ void row(Row &r, W a, uint32_t, int) { row_shifts<K>(a, r.sa); }
+    static __device__ __forceinline__ bool edge(const Row &r, W b, uint32_t, int) { return lv_edge<K>(r.sa, b); }
+};
+
+template <class W>
+struct PolMyers {
+    using Word = W;
+    struct Row {
+        W t;
+    };
+    static __device__ __forceinline__ void row(Row &r, W a, uint32_t, int) { r.t = a; }
+    static __device__ __forceinline__ bool edge(const Row &r, W b, uint32_t n, int k)
+    {
+        return myers_edge<W>(r.t, b, n, k);
+    }
+};
+
+// editDist > n: every pair conflicts
+template <class W>
+struct PolAll {
+    using Word = W;
+    struct Row {};
+    static __device__ __forceinline__ void row(Row &, W, uint32_t, int) {}
+    static __device__ __forceinline__ bool edge(const Row &, W, uint32_t, int) { return true; }
+};
 
 // ---------------------------------------------------------------------------
 // all-pairs tiling of the upper triangle
@@ -153,14 +195,15 @@ __device__ __forceinline__ void flush_stage(const uint64_t *st, uint32_t staged,
 // uniform (scalar registers), so each lane tests CPL pairs per row.  Nonzero
 // ballots (rare) are compacted through the wave's LDS stage `st`; edges are
 // written as global ids (node_off + i, node_off + j).  EMIT=false only counts.
-// Contains no workgroup barrier: a wave may return early.
-template <int K, int CPL, bool EMIT>
-__device__ __forceinline__ void wave_tile(const uint32_t *__restrict__ packed, const TileGeom &G, uint32_t rb,
-                                          uint32_t cb, uint32_t node_off, uint64_t *st, uint32_t lane,
-                                          uint32_t wave, uint64_t *__restrict__ edges, uint64_t cap,
-                                          unsigned long long *__restrict__ counter)
+// Contains no workgroup barrier: a wave may return early.  n and k are the
+// string length and threshold for the policies that take them at run time.
+template <class Pol, int CPL, bool EMIT>
+__device__ __forceinline__ void wave_tile(const typename Pol::Word *__restrict__ packed, const TileGeom &G,
+                                          uint32_t rb, uint32_t cb, uint32_t node_off, uint32_t n, int k,
+                                          uint64_t *st, uint32_t lane, uint32_t wave, uint64_t *__restrict__ edges,
+                                          uint64_t cap, unsigned long long *__restrict__ counter)
 {
-    constexpr int WD = LvShape<K>::WD;
+    using W = typename Pol::Word;
     const uint32_t N = G.N;
     const uint32_t row0 = rb * G.R;
     const uint32_t colw = cb * G.W + wave * 64u * CPL;
@@ -170,25 +213,24 @@ __device__ __forceinline__ void wave_tile(const uint32_t *__restrict__ packed, c
     if (colend - 1u <= row0) return;
     const uint32_t rows = min(min(G.R, N - row0), colend - 1u - row0);
 
-    uint32_t b[CPL];
+    W b[CPL];
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
         const uint32_t j = colw + 64u * c + lane;
-        b[c] = j < N ? packed[j] : 0u;
+        b[c] = j < N ? packed[j] : (W)0;
     }
     uint32_t staged = 0;
     uint64_t counted = 0;
 
     for (uint32_t r = 0; r < rows; ++r) {
         const uint32_t i = row0 + r;
-        const uint32_t a = packed[i];
-        uint32_t sa[WD];
-        row_shifts<K>(a, sa);
+        typename Pol::Row row;
+        Pol::row(row, packed[i], n, k);
         uint64_t mk[CPL];
         uint64_t any = 0;
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
-            mk[c] = __ballot(lv_edge<K>(sa, b[c]));
+            mk[c] = __ballot(Pol::edge(row, b[c], n, k));
             any |= mk[c];
         }
         if (any) {
@@ -221,17 +263,17 @@ __device__ __forceinline__ void wave_tile(const uint32_t *__restrict__ packed, c
 }
 
 // Single set: one workgroup per tile of the triangle (blockIdx -> tile).
-template <int K, int CPL, bool EMIT>
-__global__ __launch_bounds__(256) void k_pairs(const uint32_t *__restrict__ packed, TileGeom G, uint32_t node_off,
-                                               uint64_t *__restrict__ edges, uint64_t cap,
-                                               unsigned long long *__restrict__ counter)
+template <class Pol, int CPL, bool EMIT>
+__global__ __launch_bounds__(256) void k_pairs(const typename Pol::Word *__restrict__ packed, TileGeom G,
+                                               uint32_t node_off, uint32_t n, int k, uint64_t *__restrict__ edges,
+                                               uint64_t cap, unsigned long long *__restrict__ counter)
 {
     __shared__ uint64_t stage[kWaves][kStage];
     const uint32_t lane = lane_id();
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     uint32_t rb, cb;
     decode_tile(blockIdx.x, G, rb, cb);
-    wave_tile<K, CPL, EMIT>(packed, G, rb, cb, node_off, stage[wave], lane, wave, edges, cap, counter);
+    wave_tile<Pol, CPL, EMIT>(packed, G, rb, cb, node_off, n, k, stage[wave], lane, wave, edges, cap, counter);
 }
 
 // ---------------------------------------------------------------------------

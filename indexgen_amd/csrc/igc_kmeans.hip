@@ -4,7 +4,7 @@
 // Feature of a length-n string: the one-hot 4n-vector (dimension 4p+s is 1 iff
 // symbol p equals s).  For a centroid c, ||x - c||^2 = n - 2 sum_p C[c][4p+x_p]
 // + ||c||^2, so the assignment needs n table lookups per centroid, not a 4n-wide
-// dot product; the table (k x 4n floats, 64 KiB at k=256, n=16) sits in LDS.
+// dot product; the table (k x 4n floats) is staged through LDS in chunks.
 // Determinism: every sum runs in a fixed order with FP contraction off, ties go
 // to the lowest centroid index, and the update uses exact integer histograms
 // (cnt[c][4p+s] / size[c], one correctly rounded division), so the result does
@@ -25,14 +25,18 @@ namespace igc {
 
 constexpr int kKmThreads = 256;
 
-__device__ __forceinline__ uint32_t sym_at(uint32_t w, uint32_t p)
+// Symbol p of a packed word (32-bit words for n <= 16, 64-bit above).
+template <class W>
+__device__ __forceinline__ uint32_t sym_at(W w, uint32_t p)
 {
-    return ((w >> p) & 1u) | (((w >> (16 + p)) & 1u) << 1);
+    constexpr int H = WordTraits<W>::half;
+    return (uint32_t)((w >> p) & 1u) | ((uint32_t)((w >> (H + p)) & 1u) << 1);
 }
 
 // assign[i] = argmin_c (cn2[c] - 2 * sum_p C[c][4p + s_p]), first index on ties.
 // The centroid table goes through LDS in chunks of kc centroids (<= 32 KiB).
-static __global__ __launch_bounds__(kKmThreads) void k_km_assign(const uint32_t *__restrict__ packed, uint32_t N,
+template <class W>
+__global__ __launch_bounds__(kKmThreads) void k_km_assign(const W *__restrict__ packed, uint32_t N,
                                                                   uint32_t n, const float *__restrict__ C,
                                                                   const float *__restrict__ cn2, uint32_t k,
                                                                   uint32_t kc, int32_t *__restrict__ assign)
@@ -43,10 +47,11 @@ static __global__ __launch_bounds__(kKmThreads) void k_km_assign(const uint32_t 
     float *sN = sh + (size_t)kc * D;
     for (uint32_t base = blockIdx.x * blockDim.x; base < N; base += gridDim.x * blockDim.x) {
         const uint32_t i = base + threadIdx.x;
-        const uint32_t w = i < N ? packed[i] : 0u;
-        uint32_t off[IGC_MAX_LEN];
+        const W w = i < N ? packed[i] : (W)0;
+        constexpr uint32_t P = WordTraits<W>::half;
+        uint32_t off[P];
 #pragma unroll
-        for (uint32_t p = 0; p < IGC_MAX_LEN; ++p) off[p] = 4 * p + sym_at(w, p);
+        for (uint32_t p = 0; p < P; ++p) off[p] = 4 * p + sym_at(w, p);
         float best = 0.0f;
         int32_t bi = -1;
         for (uint32_t c0 = 0; c0 < k; c0 += kc) {
@@ -75,7 +80,8 @@ static __global__ __launch_bounds__(kKmThreads) void k_km_assign(const uint32_t 
 // counts in LDS with two 16-bit counters per word (a workgroup sees at most
 // 32768 points, so no half overflows) and flushes nonzero entries with one
 // global atomic each.
-static __global__ __launch_bounds__(kKmThreads) void k_km_accum(const uint32_t *__restrict__ packed, uint32_t N,
+template <class W>
+__global__ __launch_bounds__(kKmThreads) void k_km_accum(const W *__restrict__ packed, uint32_t N,
                                                                  uint32_t n, const int32_t *__restrict__ assign,
                                                                  uint32_t k, uint32_t *__restrict__ cnt,
                                                                  uint32_t *__restrict__ size)
@@ -88,7 +94,7 @@ static __global__ __launch_bounds__(kKmThreads) void k_km_accum(const uint32_t *
     for (uint32_t t = threadIdx.x; t < words + k; t += blockDim.x) hs[t] = 0;
     __syncthreads();
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-        const uint32_t w = packed[i];
+        const W w = packed[i];
         const uint32_t c = (uint32_t)assign[i];
         for (uint32_t p = 0; p < n; ++p) {
             const uint32_t t = c * D + 4 * p + sym_at(w, p);
@@ -126,12 +132,13 @@ static __global__ void k_km_update(const uint32_t *__restrict__ cnt, const uint3
     }
 }
 
-static __global__ void k_km_init(const uint32_t *__restrict__ packed, const uint32_t *__restrict__ seeds, uint32_t n,
-                                 uint32_t k, float *__restrict__ C, float *__restrict__ cn2)
+template <class W>
+__global__ void k_km_init(const W *__restrict__ packed, const uint32_t *__restrict__ seeds, uint32_t n, uint32_t k,
+                          float *__restrict__ C, float *__restrict__ cn2)
 {
     const uint32_t c = blockIdx.x;
     const uint32_t D = 4 * n;
-    const uint32_t w = packed[seeds[c]];
+    const W w = packed[seeds[c]];
     for (uint32_t d = threadIdx.x; d < D; d += blockDim.x) C[(size_t)c * D + d] = (sym_at(w, d >> 2) == (d & 3)) ? 1.0f : 0.0f;
     if (threadIdx.x == 0) cn2[c] = (float)n;
 }
@@ -171,6 +178,7 @@ static uint32_t km_grid(igc_ctx *ctx, uint32_t N)
 }
 
 // Launches one Lloyd iteration on ctx->stream (capturable: kernels only).
+template <class W>
 static void lloyd_iteration(igc_ctx *ctx, KmBufs &b, uint32_t N, uint32_t n, uint32_t k)
 {
     hipStream_t s = ctx->stream;
@@ -179,12 +187,12 @@ static void lloyd_iteration(igc_ctx *ctx, KmBufs &b, uint32_t N, uint32_t n, uin
     const size_t lds_assign = sizeof(float) * ((size_t)kc * D + kc);
     const size_t lds_accum = sizeof(uint32_t) * (((size_t)k * D + 1) / 2 + k);
     const uint32_t grid = km_grid(ctx, N);
-    hipLaunchKernelGGL(k_km_assign, dim3(grid), dim3(kKmThreads), lds_assign, s, ctx->packed.as<uint32_t>(), N, n,
+    hipLaunchKernelGGL(k_km_assign<W>, dim3(grid), dim3(kKmThreads), lds_assign, s, ctx->packed.as<W>(), N, n,
                        b.C.as<float>(), b.cn2.as<float>(), k, kc, b.assign.as<int32_t>());
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for((uint64_t)k * D, 256, 1024)), dim3(256), 0, s, b.cnt.as<uint32_t>(),
                        (uint64_t)k * D, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, b.size.as<uint32_t>(), (uint64_t)k, 0u);
-    hipLaunchKernelGGL(k_km_accum, dim3(grid), dim3(kKmThreads), lds_accum, s, ctx->packed.as<uint32_t>(), N, n,
+    hipLaunchKernelGGL(k_km_accum<W>, dim3(grid), dim3(kKmThreads), lds_accum, s, ctx->packed.as<W>(), N, n,
                        b.assign.as<int32_t>(), k, b.cnt.as<uint32_t>(), b.size.as<uint32_t>());
     hipLaunchKernelGGL(k_km_update, dim3(k), dim3(64), 0, s, b.cnt.as<uint32_t>(), b.size.as<uint32_t>(), D,
                        b.C.as<float>(), b.cn2.as<float>());
@@ -193,17 +201,17 @@ static void lloyd_iteration(igc_ctx *ctx, KmBufs &b, uint32_t N, uint32_t n, uin
 // K-means on symbols already in device memory: seeds -> `iters` Lloyd
 // iterations (assign, histogram, update) -> a final assign against the last
 // centroids, left in the context's assign buffer.
-int kmeans_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n, uint32_t k, uint32_t iters,
-                         uint64_t seed, bool use_graph, float *ms_out)
+template <class W>
+static int kmeans_device_w(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n, uint32_t k, uint32_t iters,
+                           uint64_t seed, bool use_graph, float *ms_out)
 {
-    if (n < 1 || n > IGC_MAX_LEN) return set_error(IGC_ERR_INVALID, "string length n=%u outside 1..%d", n, IGC_MAX_LEN);
     if (k < 1 || k > N) return set_error(IGC_ERR_INVALID, "k=%u must be in 1..N (N=%u)", k, N);
     const uint32_t D = 4 * n;
     if (((size_t)k * D + 1) / 2 * 4 + (size_t)k * 4 > 64 * 1024)
         return set_error(IGC_ERR_INVALID, "k=%u too large for the LDS histogram (k*4n/2 + k words <= 16384)", k);
     KmBufs &b = km_bufs(ctx);
     int rc;
-    if ((rc = ctx->packed.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
+    if ((rc = ctx->packed.ensure(sizeof(W) * (size_t)N))) return rc;
     if ((rc = ctx->misc.ensure(sizeof(uint32_t) * kMiscWords))) return rc;
     if ((rc = b.C.ensure(sizeof(float) * (size_t)k * D))) return rc;
     if ((rc = b.cn2.ensure(sizeof(float) * k))) return rc;
@@ -218,16 +226,16 @@ int kmeans_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32
     HIP_TRY(hipEventRecord(ctx->ev[0], s));
     HIP_TRY(hipMemcpyAsync(b.seeds.p, seeds.data(), sizeof(uint32_t) * k, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, misc, (uint64_t)kMiscWords, 0u);
-    hipLaunchKernelGGL(k_pack, dim3((N + 255) / 256), dim3(256), 0, s, d_syms, N, n, ctx->packed.as<uint32_t>(),
+    hipLaunchKernelGGL(k_pack<W>, dim3((N + 255) / 256), dim3(256), 0, s, d_syms, N, n, ctx->packed.as<W>(),
                        misc + 2);
-    hipLaunchKernelGGL(k_km_init, dim3(k), dim3(64), 0, s, ctx->packed.as<uint32_t>(), b.seeds.as<uint32_t>(), n, k,
+    hipLaunchKernelGGL(k_km_init<W>, dim3(k), dim3(64), 0, s, ctx->packed.as<W>(), b.seeds.as<uint32_t>(), n, k,
                        b.C.as<float>(), b.cn2.as<float>());
     HIP_TRY(hipGetLastError());
     if (use_graph && iters > 0) {
         hipGraph_t g = nullptr;
         hipGraphExec_t ge = nullptr;
         HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        lloyd_iteration(ctx, b, N, n, k);
+        lloyd_iteration<W>(ctx, b, N, n, k);
         hipError_t e = hipStreamEndCapture(s, &g);
         if (e != hipSuccess) return set_error(IGC_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
         e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
@@ -240,13 +248,13 @@ int kmeans_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) return set_error(IGC_ERR_DEVICE, "graph launch failed: %s", hipGetErrorString(e));
     } else {
-        for (uint32_t t = 0; t < iters; ++t) lloyd_iteration(ctx, b, N, n, k);
+        for (uint32_t t = 0; t < iters; ++t) lloyd_iteration<W>(ctx, b, N, n, k);
         HIP_TRY(hipGetLastError());
     }
     // final assignment against the last centroids
     const uint32_t kc = km_chunk(D);
-    hipLaunchKernelGGL(k_km_assign, dim3(km_grid(ctx, N)), dim3(kKmThreads), sizeof(float) * ((size_t)kc * D + kc), s,
-                       ctx->packed.as<uint32_t>(), N, n, b.C.as<float>(), b.cn2.as<float>(), k, kc,
+    hipLaunchKernelGGL(k_km_assign<W>, dim3(km_grid(ctx, N)), dim3(kKmThreads), sizeof(float) * ((size_t)kc * D + kc),
+                       s, ctx->packed.as<W>(), N, n, b.C.as<float>(), b.cn2.as<float>(), k, kc,
                        b.assign.as<int32_t>());
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[1], s));
@@ -256,6 +264,14 @@ int kmeans_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32
     if (host_misc[2]) return set_error(IGC_ERR_INVALID, "symbol code above 3 in input");
     if (ms_out) HIP_TRY(hipEventElapsedTime(ms_out, ctx->ev[0], ctx->ev[1]));
     return IGC_OK;
+}
+
+int kmeans_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n, uint32_t k, uint32_t iters,
+                         uint64_t seed, bool use_graph, float *ms_out)
+{
+    if (n < 1 || n > IGC_MAX_LEN) return set_error(IGC_ERR_INVALID, "string length n=%u outside 1..%d", n, IGC_MAX_LEN);
+    if (wide_words(n)) return kmeans_device_w<uint64_t>(ctx, d_syms, N, n, k, iters, seed, use_graph, ms_out);
+    return kmeans_device_w<uint32_t>(ctx, d_syms, N, n, k, iters, seed, use_graph, ms_out);
 }
 
 void kmeans_release(igc_ctx *ctx)

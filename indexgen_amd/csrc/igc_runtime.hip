@@ -106,52 +106,102 @@ static int size_bucket(uint32_t N)
     return b;
 }
 
-Shape pick_shape(igc_ctx *ctx, int K, uint32_t N)
+// Index of the kernel variant a (length, threshold) pair runs, for the table
+// of autotuned shapes: LV on 32-bit words K, LV on 64-bit words 10+K, Myers 20
+// (+1 on 64-bit words), every-pair 22 (+1).
+int pred_variant(uint32_t n, int K)
 {
-    const Shape t = ctx->tuned[K < 0 ? 0 : (K > 16 ? 16 : K)][size_bucket(N)];
+    const int wide = wide_words(n) ? 1 : 0;
+    switch (pred_kind(n, K)) {
+    case PredKind::LV32: return K;
+    case PredKind::LV64: return kLvLaunchMaxK + 1 + K;
+    case PredKind::MYERS: return 2 * (kLvLaunchMaxK + 1) + wide;
+    default: return 2 * (kLvLaunchMaxK + 1) + 2 + wide;
+    }
+}
+
+Shape pick_shape(igc_ctx *ctx, uint32_t n, int K, uint32_t N)
+{
+    const Shape t = ctx->tuned[pred_variant(n, K)][size_bucket(N)];
     return t.cpl ? t : default_shape(N);
 }
 
-template <int K, int CPL, bool EMIT>
-static void launch_pairs_t(hipStream_t s, uint64_t tiles, const uint32_t *packed, const TileGeom &G,
-                           uint32_t node_off, uint64_t *edges, uint64_t cap, unsigned long long *counter)
+template <class Pol, int CPL, bool EMIT>
+static void launch_pairs_t(hipStream_t s, uint64_t tiles, const void *packed, const TileGeom &G, uint32_t node_off,
+                           uint32_t n, int K, uint64_t *edges, uint64_t cap, unsigned long long *counter)
 {
-    hipLaunchKernelGGL((k_pairs<K, CPL, EMIT>), dim3((uint32_t)tiles), dim3(256), 0, s, packed, G, node_off, edges,
-                       cap, counter);
+    hipLaunchKernelGGL((k_pairs<Pol, CPL, EMIT>), dim3((uint32_t)tiles), dim3(256), 0, s,
+                       static_cast<const typename Pol::Word *>(packed), G, node_off, n, K, edges, cap, counter);
 }
 
-template <int CPL, bool EMIT>
-static int launch_pairs_k(int K, hipStream_t s, uint64_t tiles, const uint32_t *packed, const TileGeom &G,
-                          uint32_t node_off, uint64_t *edges, uint64_t cap, unsigned long long *counter)
+template <class W, int CPL, bool EMIT>
+static int launch_pairs_lv(hipStream_t s, uint64_t tiles, const void *packed, const TileGeom &G, uint32_t node_off,
+                           uint32_t n, int K, uint64_t *edges, uint64_t cap, unsigned long long *counter)
 {
     switch (K) {
 #define IGC_CASE(k)                                                                                           \
     case k:                                                                                                   \
-        launch_pairs_t<k, CPL, EMIT>(s, tiles, packed, G, node_off, edges, cap, counter);                     \
+        launch_pairs_t<PolLV<k, W>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);     \
         return IGC_OK;
         IGC_CASE(0) IGC_CASE(1) IGC_CASE(2) IGC_CASE(3) IGC_CASE(4) IGC_CASE(5) IGC_CASE(6) IGC_CASE(7)
-        IGC_CASE(8) IGC_CASE(9) IGC_CASE(10) IGC_CASE(11) IGC_CASE(12) IGC_CASE(13) IGC_CASE(14) IGC_CASE(15)
-        IGC_CASE(16)
+        IGC_CASE(8) IGC_CASE(9)
 #undef IGC_CASE
     default:
-        return set_error(IGC_ERR_INVALID, "unsupported K=%d", K);
+        return set_error(IGC_ERR_INVALID, "no Landau-Vishkin kernel for K=%d", K);
+    }
+}
+static_assert(kLvLaunchMaxK == 9, "launch_pairs_lv instantiates K = 0..9");
+
+template <int CPL, bool EMIT>
+static int launch_pairs_c(hipStream_t s, uint64_t tiles, const void *packed, const TileGeom &G, uint32_t node_off,
+                          uint32_t n, int K, uint64_t *edges, uint64_t cap, unsigned long long *counter)
+{
+    const bool wide = wide_words(n);
+    switch (pred_kind(n, K)) {
+    case PredKind::LV32:
+        return launch_pairs_lv<uint32_t, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);
+    case PredKind::LV64:
+        return launch_pairs_lv<uint64_t, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);
+    case PredKind::MYERS:
+        if (wide) launch_pairs_t<PolMyers<uint64_t>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);
+        else launch_pairs_t<PolMyers<uint32_t>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);
+        return IGC_OK;
+    default:
+        if (wide) launch_pairs_t<PolAll<uint64_t>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);
+        else launch_pairs_t<PolAll<uint32_t>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);
+        return IGC_OK;
     }
 }
 
-int launch_pairs(int K, int cpl, bool emit, hipStream_t s, uint64_t tiles, const uint32_t *packed,
+int launch_pairs(uint32_t n, int K, int cpl, bool emit, hipStream_t s, uint64_t tiles, const void *packed,
                  const TileGeom &G, uint32_t node_off, uint64_t *edges, uint64_t cap, unsigned long long *counter)
 {
     if (tiles == 0) return IGC_OK;
     if (tiles > 0x7FFFFFFFull) return set_error(IGC_ERR_INVALID, "too many tiles (%llu)", (unsigned long long)tiles);
     int rc;
-    if (cpl == 1) rc = emit ? launch_pairs_k<1, true>(K, s, tiles, packed, G, node_off, edges, cap, counter)
-                            : launch_pairs_k<1, false>(K, s, tiles, packed, G, node_off, edges, cap, counter);
-    else if (cpl == 2) rc = emit ? launch_pairs_k<2, true>(K, s, tiles, packed, G, node_off, edges, cap, counter)
-                                 : launch_pairs_k<2, false>(K, s, tiles, packed, G, node_off, edges, cap, counter);
-    else if (cpl == 4) rc = emit ? launch_pairs_k<4, true>(K, s, tiles, packed, G, node_off, edges, cap, counter)
-                                 : launch_pairs_k<4, false>(K, s, tiles, packed, G, node_off, edges, cap, counter);
+    if (cpl == 1) rc = emit ? launch_pairs_c<1, true>(s, tiles, packed, G, node_off, n, K, edges, cap, counter)
+                            : launch_pairs_c<1, false>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);
+    else if (cpl == 2) rc = emit ? launch_pairs_c<2, true>(s, tiles, packed, G, node_off, n, K, edges, cap, counter)
+                                 : launch_pairs_c<2, false>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);
+    else if (cpl == 4) rc = emit ? launch_pairs_c<4, true>(s, tiles, packed, G, node_off, n, K, edges, cap, counter)
+                                 : launch_pairs_c<4, false>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);
     else return set_error(IGC_ERR_INVALID, "unsupported columns-per-lane %d", cpl);
     if (rc) return rc;
+    HIP_TRY(hipGetLastError());
+    return IGC_OK;
+}
+
+// Packs N strings of length n into words of the width of n (32-bit for
+// n <= 16, 64-bit above); sets *bad when a symbol code exceeds 3.
+int launch_pack(hipStream_t s, const uint8_t *d_syms, uint32_t N, uint32_t n, void *packed, uint32_t *bad)
+{
+    if (N == 0) return IGC_OK;
+    if (wide_words(n))
+        hipLaunchKernelGGL(k_pack<uint64_t>, dim3((N + 255) / 256), dim3(256), 0, s, d_syms, N, n,
+                           static_cast<uint64_t *>(packed), bad);
+    else
+        hipLaunchKernelGGL(k_pack<uint32_t>, dim3((N + 255) / 256), dim3(256), 0, s, d_syms, N, n,
+                           static_cast<uint32_t *>(packed), bad);
     HIP_TRY(hipGetLastError());
     return IGC_OK;
 }
@@ -243,13 +293,13 @@ static int build_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, 
         return IGC_OK;
     }
     const int K = effective_k(n, edit_dist);
-    const Shape sh = pick_shape(ctx, K, N);
+    const Shape sh = pick_shape(ctx, n, K, N);
     const int cpl = sh.cpl;
     const TileGeom G = choose_geometry(N, sh.cpl, sh.q);
     const uint64_t tiles = tile_count(G);
 
     int rc;
-    if ((rc = ctx->packed.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
+    if ((rc = ctx->packed.ensure(word_bytes(n) * (size_t)N))) return rc;
     if ((rc = ctx->misc.ensure(sizeof(uint32_t) * kMiscWords))) return rc;
     uint32_t *misc = ctx->misc.as<uint32_t>();
     unsigned long long *counter = reinterpret_cast<unsigned long long *>(misc);
@@ -261,11 +311,9 @@ static int build_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, 
         }
         HIP_TRY(hipEventRecord(ctx->ev[0], s));
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, misc, (uint64_t)kMiscWords, 0u);
-        hipLaunchKernelGGL(k_pack, dim3((N + 255) / 256), dim3(256), 0, s, d_syms, N, n,
-                           ctx->packed.as<uint32_t>(), misc + 2);
-        HIP_TRY(hipGetLastError());
+        if ((rc = launch_pack(s, d_syms, N, n, ctx->packed.p, misc + 2))) return rc;
         HIP_TRY(hipEventRecord(ctx->ev[1], s));
-        rc = launch_pairs(K, cpl, !count_only, s, tiles, ctx->packed.as<uint32_t>(), G, 0u,
+        rc = launch_pairs(n, K, cpl, !count_only, s, tiles, ctx->packed.p, G, 0u,
                           count_only ? nullptr : ctx->edges.as<uint64_t>(), cap, counter);
         if (rc) return rc;
         HIP_TRY(hipEventRecord(ctx->ev[2], s));
@@ -484,10 +532,14 @@ IGC_API int igc_autotune(igc_ctx *ctx, uint32_t n, uint32_t edit_dist, uint32_t 
     HIP_TRY(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
     const int K = effective_k(n, edit_dist);
-    if ((rc = ctx->packed.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
+    if ((rc = ctx->packed.ensure(word_bytes(n) * (size_t)N))) return rc;
     if ((rc = ctx->misc.ensure(sizeof(uint32_t) * kMiscWords))) return rc;
-    hipLaunchKernelGGL(k_random_packed, dim3((N + 255) / 256), dim3(256), 0, s, ctx->packed.as<uint32_t>(), N, n,
-                       0x5EEDull + N);
+    if (wide_words(n))
+        hipLaunchKernelGGL(k_random_packed<uint64_t>, dim3((N + 255) / 256), dim3(256), 0, s,
+                           ctx->packed.as<uint64_t>(), N, n, 0x5EEDull + N);
+    else
+        hipLaunchKernelGGL(k_random_packed<uint32_t>, dim3((N + 255) / 256), dim3(256), 0, s,
+                           ctx->packed.as<uint32_t>(), N, n, 0x5EEDull + N);
     HIP_TRY(hipGetLastError());
     unsigned long long *counter = reinterpret_cast<unsigned long long *>(ctx->misc.as<uint32_t>());
     float best = 0;
@@ -501,7 +553,7 @@ IGC_API int igc_autotune(igc_ctx *ctx, uint32_t n, uint32_t edit_dist, uint32_t 
             float t_min = 0;
             for (uint32_t r = 0; r <= reps; ++r) { // first run warms up
                 HIP_TRY(hipEventRecord(ctx->ev[0], s));
-                if ((rc = launch_pairs(K, cpl, false, s, tiles, ctx->packed.as<uint32_t>(), G, 0u, nullptr, 0,
+                if ((rc = launch_pairs(n, K, cpl, false, s, tiles, ctx->packed.p, G, 0u, nullptr, 0,
                                        counter)))
                     return rc;
                 HIP_TRY(hipEventRecord(ctx->ev[1], s));
@@ -518,7 +570,7 @@ IGC_API int igc_autotune(igc_ctx *ctx, uint32_t n, uint32_t edit_dist, uint32_t 
             }
         }
     }
-    ctx->tuned[K < 0 ? 0 : K][size_bucket(N)] = best_shape;
+    ctx->tuned[pred_variant(n, K)][size_bucket(N)] = best_shape;
     *count = k;
     return IGC_OK;
 }
@@ -527,7 +579,7 @@ IGC_API int igc_tuned_shape(igc_ctx *ctx, uint32_t n, uint32_t edit_dist, uint32
 {
     if (!ctx || !cpl || !q) return set_error(IGC_ERR_INVALID, "null argument");
     std::lock_guard<std::mutex> lk(ctx->mu);
-    const Shape sh = pick_shape(ctx, effective_k(n, edit_dist), N);
+    const Shape sh = pick_shape(ctx, n, effective_k(n, edit_dist), N);
     *cpl = sh.cpl;
     *q = sh.q;
     return IGC_OK;

@@ -49,7 +49,7 @@ static uint64_t job_bytes(uint32_t N, uint32_t n, uint32_t ed)
 {
     const int K = ed == 0 ? -1 : effective_k(n, ed);
     const uint64_t edges = (K < 0 || N < 2) ? 0 : initial_edge_cap(N, n, K);
-    return (uint64_t)N * n + (uint64_t)N * 4 + ((uint64_t)N + 1) * 8 + (uint64_t)N * 16 + edges * 16;
+    return (uint64_t)N * n + (uint64_t)N * word_bytes(n) + ((uint64_t)N + 1) * 8 + (uint64_t)N * 16 + edges * 16;
 }
 
 } // namespace igc

@@ -44,7 +44,7 @@ def test_golden_graphs_and_codebooks(gpu_ctx, graphs_golden):
             assert np.array_equal(s["syms"][igc.greedy_order(adj)], words)
 
 
-@pytest.mark.parametrize("n", list(range(1, 17)))
+@pytest.mark.parametrize("n", list(range(1, 33)))
 def test_random_sets_every_length_and_threshold(gpu_ctx, n):
     rng = np.random.default_rng(1000 + n)
     N = 700 if n > 3 else 300
@@ -88,12 +88,19 @@ def test_edge_cases(gpu_ctx):
         adj = gpu_ctx.graph_build(s, 4)
         rp, col = O.graph_csr(s, 4)
         assert np.array_equal(adj.row_ptr, rp) and np.array_equal(adj.col, col), N
+    # the same on 64-bit words, Landau-Vishkin (n=24, K=5) and Myers (n=32, K=20)
+    for N, n, ed in ((63, 24, 6), (65, 24, 6), (257, 24, 6), (1025, 24, 6), (64, 32, 21), (1023, 32, 21)):
+        s = rng.integers(0, 4, size=(N, n), dtype=np.uint8)
+        s[: N // 4] = s[N // 4: 2 * (N // 4)]  # duplicates: edges at every threshold
+        adj = gpu_ctx.graph_build(s, ed)
+        rp, col = O.graph_csr(s, ed)
+        assert np.array_equal(adj.row_ptr, rp) and np.array_equal(adj.col, col), (N, n, ed)
     # invalid input is refused, not computed
     with pytest.raises(igc.IGCError) as e:
         gpu_ctx.graph_build(np.array([[0, 4, 1]], np.uint8).repeat(3, 0), 2)
     assert e.value.code == igc.IGC_ERR_INVALID
     with pytest.raises(igc.IGCError):
-        gpu_ctx.graph_build(np.zeros((4, 17), np.uint8), 3)
+        gpu_ctx.graph_build(np.zeros((4, 33), np.uint8), 3)
 
 
 def test_dense_rows_use_every_sort_path(gpu_ctx):
@@ -161,7 +168,7 @@ def test_batch_matches_single_and_oracle(gpu_ctx):
     rng = np.random.default_rng(21)
     sets = []
     for k in range(40):
-        n = int(rng.integers(1, 17))
+        n = int(rng.integers(1, 33))
         N = int(rng.choice([0, 1, 2, 5, 63, 64, 65, 300, 1000, 2500]))
         ed = int(rng.integers(0, n + 3))
         syms = rng.integers(0, 4, size=(N, n), dtype=np.uint8)
@@ -170,9 +177,12 @@ def test_batch_matches_single_and_oracle(gpu_ctx):
             src[int(rng.integers(0, n))] = int(rng.integers(0, 4))
             syms[r] = src
         sets.append((syms, ed))
-    # sets with 9 <= K <= 15 take the separate-launch path
+    # large thresholds run Myers inside the persistent kernel; n > 16 sets take
+    # the separate-launch path on 64-bit words
     sets.append((rng.integers(0, 4, size=(400, 16), dtype=np.uint8), 11))
     sets.append((rng.integers(0, 4, size=(300, 14), dtype=np.uint8), 13))
+    sets.append((rng.integers(0, 4, size=(500, 32), dtype=np.uint8), 14))
+    sets.append((rng.integers(0, 4, size=(350, 20), dtype=np.uint8), 5))
     banner(f"batch of {len(sets)} uneven sets")
     outs = gpu_ctx.graph_build_batch(sets)
     for (syms, ed), adj in zip(sets, outs):
@@ -202,7 +212,8 @@ def test_batched_config_sets_alone_and_in_the_mix(gpu_ctx):
 
 def test_kmeans_matches_oracle_bit_exact(gpu_ctx):
     rng = np.random.default_rng(31)
-    for N, n, k, iters, seed in [(20000, 16, 64, 10, 1), (5000, 11, 8, 3, 2), (30000, 13, 256, 10, 3), (777, 5, 7, 4, 9)]:
+    for N, n, k, iters, seed in [(20000, 16, 64, 10, 1), (5000, 11, 8, 3, 2), (30000, 13, 256, 10, 3), (777, 5, 7, 4, 9),
+                                 (8000, 24, 32, 5, 4), (3000, 32, 100, 3, 5), (2000, 17, 9, 4, 6)]:
         banner(f"kmeans N={N} n={n} k={k} iters={iters}")
         syms = rng.integers(0, 4, size=(N, n), dtype=np.uint8)
         a_o, C_o = O.kmeans(syms, k, iters, seed)
@@ -319,6 +330,13 @@ def test_autotuned_shapes_give_identical_graphs(gpu_ctx):
     best = min(ents, key=lambda e: e["ms"])
     assert gpu_ctx.tuned_shape(12, 4, syms.shape[0]) == (best["cpl"], best["q"])
     adj = gpu_ctx.graph_build(syms, 4)  # built with the tuned shape
+    assert np.array_equal(adj.row_ptr, rp) and np.array_equal(adj.col, col)
+    # 64-bit words (Landau-Vishkin at n=24, K=5)
+    syms = rng.integers(0, 4, size=(5000, 24), dtype=np.uint8)
+    rp, col = O.graph_csr(syms, 6)
+    ents = gpu_ctx.autotune(24, 6, syms.shape[0], reps=1)
+    assert len(ents) == 12
+    adj = gpu_ctx.graph_build(syms, 6)
     assert np.array_equal(adj.row_ptr, rp) and np.array_equal(adj.col, col)
 
 

@@ -1,6 +1,8 @@
-"""The kernels' pair predicate (indexgen_amd/csrc/igc_lv.hpp), compiled for the
-host with g++, against the oracle: exhaustive over all pairs of short strings
-and every threshold, and on random and near pairs up to n = 16."""
+"""The kernels' pair predicates (indexgen_amd/csrc/igc_lv.hpp), compiled for the
+host with g++, against the oracle: Landau-Vishkin on 32-bit words (n <= 16) and
+on 64-bit words (n <= 32) and the Myers recurrence on both, exhaustive over all
+pairs of short strings and every threshold, and on random and near pairs up to
+n = 32.  Also the dispatch rule: which predicate runs for (n, K)."""
 import ctypes
 import itertools
 import os
@@ -15,23 +17,37 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 @pytest.fixture(scope="module")
-def lv(tmp_path_factory):
+def harness(tmp_path_factory):
     so = str(tmp_path_factory.mktemp("lv") / "lv_harness.so")
     subprocess.run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-o", so, os.path.join(HERE, "lv_harness.cpp")],
                    check=True)
     L = ctypes.CDLL(so)
-    L.lv_pairs.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                           ctypes.c_void_p]
-    L.lv_pairs.restype = ctypes.c_int
+    for f in (L.lv_pairs, L.myers_pairs):
+        f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                      ctypes.c_void_p]
+        f.restype = ctypes.c_int
+    L.pred_kind_of.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.pred_kind_of.restype = ctypes.c_int
+    L.lv_launch_max_k.restype = ctypes.c_int
+    return L
 
-    def pairs(A, B, K):
-        A = np.ascontiguousarray(A, np.uint8)
-        B = np.ascontiguousarray(B, np.uint8)
-        out = np.zeros(A.shape[0], np.uint8)
-        assert L.lv_pairs(A.ctypes.data, B.ctypes.data, A.shape[0], A.shape[1], K, out.ctypes.data) == 0
-        return out
 
-    return pairs
+def _call(fn, A, B, K, wide):
+    A = np.ascontiguousarray(A, np.uint8)
+    B = np.ascontiguousarray(B, np.uint8)
+    out = np.zeros(A.shape[0], np.uint8)
+    assert fn(A.ctypes.data, B.ctypes.data, A.shape[0], A.shape[1], K, int(wide), out.ctypes.data) == 0
+    return out
+
+
+@pytest.fixture(scope="module")
+def lv(harness):
+    return lambda A, B, K, wide=False: _call(harness.lv_pairs, A, B, K, wide)
+
+
+@pytest.fixture(scope="module")
+def myers(harness):
+    return lambda A, B, K, wide=False: _call(harness.myers_pairs, A, B, K, wide)
 
 
 def all_strings(n):
@@ -39,15 +55,16 @@ def all_strings(n):
 
 
 @pytest.mark.parametrize("n", [1, 2, 3, 4, 5])
-def test_exhaustive(lv, n):
+def test_exhaustive(lv, myers, n):
     S = all_strings(n)
     N = S.shape[0]
     A = np.repeat(S, N, axis=0)
     B = np.tile(S, (N, 1))
-    for K in range(0, min(n + 2, 16)):
+    for K in range(0, n + 2):
         want = O.conflict_pairs(A, B, K + 1)
-        assert np.array_equal(lv(A, B, K), want), (n, K)
-    assert lv(A, B, 16).all()  # the every-pair variant
+        for wide in (False, True):
+            assert np.array_equal(lv(A, B, K, wide), want), ("lv", n, K, wide)
+            assert np.array_equal(myers(A, B, K, wide), want), ("myers", n, K, wide)
 
 
 def _near(rng, a, edits):
@@ -67,15 +84,77 @@ def _near(rng, a, edits):
     return b
 
 
-@pytest.mark.parametrize("n", [6, 7, 8, 10, 11, 12, 13, 14, 15, 16])
-def test_random_and_near_pairs(lv, n):
-    rng = np.random.default_rng(100 + n)
-    M = 20000
+def _pairs(n, M=20000, seed=0):
+    rng = np.random.default_rng(100 + n + seed)
     A = rng.integers(0, 4, size=(M, n), dtype=np.uint8)
     B = rng.integers(0, 4, size=(M, n), dtype=np.uint8)
     for r in range(0, M, 2):
-        B[r] = _near(rng, A[r], int(rng.integers(1, 8)))
+        B[r] = _near(rng, A[r], int(rng.integers(1, 8 if n <= 16 else 14)))
+    # a few strings with long runs of one symbol (all-equal planes)
+    A[1:200:10] = 0
+    B[3:300:10] = 3
+    return A, B
+
+
+@pytest.mark.parametrize("n", [6, 7, 8, 10, 11, 12, 13, 14, 15, 16])
+def test_random_and_near_pairs(lv, myers, n):
+    A, B = _pairs(n)
     for K in range(0, 16):
         want = O.conflict_pairs(A, B, K + 1)
-        got = lv(A, B, K)
-        assert np.array_equal(got, want), (n, K, int((got != want).sum()))
+        for wide in (False, True):
+            got = lv(A, B, K, wide)
+            assert np.array_equal(got, want), ("lv", n, K, wide, int((got != want).sum()))
+        for wide in (False, True):
+            got = myers(A, B, K, wide)
+            assert np.array_equal(got, want), ("myers", n, K, wide, int((got != want).sum()))
+
+
+@pytest.mark.parametrize("n", [17, 18, 20, 23, 24, 27, 31, 32])
+def test_random_and_near_pairs_wide(lv, myers, n):
+    A, B = _pairs(n, M=12000)
+    for K in range(0, n + 1):
+        want = O.conflict_pairs(A, B, K + 1)
+        got = myers(A, B, K, True)
+        assert np.array_equal(got, want), ("myers", n, K, int((got != want).sum()))
+        if K <= 15:
+            got = lv(A, B, K, True)
+            assert np.array_equal(got, want), ("lv", n, K, int((got != want).sum()))
+
+
+def test_oracle_pins_wide_distances():
+    """The oracle's distance at n > 16 against the textbook dynamic program."""
+    rng = np.random.default_rng(7)
+    for n in (17, 24, 32):
+        for _ in range(60):
+            a = rng.integers(0, 4, n).astype(np.uint8)
+            b = np.array(_near(rng, list(a), int(rng.integers(0, 12))), np.uint8)
+            D = np.arange(n + 1)
+            for i in range(1, n + 1):
+                prev, D = D, np.empty(n + 1, np.int64)
+                D[0] = i
+                for j in range(1, n + 1):
+                    D[j] = min(prev[j] + 1, D[j - 1] + 1, prev[j - 1] + (a[i - 1] != b[j - 1]))
+            assert O.levenshtein(a, b) == D[n]
+
+
+def test_dispatch_rule(harness):
+    """pred_kind: every pair when K >= n; Landau-Vishkin only up to the largest
+    K the kernels are instantiated for; 32-bit Landau-Vishkin only for n <= 16
+    and 64-bit only above; at the benchmark shape (n=13, K=3) Landau-Vishkin."""
+    kmax = harness.lv_launch_max_k()
+    seen = set()
+    for n in range(1, 33):
+        for K in range(0, 40):
+            k = harness.pred_kind_of(n, K)
+            seen.add(k)
+            if K >= n:
+                assert k == 3, (n, K)
+                continue
+            assert k in (0, 1, 2), (n, K)
+            if k in (0, 1):
+                assert K <= kmax, (n, K)
+                assert (k == 1) == (n > 16), (n, K)
+    assert seen == {0, 1, 2, 3}
+    assert harness.pred_kind_of(13, 3) == 0
+    assert harness.pred_kind_of(32, 3) == 1
+    assert harness.pred_kind_of(16, 14) == 2
