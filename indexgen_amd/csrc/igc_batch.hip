@@ -24,10 +24,10 @@
 
 namespace igc {
 
-// Predicates the persistent kernel dispatches at run time (sets with n <= 16):
-// Landau-Vishkin for K <= kBatchMaxK (pred_kind never picks it above 8 for
-// n <= 16), Myers, every pair.
-constexpr int kBatchMaxK = 8;
+// Predicates the persistent kernel dispatches at run time (sets with n <= 16,
+// two pairs per register): Landau-Vishkin for K <= kBatchMaxK (pred_kind never
+// picks it above 9 for n <= 16), Myers, every pair.
+constexpr int kBatchMaxK = 9;
 enum BatchPred : uint32_t { kBatchLV = 0, kBatchMyers = 1, kBatchAll = 2, kBatchNone = 3 };
 
 struct SetDesc {
@@ -95,7 +95,7 @@ __device__ __forceinline__ void batch_tile(const SetDesc &d, const uint32_t *__r
                                                  edges, cap, counter);                                        \
         break;
         IGC_BCASE(0) IGC_BCASE(1) IGC_BCASE(2) IGC_BCASE(3) IGC_BCASE(4) IGC_BCASE(5) IGC_BCASE(6)
-        IGC_BCASE(7) IGC_BCASE(8)
+        IGC_BCASE(7) IGC_BCASE(8) IGC_BCASE(9)
 #undef IGC_BCASE
     default:
         break;
@@ -185,10 +185,10 @@ static int plan_batch(igc_ctx *ctx, const igc_set *sets, uint32_t count, int cpl
     // heaviest tiles first: costlier predicate per pair first, then larger sets
     auto cost = [&](uint32_t s) -> uint32_t {
         if (K[s] < 0) return 0;
+        const bool wide = wide_words(sets[s].n);
         switch (pred_kind(sets[s].n, K[s])) {
-        case PredKind::LV32: return lv_cost(K[s], false);
-        case PredKind::LV64: return lv_cost(K[s], true);
-        case PredKind::MYERS: return myers_cost(sets[s].n);
+        case PredKind::LV: return lv_cost2(K[s], wide);
+        case PredKind::MYERS: return myers_cost2(sets[s].n, wide);
         default: return 1;
         }
     };
@@ -202,7 +202,7 @@ static int plan_batch(igc_ctx *ctx, const igc_set *sets, uint32_t count, int cpl
         const igc_set &x = sets[s];
         SetDesc d{};
         const Shape sh{cpl, 1};
-        d.G = choose_geometry(x.N, sh.cpl, x.N >= 8192 ? 1u : (x.N >= 2048 ? 2u : 4u));
+        d.G = choose_geometry(x.N, sh.cpl * lane_cols(x.n), x.N >= 8192 ? 1u : (x.N >= 2048 ? 2u : 4u));
         d.sym_off = sym_off[s];
         d.word_off = word_off[s];
         d.node_off = P.node_off_by_set[s];

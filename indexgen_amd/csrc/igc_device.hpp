@@ -77,13 +77,16 @@ struct HostBuf {
 // flag, [4] mid-row count, [5] big-row count, [6..7] batch work counter (u64)
 constexpr size_t kMiscWords = 8;
 
-// Tile shape of the single-set kernel: columns per lane and rows per tile W/q.
+// Tile shape of the single-set kernel: column slots per lane (each slot one
+// register of lane_cols(n) columns) and rows per tile W/q.
 struct Shape {
     int cpl;
     uint32_t q;
 };
 
-TileGeom choose_geometry(uint32_t N, int cpl, uint32_t q);
+// Tile geometry for a workgroup of 4 waves with cols_per_lane columns per lane
+// (slots per lane x the policy's lanes per register) and W/q rows per tile.
+TileGeom choose_geometry(uint32_t N, int cols_per_lane, uint32_t q);
 uint64_t tile_count(const TileGeom &G);
 Shape default_shape(uint32_t N);
 uint64_t initial_edge_cap(uint32_t N, uint32_t n, int K);
@@ -93,6 +96,8 @@ uint32_t grid_for(uint64_t work, uint32_t per_block, uint32_t cap_blocks);
 // Packed words are 32-bit for n <= 16 and 64-bit for 17 <= n <= 32.
 inline bool wide_words(uint32_t n) { return n > 16; }
 inline size_t word_bytes(uint32_t n) { return wide_words(n) ? 8 : 4; }
+// Columns a pair test covers per register: two 16-bit lanes for n <= 16.
+inline int lane_cols(uint32_t n) { return wide_words(n) ? 1 : 2; }
 
 // Launches the pair kernel of the predicate pred_kind(n, K) picks, with cpl
 // columns per lane, over `tiles` tiles of geometry G; `packed` holds words of

@@ -16,18 +16,13 @@
 //   mism_d bit j = (A[j+d] != B[j])   for the 2*floor(K/2)+1 diagonals |d|<=K/2
 //   FR[e][d] = slide_d(max(FR[e-1][d]+1, FR[e-1][d-1], FR[e-1][d+1]+1))
 //   slide_d(L) = L + ctz(mism_d >> L)
-// restricted to |d| <= min(e, K-e), and edge iff FR[K][0] >= n.  Each cell is a
-// max3, a shift, a find-first-bit and an add on one register, so a pair costs
-// O(K^2) VALU ops independent of n (about 40 at K=3 on 32-bit words) instead of
-// the 17*n of one Myers column step per symbol.  Where that quadratic cost
-// exceeds the Myers recurrence (large K), the kernels run Myers instead
-// (pred_kind in igc_lv.hpp).
-// The upper half of every mask is forced to 1: a slide can never run off the
-// word, and any furthest-reaching value past a diagonal's end only means
-// "end reached", which propagates correctly because neighbouring DP cells
-// differ by at most one.  For diagonal 0 the bits n..half-1 are 0 (both strings
-// are zero-padded), so FR[K][0] >= n  <=>  (q0 >> L) == 0 with q0 the low-half
-// mask and L the last level's start.
+// restricted to |d| <= min(e, K-e), and edge iff FR[K][0] >= n.  Positions
+// are held as thermometer codes (mask of bits >= q), so each cell is one AND
+// chain, a subtract and one bitwise merge, and for n <= 16 two pairs share a
+// register as 16-bit lanes: about 22 VALU instructions per pair at K=3,
+// independent of n, instead of the 17*n of one Myers column step per symbol.
+// Where that quadratic cost exceeds the Myers recurrence (large K), the
+// kernels run Myers instead (pred_kind and the derivation in igc_lv.hpp).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -85,29 +80,60 @@ static __global__ void k_fill_u32(uint32_t *__restrict__ p, uint64_t count, uint
 }
 
 // ---------------------------------------------------------------------------
-// pair predicates as policies of the tile code: Row holds what a wave derives
-// once from the (uniform) row word, edge() tests one column word against it.
+// pair predicates as policies of the tile code.  A policy works on packed words
+// W and tests P columns per register (P = 2 on 16-bit lanes for n <= 16, 1 on
+// 32-bit lanes above): Col holds a lane's P columns, Row what a wave derives
+// once from the (uniform) row word, test() returns the edge lanes as a mask.
+
+template <class W> struct LaneOf;
+template <> struct LaneOf<uint32_t> { using type = u16x2; };
+template <> struct LaneOf<uint64_t> { using type = uint32_t; };
+
+template <class W>
+struct ColPlanes {
+    using V = typename LaneOf<W>::type;
+    static constexpr int P = LaneCount<V>::value;
+    V lo, hi;
+    // columns j0 + 64h for h < P (zero words past N)
+    __device__ __forceinline__ void load(const W *__restrict__ packed, uint32_t j0, uint32_t N)
+    {
+        if constexpr (P == 2) {
+            const uint32_t w0 = j0 < N ? packed[j0] : 0u;
+            const uint32_t w1 = j0 + 64u < N ? packed[j0 + 64u] : 0u;
+            col_planes(w0, w1, lo, hi);
+        } else {
+            col_planes(j0 < N ? packed[j0] : (W)0, lo, hi);
+        }
+    }
+};
 
 template <int K, class W>
 struct PolLV {
     using Word = W;
-    struct Row {
-        W sa[LvShape<K>::WD];
-    };
-    static __device__ __forceinline__ void row(Row &r, W a, uint32_t, int) { row_shifts<K>(a, r.sa); }
-    static __device__ __forceinline__ bool edge(const Row &r, W b, uint32_t, int) { return lv_edge<K>(r.sa, b); }
+    using V = typename LaneOf<W>::type;
+    using Col = ColPlanes<W>;
+    static constexpr int P = Col::P;
+    using Row = LvtRow<K, V>;
+    static __device__ __forceinline__ void row(Row &r, W a, uint32_t, int) { lvt_row<K>(a, r); }
+    static __device__ __forceinline__ uint32_t test(const Row &r, const Col &c, uint32_t, int)
+    {
+        return lvt_test<K, V>(r, c.lo, c.hi);
+    }
 };
 
 template <class W>
 struct PolMyers {
     using Word = W;
+    using V = typename LaneOf<W>::type;
+    using Col = ColPlanes<W>;
+    static constexpr int P = Col::P;
     struct Row {
-        W t;
+        uint32_t lo, hi;
     };
-    static __device__ __forceinline__ void row(Row &r, W a, uint32_t, int) { r.t = a; }
-    static __device__ __forceinline__ bool edge(const Row &r, W b, uint32_t n, int k)
+    static __device__ __forceinline__ void row(Row &r, W a, uint32_t, int) { text_planes(a, r.lo, r.hi); }
+    static __device__ __forceinline__ uint32_t test(const Row &r, const Col &c, uint32_t n, int k)
     {
-        return myers_edge<W>(r.t, b, n, k);
+        return myers_lanes<V>(r.lo, r.hi, c.lo, c.hi, n, k);
     }
 };
 
@@ -115,9 +141,11 @@ struct PolMyers {
 template <class W>
 struct PolAll {
     using Word = W;
+    using Col = ColPlanes<W>;
+    static constexpr int P = Col::P;
     struct Row {};
     static __device__ __forceinline__ void row(Row &, W, uint32_t, int) {}
-    static __device__ __forceinline__ bool edge(const Row &, W, uint32_t, int) { return true; }
+    static __device__ __forceinline__ uint32_t test(const Row &, const Col &, uint32_t, int) { return (1u << P) - 1u; }
 };
 
 // ---------------------------------------------------------------------------
@@ -190,35 +218,34 @@ __device__ __forceinline__ void flush_stage(const uint64_t *st, uint32_t staged,
 }
 
 // One wave's share of tile (rb, cb) of a candidate set: rows rb*R .. rb*R+R-1,
-// columns colw .. colw+64*CPL-1 with colw = cb*W + wave*64*CPL (lane l, slot c
-// -> column colw + 64c + l).  Rows are walked in order with the row word
-// uniform (scalar registers), so each lane tests CPL pairs per row.  Nonzero
-// ballots (rare) are compacted through the wave's LDS stage `st`; edges are
-// written as global ids (node_off + i, node_off + j).  EMIT=false only counts.
-// Contains no workgroup barrier: a wave may return early.  n and k are the
-// string length and threshold for the policies that take them at run time.
+// columns colw .. colw+64*CPL*P-1 with colw = cb*W + wave*64*CPL*P (lane l,
+// slot s -> column colw + 64s + l; slots c*P .. c*P+P-1 share register group c).
+// Rows are walked in order with the row word uniform (scalar registers), so
+// each lane tests CPL*P pairs per row.  Nonzero ballots (rare) are compacted
+// through the wave's LDS stage `st`; edges are written as global ids
+// (node_off + i, node_off + j).  EMIT=false only counts.  Contains no
+// workgroup barrier: a wave may return early.  n and k are the string length
+// and threshold for the policies that take them at run time.
 template <class Pol, int CPL, bool EMIT>
 __device__ __forceinline__ void wave_tile(const typename Pol::Word *__restrict__ packed, const TileGeom &G,
                                           uint32_t rb, uint32_t cb, uint32_t node_off, uint32_t n, int k,
                                           uint64_t *st, uint32_t lane, uint32_t wave, uint64_t *__restrict__ edges,
                                           uint64_t cap, unsigned long long *__restrict__ counter)
 {
-    using W = typename Pol::Word;
+    constexpr int P = Pol::P;
+    constexpr int SL = CPL * P; // columns per lane
     const uint32_t N = G.N;
     const uint32_t row0 = rb * G.R;
-    const uint32_t colw = cb * G.W + wave * 64u * CPL;
+    const uint32_t colw = cb * G.W + wave * 64u * SL;
     if (row0 >= N || colw >= N) return;
-    const uint32_t colend = min(colw + 64u * CPL, N); // exclusive
+    const uint32_t colend = min(colw + 64u * SL, N); // exclusive
     // rows i with some column j > i in this wave: i < colend - 1
     if (colend - 1u <= row0) return;
     const uint32_t rows = min(min(G.R, N - row0), colend - 1u - row0);
 
-    W b[CPL];
+    typename Pol::Col col[CPL];
 #pragma unroll
-    for (int c = 0; c < CPL; ++c) {
-        const uint32_t j = colw + 64u * c + lane;
-        b[c] = j < N ? packed[j] : (W)0;
-    }
+    for (int c = 0; c < CPL; ++c) col[c].load(packed, colw + 64u * P * c + lane, N);
     uint32_t staged = 0;
     uint64_t counted = 0;
 
@@ -226,19 +253,23 @@ __device__ __forceinline__ void wave_tile(const typename Pol::Word *__restrict__
         const uint32_t i = row0 + r;
         typename Pol::Row row;
         Pol::row(row, packed[i], n, k);
-        uint64_t mk[CPL];
+        uint64_t mk[SL];
         uint64_t any = 0;
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
-            mk[c] = __ballot(Pol::edge(row, b[c], n, k));
-            any |= mk[c];
+            const uint32_t e = Pol::test(row, col[c], n, k);
+#pragma unroll
+            for (int h = 0; h < P; ++h) {
+                mk[c * P + h] = __ballot((e >> h) & 1u);
+                any |= mk[c * P + h];
+            }
         }
         if (any) {
 #pragma unroll
-            for (int c = 0; c < CPL; ++c) {
-                if (!mk[c]) continue;
-                const uint32_t j = colw + 64u * c + lane;
-                const bool mine = ((mk[c] >> lane) & 1ull) && j > i && j < N;
+            for (int sl = 0; sl < SL; ++sl) {
+                if (!mk[sl]) continue;
+                const uint32_t j = colw + 64u * sl + lane;
+                const bool mine = ((mk[sl] >> lane) & 1ull) && j > i && j < N;
                 const uint64_t m2 = __ballot(mine);
                 if (!m2) continue;
                 const uint32_t cnt = (uint32_t)__popcll(m2);

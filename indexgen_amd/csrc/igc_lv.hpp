@@ -7,11 +7,14 @@
 //   64-bit word (n <= 32): bit p = sym[p] & 1, bit 32+p = sym[p] >> 1
 // all other bits 0.  A shift by d moves whole symbols in both planes.
 //
-// lv_edge<K>(row shifts of A, B) decides Levenshtein(A, B) <= K by
-// Landau-Vishkin furthest-reaching diagonals over mismatch bitmasks (see the
-// header of igc_kernels.hpp); myers_edge() decides it with the bit-parallel
-// Myers recurrence (one step per symbol of B) for thresholds where
-// Landau-Vishkin is costlier or out of its range.
+// The tests run on a lane type V: u16x2 holds two pairs, one per 16-bit half
+// of a 32-bit register (n <= 16; packed 16-bit subtract, add and shift, bitwise
+// ops on the whole register), uint32_t holds one pair (n <= 32).  Both tests
+// give Levenshtein(A, B) <= K exactly:
+//   lvt_test     Landau-Vishkin furthest-reaching diagonals on thermometer
+//                codes, O(K^2) instructions independent of n;
+//   myers_lanes  the bit-parallel Myers recurrence, O(n) instructions.
+// pred_kind() picks the cheaper one for (n, K).
 #pragma once
 #include <stdint.h>
 
@@ -27,26 +30,19 @@ namespace igc {
 
 // K value meaning "every pair is an edge" (editDist > n)
 constexpr int kAllPairs = 1000;
-// largest K lv_edge is defined for
+// largest K lvt_residue is defined for
 constexpr int kLvMaxK = 15;
 // largest K for which pred_kind() picks Landau-Vishkin at any n <= 32 (the
 // kernels are instantiated for 0..kLvLaunchMaxK)
-constexpr int kLvLaunchMaxK = 9;
+constexpr int kLvLaunchMaxK = 13;
 
 template <class W> struct WordTraits;
 template <> struct WordTraits<uint32_t> {
     static constexpr int half = 16;
-    static constexpr uint32_t hi_ones = 0xFFFF0000u;
-    static constexpr uint32_t lo_ones = 0x0000FFFFu;
 };
 template <> struct WordTraits<uint64_t> {
     static constexpr int half = 32;
-    static constexpr uint64_t hi_ones = 0xFFFFFFFF00000000ull;
-    static constexpr uint64_t lo_ones = 0x00000000FFFFFFFFull;
 };
-
-IGC_HD inline uint32_t ctz_word(uint32_t x) { return (uint32_t)__builtin_ctz(x); }
-IGC_HD inline uint32_t ctz_word(uint64_t x) { return (uint32_t)__builtin_ctzll(x); }
 
 IGC_HD inline uint32_t pack_word(const uint8_t *s, uint32_t n)
 {
@@ -68,149 +64,240 @@ IGC_HD inline uint64_t pack_word64(const uint8_t *s, uint32_t n)
     return w;
 }
 
+// ---------------------------------------------------------------------------
+// lane types
+
+#if defined(__clang__)
+typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+#else
+typedef uint16_t u16x2 __attribute__((vector_size(4)));
+#endif
+
+template <class V> struct LaneCount;
+template <> struct LaneCount<uint32_t> { static constexpr int value = 1; };
+template <> struct LaneCount<u16x2> { static constexpr int value = 2; };
+
+IGC_HD inline u16x2 dup16(uint32_t x)
+{
+    u16x2 v;
+    v[0] = (uint16_t)x;
+    v[1] = (uint16_t)x;
+    return v;
+}
+
+template <class V> IGC_HD inline V splat_lane(uint32_t x);
+template <> IGC_HD inline uint32_t splat_lane<uint32_t>(uint32_t x) { return x; }
+template <> IGC_HD inline u16x2 splat_lane<u16x2>(uint32_t x) { return dup16(x); }
+
+IGC_HD inline uint32_t lane_bits(uint32_t v, int) { return v; }
+IGC_HD inline uint32_t lane_bits(u16x2 v, int h) { return (uint32_t)v[h]; }
+
+// lanes whose value is zero, as a bit mask
+IGC_HD inline uint32_t zero_lanes(uint32_t z) { return z == 0 ? 1u : 0u; }
+IGC_HD inline uint32_t zero_lanes(u16x2 z) { return (z[0] == 0 ? 1u : 0u) | (z[1] == 0 ? 2u : 0u); }
+
+// Planes of one row word shifted by s (s >= 0: towards bit 0), one lane's width.
+IGC_HD inline void row_planes(uint32_t a, int s, u16x2 &lo, u16x2 &hi)
+{
+    const uint32_t l = a & 0xFFFFu, h = a >> 16;
+    lo = dup16(s >= 0 ? (l >> s) : ((l << (-s)) & 0xFFFFu));
+    hi = dup16(s >= 0 ? (h >> s) : ((h << (-s)) & 0xFFFFu));
+}
+IGC_HD inline void row_planes(uint64_t a, int s, uint32_t &lo, uint32_t &hi)
+{
+    const uint32_t l = (uint32_t)a, h = (uint32_t)(a >> 32);
+    lo = s >= 0 ? (l >> s) : (l << (-s));
+    hi = s >= 0 ? (h >> s) : (h << (-s));
+}
+
+// Column planes: two 32-bit packed words, one per 16-bit lane, or one 64-bit word.
+IGC_HD inline void col_planes(uint32_t w0, uint32_t w1, u16x2 &lo, u16x2 &hi)
+{
+    lo[0] = (uint16_t)w0;
+    lo[1] = (uint16_t)w1;
+    hi[0] = (uint16_t)(w0 >> 16);
+    hi[1] = (uint16_t)(w1 >> 16);
+}
+IGC_HD inline void col_planes(uint64_t w, uint32_t &lo, uint32_t &hi)
+{
+    lo = (uint32_t)w;
+    hi = (uint32_t)(w >> 32);
+}
+
+// ---------------------------------------------------------------------------
+// Landau-Vishkin on thermometer codes
+//
+// Diagonal d compares A[j+d] with B[j]; its mismatch mask has bit j set iff
+// they differ.  FR[e][d], the furthest position of diagonal d reachable with e
+// edits, obeys
+//   FR[e][d] = slide_d(max(FR[e-1][d] + 1, FR[e-1][d-1], FR[e-1][d+1] + 1))
+// with slide_d(L) = the first mismatch of diagonal d at or after L, restricted
+// to |d| <= min(e, K-e) (a path through diagonal d to diagonal 0 needs |d| edits
+// each way), and Levenshtein(A, B) <= K iff slide_0 at level K reaches n.
+// A position q is held as its thermometer code, the mask of all bits >= q
+// (0 = past the lane), so
+//   max of positions   = AND of their masks,
+//   position + 1       = mask << 1,
+//   slide from mask L  = ~(x - 1) | x  with x = m & L (its lowest set bit and
+//                        everything above; 0 when no mismatch is left),
+// which needs no count-trailing-zeros and no max.  Positions past a diagonal's
+// end (garbage bits of the shifted word) only ever mean "end reached": the DP
+// cells of neighbouring diagonals differ by at most one.  On diagonal 0 the
+// padding positions n.. match (both strings are zero-padded), so the final
+// residue m_0 & L_K is zero iff the last slide reaches the lane's end.
+
 template <int K>
 struct LvShape {
     static constexpr int C = (K > kLvMaxK) ? 0 : K / 2; // band half-width
     static constexpr int WD = 2 * C + 1;               // diagonals used
 };
 
-// Row word shifted for each diagonal d = t - C (uniform across the wave).
-template <int K, class W>
-IGC_HD inline void row_shifts(W a, W (&sa)[LvShape<K>::WD])
+// m[t] is the mismatch mask of diagonal d = t - C; returns the residue
+// (a lane is an edge iff it is zero).
+template <int K, class V>
+IGC_HD inline V lvt_residue(const V (&m)[LvShape<K>::WD])
+{
+    constexpr int C = LvShape<K>::C;
+    constexpr int WD = LvShape<K>::WD;
+    if constexpr (K == 0) {
+        return m[0];
+    } else {
+        V T[WD], S[WD]; // masks of level e-1, and the same plus one
+IGC_UNROLL
+        for (int t = 0; t < WD; ++t) {
+            T[t] = V{}; // placeholders; only in-range entries are read
+            S[t] = V{};
+        }
+        {
+            const V x = m[C];
+            T[C] = ~(x - 1) | x;
+            S[C] = (V)(T[C] << 1);
+        }
+IGC_UNROLL
+        for (int e = 1; e < K; ++e) {
+            const int r = (e < K - e) ? e : (K - e);                    // range at level e
+            const int rp = (e - 1 < K - e + 1) ? (e - 1) : (K - e + 1); // range at level e-1
+            V NT[WD];
+IGC_UNROLL
+            for (int d = -C; d <= C; ++d) {
+                NT[d + C] = T[d + C];
+                if (d < -r || d > r) continue;
+                V x = m[d + C];
+                if (d >= -rp && d <= rp) x &= S[d + C];
+                if (d - 1 >= -rp && d - 1 <= rp) x &= T[d - 1 + C];
+                if (d + 1 >= -rp && d + 1 <= rp) x &= S[d + 1 + C];
+                NT[d + C] = ~(x - 1) | x;
+            }
+IGC_UNROLL
+            for (int t = 0; t < WD; ++t) {
+                T[t] = NT[t];
+                S[t] = (V)(NT[t] << 1);
+            }
+        }
+        V L = S[C];
+        if constexpr (K >= 2) L = L & T[C - 1] & S[C + 1];
+        return m[C] & L;
+    }
+}
+
+// Row side: the row word's planes shifted for each diagonal, uniform across a
+// wave (scalar registers in the kernels).
+template <int K, class V>
+struct LvtRow {
+    V lo[LvShape<K>::WD];
+    V hi[LvShape<K>::WD];
+};
+
+template <int K, class V, class W>
+IGC_HD inline void lvt_row(W a, LvtRow<K, V> &r)
 {
     constexpr int C = LvShape<K>::C;
 IGC_UNROLL
-    for (int t = 0; t < LvShape<K>::WD; ++t) {
-        const int d = t - C;
-        sa[t] = d >= 0 ? (W)(a >> d) : (W)(a << (-d));
-    }
+    for (int t = 0; t < LvShape<K>::WD; ++t) row_planes(a, t - C, r.lo[t], r.hi[t]);
 }
 
-// Levenshtein(A,B) <= K for packed words of one width (see the file header).
-// Mismatch mask of diagonal d: low half = symbol mismatches, high half forced
-// to 1 so a slide never leaves the word (furthest-reaching values stay below
-// half + K + 1 <= width).
-template <int K, class W>
-IGC_HD inline bool lv_edge(const W (&sa)[LvShape<K>::WD], W b)
+// Edge lanes (bit h: the pair in lane h has Levenshtein <= K).
+template <int K, class V>
+IGC_HD inline uint32_t lvt_test(const LvtRow<K, V> &r, V blo, V bhi)
 {
-    constexpr int H = WordTraits<W>::half;
-    constexpr W HI = WordTraits<W>::hi_ones;
-    constexpr W LO = WordTraits<W>::lo_ones;
-    if constexpr (K == 0) {
-        const W x = sa[0] ^ b;
-        return ((x | (x >> H)) & LO) == 0;
-    } else {
-        constexpr int C = LvShape<K>::C;
-        constexpr int WD = LvShape<K>::WD;
-        W m[WD];
-        W q0 = 0;
+    V m[LvShape<K>::WD];
 IGC_UNROLL
-        for (int t = 0; t < WD; ++t) {
-            const W x = sa[t] ^ b;
-            const W h = x >> H;
-            m[t] = x | h | HI;
-            if (t == C) q0 = (x & LO) | h;
-        }
-        uint32_t fr[WD], fv[WD]; // FR and FR+1 of the previous level
-IGC_UNROLL
-        for (int t = 0; t < WD; ++t) {
-            fr[t] = 0u;
-            fv[t] = 0u;
-        }
-        fr[C] = ctz_word(m[C]);
-        fv[C] = fr[C] + 1u;
-IGC_UNROLL
-        for (int e = 1; e < K; ++e) {
-            const int r = (e < K - e) ? e : (K - e);                  // range at level e
-            const int rp = (e - 1 < K - e + 1) ? (e - 1) : (K - e + 1); // range at level e-1
-            uint32_t nr[WD], nv[WD];
-IGC_UNROLL
-            for (int t = 0; t < WD; ++t) {
-                nr[t] = 0u;
-                nv[t] = 0u;
-            }
-IGC_UNROLL
-            for (int d = -C; d <= C; ++d) {
-                if (d < -r || d > r) continue;
-                uint32_t L = 0u;
-                if (d >= -rp && d <= rp) L = fv[d + C];
-                if (d - 1 >= -rp && d - 1 <= rp) L = L > fr[d - 1 + C] ? L : fr[d - 1 + C];
-                if (d + 1 >= -rp && d + 1 <= rp) L = L > fv[d + 1 + C] ? L : fv[d + 1 + C];
-                const uint32_t s = L + ctz_word((W)(m[d + C] >> L));
-                nr[d + C] = s;
-                nv[d + C] = s + 1u;
-            }
-IGC_UNROLL
-            for (int t = 0; t < WD; ++t) {
-                fr[t] = nr[t];
-                fv[t] = nv[t];
-            }
-        }
-        uint32_t L = fv[C];
-        if constexpr (K >= 2) {
-            L = L > fr[C - 1] ? L : fr[C - 1];
-            L = L > fv[C + 1] ? L : fv[C + 1];
-        }
-        return (q0 >> L) == 0;
-    }
+    for (int t = 0; t < LvShape<K>::WD; ++t) m[t] = (r.lo[t] ^ blo) | (r.hi[t] ^ bhi);
+    return zero_lanes(lvt_residue<K, V>(m));
 }
 
-// Bit-parallel Myers (Hyyro's formulation of Myers 1999) deciding
-// Levenshtein(T, P) <= K for a text T and a pattern P of one length n <= 32,
-// both packed words of one width.  One step per text symbol; the match mask of
-// text symbol T[j] is the set of pattern positions p with P_lo[p] == T_lo[j] and
-// P_hi[p] == T_hi[j].  Vertical deltas vp/vn of the last column give
-// D[n][n] = n + |vp| - |vn| over bits 0..n-1.  Bits at and above n hold
-// padding; carries and shifts only move upward, so they never reach the
-// counted bits.  In the kernels T is the row (uniform across a wave), so the
-// per-step symbol bits are scalar work.
-template <class W>
-IGC_HD inline bool myers_edge(W t, W p, uint32_t n, int K)
+// ---------------------------------------------------------------------------
+// Bit-parallel Myers (Hyyro's formulation of Myers 1999): Levenshtein(T, P)
+// for a text T and patterns P of one length n.  One step per text symbol; the
+// match mask of text symbol T[j] is the set of pattern positions p with
+// P_lo[p] == T_lo[j] and P_hi[p] == T_hi[j].  The vertical deltas vp/vn of the
+// last column give D[n][n] = n + |vp| - |vn| over bits 0..n-1.  Bits at and
+// above n hold padding; carries and shifts only move upward within a lane, so
+// they never reach the counted bits.  T is the row (uniform across a wave), so
+// its symbol bits are scalar work.  Returns the edge lanes (distance <= K).
+template <class V>
+IGC_HD inline uint32_t myers_lanes(uint32_t tlo, uint32_t thi, V plo, V phi, uint32_t n, int K)
 {
-    constexpr int H = WordTraits<W>::half;
-    constexpr W LO = WordTraits<W>::lo_ones;
-    const uint32_t tlo = (uint32_t)(t & LO), thi = (uint32_t)(t >> H);
-    const uint32_t plo = (uint32_t)(p & LO), phi = (uint32_t)(p >> H);
-    uint32_t vp = ~0u, vn = 0u;
+    V vp = splat_lane<V>(0xFFFFFFFFu), vn = splat_lane<V>(0u);
+    const V one = splat_lane<V>(1u);
     for (uint32_t j = 0; j < n; ++j) {
-        const uint32_t s0 = 0u - ((tlo >> j) & 1u);
-        const uint32_t s1 = 0u - ((thi >> j) & 1u);
-        const uint32_t eq = ~((plo ^ s0) | (phi ^ s1));
-        const uint32_t d0 = (((eq & vp) + vp) ^ vp) | eq | vn;
-        const uint32_t hp = vn | ~(d0 | vp);
-        const uint32_t hn = vp & d0;
-        const uint32_t x = (hp << 1) | 1u;
+        const V s0 = splat_lane<V>(0u - ((tlo >> j) & 1u));
+        const V s1 = splat_lane<V>(0u - ((thi >> j) & 1u));
+        const V eq = ~((plo ^ s0) | (phi ^ s1));
+        const V d0 = (((eq & vp) + vp) ^ vp) | eq | vn;
+        const V hp = vn | ~(d0 | vp);
+        const V hn = vp & d0;
+        const V x = (V)(hp << 1) | one;
         vn = x & d0;
-        vp = (hn << 1) | ~(x | d0);
+        vp = (V)(hn << 1) | ~(x | d0);
     }
     const uint32_t mask = n >= 32 ? ~0u : ((1u << n) - 1u);
-    const int dist = (int)n + __builtin_popcount(vp & mask) - __builtin_popcount(vn & mask);
-    return dist <= K;
+    uint32_t out = 0;
+IGC_UNROLL
+    for (int h = 0; h < LaneCount<V>::value; ++h) {
+        const int dist = (int)n + __builtin_popcount(lane_bits(vp, h) & mask) -
+                         __builtin_popcount(lane_bits(vn, h) & mask);
+        out |= (uint32_t)(dist <= K) << h;
+    }
+    return out;
 }
 
-// Which predicate a (length, threshold) pair uses: Landau-Vishkin on the word
-// width of n while its cost (VALU instructions per pair) is below Myers', the
-// Myers recurrence otherwise, and no test at all when K >= n.
-enum class PredKind : int { LV32 = 0, LV64 = 1, MYERS = 2, ALL = 3 };
-
-// VALU instructions of one pair test on gfx950, counted in the hipcc -O3
-// assembly of a kernel that runs just the predicate (v_bitop3_b32 merges most
-// of the Myers step's logic): Landau-Vishkin per K on 32- and 64-bit words,
-// Myers 15 per symbol plus 10.
-IGC_HD inline uint32_t lv_cost(int K, bool wide)
+// Row planes of the text for myers_lanes.
+IGC_HD inline void text_planes(uint32_t a, uint32_t &lo, uint32_t &hi)
 {
-    const uint32_t c32[kLvMaxK + 1] = {6, 10, 26, 40, 67, 91, 128, 162, 209, 253, 310, 364, 431, 495, 572, 646};
-    const uint32_t c64[kLvMaxK + 1] = {8, 12, 40, 63, 107, 146, 206, 261, 337, 408, 500, 587, 695, 798, 922, 1041};
-    return wide ? c64[K] : c32[K];
+    lo = a & 0xFFFFu;
+    hi = a >> 16;
+}
+IGC_HD inline void text_planes(uint64_t a, uint32_t &lo, uint32_t &hi)
+{
+    lo = (uint32_t)a;
+    hi = (uint32_t)(a >> 32);
 }
 
-IGC_HD inline uint32_t myers_cost(uint32_t n) { return 15u * n + 10u; }
+// ---------------------------------------------------------------------------
+// Which predicate a (length, threshold) pair uses: Landau-Vishkin while its
+// cost is below Myers', Myers otherwise, and no test at all when K >= n.
+// n <= 16 runs two pairs per register (u16x2), 17 <= n <= 32 one (uint32_t).
+enum class PredKind : int { LV = 0, MYERS = 1, ALL = 2 };
+
+// VALU instructions per two pairs on gfx950, counted in the hipcc -O3
+// assembly of kernels that run just one test (v_bitop3_b32 merges most of the
+// logic): Landau-Vishkin per K on u16x2 and on uint32_t (twice its one-pair
+// count), Myers per symbol plus a fixed part.
+IGC_HD inline uint32_t lv_cost2(int K, bool wide)
+{
+    const uint32_t c16[kLvMaxK + 1] = {11, 16, 30, 44, 70, 94, 130, 164, 210, 254, 310, 364, 430, 494, 570, 644};
+    const uint32_t c32[kLvMaxK + 1] = {8, 11, 25, 39, 64, 89, 124, 159, 204, 249, 305, 359, 425, 489, 565, 639};
+    return wide ? 2u * c32[K] : c16[K];
+}
+
+IGC_HD inline uint32_t myers_cost2(uint32_t n, bool wide) { return wide ? 2u * (15u * n + 10u) : 16u * n + 12u; }
 
 IGC_HD inline PredKind pred_kind(uint32_t n, int K)
 {
     if (K >= kAllPairs || K >= (int)n) return PredKind::ALL;
     const bool wide = n > 16;
-    if (K <= kLvMaxK && lv_cost(K, wide) <= myers_cost(n)) return wide ? PredKind::LV64 : PredKind::LV32;
+    if (K <= kLvMaxK && lv_cost2(K, wide) <= myers_cost2(n, wide)) return PredKind::LV;
     return PredKind::MYERS;
 }
 

@@ -67,11 +67,11 @@ uint64_t initial_edge_cap(uint32_t N, uint32_t n, int K)
     return (uint64_t)est;
 }
 
-TileGeom choose_geometry(uint32_t N, int cpl, uint32_t q)
+TileGeom choose_geometry(uint32_t N, int cols_per_lane, uint32_t q)
 {
     TileGeom G;
     G.N = N;
-    G.W = 256u * (uint32_t)cpl;
+    G.W = 256u * (uint32_t)cols_per_lane;
     G.q = q;
     G.R = G.W / q;
     G.NBr = (N + G.R - 1) / G.R;
@@ -107,14 +107,13 @@ static int size_bucket(uint32_t N)
 }
 
 // Index of the kernel variant a (length, threshold) pair runs, for the table
-// of autotuned shapes: LV on 32-bit words K, LV on 64-bit words 10+K, Myers 20
-// (+1 on 64-bit words), every-pair 22 (+1).
+// of autotuned shapes: Landau-Vishkin on 32-bit words K, on 64-bit words
+// kLvLaunchMaxK+1+K, then Myers and every-pair on each width.
 int pred_variant(uint32_t n, int K)
 {
     const int wide = wide_words(n) ? 1 : 0;
     switch (pred_kind(n, K)) {
-    case PredKind::LV32: return K;
-    case PredKind::LV64: return kLvLaunchMaxK + 1 + K;
+    case PredKind::LV: return wide * (kLvLaunchMaxK + 1) + K;
     case PredKind::MYERS: return 2 * (kLvLaunchMaxK + 1) + wide;
     default: return 2 * (kLvLaunchMaxK + 1) + 2 + wide;
     }
@@ -144,13 +143,13 @@ static int launch_pairs_lv(hipStream_t s, uint64_t tiles, const void *packed, co
         launch_pairs_t<PolLV<k, W>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);     \
         return IGC_OK;
         IGC_CASE(0) IGC_CASE(1) IGC_CASE(2) IGC_CASE(3) IGC_CASE(4) IGC_CASE(5) IGC_CASE(6) IGC_CASE(7)
-        IGC_CASE(8) IGC_CASE(9)
+        IGC_CASE(8) IGC_CASE(9) IGC_CASE(10) IGC_CASE(11) IGC_CASE(12) IGC_CASE(13)
 #undef IGC_CASE
     default:
         return set_error(IGC_ERR_INVALID, "no Landau-Vishkin kernel for K=%d", K);
     }
 }
-static_assert(kLvLaunchMaxK == 9, "launch_pairs_lv instantiates K = 0..9");
+static_assert(kLvLaunchMaxK == 13, "launch_pairs_lv instantiates K = 0..13");
 
 template <int CPL, bool EMIT>
 static int launch_pairs_c(hipStream_t s, uint64_t tiles, const void *packed, const TileGeom &G, uint32_t node_off,
@@ -158,10 +157,9 @@ static int launch_pairs_c(hipStream_t s, uint64_t tiles, const void *packed, con
 {
     const bool wide = wide_words(n);
     switch (pred_kind(n, K)) {
-    case PredKind::LV32:
+    case PredKind::LV:
+        if (wide) return launch_pairs_lv<uint64_t, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);
         return launch_pairs_lv<uint32_t, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);
-    case PredKind::LV64:
-        return launch_pairs_lv<uint64_t, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);
     case PredKind::MYERS:
         if (wide) launch_pairs_t<PolMyers<uint64_t>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);
         else launch_pairs_t<PolMyers<uint32_t>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);
@@ -295,7 +293,7 @@ static int build_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, 
     const int K = effective_k(n, edit_dist);
     const Shape sh = pick_shape(ctx, n, K, N);
     const int cpl = sh.cpl;
-    const TileGeom G = choose_geometry(N, sh.cpl, sh.q);
+    const TileGeom G = choose_geometry(N, sh.cpl * lane_cols(n), sh.q);
     const uint64_t tiles = tile_count(G);
 
     int rc;
@@ -547,7 +545,7 @@ IGC_API int igc_autotune(igc_ctx *ctx, uint32_t n, uint32_t edit_dist, uint32_t 
     uint32_t k = 0;
     for (int cpl : {1, 2, 4}) {
         for (uint32_t q : {1u, 2u, 4u, 8u}) {
-            const TileGeom G = choose_geometry(N, cpl, q);
+            const TileGeom G = choose_geometry(N, cpl * lane_cols(n), q);
             const uint64_t tiles = tile_count(G);
             if (tiles == 0 || tiles > 0x7FFFFFFFull) continue;
             float t_min = 0;

@@ -1,5 +1,11 @@
 // Host build of the GPU pair predicates (indexgen_amd/csrc/igc_lv.hpp) for the
 // CPU tests: the same template code the kernels run, compiled with g++.
+//
+// Each predicate runs on the lane type the kernels use: two pairs per call on
+// 16-bit lanes (n <= 16) or one pair on a 32-bit lane (wide, n <= 32).  For
+// 16-bit lanes every pair (A[r], B[r]) is run twice, once in each lane, with an
+// unrelated column string in the other lane; out[r] is 2 when the lanes
+// disagree.
 #include "../indexgen_amd/csrc/igc_lv.hpp"
 
 #include <stddef.h>
@@ -7,33 +13,43 @@
 
 using namespace igc;
 
-template <int K>
-static void run32(const uint8_t *A, const uint8_t *B, int M, int n, uint8_t *out)
-{
-    for (int r = 0; r < M; ++r) {
-        uint32_t sa[LvShape<K>::WD];
-        row_shifts<K>(pack_word(A + (size_t)r * n, n), sa);
-        out[r] = lv_edge<K>(sa, pack_word(B + (size_t)r * n, n)) ? 1 : 0;
-    }
-}
+namespace {
 
-template <int K>
-static void run64(const uint8_t *A, const uint8_t *B, int M, int n, uint8_t *out)
-{
-    for (int r = 0; r < M; ++r) {
-        uint64_t sa[LvShape<K>::WD];
-        row_shifts<K>(pack_word64(A + (size_t)r * n, n), sa);
-        out[r] = lv_edge<K>(sa, pack_word64(B + (size_t)r * n, n)) ? 1 : 0;
+struct LvPred {
+    template <int K>
+    static void run(const uint8_t *A, const uint8_t *B, int M, int n, int wide, uint8_t *out)
+    {
+        for (int r = 0; r < M; ++r) {
+            const uint8_t *a = A + (size_t)r * n, *b = B + (size_t)r * n;
+            const uint8_t *o = B + (size_t)((r + 1) % M) * n;
+            if (wide) {
+                LvtRow<K, uint32_t> row;
+                lvt_row<K>(pack_word64(a, n), row);
+                uint32_t lo, hi;
+                col_planes(pack_word64(b, n), lo, hi);
+                out[r] = (uint8_t)lvt_test<K, uint32_t>(row, lo, hi);
+            } else {
+                LvtRow<K, u16x2> row;
+                lvt_row<K>(pack_word(a, n), row);
+                u16x2 lo, hi;
+                col_planes(pack_word(b, n), pack_word(o, n), lo, hi);
+                const uint32_t e0 = lvt_test<K, u16x2>(row, lo, hi) & 1u;
+                col_planes(pack_word(o, n), pack_word(b, n), lo, hi);
+                const uint32_t e1 = (lvt_test<K, u16x2>(row, lo, hi) >> 1) & 1u;
+                out[r] = e0 == e1 ? (uint8_t)e0 : 2;
+            }
+        }
     }
-}
+};
 
-// out[r] = (Levenshtein(A[r], B[r]) <= K) by Landau-Vishkin, 0 <= K <= 15, on
-// 32-bit words (n <= 16) or 64-bit words (wide, n <= 32)
+} // namespace
+
+// out[r] = (Levenshtein(A[r], B[r]) <= K) by Landau-Vishkin, 0 <= K <= 15
 extern "C" int lv_pairs(const uint8_t *A, const uint8_t *B, int M, int n, int K, int wide, uint8_t *out)
 {
     if (wide ? n > 32 : n > 16) return -1;
     switch (K) {
-#define C_(k) case k: if (wide) run64<k>(A, B, M, n, out); else run32<k>(A, B, M, n, out); return 0;
+#define C_(k) case k: LvPred::run<k>(A, B, M, n, wide, out); return 0;
         C_(0) C_(1) C_(2) C_(3) C_(4) C_(5) C_(6) C_(7) C_(8) C_(9) C_(10) C_(11) C_(12) C_(13) C_(14) C_(15)
 #undef C_
     }
@@ -46,13 +62,26 @@ extern "C" int myers_pairs(const uint8_t *A, const uint8_t *B, int M, int n, int
     if (wide ? n > 32 : n > 16) return -1;
     for (int r = 0; r < M; ++r) {
         const uint8_t *a = A + (size_t)r * n, *b = B + (size_t)r * n;
-        const bool e = wide ? myers_edge<uint64_t>(pack_word64(a, n), pack_word64(b, n), (uint32_t)n, K)
-                            : myers_edge<uint32_t>(pack_word(a, n), pack_word(b, n), (uint32_t)n, K);
-        out[r] = e ? 1 : 0;
+        const uint8_t *o = B + (size_t)((r + 1) % M) * n;
+        uint32_t tlo, thi;
+        if (wide) {
+            text_planes(pack_word64(a, n), tlo, thi);
+            uint32_t lo, hi;
+            col_planes(pack_word64(b, n), lo, hi);
+            out[r] = (uint8_t)myers_lanes<uint32_t>(tlo, thi, lo, hi, (uint32_t)n, K);
+        } else {
+            text_planes(pack_word(a, n), tlo, thi);
+            u16x2 lo, hi;
+            col_planes(pack_word(b, n), pack_word(o, n), lo, hi);
+            const uint32_t e0 = myers_lanes<u16x2>(tlo, thi, lo, hi, (uint32_t)n, K) & 1u;
+            col_planes(pack_word(o, n), pack_word(b, n), lo, hi);
+            const uint32_t e1 = (myers_lanes<u16x2>(tlo, thi, lo, hi, (uint32_t)n, K) >> 1) & 1u;
+            out[r] = e0 == e1 ? (uint8_t)e0 : 2;
+        }
     }
     return 0;
 }
 
-// the predicate the kernels pick for (n, K): 0 LV32, 1 LV64, 2 Myers, 3 every pair
+// the predicate the kernels pick for (n, K): 0 Landau-Vishkin, 1 Myers, 2 every pair
 extern "C" int pred_kind_of(int n, int K) { return (int)pred_kind((uint32_t)n, K); }
 extern "C" int lv_launch_max_k(void) { return kLvLaunchMaxK; }

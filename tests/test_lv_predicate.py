@@ -1,8 +1,9 @@
 """The kernels' pair predicates (indexgen_amd/csrc/igc_lv.hpp), compiled for the
-host with g++, against the oracle: Landau-Vishkin on 32-bit words (n <= 16) and
-on 64-bit words (n <= 32) and the Myers recurrence on both, exhaustive over all
-pairs of short strings and every threshold, and on random and near pairs up to
-n = 32.  Also the dispatch rule: which predicate runs for (n, K)."""
+host with g++, against the oracle: Landau-Vishkin on thermometer codes and the
+Myers recurrence, each on two-pairs-per-register 16-bit lanes (n <= 16) and on
+32-bit lanes (n <= 32), exhaustive over all pairs of short strings and every
+threshold, and on random and near pairs up to n = 32.  Also the dispatch rule:
+which predicate runs for (n, K)."""
 import ctypes
 import itertools
 import os
@@ -60,7 +61,7 @@ def test_exhaustive(lv, myers, n):
     N = S.shape[0]
     A = np.repeat(S, N, axis=0)
     B = np.tile(S, (N, 1))
-    for K in range(0, n + 2):
+    for K in range(0, min(n + 2, 16)):
         want = O.conflict_pairs(A, B, K + 1)
         for wide in (False, True):
             assert np.array_equal(lv(A, B, K, wide), want), ("lv", n, K, wide)
@@ -139,22 +140,23 @@ def test_oracle_pins_wide_distances():
 
 def test_dispatch_rule(harness):
     """pred_kind: every pair when K >= n; Landau-Vishkin only up to the largest
-    K the kernels are instantiated for; 32-bit Landau-Vishkin only for n <= 16
-    and 64-bit only above; at the benchmark shape (n=13, K=3) Landau-Vishkin."""
+    K the kernels are instantiated for, and for every K up to it at some n;
+    at the benchmark shape (n=13, K=3) Landau-Vishkin."""
     kmax = harness.lv_launch_max_k()
-    seen = set()
+    seen, lv_k = set(), set()
     for n in range(1, 33):
         for K in range(0, 40):
             k = harness.pred_kind_of(n, K)
             seen.add(k)
             if K >= n:
-                assert k == 3, (n, K)
+                assert k == 2, (n, K)
                 continue
-            assert k in (0, 1, 2), (n, K)
-            if k in (0, 1):
+            assert k in (0, 1), (n, K)
+            if k == 0:
                 assert K <= kmax, (n, K)
-                assert (k == 1) == (n > 16), (n, K)
-    assert seen == {0, 1, 2, 3}
+                lv_k.add(K)
+    assert seen == {0, 1, 2}
+    assert lv_k == set(range(kmax + 1))
     assert harness.pred_kind_of(13, 3) == 0
-    assert harness.pred_kind_of(32, 3) == 1
-    assert harness.pred_kind_of(16, 14) == 2
+    assert harness.pred_kind_of(32, 3) == 0
+    assert harness.pred_kind_of(16, 14) == 1
