@@ -235,7 +235,7 @@ def main():
     ctx = igc.Context(0)
     ptr = d_syms.data_ptr()
     tune = ctx.autotune(n, min_ed, N, reps=2) if not args.no_autotune else []
-    shape = ctx.tuned_shape(n, min_ed, N)
+    plan = ctx.pair_plan(n, min_ed, N)
 
     for _ in range(args.warmup):
         ctx.graph_build_device(ptr, N, n, min_ed)
@@ -263,21 +263,22 @@ def main():
     order = igc.greedy_order(adj)
     greedy_ms = (time.perf_counter() - g1) * 1e3
 
-    K = min(min_ed - 1, 16) if min_ed - 1 < n else 16
     kern_ms = float(np.mean(pairs_ms))
     edge_bytes = dcsr.n_edges * 8  # the kernel's compacted (i,j) writes
     # Algorithmic ops per pair as the measurement plan defines them for this
     # stage: one Myers column step per symbol, 17 int ops, so 17*n per pair.
     # This kernel computes the same predicate with fewer instructions (the
-    # Landau-Vishkin recurrence), so the fraction below exceeds 1 by the
-    # algorithmic gain; the physical issue rate comes from the rocprof counters.
+    # Landau-Vishkin recurrence, two pairs per register), so the fraction below
+    # exceeds 1 by the algorithmic gain; the physical issue rate comes from the
+    # rocprof counters.
     myers_ops = 17 * n
     achieved = pairs * myers_ops / (kern_ms / 1e3) / 1e12
     traffic, issued = None, None
     tpath = os.path.join(ROOT, "profiles", "k_pairs_traffic.json")
     if os.path.exists(tpath):
         rec = json.load(open(tpath)).get(args.workload)
-        if rec and (rec.get("cols_per_lane"), rec.get("q")) == tuple(shape):
+        if rec and (rec.get("slots"), rec.get("q"), rec.get("lane_cols")) == (plan["slots"], plan["q"],
+                                                                             plan["lane_cols"]):
             # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE passes
             traffic = int(rec["fetch_size_kib"] * 1024 * rec["fetch_scale"]
                           + rec["write_size_kib"] * 1024 * rec["write_scale"])
@@ -297,9 +298,10 @@ def main():
         "frac": round(achieved / PEAK_VALU_TOPS, 4),
         "traffic": traffic,
         "algorithmic_bytes": int(N * 4 + edge_bytes),
-        "kernel": f"k_pairs<K={K},CPL={shape[0]}>",
-        "tile": {"cols_per_lane": shape[0], "rows_per_tile": 256 * shape[0] // shape[1],
-                 "cols_per_tile": 256 * shape[0]},
+        "kernel": f"k_pairs<{plan['predicate']} K={plan['k']}, {plan['lane_cols']} pairs/register, "
+                  f"{plan['slots']} slots>",
+        "tile": {"cols_per_lane": plan["slots"] * plan["lane_cols"], "rows_per_tile": plan["tile_rows"],
+                 "cols_per_tile": plan["tile_cols"], "workgroups": plan["tiles"]},
         "kernel_ms": round(kern_ms, 4),
         "pairs_per_launch": pairs,
         "algorithmic_ops_per_pair": myers_ops,

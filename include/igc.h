@@ -122,8 +122,10 @@ IGC_API int igc_graph_build(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint3
 IGC_API int igc_graph_build_device(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n,
                                    uint32_t edit_dist, igc_csr *out);
 
-/* One timed tile shape of the pair kernel: columns per lane `cpl` (1, 2, 4),
- * rows per tile 256*cpl/q, best time over the repetitions. */
+/* One timed tile shape of the pair kernel: `cpl` register slots per lane
+ * (1, 2, 4; each slot tests lane_cols columns: 2 for n <= 16, 1 above), tiles
+ * of 256*cpl*lane_cols columns and 1/q as many rows, best time over the
+ * repetitions. */
 typedef struct igc_tune_entry {
     int cpl;
     uint32_t q;
@@ -140,6 +142,25 @@ IGC_API int igc_autotune(igc_ctx *ctx, uint32_t n, uint32_t edit_dist, uint32_t 
                          igc_tune_entry *entries, uint32_t cap, uint32_t *count);
 /* Shape the context will use for a set of N strings at (n, edit_dist). */
 IGC_API int igc_tuned_shape(igc_ctx *ctx, uint32_t n, uint32_t edit_dist, uint32_t N, int *cpl, uint32_t *q);
+
+/* The single-set pair launch the context would make for N strings at
+ * (n, edit_dist): which test, how many pairs per register, the tile shape and
+ * the workgroup count.  Introspection for benchmarks and profiling; no
+ * counterpart in the reference. */
+#define IGC_PRED_LANDAU_VISHKIN 0
+#define IGC_PRED_MYERS 1
+#define IGC_PRED_ALL_PAIRS 2
+typedef struct igc_pair_plan {
+    int predicate;      /* IGC_PRED_* */
+    int k;              /* Levenshtein threshold edit_dist - 1 */
+    int lane_cols;      /* pairs per register: 2 (16-bit lanes, n <= 16) or 1 */
+    int slots;          /* registers of columns per lane */
+    uint32_t q;         /* tile_rows = tile_cols / q */
+    uint32_t tile_cols; /* 4 waves x 64 lanes x slots x lane_cols */
+    uint32_t tile_rows;
+    uint64_t tiles;     /* workgroups of the launch (upper triangle) */
+} igc_pair_plan;
+IGC_API int igc_pair_plan_of(igc_ctx *ctx, uint32_t n, uint32_t edit_dist, uint32_t N, igc_pair_plan *out);
 
 /* Counts the edges without materialising them: the verify mode of the
  * reference (VerifyDist, src/Utils.cpp:203-309: zero edges == SUCCESS). */

@@ -135,6 +135,15 @@ class _TuneEntry(ctypes.Structure):
     _fields_ = [("cpl", ctypes.c_int), ("q", ctypes.c_uint32), ("ms", ctypes.c_float), ("gpairs_s", ctypes.c_float)]
 
 
+class _PairPlan(ctypes.Structure):
+    _fields_ = [("predicate", ctypes.c_int), ("k", ctypes.c_int), ("lane_cols", ctypes.c_int), ("slots", ctypes.c_int),
+                ("q", ctypes.c_uint32), ("tile_cols", ctypes.c_uint32), ("tile_rows", ctypes.c_uint32),
+                ("tiles", ctypes.c_uint64)]
+
+
+PREDICATES = {0: "landau_vishkin", 1: "myers", 2: "all_pairs"}
+
+
 class _CodebookInfo(ctypes.Structure):
     _fields_ = [
         ("code_len", ctypes.c_int),
@@ -198,6 +207,7 @@ def lib() -> ctypes.CDLL:
         "igc_server_destroy": (i, [vp]),
         "igc_autotune": (i, [vp, u32, u32, u32, u32, ctypes.POINTER(_TuneEntry), u32, ctypes.POINTER(u32)]),
         "igc_tuned_shape": (i, [vp, u32, u32, u32, ctypes.POINTER(i), ctypes.POINTER(u32)]),
+        "igc_pair_plan_of": (i, [vp, u32, u32, u32, ctypes.POINTER(_PairPlan)]),
         "igc_valu_probe": (i, [vp, vp]),
         "igc_last_error": (ctypes.c_char_p, []),
         "igc_backend": (ctypes.c_char_p, []),
@@ -444,6 +454,13 @@ class Context:
         cpl, q = ctypes.c_int(), ctypes.c_uint32()
         _check(lib().igc_tuned_shape(self._h, int(n), int(min_ed), int(N), ctypes.byref(cpl), ctypes.byref(q)))
         return cpl.value, q.value
+
+    def pair_plan(self, n: int, min_ed: int, N: int) -> dict:
+        """The pair launch this context makes for N strings at (n, min_ed)."""
+        p = _PairPlan()
+        _check(lib().igc_pair_plan_of(self._h, int(n), int(min_ed), int(N), ctypes.byref(p)))
+        return {"predicate": PREDICATES[p.predicate], "k": p.k, "lane_cols": p.lane_cols, "slots": p.slots,
+                "q": p.q, "tile_cols": p.tile_cols, "tile_rows": p.tile_rows, "tiles": int(p.tiles)}
 
     def valu_probe(self) -> dict:
         """Wave-level VALU instructions per second for each int32 op type of the pair kernel."""

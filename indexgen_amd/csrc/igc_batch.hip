@@ -119,7 +119,7 @@ __global__ __launch_bounds__(256) void k_pairs_batch(const uint8_t *__restrict__
     for (;;) {
         if (threadIdx.x == 0) next = atomicAdd(work, 1ull);
         __syncthreads();
-        const uint64_t t = next;
+        const uint64_t t = uniform64(next);
         __syncthreads();
         if (t >= total_tiles) break;
         const uint32_t s = find_set_by(sets, nsets, t, false);

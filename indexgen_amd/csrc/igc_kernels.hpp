@@ -178,8 +178,10 @@ __device__ __forceinline__ void decode_tile(uint64_t t, const TileGeom &G, uint3
     while (g + 1 < (int64_t)G.NBc && tiles_before_group((uint64_t)g + 1, G) <= t) ++g;
     const uint64_t tp = t - tiles_before_group((uint64_t)g, G);
     const uint32_t per = G.NBc - (uint32_t)g;
-    rb = (uint32_t)g * G.q + (uint32_t)(tp / per);
-    cb = (uint32_t)g + (uint32_t)(tp % per);
+    // the double-precision estimate runs on the vector unit; the result is
+    // uniform, so move it to scalar registers (the row loop bounds follow it)
+    rb = __builtin_amdgcn_readfirstlane((uint32_t)g * G.q + (uint32_t)(tp / per));
+    cb = __builtin_amdgcn_readfirstlane((uint32_t)g + (uint32_t)(tp % per));
 }
 
 __device__ __forceinline__ uint32_t lane_id()
@@ -241,7 +243,8 @@ __device__ __forceinline__ void wave_tile(const typename Pol::Word *__restrict__
     const uint32_t colend = min(colw + 64u * SL, N); // exclusive
     // rows i with some column j > i in this wave: i < colend - 1
     if (colend - 1u <= row0) return;
-    const uint32_t rows = min(min(G.R, N - row0), colend - 1u - row0);
+    // uniform; kept in a scalar register so the row loop's exit test is scalar
+    const uint32_t rows = __builtin_amdgcn_readfirstlane(min(min(G.R, N - row0), colend - 1u - row0));
 
     typename Pol::Col col[CPL];
 #pragma unroll

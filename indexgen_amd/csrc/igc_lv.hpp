@@ -222,7 +222,14 @@ IGC_HD inline uint32_t lvt_test(const LvtRow<K, V> &r, V blo, V bhi)
 {
     V m[LvShape<K>::WD];
 IGC_UNROLL
-    for (int t = 0; t < LvShape<K>::WD; ++t) m[t] = (r.lo[t] ^ blo) | (r.hi[t] ^ bhi);
+    for (int t = 0; t < LvShape<K>::WD; ++t) {
+        m[t] = (r.lo[t] ^ blo) | (r.hi[t] ^ bhi);
+#if defined(__HIP_DEVICE_COMPILE__)
+        // keep each mask in one register: without this the compiler folds the
+        // OR into later ANDs and recomputes one plane's XOR per use
+        asm volatile("" : "+v"(m[t]));
+#endif
+    }
     return zero_lanes(lvt_residue<K, V>(m));
 }
 
@@ -286,8 +293,8 @@ enum class PredKind : int { LV = 0, MYERS = 1, ALL = 2 };
 // count), Myers per symbol plus a fixed part.
 IGC_HD inline uint32_t lv_cost2(int K, bool wide)
 {
-    const uint32_t c16[kLvMaxK + 1] = {11, 16, 30, 44, 70, 94, 130, 164, 210, 254, 310, 364, 430, 494, 570, 644};
-    const uint32_t c32[kLvMaxK + 1] = {8, 11, 25, 39, 64, 89, 124, 159, 204, 249, 305, 359, 425, 489, 565, 639};
+    const uint32_t c16[kLvMaxK + 1] = {11, 15, 29, 42, 67, 90, 125, 158, 203, 246, 301, 354, 419, 482, 557, 630};
+    const uint32_t c32[kLvMaxK + 1] = {7, 11, 24, 38, 62, 86, 120, 154, 198, 242, 296, 349, 414, 477, 552, 625};
     return wide ? 2u * c32[K] : c16[K];
 }
 
