@@ -290,18 +290,23 @@ static int batch_device_locked(igc_ctx *ctx, const uint8_t *d_syms, const BatchP
             launches++;
         }
         HIP_TRY(hipEventRecord(ctx->ev[2], s));
-        if ((rc = launch_csr(ctx, P.total, cap, counter, misc, &launches))) return rc;
+        uint32_t host_misc[kMiscWords];
+        HIP_TRY(hipMemcpyAsync(host_misc, misc, sizeof host_misc, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (host_misc[2]) return set_error(IGC_ERR_INVALID, "symbol code above 3 in input");
+        const uint64_t E = (uint64_t)host_misc[0] | ((uint64_t)host_misc[1] << 32);
+        if (E > cap) {
+            cap = E + E / 64 + 1024; // exact regrow, then replay the pairs
+            continue;
+        }
+        if ((rc = launch_csr(ctx, P.total, E, counter, misc, &launches))) return rc;
         hipLaunchKernelGGL(k_gather_bounds, dim3((nd + 1 + 255) / 256), dim3(256), 0, s, ctx->row_ptr.as<uint64_t>(),
                            d_node_order, nd, P.total, ctx->bounds.as<uint64_t>());
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(ctx->ev[3], s));
-        uint32_t host_misc[kMiscWords];
         std::vector<uint64_t> b(nd + 1);
-        HIP_TRY(hipMemcpyAsync(host_misc, misc, sizeof host_misc, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipMemcpyAsync(b.data(), ctx->bounds.p, sizeof(uint64_t) * (nd + 1), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        if (host_misc[2]) return set_error(IGC_ERR_INVALID, "symbol code above 3 in input");
-        const uint64_t E = (uint64_t)host_misc[0] | ((uint64_t)host_misc[1] << 32);
         HIP_TRY(hipEventElapsedTime(&tm.pack_ms, ctx->ev[0], ctx->ev[1]));
         HIP_TRY(hipEventElapsedTime(&tm.pairs_ms, ctx->ev[1], ctx->ev[2]));
         HIP_TRY(hipEventElapsedTime(&tm.csr_ms, ctx->ev[2], ctx->ev[3]));
@@ -309,13 +314,10 @@ static int batch_device_locked(igc_ctx *ctx, const uint8_t *d_syms, const BatchP
         tm.launches = launches + 1;
         tm.retries = (uint32_t)attempt;
         tm.n_edges = E;
-        if (E <= cap) {
-            bounds_out = b; // caller order: the gather walked the node-ordered table
-            *n_edges = E;
-            ctx->last = tm;
-            return IGC_OK;
-        }
-        cap = E + E / 64 + 1024;
+        bounds_out = b; // caller order: the gather walked the node-ordered table
+        *n_edges = E;
+        ctx->last = tm;
+        return IGC_OK;
     }
     return set_error(IGC_ERR_DEVICE, "edge buffer kept overflowing");
 }

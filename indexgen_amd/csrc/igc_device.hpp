@@ -2,6 +2,7 @@
 // libigc.so: the context, grow-only device buffers and the launch helpers.
 #pragma once
 #include "igc_internal.hpp"
+#include "igc_csr.hpp"
 #include "igc_kernels.hpp"
 
 #include <hip/hip_runtime.h>
@@ -74,7 +75,7 @@ struct HostBuf {
 };
 
 // misc buffer layout (uint32 words): [0..1] edge counter (u64), [2] bad symbol
-// flag, [4] mid-row count, [5] big-row count, [6..7] batch work counter (u64)
+// flag, [6..7] batch work counter (u64)
 constexpr size_t kMiscWords = 8;
 
 // Tile shape of the single-set kernel: column slots per lane (each slot one
@@ -115,7 +116,9 @@ struct igc_ctx {
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     std::mutex mu;
-    igc::DevBuf syms, packed, edges, misc, deg, cursor, row_ptr, col, mid, big, bsum, sets, bounds;
+    igc::DevBuf syms, packed, edges, misc, row_ptr, col, bsum, sets, bounds;
+    igc::DevBuf keys[2];         // radix-sort ping-pong buffers (2 entries per edge)
+    igc::DevBuf rcount, roffset; // per-tile digit counts and their scan
     igc::HostBuf host_rp, host_col;
     hipEvent_t ev[5] = {};
     igc_timing last{};

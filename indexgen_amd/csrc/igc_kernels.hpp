@@ -311,24 +311,7 @@ __global__ __launch_bounds__(256) void k_pairs(const typename Pol::Word *__restr
 }
 
 // ---------------------------------------------------------------------------
-// CSR assembly
-
-__device__ __forceinline__ uint64_t edge_total(const unsigned long long *counter, uint64_t cap)
-{
-    const uint64_t e = (uint64_t)*counter;
-    return e < cap ? e : cap;
-}
-
-static __global__ void k_degree(const uint64_t *__restrict__ edges, const unsigned long long *__restrict__ counter,
-                         uint64_t cap, uint32_t *__restrict__ deg)
-{
-    const uint64_t E = edge_total(counter, cap);
-    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t v = edges[e];
-        atomicAdd(&deg[(uint32_t)(v >> 32)], 1u);
-        atomicAdd(&deg[(uint32_t)v], 1u);
-    }
-}
+// exclusive u64 scan of a u32 array (the radix sort's digit counts)
 
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
@@ -407,148 +390,6 @@ static __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32
             row_ptr[base + k] = run;
             run += v[k];
             if (base + k == (uint64_t)N - 1) row_ptr[N] = run;
-        }
-    }
-}
-
-static __global__ void k_scatter(const uint64_t *__restrict__ edges, const unsigned long long *__restrict__ counter,
-                          uint64_t cap, const uint64_t *__restrict__ row_ptr, uint32_t *__restrict__ cursor,
-                          uint32_t *__restrict__ col)
-{
-    const uint64_t E = edge_total(counter, cap);
-    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t v = edges[e];
-        const uint32_t i = (uint32_t)(v >> 32), j = (uint32_t)v;
-        const uint32_t p = atomicAdd(&cursor[i], 1u);
-        col[row_ptr[i] + p] = j;
-        const uint32_t q = atomicAdd(&cursor[j], 1u);
-        col[row_ptr[j] + q] = i;
-    }
-}
-
-// Row sort, short rows: one wave per row, bitonic network across lanes for
-// rows of <= 64 entries; longer rows are queued for the block kernels.
-constexpr uint32_t kMidMax = 4096;
-
-static __global__ __launch_bounds__(256) void k_sort_short(const uint64_t *__restrict__ row_ptr, uint32_t N,
-                                                     uint32_t *__restrict__ col, uint32_t *__restrict__ mid_list,
-                                                     uint32_t *__restrict__ big_list, uint32_t *__restrict__ counts)
-{
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave_global = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
-    for (uint32_t x = wave_global; x < N; x += waves) {
-        const uint64_t s = row_ptr[x], len = row_ptr[x + 1] - s;
-        if (len <= 1) continue;
-        if (len > 64) {
-            if (lane == 0) {
-                if (len <= kMidMax) mid_list[atomicAdd(&counts[0], 1u)] = x;
-                else big_list[atomicAdd(&counts[1], 1u)] = x;
-            }
-            continue;
-        }
-        uint32_t v = lane < len ? col[s + lane] : 0xFFFFFFFFu;
-        for (uint32_t k = 2; k <= 64; k <<= 1) {
-            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                const uint32_t o = __shfl_xor(v, (int)j, 64);
-                const bool up = (lane & k) == 0;
-                const bool low = (lane & j) == 0;
-                const uint32_t mn = v < o ? v : o, mx = v < o ? o : v;
-                v = (low == up) ? mn : mx;
-            }
-        }
-        if (lane < len) col[s + lane] = v;
-    }
-}
-
-// Rows of 65..4096 entries: one workgroup per row, bitonic sort in LDS.
-static __global__ __launch_bounds__(256) void k_sort_mid(const uint64_t *__restrict__ row_ptr, uint32_t *__restrict__ col,
-                                                   const uint32_t *__restrict__ mid_list,
-                                                   const uint32_t *__restrict__ counts)
-{
-    __shared__ uint32_t sh[kMidMax];
-    const uint32_t nrows = counts[0];
-    for (uint32_t w = blockIdx.x; w < nrows; w += gridDim.x) {
-        const uint32_t x = mid_list[w];
-        const uint64_t s = row_ptr[x];
-        const uint32_t len = (uint32_t)(row_ptr[x + 1] - s);
-        uint32_t P = 64;
-        while (P < len) P <<= 1;
-        for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) sh[t] = t < len ? col[s + t] : 0xFFFFFFFFu;
-        __syncthreads();
-        for (uint32_t k = 2; k <= P; k <<= 1) {
-            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) {
-                    const uint32_t u = t ^ j;
-                    if (u > t) {
-                        const uint32_t a = sh[t], b = sh[u];
-                        const bool up = (t & k) == 0;
-                        if ((a > b) == up) {
-                            sh[t] = b;
-                            sh[u] = a;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-        }
-        for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) col[s + t] = sh[t];
-        __syncthreads();
-    }
-}
-
-// Rows over 4096 entries: values are distinct node ids < N, so a bitmap over
-// value chunks of 2^19 ids in LDS, read back in order, sorts the row.
-constexpr uint32_t kBigChunkWords = 16384; // 64 KiB of LDS = 2^19 ids
-
-static __global__ __launch_bounds__(256) void k_sort_big(const uint64_t *__restrict__ row_ptr, uint32_t N,
-                                                   uint32_t *__restrict__ col, uint32_t *__restrict__ tmp,
-                                                   const uint32_t *__restrict__ big_list,
-                                                   const uint32_t *__restrict__ counts)
-{
-    __shared__ uint32_t bm[kBigChunkWords];
-    __shared__ uint32_t wsum[256];
-    const uint32_t nrows = counts[1];
-    constexpr uint32_t per = kBigChunkWords / 256; // words owned by each thread in the read-back
-    for (uint32_t w = blockIdx.x; w < nrows; w += gridDim.x) {
-        const uint32_t x = big_list[w];
-        const uint64_t s = row_ptr[x];
-        const uint32_t len = (uint32_t)(row_ptr[x + 1] - s);
-        // stage the row in tmp (same offsets) so the writes below cannot race the reads
-        for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) tmp[s + t] = col[s + t];
-        __syncthreads();
-        uint32_t written = 0;
-        for (uint64_t base = 0; base < N; base += (uint64_t)kBigChunkWords * 32) {
-            for (uint32_t t = threadIdx.x; t < kBigChunkWords; t += blockDim.x) bm[t] = 0;
-            __syncthreads();
-            for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) {
-                const uint64_t v = tmp[s + t];
-                if (v >= base && v < base + (uint64_t)kBigChunkWords * 32) {
-                    const uint32_t o = (uint32_t)(v - base);
-                    atomicOr(&bm[o >> 5], 1u << (o & 31));
-                }
-            }
-            __syncthreads();
-            uint32_t mine = 0;
-            for (uint32_t k = 0; k < per; ++k) mine += __popc(bm[threadIdx.x * per + k]);
-            wsum[threadIdx.x] = mine;
-            __syncthreads();
-            uint32_t before = 0, total = 0;
-            for (uint32_t t = 0; t < 256; ++t) {
-                if (t < threadIdx.x) before += wsum[t];
-                total += wsum[t];
-            }
-            uint32_t out = written + before;
-            for (uint32_t k = 0; k < per; ++k) {
-                uint32_t word = bm[threadIdx.x * per + k];
-                while (word) {
-                    const uint32_t bit = (uint32_t)__builtin_ctz(word);
-                    word &= word - 1;
-                    col[s + out++] = (uint32_t)(base + (uint64_t)(threadIdx.x * per + k) * 32 + bit);
-                }
-            }
-            written += total;
-            __syncthreads();
         }
     }
 }

@@ -42,10 +42,10 @@ template <int OP>
 static int probe_one(igc_ctx *ctx, uint32_t blocks, float *ms_out)
 {
     hipStream_t s = ctx->stream;
-    hipLaunchKernelGGL(k_valu_probe<OP>, dim3(blocks), dim3(256), 0, s, ctx->deg.as<uint32_t>(), 7u);
+    hipLaunchKernelGGL(k_valu_probe<OP>, dim3(blocks), dim3(256), 0, s, ctx->rcount.as<uint32_t>(), 7u);
     HIP_TRY(hipEventRecord(ctx->ev[0], s));
     for (int r = 0; r < 5; ++r)
-        hipLaunchKernelGGL(k_valu_probe<OP>, dim3(blocks), dim3(256), 0, s, ctx->deg.as<uint32_t>(), 8u + r);
+        hipLaunchKernelGGL(k_valu_probe<OP>, dim3(blocks), dim3(256), 0, s, ctx->rcount.as<uint32_t>(), 8u + r);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[1], s));
     HIP_TRY(hipEventSynchronize(ctx->ev[1]));
@@ -66,7 +66,7 @@ extern "C" IGC_API int igc_valu_probe(igc_ctx *ctx, double *wave_instr_per_s /* 
     HIP_TRY(hipSetDevice(ctx->device));
     const uint32_t blocks = (uint32_t)ctx->cu_count * 32;
     int rc;
-    if ((rc = ctx->deg.ensure(sizeof(uint32_t) * (size_t)blocks * 256))) return rc;
+    if ((rc = ctx->rcount.ensure(sizeof(uint32_t) * (size_t)blocks * 256))) return rc;
     float ms[7];
     if ((rc = probe_one<0>(ctx, blocks, &ms[0])) || (rc = probe_one<1>(ctx, blocks, &ms[1])) ||
         (rc = probe_one<2>(ctx, blocks, &ms[2])) || (rc = probe_one<3>(ctx, blocks, &ms[3])) ||

@@ -1,9 +1,9 @@
 // igc_runtime.hip -- device orchestration and C ABI of libigc.so (gfx950).
 //
-// One build = pack -> pairs -> degree -> scan -> scatter -> row sort, all on the
-// context's stream with no host synchronisation until the final read of the
-// edge count.  If the edge buffer overflowed (the count is exact even then),
-// the buffer is regrown to the exact count and the build is replayed.
+// One build = pack -> pairs -> (read of the exact edge count) -> CSR assembly by
+// radix sort, all on the context's stream.  If the edge buffer overflowed (the
+// count is exact even then), the buffer is regrown to the exact count and the
+// pairs are replayed before any CSR work.
 #include "igc_device.hpp"
 
 #include <algorithm>
@@ -228,44 +228,57 @@ static int check_args(igc_ctx *ctx, uint32_t N, uint32_t n)
     return IGC_OK;
 }
 
-// CSR of the edge list: degree -> scan -> scatter -> sort, sized by N and cap so
-// no host round trip is needed before the final count.
+// CSR of the edge list by LSD radix sort of the 2E directed entries (igc_csr.hpp),
+// sized by N and cap so no host round trip is needed before the final count.
 int launch_csr(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *counter, uint32_t *misc,
                uint32_t *launches)
 {
+    (void)misc;
     hipStream_t s = ctx->stream;
     int rc;
-    const uint32_t nsb = (uint32_t)(((uint64_t)N + kScanTile - 1) / kScanTile);
-    if ((rc = ctx->deg.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
-    if ((rc = ctx->cursor.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
+    uint32_t bsh = 1;
+    while (bsh < 32 && ((uint64_t)1 << bsh) < (uint64_t)N) ++bsh; // node ids < 2^bsh
+    const uint32_t passes = (2 * bsh + kRadixBits - 1) / kRadixBits;
+    const uint64_t items_cap = 2 * std::max<uint64_t>(cap, 1);
+    const uint64_t tiles64 = (items_cap + kRadixTile - 1) / kRadixTile;
+    if (tiles64 * kRadixDigits > 0xFFFFFFFFull)
+        return set_error(IGC_ERR_INVALID, "edge buffer of %llu entries is too large", (unsigned long long)cap);
+    const uint32_t tiles = (uint32_t)tiles64;
+    const uint32_t L = tiles * kRadixDigits; // digit-major counts
+    const uint32_t nsb = (uint32_t)(((uint64_t)L + kScanTile - 1) / kScanTile);
     if ((rc = ctx->row_ptr.ensure(sizeof(uint64_t) * ((size_t)N + 1)))) return rc;
-    if ((rc = ctx->col.ensure(sizeof(uint32_t) * 2 * (size_t)cap))) return rc;
+    if ((rc = ctx->col.ensure(sizeof(uint32_t) * items_cap))) return rc;
+    if ((rc = ctx->keys[0].ensure(sizeof(uint64_t) * items_cap))) return rc;
+    if (passes > 1 && (rc = ctx->keys[1].ensure(sizeof(uint64_t) * items_cap))) return rc;
+    if ((rc = ctx->rcount.ensure(sizeof(uint32_t) * (size_t)L))) return rc;
+    if ((rc = ctx->roffset.ensure(sizeof(uint64_t) * ((size_t)L + 1)))) return rc;
     if ((rc = ctx->bsum.ensure(sizeof(uint64_t) * (size_t)nsb))) return rc;
-    if ((rc = ctx->mid.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
-    if ((rc = ctx->big.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
-    const uint32_t fillg = grid_for(N, 256, 8192);
-    hipLaunchKernelGGL(k_fill_u32, dim3(fillg), dim3(256), 0, s, ctx->deg.as<uint32_t>(), (uint64_t)N, 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(fillg), dim3(256), 0, s, ctx->cursor.as<uint32_t>(), (uint64_t)N, 0u);
-    const uint32_t eg = grid_for(cap, 256, (uint32_t)ctx->cu_count * 16);
-    hipLaunchKernelGGL(k_degree, dim3(eg), dim3(256), 0, s, ctx->edges.as<uint64_t>(), counter, cap,
-                       ctx->deg.as<uint32_t>());
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(kScanThreads), 0, s, ctx->deg.as<uint32_t>(), N,
-                       ctx->bsum.as<uint64_t>());
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, s, ctx->bsum.as<uint64_t>(), nsb);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(kScanThreads), 0, s, ctx->deg.as<uint32_t>(), N,
-                       ctx->bsum.as<uint64_t>(), ctx->row_ptr.as<uint64_t>());
-    hipLaunchKernelGGL(k_scatter, dim3(eg), dim3(256), 0, s, ctx->edges.as<uint64_t>(), counter, cap,
-                       ctx->row_ptr.as<uint64_t>(), ctx->cursor.as<uint32_t>(), ctx->col.as<uint32_t>());
-    hipLaunchKernelGGL(k_sort_short, dim3(grid_for((uint64_t)N * 64, 256, 8192)), dim3(256), 0, s,
-                       ctx->row_ptr.as<uint64_t>(), N, ctx->col.as<uint32_t>(), ctx->mid.as<uint32_t>(),
-                       ctx->big.as<uint32_t>(), misc + 4);
-    hipLaunchKernelGGL(k_sort_mid, dim3((uint32_t)ctx->cu_count * 4), dim3(256), 0, s, ctx->row_ptr.as<uint64_t>(),
-                       ctx->col.as<uint32_t>(), ctx->mid.as<uint32_t>(), misc + 4);
-    hipLaunchKernelGGL(k_sort_big, dim3((uint32_t)ctx->cu_count * 2), dim3(256), 0, s, ctx->row_ptr.as<uint64_t>(),
-                       N, ctx->col.as<uint32_t>(), ctx->edges.as<uint32_t>() /* edge list is dead after the scatter */,
-                       ctx->big.as<uint32_t>(), misc + 4);
+    uint32_t *counts = ctx->rcount.as<uint32_t>();
+    uint64_t *offsets = ctx->roffset.as<uint64_t>();
+    for (uint32_t pass = 0; pass < passes; ++pass) {
+        RadixPass rp;
+        rp.keys = pass ? ctx->keys[(pass - 1) & 1].as<uint64_t>() : nullptr;
+        rp.edges = ctx->edges.as<uint64_t>();
+        rp.cnt = counter;
+        rp.cap = cap;
+        rp.bsh = bsh;
+        rp.shift = pass * kRadixBits;
+        rp.from_edges = pass == 0;
+        const bool last = pass + 1 == passes;
+        hipLaunchKernelGGL(k_radix_count, dim3(tiles), dim3(kRadixThreads), 0, s, rp, tiles, counts);
+        hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(kScanThreads), 0, s, counts, L, ctx->bsum.as<uint64_t>());
+        hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, s, ctx->bsum.as<uint64_t>(), nsb);
+        hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(kScanThreads), 0, s, counts, L, ctx->bsum.as<uint64_t>(),
+                           offsets);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kRadixThreads), 0, s, rp, tiles, offsets,
+                           (uint32_t)last, last ? nullptr : ctx->keys[pass & 1].as<uint64_t>(),
+                           ctx->col.as<uint32_t>(), ctx->keys[pass & 1].as<uint32_t>() /* src, last pass */);
+        *launches += 5;
+    }
+    hipLaunchKernelGGL(k_csr_row_ptr, dim3(grid_for((uint64_t)N + 1, 256, 8192)), dim3(256), 0, s,
+                       ctx->keys[(passes - 1) & 1].as<uint32_t>(), counter, cap, N, ctx->row_ptr.as<uint64_t>());
     HIP_TRY(hipGetLastError());
-    *launches += 11;
+    *launches += 1;
     return IGC_OK;
 }
 
@@ -315,16 +328,23 @@ static int build_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, 
                           count_only ? nullptr : ctx->edges.as<uint64_t>(), cap, counter);
         if (rc) return rc;
         HIP_TRY(hipEventRecord(ctx->ev[2], s));
-        uint32_t launches = 3;
-        if (!count_only) {
-            if ((rc = launch_csr(ctx, N, cap, counter, misc, &launches))) return rc;
-        }
-        HIP_TRY(hipEventRecord(ctx->ev[3], s));
+        // exact edge count: sizes the CSR sort, and an overflowed attempt is
+        // replayed without assembling anything
         uint32_t host_misc[kMiscWords];
         HIP_TRY(hipMemcpyAsync(host_misc, misc, sizeof host_misc, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         if (host_misc[2]) return set_error(IGC_ERR_INVALID, "symbol code above 3 in input");
-        uint64_t E = (uint64_t)host_misc[0] | ((uint64_t)host_misc[1] << 32);
+        const uint64_t E = (uint64_t)host_misc[0] | ((uint64_t)host_misc[1] << 32);
+        if (!count_only && E > cap) {
+            cap = E + E / 64 + 1024; // exact regrow, then replay
+            continue;
+        }
+        uint32_t launches = 3;
+        if (!count_only) {
+            if ((rc = launch_csr(ctx, N, E, counter, misc, &launches))) return rc;
+        }
+        HIP_TRY(hipEventRecord(ctx->ev[3], s));
+        HIP_TRY(hipEventSynchronize(ctx->ev[3]));
         HIP_TRY(hipEventElapsedTime(&tm.pack_ms, ctx->ev[0], ctx->ev[1]));
         HIP_TRY(hipEventElapsedTime(&tm.pairs_ms, ctx->ev[1], ctx->ev[2]));
         HIP_TRY(hipEventElapsedTime(&tm.csr_ms, ctx->ev[2], ctx->ev[3]));
@@ -332,12 +352,9 @@ static int build_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, 
         tm.launches = launches;
         tm.retries = (uint32_t)attempt;
         tm.n_edges = E;
-        if (count_only || E <= cap) {
-            *n_edges_out = E;
-            ctx->last = tm;
-            return IGC_OK;
-        }
-        cap = E + E / 64 + 1024; // exact regrow, then replay
+        *n_edges_out = E;
+        ctx->last = tm;
+        return IGC_OK;
     }
     return set_error(IGC_ERR_DEVICE, "edge buffer kept overflowing");
 }
@@ -402,8 +419,8 @@ IGC_API int igc_ctx_destroy(igc_ctx *ctx)
         std::lock_guard<std::mutex> lk(ctx->mu);
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
-        for (DevBuf *b : {&ctx->syms, &ctx->packed, &ctx->edges, &ctx->misc, &ctx->deg, &ctx->cursor, &ctx->row_ptr,
-                          &ctx->col, &ctx->mid, &ctx->big, &ctx->bsum})
+        for (DevBuf *b : {&ctx->syms, &ctx->packed, &ctx->edges, &ctx->misc, &ctx->row_ptr, &ctx->col, &ctx->keys[0],
+                          &ctx->keys[1], &ctx->rcount, &ctx->roffset, &ctx->bsum, &ctx->sets, &ctx->bounds})
             b->release();
         kmeans_release(ctx);
         for (auto &e : ctx->ev)

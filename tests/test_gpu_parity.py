@@ -307,7 +307,7 @@ def test_server_results_same_alone_and_in_the_mix(gpu_ctx):
 def test_server_admission_budget(gpu_ctx):
     rng = np.random.default_rng(41)
     big = W.filtered_uniform(rng, 20000, 12)
-    srv = igc.Server(0, hbm_budget_bytes=1 << 20, pipelines=1, host_threads=2)   # 1 MiB
+    srv = igc.Server(0, hbm_budget_bytes=6 << 20, pipelines=1, host_threads=2)   # 6 MiB
     try:
         r = srv.wait(srv.submit(big, 4))                      # can never fit: refused
         assert r["status"] == igc.IGC_ERR_ADMISSION
