@@ -119,7 +119,7 @@ struct igc_ctx {
     igc::DevBuf syms, packed, edges, misc, row_ptr, col, bsum, sets, bounds;
     igc::DevBuf keys[2];         // radix-sort ping-pong buffers (2 entries per edge)
     igc::DevBuf rcount, roffset; // per-tile digit counts and their scan
-    igc::HostBuf host_rp, host_col;
+    igc::HostBuf host_rp, host_col, stage; // pinned: batch CSR copies, staged copies
     hipEvent_t ev[5] = {};
     igc_timing last{};
     int cu_count = 256;

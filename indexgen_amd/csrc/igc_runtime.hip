@@ -422,6 +422,7 @@ IGC_API int igc_ctx_destroy(igc_ctx *ctx)
         for (DevBuf *b : {&ctx->syms, &ctx->packed, &ctx->edges, &ctx->misc, &ctx->row_ptr, &ctx->col, &ctx->keys[0],
                           &ctx->keys[1], &ctx->rcount, &ctx->roffset, &ctx->bsum, &ctx->sets, &ctx->bounds})
             b->release();
+        for (HostBuf *b : {&ctx->host_rp, &ctx->host_col, &ctx->stage}) b->release();
         kmeans_release(ctx);
         for (auto &e : ctx->ev)
             if (e) (void)hipEventDestroy(e);
@@ -481,6 +482,37 @@ IGC_API int igc_csr_free(igc_csr *csr)
     return IGC_OK;
 }
 
+// Device-to-host copy into pageable memory through two pinned staging chunks:
+// the copy engine fills one chunk while the host moves the other out, so the
+// transfer runs at PCIe speed rather than at the pageable-copy rate.
+static int d2h_staged(igc_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    constexpr size_t kChunk = 8u << 20;
+    hipStream_t s = ctx->stream;
+    if (bytes <= kChunk) {
+        HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        return IGC_OK;
+    }
+    int rc = ctx->stage.ensure(2 * kChunk);
+    if (rc) return rc;
+    const size_t chunks = (bytes + kChunk - 1) / kChunk;
+    auto part = [&](size_t c) { return std::min(kChunk, bytes - c * kChunk); };
+    for (size_t c = 0; c <= chunks; ++c) {
+        if (c < chunks) {
+            HIP_TRY(hipMemcpyAsync(ctx->stage.as<uint8_t>() + (c & 1) * kChunk, static_cast<const uint8_t *>(src) + c * kChunk,
+                                   part(c), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipEventRecord(ctx->ev[c & 1], s));
+        }
+        if (c >= 1) {
+            const size_t p = c - 1;
+            HIP_TRY(hipEventSynchronize(ctx->ev[p & 1]));
+            std::memcpy(static_cast<uint8_t *>(dst) + p * kChunk, ctx->stage.as<uint8_t>() + (p & 1) * kChunk, part(p));
+        }
+    }
+    return IGC_OK;
+}
+
 static int csr_to_host_locked(igc_ctx *ctx, uint32_t N, uint64_t E, const uint64_t *d_row_ptr, const uint32_t *d_col,
                               uint32_t flags, igc_csr *host)
 {
@@ -497,12 +529,12 @@ static int csr_to_host_locked(igc_ctx *ctx, uint32_t N, uint64_t E, const uint64
         host->row_ptr[0] = 0;
         return IGC_OK;
     }
-    HIP_TRY(hipMemcpyAsync(host->row_ptr, d_row_ptr, sizeof(uint64_t) * ((size_t)N + 1), hipMemcpyDeviceToHost,
-                           ctx->stream));
-    if (E)
-        HIP_TRY(hipMemcpyAsync(host->col, d_col, sizeof(uint32_t) * 2 * (size_t)E, hipMemcpyDeviceToHost,
-                               ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    int rc = d2h_staged(ctx, host->row_ptr, d_row_ptr, sizeof(uint64_t) * ((size_t)N + 1));
+    if (!rc && E) rc = d2h_staged(ctx, host->col, d_col, sizeof(uint32_t) * 2 * (size_t)E);
+    if (rc) {
+        igc_csr_free(host);
+        return rc;
+    }
     const uint32_t T = (flags >> 16) & 0xFFFFu;
     if (T > 1) igc::reorder_threads(*host, T);
     return IGC_OK;
