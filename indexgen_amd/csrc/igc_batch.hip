@@ -330,8 +330,8 @@ static int split_set(const uint64_t *rp, const uint32_t *col, uint32_t off, uint
     out->n_nodes = N;
     out->location = IGC_ON_HOST;
     out->n_edges = ents / 2;
-    out->row_ptr = static_cast<uint64_t *>(std::malloc(sizeof(uint64_t) * ((size_t)N + 1)));
-    out->col = static_cast<uint32_t *>(std::malloc(sizeof(uint32_t) * std::max<uint64_t>(ents, 1)));
+    out->row_ptr = static_cast<uint64_t *>(host_alloc(sizeof(uint64_t) * ((size_t)N + 1)));
+    out->col = static_cast<uint32_t *>(host_alloc(sizeof(uint32_t) * std::max<uint64_t>(ents, 1)));
     if (!out->row_ptr || !out->col) {
         igc_csr_free(out);
         return set_error(IGC_ERR_OOM, "host CSR allocation failed");

@@ -4,15 +4,17 @@
 // The pair kernels leave E undirected edges (i << 32) | j, i < j, in no order.
 // The CSR wants, for every row r, its neighbours ascending.  Each edge gives two
 // directed entries, keyed src << b | dst with b = bits of N-1, so sorting the
-// 2E keys orders them by row and then by neighbour.  The sort is LSD radix on
-// 8-bit digits, ceil(2b/8) passes, each pass three kernels:
+// 2E keys orders them by row and then by neighbour.  The sort is LSD radix in
+// P = ceil(2b/9) passes of ceil(2b/P)-bit digits (at most 512 digits), each pass
+// three kernels:
 //   k_radix_count    digit histogram of each 4096-entry tile (LDS counters),
 //   k_scan_*         exclusive scan of the digit-major counts (the u64 scan the
 //                    degree array used to take),
 //   k_radix_scatter  stable scatter: within a tile, entries are ranked in input
-//                    order by a wave-level multisplit (8 ballots give each lane
-//                    the set of lanes holding its digit) plus per-wave digit
-//                    counts in LDS.
+//                    order by a wave-level multisplit (one ballot per digit bit
+//                    gives each lane the set of lanes holding its digit) plus per-wave digit
+//                    counts in LDS, placed in digit order in LDS, and written
+//                    out one contiguous digit run after another.
 // Pass 0 reads the edge list directly (entry t is edge t/2, direction t%2).  The
 // last pass writes the neighbour (col) and the row (src) of each entry at its
 // final place instead of the key, and row_ptr[r] is the first position whose
@@ -28,8 +30,9 @@ namespace igc {
 constexpr int kRadixThreads = 256;
 constexpr int kRadixRounds = 16; // entries per thread and tile
 constexpr uint32_t kRadixTile = kRadixThreads * kRadixRounds;
-constexpr int kRadixBits = 8;
-constexpr uint32_t kRadixDigits = 1u << kRadixBits;
+constexpr uint32_t kRadixMaxBits = 9; // digit width of a pass: 1..9 bits
+constexpr uint32_t kRadixMaxDigits = 1u << kRadixMaxBits;
+constexpr uint32_t kDigitsPerThread = kRadixMaxDigits / kRadixThreads;
 
 struct RadixPass {
     const uint64_t *keys;          // input keys (pass > 0)
@@ -38,6 +41,7 @@ struct RadixPass {
     uint64_t cap;                  // edge buffer capacity (entries beyond are ignored)
     uint32_t bsh;                  // bits of a node id in the key
     uint32_t shift;                // digit position of this pass
+    uint32_t bits;                 // digit width of this pass
     uint32_t from_edges;           // pass 0
 };
 
@@ -57,82 +61,147 @@ __device__ __forceinline__ uint64_t radix_key(const RadixPass &p, uint64_t t)
     return p.keys[t];
 }
 
-// counts[d * tiles + b] = entries of tile b with digit d
+__device__ __forceinline__ uint32_t radix_digit(const RadixPass &p, uint64_t key)
+{
+    return (uint32_t)(key >> p.shift) & ((1u << p.bits) - 1u);
+}
+
+// counts[d * tiles + b] = entries of tile b with digit d, d < 2^bits
 static __global__ __launch_bounds__(kRadixThreads) void k_radix_count(RadixPass p, uint32_t tiles,
                                                                        uint32_t *__restrict__ counts)
 {
-    __shared__ uint32_t h[kRadixDigits];
+    __shared__ uint32_t h[kRadixMaxDigits];
     const uint32_t b = blockIdx.x;
-    for (uint32_t d = threadIdx.x; d < kRadixDigits; d += blockDim.x) h[d] = 0;
+    const uint32_t nd = 1u << p.bits;
+    for (uint32_t d = threadIdx.x; d < nd; d += blockDim.x) h[d] = 0;
     __syncthreads();
     const uint64_t n = radix_items(p);
     const uint64_t start = (uint64_t)b * kRadixTile;
     for (int r = 0; r < kRadixRounds; ++r) {
         const uint64_t t = start + (uint64_t)r * kRadixThreads + threadIdx.x;
-        if (t < n) atomicAdd(&h[(uint32_t)(radix_key(p, t) >> p.shift) & (kRadixDigits - 1)], 1u);
+        if (t < n) atomicAdd(&h[radix_digit(p, radix_key(p, t))], 1u);
     }
     __syncthreads();
-    for (uint32_t d = threadIdx.x; d < kRadixDigits; d += blockDim.x) counts[(uint64_t)d * tiles + b] = h[d];
+    for (uint32_t d = threadIdx.x; d < nd; d += blockDim.x) counts[(uint64_t)d * tiles + b] = h[d];
 }
 
-// Stable scatter of tile b.  offsets[d * tiles + b] is where the tile's first
-// entry with digit d goes.  last: write col/src instead of keys.
+// Stable scatter of tile b; offsets[d * tiles + b] is where the tile's first
+// entry with digit d goes.  Wave w owns the tile's entries w*1024 .. w*1024+1023
+// (16 rounds of 64), so input order within the tile is wave-major.  Each wave
+// counts its digits into LDS; one scan over the tile's digits gives every wave
+// its running position per digit; the waves then rank their entries round by
+// round with wave-level operations only (one ballot per digit bit gives each
+// lane the lanes holding its digit), placing the tile in digit order in LDS;
+// finally each digit run is written out contiguously by consecutive threads.
+// last: write col/src instead of keys.
 static __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(RadixPass p, uint32_t tiles,
                                                                          const uint64_t *__restrict__ offsets,
                                                                          uint32_t last, uint64_t *__restrict__ out,
                                                                          uint32_t *__restrict__ col,
                                                                          uint32_t *__restrict__ src)
 {
-    __shared__ uint64_t base[kRadixDigits];
-    __shared__ uint32_t wcnt[kRadixThreads / 64][kRadixDigits];
+    constexpr int kWaves = kRadixThreads / 64;
+    constexpr uint32_t kPerWave = kRadixTile / kWaves;
+    __shared__ uint64_t stage[kRadixTile];
+    __shared__ uint64_t gbase[kRadixMaxDigits];        // global position of the tile's first entry per digit
+    __shared__ uint32_t lstart[kRadixMaxDigits];       // local position of the same
+    __shared__ uint32_t wpos[kWaves][kRadixMaxDigits]; // per wave: digit count, then next local position
+    __shared__ uint32_t wsum[kWaves];
     const uint32_t b = blockIdx.x;
     const uint64_t n = radix_items(p);
     const uint64_t start = (uint64_t)b * kRadixTile;
     if (start >= n) return; // whole workgroup: no barrier is skipped by part of it
+    const uint32_t tile_n = (uint32_t)((n - start) < kRadixTile ? (n - start) : kRadixTile);
+    const uint32_t nd = 1u << p.bits;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (uint32_t d = threadIdx.x; d < kRadixDigits; d += blockDim.x) {
-        base[d] = offsets[(uint64_t)d * tiles + b];
-        for (int w = 0; w < kRadixThreads / 64; ++w) wcnt[w][d] = 0;
+    for (uint32_t d = threadIdx.x; d < kRadixMaxDigits; d += kRadixThreads)
+        for (int w = 0; w < kWaves; ++w) wpos[w][d] = 0;
+    __syncthreads();
+    uint64_t key[kRadixRounds];
+#pragma unroll
+    for (int r = 0; r < kRadixRounds; ++r) {
+        const uint32_t li = wave * kPerWave + (uint32_t)r * 64 + lane;
+        key[r] = li < tile_n ? radix_key(p, start + li) : 0ull;
+        if (li < tile_n) atomicAdd(&wpos[wave][radix_digit(p, key[r])], 1u);
+    }
+    __syncthreads();
+    {
+        // tile counts, their exclusive scan over digits, and each wave's start;
+        // thread t owns digits t*kDigitsPerThread .. +kDigitsPerThread-1
+        uint32_t c[kDigitsPerThread], tsum = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kDigitsPerThread; ++k) {
+            const uint32_t d = threadIdx.x * kDigitsPerThread + k;
+            c[k] = 0;
+            for (int w = 0; w < kWaves; ++w) c[k] += wpos[w][d];
+            gbase[d] = d < nd ? offsets[(uint64_t)d * tiles + b] : 0ull;
+            tsum += c[k];
+        }
+        uint32_t incl = tsum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o, 64);
+            if (lane >= (uint32_t)o) incl += y;
+        }
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        uint32_t run = incl - tsum;
+        for (uint32_t w = 0; w < wave; ++w) run += wsum[w];
+#pragma unroll
+        for (uint32_t k = 0; k < kDigitsPerThread; ++k) {
+            const uint32_t d = threadIdx.x * kDigitsPerThread + k;
+            lstart[d] = run;
+            uint32_t pos = run;
+            for (int w = 0; w < kWaves; ++w) {
+                const uint32_t cw = wpos[w][d];
+                wpos[w][d] = pos;
+                pos += cw;
+            }
+            run += c[k];
+        }
     }
     __syncthreads();
     const uint64_t lt = (1ull << lane) - 1ull; // lanes below this one
-    const uint64_t mask = (1ull << p.bsh) - 1ull;
+    uint32_t *mypos = wpos[wave];
+#pragma unroll
     for (int r = 0; r < kRadixRounds; ++r) {
-        const uint64_t t = start + (uint64_t)r * kRadixThreads + threadIdx.x;
-        const bool valid = t < n;
-        const uint64_t key = valid ? radix_key(p, t) : 0ull;
-        const uint32_t digit = (uint32_t)(key >> p.shift) & (kRadixDigits - 1);
-        // lanes of this wave holding the same digit
+        const uint32_t li = wave * kPerWave + (uint32_t)r * 64 + lane;
+        const bool valid = li < tile_n;
+        const uint32_t digit = radix_digit(p, key[r]);
         uint64_t match = __ballot(valid);
 #pragma unroll
-        for (int bit = 0; bit < kRadixBits; ++bit) {
+        for (uint32_t bit = 0; bit < kRadixMaxBits; ++bit) {
+            if (bit >= p.bits) break; // uniform
             const bool set = (digit >> bit) & 1u;
             const uint64_t bb = __ballot(set);
             match &= set ? bb : ~bb;
         }
         const uint32_t rank = (uint32_t)__popcll(match & lt);
-        if (valid && rank == 0) wcnt[wave][digit] = (uint32_t)__popcll(match);
-        __syncthreads();
+        // every lane reads its digit's position before the group's first lane
+        // advances it (one wave: LDS operations complete in issue order)
+        const uint32_t base = valid ? mypos[digit] : 0u;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
         if (valid) {
-            uint64_t off = base[digit] + rank;
-            for (uint32_t w = 0; w < wave; ++w) off += wcnt[w][digit];
-            if (last) {
-                col[off] = (uint32_t)(key & mask);
-                src[off] = (uint32_t)(key >> p.bsh);
-            } else {
-                out[off] = key;
-            }
+            stage[base + rank] = key[r];
+            if (rank == 0) mypos[digit] = base + (uint32_t)__popcll(match);
         }
-        __syncthreads();
-        for (uint32_t d = threadIdx.x; d < kRadixDigits; d += blockDim.x) {
-            uint32_t s = 0;
-            for (int w = 0; w < kRadixThreads / 64; ++w) {
-                s += wcnt[w][d];
-                wcnt[w][d] = 0;
-            }
-            base[d] += s;
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // global position = gbase[d] + (local position - start of digit d's run)
+    const uint64_t mask = (1ull << p.bsh) - 1ull;
+    for (uint32_t li = threadIdx.x; li < tile_n; li += kRadixThreads) {
+        const uint64_t k = stage[li];
+        const uint32_t digit = radix_digit(p, k);
+        const uint64_t g = gbase[digit] + (li - lstart[digit]);
+        if (last) {
+            col[g] = (uint32_t)(k & mask);
+            src[g] = (uint32_t)(k >> p.bsh);
+        } else {
+            out[g] = k;
         }
-        __syncthreads();
     }
 }
 

@@ -2,6 +2,8 @@
 // order policies and the reference's output formats.  Plain C++17, no device code.
 #include "igc_internal.hpp"
 
+#include <sys/mman.h>
+
 #include <algorithm>
 #include <cerrno>
 #include <cstdio>
@@ -13,6 +15,17 @@
 #include <vector>
 
 namespace igc {
+
+void *host_alloc(size_t bytes)
+{
+    constexpr size_t kHuge = 2u << 20;
+    if (bytes < 2 * kHuge) return std::malloc(bytes);
+    const size_t sz = (bytes + kHuge - 1) / kHuge * kHuge;
+    void *p = std::aligned_alloc(kHuge, sz);
+    if (p) (void)madvise(p, sz, MADV_HUGEPAGE);
+    return p;
+}
+
 
 void reorder_threads(igc_csr &csr, uint32_t T)
 {

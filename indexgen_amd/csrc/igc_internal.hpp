@@ -2,6 +2,7 @@
 #pragma once
 #include "../../include/igc.h"
 
+#include <cstddef>
 #include <cstdint>
 
 namespace igc {
@@ -14,5 +15,11 @@ const char *last_error();
 // the reference's T-thread FillAdjList merge (src/SparseMat.cpp:599-609):
 // within row x, neighbour y is keyed by (min(x,y) mod T, min(x,y), max(x,y)).
 void reorder_threads(igc_csr &csr, uint32_t T);
+
+// Host memory for arrays handed to the caller (released with std::free by
+// igc_csr_free).  Blocks of 4 MiB and more are 2 MiB aligned and advised for
+// transparent huge pages, so filling them takes a few page faults instead of
+// one per 4 KiB page.
+void *host_alloc(size_t bytes);
 
 } // namespace igc

@@ -238,13 +238,15 @@ int launch_csr(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *count
     int rc;
     uint32_t bsh = 1;
     while (bsh < 32 && ((uint64_t)1 << bsh) < (uint64_t)N) ++bsh; // node ids < 2^bsh
-    const uint32_t passes = (2 * bsh + kRadixBits - 1) / kRadixBits;
+    // fewest passes of at most kRadixMaxBits-bit digits, then the narrowest digit
+    const uint32_t passes = (2 * bsh + kRadixMaxBits - 1) / kRadixMaxBits;
+    const uint32_t bits = (2 * bsh + passes - 1) / passes;
     const uint64_t items_cap = 2 * std::max<uint64_t>(cap, 1);
     const uint64_t tiles64 = (items_cap + kRadixTile - 1) / kRadixTile;
-    if (tiles64 * kRadixDigits > 0xFFFFFFFFull)
+    if (tiles64 << bits > 0xFFFFFFFFull)
         return set_error(IGC_ERR_INVALID, "edge buffer of %llu entries is too large", (unsigned long long)cap);
     const uint32_t tiles = (uint32_t)tiles64;
-    const uint32_t L = tiles * kRadixDigits; // digit-major counts
+    const uint32_t L = tiles << bits; // digit-major counts
     const uint32_t nsb = (uint32_t)(((uint64_t)L + kScanTile - 1) / kScanTile);
     if ((rc = ctx->row_ptr.ensure(sizeof(uint64_t) * ((size_t)N + 1)))) return rc;
     if ((rc = ctx->col.ensure(sizeof(uint32_t) * items_cap))) return rc;
@@ -262,7 +264,8 @@ int launch_csr(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *count
         rp.cnt = counter;
         rp.cap = cap;
         rp.bsh = bsh;
-        rp.shift = pass * kRadixBits;
+        rp.shift = pass * bits;
+        rp.bits = std::min(bits, 2 * bsh - rp.shift);
         rp.from_edges = pass == 0;
         const bool last = pass + 1 == passes;
         hipLaunchKernelGGL(k_radix_count, dim3(tiles), dim3(kRadixThreads), 0, s, rp, tiles, counts);
@@ -519,8 +522,8 @@ static int csr_to_host_locked(igc_ctx *ctx, uint32_t N, uint64_t E, const uint64
     host->n_nodes = N;
     host->location = IGC_ON_HOST;
     host->n_edges = E;
-    host->row_ptr = static_cast<uint64_t *>(std::malloc(sizeof(uint64_t) * ((size_t)N + 1)));
-    host->col = static_cast<uint32_t *>(std::malloc(sizeof(uint32_t) * std::max<size_t>(2 * (size_t)E, 1)));
+    host->row_ptr = static_cast<uint64_t *>(host_alloc(sizeof(uint64_t) * ((size_t)N + 1)));
+    host->col = static_cast<uint32_t *>(host_alloc(sizeof(uint32_t) * std::max<size_t>(2 * (size_t)E, 1)));
     if (!host->row_ptr || !host->col) {
         igc_csr_free(host);
         return set_error(IGC_ERR_OOM, "host CSR allocation failed");
