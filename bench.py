@@ -255,6 +255,16 @@ def main():
     ms_per_step = elapsed * 1e3 / args.steps
     value = pairs / (elapsed / args.steps) / 1e9
 
+    # the same builds without the exact row filter, for comparison
+    ctx.set_prefilter(False)
+    ctx.graph_build_device(ptr, N, n, min_ed)
+    unf_ms = []
+    for _ in range(max(2, min(args.steps, 5))):
+        ctx.graph_build_device(ptr, N, n, min_ed)
+        unf_ms.append(ctx.last_timing().pairs_ms)
+    ctx.set_prefilter(True)
+    dcsr = ctx.graph_build_device(ptr, N, n, min_ed)
+
     # outputs: greedy codebook of the last build (host), reported separately
     g0 = time.perf_counter()
     adj = ctx.device_csr_to_host(dcsr)
@@ -330,6 +340,10 @@ def main():
                          "d2h_csr": round(d2h_ms, 2), "greedy_host": round(greedy_ms, 2),
                          "codewords": int(order.size)},
         "roofline": roofline,
+        "prefilter": {"pairs_kernel_ms": round(kern_ms, 4),
+                      "pairs_kernel_ms_without": round(float(np.mean(unf_ms)), 4),
+                      "speedup": round(float(np.mean(unf_ms)) / kern_ms, 3),
+                      "note": "exact row skipping by half-string symbol counts; same graph"},
     }
     if args.serving_jobs > 0:
         result["serving"] = serving_block(args.serving_jobs, host_threads=host_threads())

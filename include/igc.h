@@ -102,6 +102,12 @@ IGC_API int igc_ctx_destroy(igc_ctx *ctx);
  * restores the context's own stream. */
 IGC_API int igc_ctx_set_stream(igc_ctx *ctx, void *hip_stream);
 IGC_API void *igc_ctx_stream(igc_ctx *ctx);
+/* Exact row skipping in single-set builds of 4096 or more strings (default on):
+ * candidates are ordered by the symbol counts of their two halves, and a row is
+ * skipped for a wave's column window when those counts prove no column of the
+ * window is within the threshold.  The graph is the same either way.  No
+ * counterpart in the reference (an internal speed-up of FillAdjListGPU). */
+IGC_API int igc_ctx_set_prefilter(igc_ctx *ctx, int enable);
 IGC_API int igc_last_timing(igc_ctx *ctx, igc_timing *out);
 
 /* ---- single candidate set ---- */

@@ -208,6 +208,7 @@ def lib() -> ctypes.CDLL:
         "igc_autotune": (i, [vp, u32, u32, u32, u32, ctypes.POINTER(_TuneEntry), u32, ctypes.POINTER(u32)]),
         "igc_tuned_shape": (i, [vp, u32, u32, u32, ctypes.POINTER(i), ctypes.POINTER(u32)]),
         "igc_pair_plan_of": (i, [vp, u32, u32, u32, ctypes.POINTER(_PairPlan)]),
+        "igc_ctx_set_prefilter": (i, [vp, i]),
         "igc_valu_probe": (i, [vp, vp]),
         "igc_last_error": (ctypes.c_char_p, []),
         "igc_backend": (ctypes.c_char_p, []),
@@ -454,6 +455,10 @@ class Context:
         cpl, q = ctypes.c_int(), ctypes.c_uint32()
         _check(lib().igc_tuned_shape(self._h, int(n), int(min_ed), int(N), ctypes.byref(cpl), ctypes.byref(q)))
         return cpl.value, q.value
+
+    def set_prefilter(self, enable: bool) -> None:
+        """Exact row skipping by half-string symbol counts (default on; same graph)."""
+        _check(lib().igc_ctx_set_prefilter(self._h, 1 if enable else 0))
 
     def pair_plan(self, n: int, min_ed: int, N: int) -> dict:
         """The pair launch this context makes for N strings at (n, min_ed)."""

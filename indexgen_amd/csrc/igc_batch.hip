@@ -78,20 +78,21 @@ __device__ __forceinline__ void batch_tile(const SetDesc &d, const uint32_t *__r
                                            uint64_t *__restrict__ edges, uint64_t cap,
                                            unsigned long long *__restrict__ counter)
 {
+    const RowFilter none{};
     if (d.pred == kBatchMyers) {
-        wave_tile<PolMyers<uint32_t>, CPL, true>(packed, d.G, rb, cb, d.node_off, d.n, d.K, st, lane, wave, edges,
+        wave_tile<PolMyers<uint32_t>, CPL, true>(packed, d.G, rb, cb, d.node_off, d.n, d.K, none, st, lane, wave, edges,
                                                  cap, counter);
         return;
     }
     if (d.pred == kBatchAll) {
-        wave_tile<PolAll<uint32_t>, CPL, true>(packed, d.G, rb, cb, d.node_off, d.n, d.K, st, lane, wave, edges,
+        wave_tile<PolAll<uint32_t>, CPL, true>(packed, d.G, rb, cb, d.node_off, d.n, d.K, none, st, lane, wave, edges,
                                                cap, counter);
         return;
     }
     switch (d.K) {
 #define IGC_BCASE(k)                                                                                          \
     case k:                                                                                                   \
-        wave_tile<PolLV<k, uint32_t>, CPL, true>(packed, d.G, rb, cb, d.node_off, d.n, d.K, st, lane, wave,     \
+        wave_tile<PolLV<k, uint32_t>, CPL, true>(packed, d.G, rb, cb, d.node_off, d.n, d.K, none, st, lane, wave, \
                                                  edges, cap, counter);                                        \
         break;
         IGC_BCASE(0) IGC_BCASE(1) IGC_BCASE(2) IGC_BCASE(3) IGC_BCASE(4) IGC_BCASE(5) IGC_BCASE(6)

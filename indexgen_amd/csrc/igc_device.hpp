@@ -104,7 +104,8 @@ inline int lane_cols(uint32_t n) { return wide_words(n) ? 1 : 2; }
 // columns per lane, over `tiles` tiles of geometry G; `packed` holds words of
 // the width of n.
 int launch_pairs(uint32_t n, int K, int cpl, bool emit, hipStream_t s, uint64_t tiles, const void *packed,
-                 const TileGeom &G, uint32_t node_off, uint64_t *edges, uint64_t cap, unsigned long long *counter);
+                 const TileGeom &G, uint32_t node_off, uint64_t *edges, uint64_t cap, unsigned long long *counter,
+                 const RowFilter &f = RowFilter{});
 int launch_pack(hipStream_t s, const uint8_t *d_syms, uint32_t N, uint32_t n, void *packed, uint32_t *bad);
 int pred_variant(uint32_t n, int K);
 constexpr int kPredVariants = 2 * (kLvLaunchMaxK + 1) + 4;
@@ -119,6 +120,10 @@ struct igc_ctx {
     igc::DevBuf syms, packed, edges, misc, row_ptr, col, bsum, sets, bounds;
     igc::DevBuf keys[2];         // radix-sort ping-pong buffers (2 entries per edge)
     igc::DevBuf rcount, roffset; // per-tile digit counts and their scan
+    // row filter: half counts (unsorted, sorted), bucket keys, permutation,
+    // sorted packed words, window boxes and thresholds
+    igc::DevBuf feat, feat_s, nkey, perm, packed_s, fbox, fthr;
+    bool prefilter = true; // exact row skipping in single-set builds
     igc::HostBuf host_rp, host_col, stage; // pinned: batch CSR copies, staged copies
     hipEvent_t ev[5] = {};
     igc_timing last{};

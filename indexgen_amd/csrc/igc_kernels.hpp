@@ -28,6 +28,7 @@
 #include <stdint.h>
 
 #include "igc_lv.hpp"
+#include "igc_prefilter.hpp"
 
 namespace igc {
 
@@ -228,11 +229,17 @@ __device__ __forceinline__ void flush_stage(const uint64_t *st, uint32_t staged,
 // (node_off + i, node_off + j).  EMIT=false only counts.  Contains no
 // workgroup barrier: a wave may return early.  n and k are the string length
 // and threshold for the policies that take them at run time.
+//
+// With a RowFilter (single sets), `packed` is in the filter's sorted order, a
+// row is skipped for this wave's whole column window when its half-count
+// distance to the window's box exceeds 2K (igc_prefilter.hpp), and emitted ids
+// are mapped back to the caller's order through the permutation.
 template <class Pol, int CPL, bool EMIT>
 __device__ __forceinline__ void wave_tile(const typename Pol::Word *__restrict__ packed, const TileGeom &G,
                                           uint32_t rb, uint32_t cb, uint32_t node_off, uint32_t n, int k,
-                                          uint64_t *st, uint32_t lane, uint32_t wave, uint64_t *__restrict__ edges,
-                                          uint64_t cap, unsigned long long *__restrict__ counter)
+                                          const RowFilter &flt, uint64_t *st, uint32_t lane, uint32_t wave,
+                                          uint64_t *__restrict__ edges, uint64_t cap,
+                                          unsigned long long *__restrict__ counter)
 {
     constexpr int P = Pol::P;
     constexpr int SL = CPL * P; // columns per lane
@@ -251,40 +258,75 @@ __device__ __forceinline__ void wave_tile(const typename Pol::Word *__restrict__
     for (int c = 0; c < CPL; ++c) col[c].load(packed, colw + 64u * P * c + lane, N);
     uint32_t staged = 0;
     uint64_t counted = 0;
+    uint4 box = make_uint4(0u, 0u, 0u, 0u);
+    uint32_t thr = 0;
+    if (flt.box) {
+        const uint32_t w = colw / flt.window;
+        box = flt.box[w];
+        thr = flt.thr[w];
+    }
 
-    for (uint32_t r = 0; r < rows; ++r) {
-        const uint32_t i = row0 + r;
-        typename Pol::Row row;
-        Pol::row(row, packed[i], n, k);
-        uint64_t mk[SL];
-        uint64_t any = 0;
-#pragma unroll
-        for (int c = 0; c < CPL; ++c) {
-            const uint32_t e = Pol::test(row, col[c], n, k);
-#pragma unroll
-            for (int h = 0; h < P; ++h) {
-                mk[c * P + h] = __ballot((e >> h) & 1u);
-                any |= mk[c * P + h];
+    // rows in chunks of 64: with a filter, lane l tests row r0 + l against the
+    // window's box and one ballot gives the chunk's rows that can conflict
+    for (uint32_t r0 = 0; r0 < rows; r0 += 64) {
+        const uint32_t left = rows - r0;
+        uint64_t todo = left >= 64 ? ~0ull : ((1ull << left) - 1ull);
+        if (flt.box) {
+            bool keep = false;
+            if (lane < left) {
+                const uint2 f = flt.feat[row0 + r0 + lane];
+                uint32_t sad = __builtin_amdgcn_sad_u8(f.x, box.x, 0u);
+                sad = __builtin_amdgcn_sad_u8(f.x, box.z, sad);
+                sad = __builtin_amdgcn_sad_u8(f.y, box.y, sad);
+                sad = __builtin_amdgcn_sad_u8(f.y, box.w, sad);
+                keep = sad <= thr;
             }
+            todo = __ballot(keep);
         }
-        if (any) {
+        // the next row's word is loaded one row ahead (scalar load latency)
+        using Word = typename Pol::Word;
+        Word next = todo ? packed[row0 + r0 + (uint32_t)__builtin_ctzll(todo)] : (Word)0;
+        while (todo) {
+            const uint32_t i = row0 + r0 + (uint32_t)__builtin_ctzll(todo);
+            todo &= todo - 1ull;
+            const Word a = next;
+            if (todo) next = packed[row0 + r0 + (uint32_t)__builtin_ctzll(todo)];
+            typename Pol::Row row;
+            Pol::row(row, a, n, k);
+            uint64_t mk[SL];
+            uint64_t any = 0;
 #pragma unroll
-            for (int sl = 0; sl < SL; ++sl) {
-                if (!mk[sl]) continue;
-                const uint32_t j = colw + 64u * sl + lane;
-                const bool mine = ((mk[sl] >> lane) & 1ull) && j > i && j < N;
-                const uint64_t m2 = __ballot(mine);
-                if (!m2) continue;
-                const uint32_t cnt = (uint32_t)__popcll(m2);
-                if constexpr (EMIT) {
-                    if (staged + cnt > (uint32_t)kStage) {
-                        flush_stage(st, staged, lane, edges, cap, counter);
-                        staged = 0;
+            for (int c = 0; c < CPL; ++c) {
+                const uint32_t e = Pol::test(row, col[c], n, k);
+#pragma unroll
+                for (int h = 0; h < P; ++h) {
+                    mk[c * P + h] = __ballot((e >> h) & 1u);
+                    any |= mk[c * P + h];
+                }
+            }
+            if (any) {
+#pragma unroll
+                for (int sl = 0; sl < SL; ++sl) {
+                    if (!mk[sl]) continue;
+                    const uint32_t j = colw + 64u * sl + lane;
+                    const bool mine = ((mk[sl] >> lane) & 1ull) && j > i && j < N;
+                    const uint64_t m2 = __ballot(mine);
+                    if (!m2) continue;
+                    const uint32_t cnt = (uint32_t)__popcll(m2);
+                    if constexpr (EMIT) {
+                        if (staged + cnt > (uint32_t)kStage) {
+                            flush_stage(st, staged, lane, edges, cap, counter);
+                            staged = 0;
+                        }
+                        if (mine) {
+                            const uint32_t oi = flt.perm ? flt.perm[i] : i;
+                            const uint32_t oj = flt.perm ? flt.perm[j] : j;
+                            st[staged + rank_in_mask(m2)] = ((uint64_t)(node_off + oi) << 32) | (node_off + oj);
+                        }
+                        staged += cnt;
+                    } else {
+                        counted += cnt;
                     }
-                    if (mine) st[staged + rank_in_mask(m2)] = ((uint64_t)(node_off + i) << 32) | (node_off + j);
-                    staged += cnt;
-                } else {
-                    counted += cnt;
                 }
             }
         }
@@ -299,15 +341,16 @@ __device__ __forceinline__ void wave_tile(const typename Pol::Word *__restrict__
 // Single set: one workgroup per tile of the triangle (blockIdx -> tile).
 template <class Pol, int CPL, bool EMIT>
 __global__ __launch_bounds__(256) void k_pairs(const typename Pol::Word *__restrict__ packed, TileGeom G,
-                                               uint32_t node_off, uint32_t n, int k, uint64_t *__restrict__ edges,
-                                               uint64_t cap, unsigned long long *__restrict__ counter)
+                                               uint32_t node_off, uint32_t n, int k, RowFilter flt,
+                                               uint64_t *__restrict__ edges, uint64_t cap,
+                                               unsigned long long *__restrict__ counter)
 {
     __shared__ uint64_t stage[kWaves][kStage];
     const uint32_t lane = lane_id();
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     uint32_t rb, cb;
     decode_tile(blockIdx.x, G, rb, cb);
-    wave_tile<Pol, CPL, EMIT>(packed, G, rb, cb, node_off, n, k, stage[wave], lane, wave, edges, cap, counter);
+    wave_tile<Pol, CPL, EMIT>(packed, G, rb, cb, node_off, n, k, flt, stage[wave], lane, wave, edges, cap, counter);
 }
 
 // ---------------------------------------------------------------------------

@@ -127,20 +127,22 @@ Shape pick_shape(igc_ctx *ctx, uint32_t n, int K, uint32_t N)
 
 template <class Pol, int CPL, bool EMIT>
 static void launch_pairs_t(hipStream_t s, uint64_t tiles, const void *packed, const TileGeom &G, uint32_t node_off,
-                           uint32_t n, int K, uint64_t *edges, uint64_t cap, unsigned long long *counter)
+                           uint32_t n, int K, const RowFilter &f, uint64_t *edges, uint64_t cap,
+                           unsigned long long *counter)
 {
     hipLaunchKernelGGL((k_pairs<Pol, CPL, EMIT>), dim3((uint32_t)tiles), dim3(256), 0, s,
-                       static_cast<const typename Pol::Word *>(packed), G, node_off, n, K, edges, cap, counter);
+                       static_cast<const typename Pol::Word *>(packed), G, node_off, n, K, f, edges, cap, counter);
 }
 
 template <class W, int CPL, bool EMIT>
 static int launch_pairs_lv(hipStream_t s, uint64_t tiles, const void *packed, const TileGeom &G, uint32_t node_off,
-                           uint32_t n, int K, uint64_t *edges, uint64_t cap, unsigned long long *counter)
+                           uint32_t n, int K, const RowFilter &f, uint64_t *edges, uint64_t cap,
+                           unsigned long long *counter)
 {
     switch (K) {
 #define IGC_CASE(k)                                                                                           \
     case k:                                                                                                   \
-        launch_pairs_t<PolLV<k, W>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);     \
+        launch_pairs_t<PolLV<k, W>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);  \
         return IGC_OK;
         IGC_CASE(0) IGC_CASE(1) IGC_CASE(2) IGC_CASE(3) IGC_CASE(4) IGC_CASE(5) IGC_CASE(6) IGC_CASE(7)
         IGC_CASE(8) IGC_CASE(9) IGC_CASE(10) IGC_CASE(11) IGC_CASE(12) IGC_CASE(13)
@@ -153,36 +155,38 @@ static_assert(kLvLaunchMaxK == 13, "launch_pairs_lv instantiates K = 0..13");
 
 template <int CPL, bool EMIT>
 static int launch_pairs_c(hipStream_t s, uint64_t tiles, const void *packed, const TileGeom &G, uint32_t node_off,
-                          uint32_t n, int K, uint64_t *edges, uint64_t cap, unsigned long long *counter)
+                          uint32_t n, int K, const RowFilter &f, uint64_t *edges, uint64_t cap,
+                          unsigned long long *counter)
 {
     const bool wide = wide_words(n);
     switch (pred_kind(n, K)) {
     case PredKind::LV:
-        if (wide) return launch_pairs_lv<uint64_t, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);
-        return launch_pairs_lv<uint32_t, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);
+        if (wide) return launch_pairs_lv<uint64_t, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
+        return launch_pairs_lv<uint32_t, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
     case PredKind::MYERS:
-        if (wide) launch_pairs_t<PolMyers<uint64_t>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);
-        else launch_pairs_t<PolMyers<uint32_t>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);
+        if (wide) launch_pairs_t<PolMyers<uint64_t>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
+        else launch_pairs_t<PolMyers<uint32_t>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
         return IGC_OK;
     default:
-        if (wide) launch_pairs_t<PolAll<uint64_t>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);
-        else launch_pairs_t<PolAll<uint32_t>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);
+        if (wide) launch_pairs_t<PolAll<uint64_t>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
+        else launch_pairs_t<PolAll<uint32_t>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
         return IGC_OK;
     }
 }
 
 int launch_pairs(uint32_t n, int K, int cpl, bool emit, hipStream_t s, uint64_t tiles, const void *packed,
-                 const TileGeom &G, uint32_t node_off, uint64_t *edges, uint64_t cap, unsigned long long *counter)
+                 const TileGeom &G, uint32_t node_off, uint64_t *edges, uint64_t cap, unsigned long long *counter,
+                 const RowFilter &f)
 {
     if (tiles == 0) return IGC_OK;
     if (tiles > 0x7FFFFFFFull) return set_error(IGC_ERR_INVALID, "too many tiles (%llu)", (unsigned long long)tiles);
     int rc;
-    if (cpl == 1) rc = emit ? launch_pairs_c<1, true>(s, tiles, packed, G, node_off, n, K, edges, cap, counter)
-                            : launch_pairs_c<1, false>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);
-    else if (cpl == 2) rc = emit ? launch_pairs_c<2, true>(s, tiles, packed, G, node_off, n, K, edges, cap, counter)
-                                 : launch_pairs_c<2, false>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);
-    else if (cpl == 4) rc = emit ? launch_pairs_c<4, true>(s, tiles, packed, G, node_off, n, K, edges, cap, counter)
-                                 : launch_pairs_c<4, false>(s, tiles, packed, G, node_off, n, K, edges, cap, counter);
+    if (cpl == 1) rc = emit ? launch_pairs_c<1, true>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter)
+                            : launch_pairs_c<1, false>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
+    else if (cpl == 2) rc = emit ? launch_pairs_c<2, true>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter)
+                                 : launch_pairs_c<2, false>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
+    else if (cpl == 4) rc = emit ? launch_pairs_c<4, true>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter)
+                                 : launch_pairs_c<4, false>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
     else return set_error(IGC_ERR_INVALID, "unsupported columns-per-lane %d", cpl);
     if (rc) return rc;
     HIP_TRY(hipGetLastError());
@@ -285,6 +289,62 @@ int launch_csr(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *count
     return IGC_OK;
 }
 
+// Smallest set the row filter is set up for (below it the sort costs more than
+// the rows it saves).
+constexpr uint32_t kFilterMinN = 4096;
+
+// Sorts the candidates by their half counts for the row filter (igc_prefilter.hpp)
+// and fills f; `window` is a wave's column span.  Asynchronous on ctx->stream.
+static int setup_filter(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n, int K, uint32_t window,
+                        RowFilter &f)
+{
+    hipStream_t s = ctx->stream;
+    const uint32_t h = n / 2;
+    const uint64_t B64 = (uint64_t)comp_count(h) * comp_count(n - h);
+    const uint32_t B = (uint32_t)B64; // at most 969^2 for n = 32
+    const uint32_t nwin = (N + window - 1) / window;
+    const uint32_t nsb = (uint32_t)(((uint64_t)B + kScanTile - 1) / kScanTile);
+    int rc;
+    if ((rc = ctx->feat.ensure(sizeof(uint2) * (size_t)N))) return rc;
+    if ((rc = ctx->feat_s.ensure(sizeof(uint2) * (size_t)N))) return rc;
+    if ((rc = ctx->nkey.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
+    if ((rc = ctx->perm.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
+    if ((rc = ctx->packed_s.ensure(word_bytes(n) * (size_t)N))) return rc;
+    if ((rc = ctx->fbox.ensure(sizeof(uint4) * (size_t)nwin))) return rc;
+    if ((rc = ctx->fthr.ensure(sizeof(uint32_t) * (size_t)nwin))) return rc;
+    if ((rc = ctx->rcount.ensure(sizeof(uint32_t) * (size_t)B))) return rc;
+    if ((rc = ctx->roffset.ensure(sizeof(uint64_t) * ((size_t)B + 1)))) return rc;
+    if ((rc = ctx->bsum.ensure(sizeof(uint64_t) * (size_t)nsb))) return rc;
+    uint32_t *hist = ctx->rcount.as<uint32_t>();
+    uint64_t *off = ctx->roffset.as<uint64_t>();
+    const uint32_t g = (N + 255) / 256;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(B, 256, 4096)), dim3(256), 0, s, hist, (uint64_t)B, 0u);
+    hipLaunchKernelGGL(k_node_keys, dim3(g), dim3(256), 0, s, d_syms, N, n, ctx->feat.as<uint2>(),
+                       ctx->nkey.as<uint32_t>(), hist);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(kScanThreads), 0, s, hist, B, ctx->bsum.as<uint64_t>());
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, s, ctx->bsum.as<uint64_t>(), nsb);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(kScanThreads), 0, s, hist, B, ctx->bsum.as<uint64_t>(), off);
+    hipLaunchKernelGGL(k_node_scatter, dim3(g), dim3(256), 0, s, ctx->nkey.as<uint32_t>(), N, off, hist,
+                       ctx->perm.as<uint32_t>());
+    if (wide_words(n))
+        hipLaunchKernelGGL(k_node_permute<uint64_t>, dim3(g), dim3(256), 0, s, ctx->perm.as<uint32_t>(), N,
+                           ctx->packed.as<uint64_t>(), ctx->feat.as<uint2>(), ctx->packed_s.as<uint64_t>(),
+                           ctx->feat_s.as<uint2>());
+    else
+        hipLaunchKernelGGL(k_node_permute<uint32_t>, dim3(g), dim3(256), 0, s, ctx->perm.as<uint32_t>(), N,
+                           ctx->packed.as<uint32_t>(), ctx->feat.as<uint2>(), ctx->packed_s.as<uint32_t>(),
+                           ctx->feat_s.as<uint2>());
+    hipLaunchKernelGGL(k_node_boxes, dim3((nwin + 255) / 256), dim3(256), 0, s, ctx->feat_s.as<uint2>(), N, window,
+                       (uint32_t)K, ctx->fbox.as<uint4>(), ctx->fthr.as<uint32_t>());
+    HIP_TRY(hipGetLastError());
+    f.feat = ctx->feat_s.as<uint2>();
+    f.box = ctx->fbox.as<uint4>();
+    f.thr = ctx->fthr.as<uint32_t>();
+    f.perm = ctx->perm.as<uint32_t>();
+    f.window = window;
+    return IGC_OK;
+}
+
 // Full device build on ctx->stream.  Leaves the CSR in ctx->row_ptr/col.
 static int build_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n, uint32_t edit_dist,
                                uint64_t *n_edges_out, bool count_only)
@@ -311,6 +371,9 @@ static int build_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, 
     const int cpl = sh.cpl;
     const TileGeom G = choose_geometry(N, sh.cpl * lane_cols(n), sh.q);
     const uint64_t tiles = tile_count(G);
+    // exact row skipping by half counts: worth its sort for large sets with a
+    // nontrivial threshold
+    const bool use_filter = ctx->prefilter && N >= kFilterMinN && K >= 1 && pred_kind(n, K) != PredKind::ALL;
 
     int rc;
     if ((rc = ctx->packed.ensure(word_bytes(n) * (size_t)N))) return rc;
@@ -326,9 +389,11 @@ static int build_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, 
         HIP_TRY(hipEventRecord(ctx->ev[0], s));
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, misc, (uint64_t)kMiscWords, 0u);
         if ((rc = launch_pack(s, d_syms, N, n, ctx->packed.p, misc + 2))) return rc;
+        RowFilter flt{};
+        if (use_filter && (rc = setup_filter(ctx, d_syms, N, n, K, 64u * cpl * lane_cols(n), flt))) return rc;
         HIP_TRY(hipEventRecord(ctx->ev[1], s));
-        rc = launch_pairs(n, K, cpl, !count_only, s, tiles, ctx->packed.p, G, 0u,
-                          count_only ? nullptr : ctx->edges.as<uint64_t>(), cap, counter);
+        rc = launch_pairs(n, K, cpl, !count_only, s, tiles, use_filter ? ctx->packed_s.p : ctx->packed.p, G, 0u,
+                          count_only ? nullptr : ctx->edges.as<uint64_t>(), cap, counter, flt);
         if (rc) return rc;
         HIP_TRY(hipEventRecord(ctx->ev[2], s));
         // exact edge count: sizes the CSR sort, and an overflowed attempt is
@@ -423,7 +488,8 @@ IGC_API int igc_ctx_destroy(igc_ctx *ctx)
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
         for (DevBuf *b : {&ctx->syms, &ctx->packed, &ctx->edges, &ctx->misc, &ctx->row_ptr, &ctx->col, &ctx->keys[0],
-                          &ctx->keys[1], &ctx->rcount, &ctx->roffset, &ctx->bsum, &ctx->sets, &ctx->bounds})
+                          &ctx->keys[1], &ctx->rcount, &ctx->roffset, &ctx->bsum, &ctx->sets, &ctx->bounds,
+                          &ctx->feat, &ctx->feat_s, &ctx->nkey, &ctx->perm, &ctx->packed_s, &ctx->fbox, &ctx->fthr})
             b->release();
         for (HostBuf *b : {&ctx->host_rp, &ctx->host_col, &ctx->stage}) b->release();
         kmeans_release(ctx);
@@ -444,6 +510,14 @@ IGC_API int igc_ctx_set_stream(igc_ctx *ctx, void *hip_stream)
 }
 
 IGC_API void *igc_ctx_stream(igc_ctx *ctx) { return ctx ? static_cast<void *>(ctx->stream) : nullptr; }
+
+IGC_API int igc_ctx_set_prefilter(igc_ctx *ctx, int enable)
+{
+    if (!ctx) return set_error(IGC_ERR_INVALID, "null context");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->prefilter = enable != 0;
+    return IGC_OK;
+}
 
 IGC_API int igc_last_timing(igc_ctx *ctx, igc_timing *out)
 {

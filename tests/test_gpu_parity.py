@@ -62,6 +62,42 @@ def test_random_sets_every_length_and_threshold(gpu_ctx, n):
         assert np.array_equal(adj.col, col), (n, ed)
 
 
+@pytest.mark.parametrize("n", [5, 8, 11, 13, 16, 17, 24, 32])
+def test_prefilter_sets_match_oracle(gpu_ctx, n):
+    """Sets large enough for the row filter (sorted by half counts, rows skipped
+    per column window): same CSR as the oracle and as the unfiltered build."""
+    rng = np.random.default_rng(3000 + n)
+    N = 6000
+    syms = rng.integers(0, 4, size=(N, n), dtype=np.uint8)
+    for r in range(0, N // 3):  # near copies (substitutions, indels) so thresholds see edges
+        src = list(syms[int(rng.integers(0, N))])
+        for _ in range(int(rng.integers(1, 4))):
+            p = int(rng.integers(0, n))
+            if rng.integers(0, 2):
+                src[p] = int(rng.integers(0, 4))
+            else:
+                del src[p]
+                src.insert(int(rng.integers(0, n)), int(rng.integers(0, 4)))
+        syms[r] = src
+    syms[N // 3: N // 3 + 50] = 0  # one-symbol strings: all counts in one byte
+    for ed in sorted({1, 2, 3, 4, 5, n // 2, n - 1, n, n + 1}):
+        if ed < 1:
+            continue
+        banner(f"prefilter n={n} min_ed={ed} N={N}")
+        rp, col = O.graph_csr(syms, ed)
+        adj = gpu_ctx.graph_build(syms, ed)
+        assert np.array_equal(adj.row_ptr, rp), (n, ed)
+        assert np.array_equal(adj.col, col), (n, ed)
+        assert gpu_ctx.graph_count(syms, ed) == col.size // 2
+    gpu_ctx.set_prefilter(False)
+    try:
+        adj_off = gpu_ctx.graph_build(syms, 4)
+    finally:
+        gpu_ctx.set_prefilter(True)
+    rp, col = O.graph_csr(syms, 4)
+    assert np.array_equal(adj_off.col, col)
+
+
 def test_edge_cases(gpu_ctx):
     banner("edge cases")
     # empty and single inputs
