@@ -63,6 +63,9 @@ extern "C" {
 #define IGC_ORDER_ASCENDING 0u
 /* neighbour order of the reference's FillAdjList with T threads, 1 <= T <= 65535 */
 #define IGC_ORDER_THREADS(T) (((uint32_t)(T)&0xFFFFu) << 16)
+/* CSR of the upper triangle only: row i lists its neighbours j > i, ascending
+ * (|E| entries instead of 2|E|; the pair-file order of the reference) */
+#define IGC_UPPER_TRIANGLE 1u
 
 /* where a CSR's arrays live */
 #define IGC_ON_HOST 0u
@@ -227,6 +230,15 @@ IGC_API int igc_graph_build_batch_device(igc_ctx *ctx, const igc_set *sets, uint
  * RandomCluster.cpp:42-136; src/gpu_cluster.py:16-193,287).  k*4n/2 + k <= 16384. */
 IGC_API int igc_kmeans(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint32_t n, uint32_t k, uint32_t iters,
                        uint64_t seed, uint32_t flags, int32_t *assign_out, float *centroids_out);
+
+/* Nearest-centroid assignment on one-hot features: out[i] = argmin_c
+ * ||onehot(x_i) - C_c||^2 over k caller-provided FP32 centroids (k x 4n,
+ * dimension 4p+s = symbol s at position p), ties to the lowest c; the same
+ * arithmetic as igc_kmeans' assignment step.  Replaces the reference's
+ * centre assignment call (src/clustering/impl/random_cluster/RandomCluster.cpp:
+ * 62-116 running src/gpu_cluster.py:16-193,287). */
+IGC_API int igc_assign_fp32(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint32_t n, const float *centroids,
+                            uint32_t k, int32_t *out);
 
 /* The k distinct seed indices igc_kmeans starts from: a partial Fisher-Yates
  * shuffle of 0..N-1 driven by SplitMix64(seed). */

@@ -35,6 +35,7 @@ IGC_ERR_ADMISSION = 4
 IGC_ERR_IO = 5
 IGC_ERR_BUSY = 6
 IGC_MAX_LEN = 32
+IGC_UPPER_TRIANGLE = 1  # flag: CSR rows list only neighbours above the row
 IGC_ON_HOST = 0
 IGC_ON_DEVICE = 1
 
@@ -209,6 +210,7 @@ def lib() -> ctypes.CDLL:
         "igc_tuned_shape": (i, [vp, u32, u32, u32, ctypes.POINTER(i), ctypes.POINTER(u32)]),
         "igc_pair_plan_of": (i, [vp, u32, u32, u32, ctypes.POINTER(_PairPlan)]),
         "igc_ctx_set_prefilter": (i, [vp, i]),
+        "igc_assign_fp32": (i, [vp, vp, u32, u32, vp, u32, vp]),
         "igc_valu_probe": (i, [vp, vp]),
         "igc_last_error": (ctypes.c_char_p, []),
         "igc_backend": (ctypes.c_char_p, []),
@@ -282,7 +284,8 @@ class AdjList:
     include/SparseMat.hpp:42-194, as flat arrays)."""
 
     row_ptr: np.ndarray  # uint64 [N+1]
-    col: np.ndarray      # uint32 [2|E|]
+    col: np.ndarray      # uint32 [2|E|], or [|E|] for an upper-triangle CSR
+    upper: bool = False  # rows list only neighbours above the row (IGC_UPPER_TRIANGLE)
 
     @property
     def n_nodes(self) -> int:
@@ -290,12 +293,12 @@ class AdjList:
 
     @property
     def n_edges(self) -> int:
-        return int(self.col.size // 2)
+        return int(self.col.size if self.upper else self.col.size // 2)
 
     @property
     def matrix_ones(self) -> int:
         """The reference's matrixOnesNum / "Number of Ones in Matrix" = 2|E|."""
-        return int(self.col.size)
+        return 2 * self.n_edges
 
     def neighbors(self, i: int) -> np.ndarray:
         return self.col[int(self.row_ptr[i]):int(self.row_ptr[i + 1])]
@@ -316,11 +319,12 @@ class AdjList:
 
 def _adj_from_c(c: _Csr) -> AdjList:
     N = int(c.n_nodes)
-    E2 = 2 * int(c.n_edges)
     rp = np.ctypeslib.as_array(c.row_ptr, shape=(N + 1,)).copy()
-    col = np.ctypeslib.as_array(c.col, shape=(max(E2, 1),))[:E2].copy()
+    ents = int(rp[N])
+    col = np.ctypeslib.as_array(c.col, shape=(max(ents, 1),))[:ents].copy()
+    upper = ents != 2 * int(c.n_edges)
     lib().igc_csr_free(ctypes.byref(c))
-    return AdjList(rp, col)
+    return AdjList(rp, col, upper)
 
 
 @dataclass
@@ -455,6 +459,17 @@ class Context:
         cpl, q = ctypes.c_int(), ctypes.c_uint32()
         _check(lib().igc_tuned_shape(self._h, int(n), int(min_ed), int(N), ctypes.byref(cpl), ctypes.byref(q)))
         return cpl.value, q.value
+
+    def assign_fp32(self, syms, centroids) -> np.ndarray:
+        """Nearest one-hot FP32 centroid of every string (int32[N]); centroids float32[k, 4n]."""
+        s = to_symbols(syms)
+        C = np.ascontiguousarray(centroids, dtype=np.float32)
+        N, n = s.shape
+        if C.ndim != 2 or C.shape[1] != 4 * n:
+            raise ValueError(f"centroids must be [k, {4 * n}]")
+        out = np.zeros(N, dtype=np.int32)
+        _check(lib().igc_assign_fp32(self._h, s.ctypes.data, N, n, C.ctypes.data, C.shape[0], out.ctypes.data))
+        return out
 
     def set_prefilter(self, enable: bool) -> None:
         """Exact row skipping by half-string symbol counts (default on; same graph)."""

@@ -323,19 +323,32 @@ static int batch_device_locked(igc_ctx *ctx, const uint8_t *d_syms, const BatchP
     return set_error(IGC_ERR_DEVICE, "edge buffer kept overflowing");
 }
 
-// Host CSR of set s from the union CSR already copied to pinned host memory.
+// Host CSR of set s from the union CSR already copied to pinned host memory;
+// with IGC_UPPER_TRIANGLE each row keeps its neighbours above the row.
 static int split_set(const uint64_t *rp, const uint32_t *col, uint32_t off, uint32_t N, uint32_t flags, igc_csr *out)
 {
     const uint64_t base = rp[off];
-    const uint64_t ents = rp[off + N] - base;
+    const uint64_t all = rp[off + N] - base;
+    const bool upper = (flags & IGC_UPPER_TRIANGLE) != 0;
+    const uint64_t ents = upper ? all / 2 : all;
     out->n_nodes = N;
     out->location = IGC_ON_HOST;
-    out->n_edges = ents / 2;
+    out->n_edges = all / 2;
     out->row_ptr = static_cast<uint64_t *>(host_alloc(sizeof(uint64_t) * ((size_t)N + 1)));
     out->col = static_cast<uint32_t *>(host_alloc(sizeof(uint32_t) * std::max<uint64_t>(ents, 1)));
     if (!out->row_ptr || !out->col) {
         igc_csr_free(out);
         return set_error(IGC_ERR_OOM, "host CSR allocation failed");
+    }
+    if (upper) {
+        uint64_t w = 0;
+        for (uint32_t x = 0; x < N; ++x) {
+            out->row_ptr[x] = w;
+            for (uint64_t q = rp[off + x]; q < rp[off + x + 1]; ++q)
+                if (col[q] - off > x) out->col[w++] = col[q] - off;
+        }
+        out->row_ptr[N] = w;
+        return IGC_OK;
     }
     for (uint32_t x = 0; x <= N; ++x) out->row_ptr[x] = rp[off + x] - base;
     for (uint64_t q = 0; q < ents; ++q) out->col[q] = col[base + q] - off;

@@ -43,17 +43,23 @@ struct RadixPass {
     uint32_t shift;                // digit position of this pass
     uint32_t bits;                 // digit width of this pass
     uint32_t from_edges;           // pass 0
+    uint32_t upper;                // one entry per edge, row = the smaller id
 };
 
 __device__ __forceinline__ uint64_t radix_items(const RadixPass &p)
 {
     const uint64_t e = (uint64_t)*p.cnt;
-    return 2 * (e < p.cap ? e : p.cap);
+    return (p.upper ? 1 : 2) * (e < p.cap ? e : p.cap);
 }
 
 __device__ __forceinline__ uint64_t radix_key(const RadixPass &p, uint64_t t)
 {
     if (p.from_edges) {
+        if (p.upper) {
+            const uint64_t v = p.edges[t];
+            const uint64_t i = v >> 32, j = (uint32_t)v;
+            return i < j ? ((i << p.bsh) | j) : ((j << p.bsh) | i);
+        }
         const uint64_t v = p.edges[t >> 1];
         const uint64_t i = v >> 32, j = (uint32_t)v;
         return (t & 1) ? ((j << p.bsh) | i) : ((i << p.bsh) | j);
@@ -207,10 +213,10 @@ static __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(RadixPas
 
 // row_ptr[r] = first position with src >= r, for r = 0..N (src ascending).
 static __global__ void k_csr_row_ptr(const uint32_t *__restrict__ src, const unsigned long long *__restrict__ cnt,
-                                     uint64_t cap, uint32_t N, uint64_t *__restrict__ row_ptr)
+                                     uint64_t cap, uint32_t per_edge, uint32_t N, uint64_t *__restrict__ row_ptr)
 {
     const uint64_t e = (uint64_t)*cnt;
-    const uint64_t n = 2 * (e < cap ? e : cap);
+    const uint64_t n = per_edge * (e < cap ? e : cap);
     for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r <= N; r += (uint64_t)gridDim.x * blockDim.x) {
         uint64_t lo = 0, hi = n;
         while (lo < hi) {

@@ -138,7 +138,7 @@ namespace igc {
 // Builds the CSR of the edge list in ctx->edges (count in counter, capacity cap)
 // over N nodes into ctx->row_ptr / ctx->col; asynchronous on ctx->stream.
 int launch_csr(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *counter, uint32_t *misc,
-               uint32_t *launches);
+               uint32_t *launches, bool upper = false);
 Shape pick_shape(igc_ctx *ctx, uint32_t n, int K, uint32_t N);
 
 int kmeans_attach(igc_ctx *ctx);

@@ -180,6 +180,8 @@ IGC_API int igc_greedy(const igc_csr *csr, uint32_t *order, uint32_t *count)
 {
     if (!csr || !count || (!order && csr->n_nodes)) return set_error(IGC_ERR_INVALID, "null argument");
     if (csr->location != IGC_ON_HOST) return set_error(IGC_ERR_INVALID, "igc_greedy needs a host CSR");
+    if (csr->n_nodes && csr->row_ptr[csr->n_nodes] != 2 * csr->n_edges)
+        return set_error(IGC_ERR_INVALID, "igc_greedy needs the symmetric CSR (not an upper-triangle one)");
     *count = 0;
     if (csr->n_nodes == 0) return IGC_OK;
     if (!csr->row_ptr || (!csr->col && csr->n_edges)) return set_error(IGC_ERR_INVALID, "CSR has null arrays");

@@ -143,6 +143,21 @@ __global__ void k_km_init(const W *__restrict__ packed, const uint32_t *__restri
     if (threadIdx.x == 0) cn2[c] = (float)n;
 }
 
+// cn2[c] = sum_d C[c][d]^2 in dimension order (as k_km_update computes it), for
+// caller-provided centroids.  One thread per centroid.
+static __global__ void k_km_norms(const float *__restrict__ C, uint32_t k, uint32_t D, float *__restrict__ cn2)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= k) return;
+    float s = 0.0f;
+    for (uint32_t d = 0; d < D; ++d) {
+        const float v = C[(size_t)c * D + d];
+        const float sq = v * v;
+        s = s + sq;
+    }
+    cn2[c] = s;
+}
+
 // k distinct seed indices: a partial Fisher-Yates shuffle of 0..N-1 driven by
 // SplitMix64(seed); seed index c is the c-th drawn.
 void kmeans_seeds(uint32_t N, uint32_t k, uint64_t seed, std::vector<uint32_t> &out)
@@ -320,6 +335,53 @@ IGC_API int igc_kmeans(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint32_t n
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->last = igc_timing{};
     ctx->last.total_ms = ms;
+    return IGC_OK;
+}
+
+IGC_API int igc_assign_fp32(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint32_t n, const float *centroids,
+                            uint32_t k, int32_t *out)
+{
+    if (!ctx || (!syms && N) || !centroids || (!out && N)) return set_error(IGC_ERR_INVALID, "null argument");
+    if (n < 1 || n > IGC_MAX_LEN) return set_error(IGC_ERR_INVALID, "string length n=%u outside 1..%d", n, IGC_MAX_LEN);
+    if (k < 1) return set_error(IGC_ERR_INVALID, "k must be at least 1");
+    if (N == 0) return IGC_OK;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    int rc;
+    if ((rc = kmeans_attach(ctx))) return rc;
+    KmBufs &b = km_bufs(ctx);
+    hipStream_t s = ctx->stream;
+    const uint32_t D = 4 * n;
+    const size_t bytes = (size_t)N * n;
+    const bool wide = wide_words(n);
+    if ((rc = ctx->syms.ensure(bytes))) return rc;
+    if ((rc = ctx->packed.ensure(word_bytes(n) * ((size_t)N + 1)))) return rc;
+    if ((rc = ctx->misc.ensure(sizeof(uint32_t) * kMiscWords))) return rc;
+    if ((rc = b.C.ensure(sizeof(float) * (size_t)k * D))) return rc;
+    if ((rc = b.cn2.ensure(sizeof(float) * k))) return rc;
+    if ((rc = b.assign.ensure(sizeof(int32_t) * (size_t)N))) return rc;
+    uint32_t *misc = ctx->misc.as<uint32_t>();
+    HIP_TRY(hipMemcpyAsync(ctx->syms.p, syms, bytes, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(b.C.p, centroids, sizeof(float) * (size_t)k * D, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, misc, (uint64_t)kMiscWords, 0u);
+    if ((rc = launch_pack(s, ctx->syms.as<uint8_t>(), N, n, ctx->packed.p, misc + 2))) return rc;
+    hipLaunchKernelGGL(k_km_norms, dim3((k + 255) / 256), dim3(256), 0, s, b.C.as<float>(), k, D, b.cn2.as<float>());
+    const uint32_t kc = km_chunk(D);
+    const size_t lds = sizeof(float) * ((size_t)kc * D + kc);
+    if (wide)
+        hipLaunchKernelGGL(k_km_assign<uint64_t>, dim3(km_grid(ctx, N)), dim3(kKmThreads), lds, s,
+                           ctx->packed.as<uint64_t>(), N, n, b.C.as<float>(), b.cn2.as<float>(), k, kc,
+                           b.assign.as<int32_t>());
+    else
+        hipLaunchKernelGGL(k_km_assign<uint32_t>, dim3(km_grid(ctx, N)), dim3(kKmThreads), lds, s,
+                           ctx->packed.as<uint32_t>(), N, n, b.C.as<float>(), b.cn2.as<float>(), k, kc,
+                           b.assign.as<int32_t>());
+    HIP_TRY(hipGetLastError());
+    uint32_t host_misc[kMiscWords];
+    HIP_TRY(hipMemcpyAsync(host_misc, misc, sizeof host_misc, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(out, b.assign.p, sizeof(int32_t) * (size_t)N, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (host_misc[2]) return set_error(IGC_ERR_INVALID, "symbol code above 3 in input");
     return IGC_OK;
 }
 

@@ -235,7 +235,7 @@ static int check_args(igc_ctx *ctx, uint32_t N, uint32_t n)
 // CSR of the edge list by LSD radix sort of the 2E directed entries (igc_csr.hpp),
 // sized by N and cap so no host round trip is needed before the final count.
 int launch_csr(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *counter, uint32_t *misc,
-               uint32_t *launches)
+               uint32_t *launches, bool upper)
 {
     (void)misc;
     hipStream_t s = ctx->stream;
@@ -245,7 +245,7 @@ int launch_csr(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *count
     // fewest passes of at most kRadixMaxBits-bit digits, then the narrowest digit
     const uint32_t passes = (2 * bsh + kRadixMaxBits - 1) / kRadixMaxBits;
     const uint32_t bits = (2 * bsh + passes - 1) / passes;
-    const uint64_t items_cap = 2 * std::max<uint64_t>(cap, 1);
+    const uint64_t items_cap = (upper ? 1 : 2) * std::max<uint64_t>(cap, 1);
     const uint64_t tiles64 = (items_cap + kRadixTile - 1) / kRadixTile;
     if (tiles64 << bits > 0xFFFFFFFFull)
         return set_error(IGC_ERR_INVALID, "edge buffer of %llu entries is too large", (unsigned long long)cap);
@@ -271,6 +271,7 @@ int launch_csr(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *count
         rp.shift = pass * bits;
         rp.bits = std::min(bits, 2 * bsh - rp.shift);
         rp.from_edges = pass == 0;
+        rp.upper = upper ? 1u : 0u;
         const bool last = pass + 1 == passes;
         hipLaunchKernelGGL(k_radix_count, dim3(tiles), dim3(kRadixThreads), 0, s, rp, tiles, counts);
         hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(kScanThreads), 0, s, counts, L, ctx->bsum.as<uint64_t>());
@@ -283,7 +284,8 @@ int launch_csr(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *count
         *launches += 5;
     }
     hipLaunchKernelGGL(k_csr_row_ptr, dim3(grid_for((uint64_t)N + 1, 256, 8192)), dim3(256), 0, s,
-                       ctx->keys[(passes - 1) & 1].as<uint32_t>(), counter, cap, N, ctx->row_ptr.as<uint64_t>());
+                       ctx->keys[(passes - 1) & 1].as<uint32_t>(), counter, cap, upper ? 1u : 2u, N,
+                       ctx->row_ptr.as<uint64_t>());
     HIP_TRY(hipGetLastError());
     *launches += 1;
     return IGC_OK;
@@ -347,7 +349,7 @@ static int setup_filter(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_
 
 // Full device build on ctx->stream.  Leaves the CSR in ctx->row_ptr/col.
 static int build_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n, uint32_t edit_dist,
-                               uint64_t *n_edges_out, bool count_only)
+                               uint64_t *n_edges_out, bool count_only, bool upper = false)
 {
     hipStream_t s = ctx->stream;
     igc_timing tm{};
@@ -409,7 +411,7 @@ static int build_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, 
         }
         uint32_t launches = 3;
         if (!count_only) {
-            if ((rc = launch_csr(ctx, N, E, counter, misc, &launches))) return rc;
+            if ((rc = launch_csr(ctx, N, E, counter, misc, &launches, upper))) return rc;
         }
         HIP_TRY(hipEventRecord(ctx->ev[3], s));
         HIP_TRY(hipEventSynchronize(ctx->ev[3]));
@@ -590,14 +592,18 @@ static int d2h_staged(igc_ctx *ctx, void *dst, const void *src, size_t bytes)
     return IGC_OK;
 }
 
+// Copies a device CSR of E edges (2E entries, or E for an upper-triangle CSR)
+// into newly allocated host arrays, then applies the T-order of flags.
 static int csr_to_host_locked(igc_ctx *ctx, uint32_t N, uint64_t E, const uint64_t *d_row_ptr, const uint32_t *d_col,
                               uint32_t flags, igc_csr *host)
 {
+    const bool upper = (flags & IGC_UPPER_TRIANGLE) != 0;
+    const size_t ents = (upper ? 1 : 2) * (size_t)E;
     host->n_nodes = N;
     host->location = IGC_ON_HOST;
     host->n_edges = E;
     host->row_ptr = static_cast<uint64_t *>(host_alloc(sizeof(uint64_t) * ((size_t)N + 1)));
-    host->col = static_cast<uint32_t *>(host_alloc(sizeof(uint32_t) * std::max<size_t>(2 * (size_t)E, 1)));
+    host->col = static_cast<uint32_t *>(host_alloc(sizeof(uint32_t) * std::max<size_t>(ents, 1)));
     if (!host->row_ptr || !host->col) {
         igc_csr_free(host);
         return set_error(IGC_ERR_OOM, "host CSR allocation failed");
@@ -607,13 +613,13 @@ static int csr_to_host_locked(igc_ctx *ctx, uint32_t N, uint64_t E, const uint64
         return IGC_OK;
     }
     int rc = d2h_staged(ctx, host->row_ptr, d_row_ptr, sizeof(uint64_t) * ((size_t)N + 1));
-    if (!rc && E) rc = d2h_staged(ctx, host->col, d_col, sizeof(uint32_t) * 2 * (size_t)E);
+    if (!rc && E) rc = d2h_staged(ctx, host->col, d_col, sizeof(uint32_t) * ents);
     if (rc) {
         igc_csr_free(host);
         return rc;
     }
     const uint32_t T = (flags >> 16) & 0xFFFFu;
-    if (T > 1) igc::reorder_threads(*host, T);
+    if (T > 1 && !upper) igc::reorder_threads(*host, T); // upper rows are ascending in T-order too
     return IGC_OK;
 }
 
@@ -621,6 +627,7 @@ IGC_API int igc_csr_to_host(igc_ctx *ctx, const igc_csr *dev, uint32_t flags, ig
 {
     if (!ctx || !dev || !host) return set_error(IGC_ERR_INVALID, "null argument");
     if (dev->location != IGC_ON_DEVICE) return set_error(IGC_ERR_INVALID, "source CSR is not on the device");
+    if (flags & IGC_UPPER_TRIANGLE) return set_error(IGC_ERR_INVALID, "device CSRs are symmetric; build with the flag");
     std::lock_guard<std::mutex> lk(ctx->mu);
     HIP_TRY(hipSetDevice(ctx->device));
     return csr_to_host_locked(ctx, dev->n_nodes, dev->n_edges, dev->row_ptr, dev->col, flags, host);
@@ -639,7 +646,8 @@ IGC_API int igc_graph_build(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint3
     if ((rc = ctx->syms.ensure(std::max<size_t>(bytes, 1)))) return rc;
     if (bytes) HIP_TRY(hipMemcpyAsync(ctx->syms.p, syms, bytes, hipMemcpyHostToDevice, ctx->stream));
     uint64_t E = 0;
-    rc = build_device_locked(ctx, ctx->syms.as<uint8_t>(), N, n, edit_dist, &E, false);
+    rc = build_device_locked(ctx, ctx->syms.as<uint8_t>(), N, n, edit_dist, &E, false,
+                             (flags & IGC_UPPER_TRIANGLE) != 0);
     if (rc) return rc;
     return csr_to_host_locked(ctx, N, E, ctx->row_ptr.as<uint64_t>(), ctx->col.as<uint32_t>(), flags, out);
 }

@@ -541,6 +541,25 @@ static void km_assign(const uint8_t *syms, int N, int n, int k, const float *C, 
     }
 }
 
+/* Nearest-centroid assignment with caller-provided centroids: cn2 in dimension
+ * order, then the same argmin as the Lloyd iterations above. */
+ORC_EXPORT void orc_assign_fp32(const uint8_t *syms, int N, int n, const float *C, int k, int32_t *assign)
+{
+    const int D = 4 * n;
+    float *cn2 = (float *)calloc((size_t)k, sizeof(float));
+    for (int c = 0; c < k; ++c) {
+        float acc = 0.0f;
+        for (int d = 0; d < D; ++d) {
+            float v = C[(size_t)c * D + d];
+            float sq = v * v;
+            acc = acc + sq;
+        }
+        cn2[c] = acc;
+    }
+    km_assign(syms, N, n, k, C, cn2, assign);
+    free(cn2);
+}
+
 ORC_EXPORT void orc_kmeans(const uint8_t *syms, int N, int n, int k, int iters, uint64_t seed, int32_t *assign,
                            float *C_out)
 {

@@ -69,6 +69,8 @@ def lib() -> ctypes.CDLL:
         L.orc_kmeans.argtypes = [u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                  ctypes.c_void_p, ctypes.c_void_p]
         L.orc_kmeans.restype = None
+        L.orc_assign_fp32.argtypes = [u8p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        L.orc_assign_fp32.restype = None
         _lib = L
     return _lib
 
@@ -192,3 +194,14 @@ def kmeans(syms: np.ndarray, k: int, iters: int, seed: int):
     C = np.zeros((k, 4 * n), dtype=np.float32)
     lib().orc_kmeans(ps, N, n, int(k), int(iters), int(seed), assign.ctypes.data, C.ctypes.data)
     return assign, C
+
+
+def assign_fp32(syms: np.ndarray, C: np.ndarray) -> np.ndarray:
+    """Nearest-centroid ids (int32[N]) for FP32 centroids C[k, 4n], first index on ties."""
+    s, ps = _u8(syms)
+    N, n = s.shape
+    C = np.ascontiguousarray(C, dtype=np.float32)
+    assert C.shape[1] == 4 * n
+    out = np.zeros(N, dtype=np.int32)
+    lib().orc_assign_fp32(ps, N, n, C.ctypes.data, C.shape[0], out.ctypes.data)
+    return out

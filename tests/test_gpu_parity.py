@@ -273,6 +273,44 @@ def test_kmeans_graph_replays_equal_eager(gpu_ctx):
         assert np.array_equal(a_g, a_o)
 
 
+def test_assign_fp32_matches_oracle(gpu_ctx):
+    rng = np.random.default_rng(62)
+    for N, n, k in [(5000, 13, 37), (3000, 24, 9), (700, 5, 60), (20000, 16, 256)]:
+        banner(f"assign_fp32 N={N} n={n} k={k}")
+        syms = rng.integers(0, 4, size=(N, n), dtype=np.uint8)
+        C = rng.random((k, 4 * n), dtype=np.float32)
+        for c in range(min(k, 5)):  # one-hot copies of real strings
+            C[c] = 0.0
+            C[c, 4 * np.arange(n) + syms[c]] = 1.0
+        assert np.array_equal(gpu_ctx.assign_fp32(syms, C), O.assign_fp32(syms, C))
+    # the K-means result's own centroids assign like its final step
+    syms = rng.integers(0, 4, size=(8000, 12), dtype=np.uint8)
+    a, C = gpu_ctx.kmeans(syms, 16, 4, 7)
+    assert np.array_equal(gpu_ctx.assign_fp32(syms, C), a)
+
+
+def test_upper_triangle_csr(gpu_ctx):
+    rng = np.random.default_rng(63)
+    for N, n, ed in [(700, 9, 4), (6000, 13, 5), (5000, 20, 6)]:  # 6000: with the row prefilter
+        banner(f"upper triangle N={N} n={n} ed={ed}")
+        syms = rng.integers(0, 4, size=(N, n), dtype=np.uint8)
+        syms[: N // 4] = syms[N // 4: 2 * (N // 4)]
+        full = gpu_ctx.graph_build(syms, ed)
+        up = gpu_ctx.graph_build(syms, ed, igc.IGC_UPPER_TRIANGLE)
+        assert up.n_edges == full.n_edges and up.col.size == full.n_edges
+        for x in range(0, N, max(1, N // 200)):
+            row = full.col[full.row_ptr[x]:full.row_ptr[x + 1]]
+            assert np.array_equal(up.col[up.row_ptr[x]:up.row_ptr[x + 1]], row[row > x]), x
+    sets = [(rng.integers(0, 4, size=(400, 10), dtype=np.uint8), 3), (rng.integers(0, 4, size=(300, 12), dtype=np.uint8), 4)]
+    outs = gpu_ctx.graph_build_batch(sets, flags=igc.IGC_UPPER_TRIANGLE)
+    for (s, ed), o in zip(sets, outs):
+        full = gpu_ctx.graph_build(s, ed)
+        assert o.col.size == full.n_edges
+        for x in range(s.shape[0]):
+            row = full.col[full.row_ptr[x]:full.row_ptr[x + 1]]
+            assert np.array_equal(o.col[o.row_ptr[x]:o.row_ptr[x + 1]], row[row > x])
+
+
 def refine_oracle(syms, ed, k, iters, rounds, conv, seed):
     """The clustered loop (src/SparseMat.cpp:918-1095) with the oracle's pieces."""
     cur = np.arange(syms.shape[0])

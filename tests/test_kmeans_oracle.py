@@ -63,3 +63,22 @@ def test_seed_selection_matches_oracle():
     for N, k, seed in [(10, 10, 0), (1000, 37, 5), (500000, 256, 4)]:
         assert np.array_equal(igc.kmeans_seeds(N, k, seed), O.kmeans_seeds(N, k, seed))
         assert np.unique(O.kmeans_seeds(N, k, seed)).size == k
+
+
+def test_assign_fp32_oracle_matches_brute_force():
+    """orc_assign_fp32 (the GPU assignment's arithmetic) against a float64
+    brute-force argmin over explicit one-hot vectors, with centroids whose
+    entries are multiples of 1/8 (exact in float32, so no rounding ties)."""
+    rng = np.random.default_rng(61)
+    for N, n, k in [(3000, 13, 37), (1000, 24, 9), (500, 5, 60)]:
+        syms = rng.integers(0, 4, size=(N, n), dtype=np.uint8)
+        C = (rng.integers(0, 9, size=(k, 4 * n)) / 8.0).astype(np.float32)
+        C[: min(k, 5)] = 0.0
+        for c in range(min(k, 5)):  # one-hot copies of real strings: exact ties with other rows
+            C[c, 4 * np.arange(n) + syms[c]] = 1.0
+        X = np.zeros((N, 4 * n))
+        X[np.repeat(np.arange(N), n), (4 * np.arange(n))[None, :].repeat(N, 0).ravel() + syms.ravel()] = 1.0
+        d = ((X[:, None, :] - C[None, :, :].astype(np.float64)) ** 2).sum(-1)
+        want = d.argmin(1)  # first index on ties
+        got = O.assign_fp32(syms, C)
+        assert np.array_equal(got, want), (N, n, k)
