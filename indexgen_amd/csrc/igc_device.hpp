@@ -122,7 +122,8 @@ struct igc_ctx {
     igc::DevBuf rcount, roffset; // per-tile digit counts and their scan
     // row filter: half counts (unsorted, sorted), bucket keys, permutation,
     // sorted packed words, window boxes and thresholds
-    igc::DevBuf feat, feat_s, nkey, perm, packed_s, fbox, fthr;
+    igc::DevBuf feat, feat_s, nkey, perm, packed_s, fitems, fbias, fcount, corder;
+    uint32_t corder_n = 0; // string length corder was built for
     bool prefilter = true; // exact row skipping in single-set builds
     igc::HostBuf host_rp, host_col, stage; // pinned: batch CSR copies, staged copies
     hipEvent_t ev[5] = {};

@@ -232,8 +232,8 @@ __device__ __forceinline__ void flush_stage(const uint64_t *st, uint32_t staged,
 //
 // With a RowFilter (single sets), `packed` is in the filter's sorted order, a
 // row is skipped for this wave's whole column window when its half-count
-// distance to the window's box exceeds 2K (igc_prefilter.hpp), and emitted ids
-// are mapped back to the caller's order through the permutation.
+// distance to every item of the window exceeds 2K (igc_prefilter.hpp), and
+// emitted ids are mapped back to the caller's order through the permutation.
 template <class Pol, int CPL, bool EMIT>
 __device__ __forceinline__ void wave_tile(const typename Pol::Word *__restrict__ packed, const TileGeom &G,
                                           uint32_t rb, uint32_t cb, uint32_t node_off, uint32_t n, int k,
@@ -258,30 +258,35 @@ __device__ __forceinline__ void wave_tile(const typename Pol::Word *__restrict__
     for (int c = 0; c < CPL; ++c) col[c].load(packed, colw + 64u * P * c + lane, N);
     uint32_t staged = 0;
     uint64_t counted = 0;
-    uint4 box = make_uint4(0u, 0u, 0u, 0u);
-    uint32_t thr = 0;
-    if (flt.box) {
+    const uint4 *items = nullptr;
+    const uint32_t *ibias = nullptr;
+    uint32_t nitems = 0;
+    if (flt.items) {
         const uint32_t w = colw / flt.window;
-        box = flt.box[w];
-        thr = flt.thr[w];
+        items = flt.items + (size_t)w * kItemCap;
+        ibias = flt.bias + (size_t)w * kItemCap;
+        nitems = __builtin_amdgcn_readfirstlane(flt.count[w]);
     }
 
     // rows in chunks of 64: with a filter, lane l tests row r0 + l against the
-    // window's box and one ballot gives the chunk's rows that can conflict
+    // window's items and one ballot gives the chunk's rows that can conflict
     for (uint32_t r0 = 0; r0 < rows; r0 += 64) {
         const uint32_t left = rows - r0;
         uint64_t todo = left >= 64 ? ~0ull : ((1ull << left) - 1ull);
-        if (flt.box) {
-            bool keep = false;
-            if (lane < left) {
-                const uint2 f = flt.feat[row0 + r0 + lane];
-                uint32_t sad = __builtin_amdgcn_sad_u8(f.x, box.x, 0u);
-                sad = __builtin_amdgcn_sad_u8(f.x, box.z, sad);
-                sad = __builtin_amdgcn_sad_u8(f.y, box.y, sad);
-                sad = __builtin_amdgcn_sad_u8(f.y, box.w, sad);
-                keep = sad <= thr;
+        if (flt.items) {
+            const uint2 f = lane < left ? flt.feat[row0 + r0 + lane] : make_uint2(0u, 0u);
+            uint32_t best = 0xFFFFFFFFu;
+            // the window's items are wave-uniform and read-only: scalar loads
+            using CU32 = const __attribute__((address_space(4))) uint32_t;
+            CU32 *si = (CU32 *)items, *sb = (CU32 *)ibias;
+            for (uint32_t q = 0; q < nitems; ++q) {
+                uint32_t v = __builtin_amdgcn_sad_u8(f.x, si[4 * q], sb[q]);
+                v = __builtin_amdgcn_sad_u8(f.x, si[4 * q + 2], v);
+                v = __builtin_amdgcn_sad_u8(f.y, si[4 * q + 1], v);
+                v = __builtin_amdgcn_sad_u8(f.y, si[4 * q + 3], v);
+                best = min(best, v);
             }
-            todo = __ballot(keep);
+            todo &= __ballot(best <= flt.limit);
         }
         // the next row's word is loaded one row ahead (scalar load latency)
         using Word = typename Pol::Word;

@@ -9,29 +9,42 @@
 // and the deleted symbol at most one symbol crosses the split: 4 per pair, 2
 // per edit.  So an edge (Levenshtein <= K) needs that L1 distance <= 2K.
 //
-// Use.  Candidates are bucket-sorted by (prefix counts, suffix counts) in
-// lexicographic order, so the 64*slots*P columns a wave owns (a "window")
-// have similar counts.  Per window the byte-wise minimum and maximum of the
-// eight counts form a box; for a row r the L1 distance to the box is a lower
-// bound for its distance to every column in the window, and
-//   sum_d |r_d - lo_d| + |r_d - hi_d| = sum_d (hi_d - lo_d) + 2 * dist(r, box),
-// i.e. four v_sad_u8 on the packed counts.  The row is skipped for the whole
-// window when dist > 2K.  Skipping is exact: no pair that can be an edge is
+// Order.  Candidates are bucket-sorted by their count class (p, s), the
+// classes taken along a Hilbert curve through (p0, p1, p2, s0, s1, s2) (the
+// fourth count of each half follows from the other three), so the 64*slots*P
+// columns a wave owns (a "window") hold a few neighbouring classes.
+//
+// Items.  Each window keeps up to kItemCap items: one per run of equal
+// classes (a point), or, when a window holds more runs, kItemCap boxes of
+// consecutive runs (byte-wise min and max).  For a row r and an item [lo, hi]
+//   sum_d |r_d - lo_d| + |r_d - hi_d| = sum_d (hi_d - lo_d) + 2 * dist(r, item),
+// i.e. four v_sad_u8 on the packed counts started from a per-item bias that
+// cancels the width.  The row is skipped for the whole window when every
+// item is farther than 2K.  Skipping is exact: no pair that can be an edge is
 // skipped, so the graph is the same; edges are emitted with the original ids
 // through the permutation.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <utility>
+#include <vector>
+
 namespace igc {
+
+constexpr uint32_t kItemCap = 16;          // items per window
+constexpr uint32_t kItemBias = 1u << 16;   // above any width (8 counts of at most 32)
 
 // Pair-kernel view of the prefilter (all null: no filtering, identity ids).
 struct RowFilter {
     const uint2 *feat;    // counts of the sorted candidates: x = prefix, y = suffix (a byte per symbol)
-    const uint4 *box;     // per window: x/y = byte-wise min prefix/suffix, z/w = max
-    const uint32_t *thr;  // per window: sum(hi - lo) + 4K (skip when the SAD sum exceeds it)
+    const uint4 *items;   // window w's items at w * kItemCap: x/y = byte-wise min prefix/suffix, z/w = max
+    const uint32_t *bias; // per item: kItemBias - sum(hi - lo)
+    const uint32_t *count;// items per window
     const uint32_t *perm; // sorted position -> original id
     uint32_t window;      // columns per window (a wave's column span)
+    uint32_t limit;       // kItemBias + 4K: a row is kept when some item's SAD chain is at most this
 };
 
 __device__ __forceinline__ uint32_t half_counts(const uint8_t *s, uint32_t a, uint32_t b)
@@ -57,11 +70,12 @@ __device__ __forceinline__ uint32_t comp_rank(uint32_t packed, uint32_t len)
 
 __host__ __device__ inline uint32_t comp_count(uint32_t len) { return (len + 3) * (len + 2) * (len + 1) / 6; }
 
-// feat[i], bucket key[i] = rank(prefix) * comp_count(n-h) + rank(suffix), and
-// the bucket histogram.
+// feat[i], bucket key[i] = the position of i's count class along the curve
+// (order[rank(prefix) * comp_count(n-h) + rank(suffix)]), and the bucket
+// histogram.
 static __global__ void k_node_keys(const uint8_t *__restrict__ syms, uint32_t N, uint32_t n,
-                                   uint2 *__restrict__ feat, uint32_t *__restrict__ key,
-                                   uint32_t *__restrict__ hist)
+                                   const uint32_t *__restrict__ order, uint2 *__restrict__ feat,
+                                   uint32_t *__restrict__ key, uint32_t *__restrict__ hist)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
@@ -69,7 +83,7 @@ static __global__ void k_node_keys(const uint8_t *__restrict__ syms, uint32_t N,
     const uint8_t *s = syms + (size_t)i * n;
     const uint32_t p = half_counts(s, 0, h), q = half_counts(s, h, n);
     feat[i] = make_uint2(p, q);
-    const uint32_t k = comp_rank(p, h) * comp_count(n - h) + comp_rank(q, n - h);
+    const uint32_t k = order[comp_rank(p, h) * comp_count(n - h) + comp_rank(q, n - h)];
     key[i] = k;
     atomicAdd(&hist[k], 1u);
 }
@@ -97,49 +111,139 @@ __global__ void k_node_permute(const uint32_t *__restrict__ perm, uint32_t N, co
     feat_s[k] = feat[i];
 }
 
-__device__ __forceinline__ uint32_t bytes_min(uint32_t a, uint32_t b)
+__device__ __forceinline__ uint32_t byte_of(const uint2 f, uint32_t b)
 {
-    uint32_t r = 0;
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t x = (a >> (8 * k)) & 0xFF, y = (b >> (8 * k)) & 0xFF;
-        r |= (x < y ? x : y) << (8 * k);
-    }
-    return r;
+    return ((b < 4 ? f.x : f.y) >> (8 * (b & 3))) & 0xFFu;
 }
 
-__device__ __forceinline__ uint32_t bytes_max(uint32_t a, uint32_t b)
+// Items of every window of `window` sorted candidates: one workgroup of 64
+// per window.  A column starts a run when its class differs from the column
+// before it; run r goes to item r (at most kItemCap runs) or to item
+// r * kItemCap / runs, and each item's byte-wise min and max are gathered in
+// LDS.
+static __global__ __launch_bounds__(64) void k_node_items(const uint2 *__restrict__ feat_s, uint32_t N,
+                                                          uint32_t window, uint4 *__restrict__ items,
+                                                          uint32_t *__restrict__ bias, uint32_t *__restrict__ count)
 {
-    uint32_t r = 0;
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t x = (a >> (8 * k)) & 0xFF, y = (b >> (8 * k)) & 0xFF;
-        r |= (x > y ? x : y) << (8 * k);
-    }
-    return r;
-}
-
-__device__ __forceinline__ uint32_t bytes_sum(uint32_t a)
-{
-    return (a & 0xFF) + ((a >> 8) & 0xFF) + ((a >> 16) & 0xFF) + (a >> 24);
-}
-
-// Box and threshold of every window of `window` sorted candidates.
-static __global__ void k_node_boxes(const uint2 *__restrict__ feat_s, uint32_t N, uint32_t window, uint32_t K,
-                                    uint4 *__restrict__ box, uint32_t *__restrict__ thr)
-{
-    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t lo[kItemCap * 8], hi[kItemCap * 8];
+    const uint32_t w = blockIdx.x, lane = threadIdx.x;
     const uint32_t first = w * window;
     if (first >= N) return;
     const uint32_t last = min(first + window, N);
-    uint4 b = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u);
-    for (uint32_t k = first; k < last; ++k) {
-        const uint2 f = feat_s[k];
-        b.x = bytes_min(b.x, f.x);
-        b.y = bytes_min(b.y, f.y);
-        b.z = bytes_max(b.z, f.x);
-        b.w = bytes_max(b.w, f.y);
+    for (uint32_t t = lane; t < kItemCap * 8; t += 64) {
+        lo[t] = 0xFFu;
+        hi[t] = 0u;
     }
-    box[w] = b;
-    thr[w] = bytes_sum(b.z) - bytes_sum(b.x) + bytes_sum(b.w) - bytes_sum(b.y) + 4 * K;
+    uint32_t runs = 0;
+    for (uint32_t c0 = first; c0 < last; c0 += 64) {
+        const uint32_t c = c0 + lane;
+        bool start = false;
+        if (c < last) {
+            const uint2 f = feat_s[c];
+            start = c == first || f.x != feat_s[c - 1].x || f.y != feat_s[c - 1].y;
+        }
+        runs += (uint32_t)__popcll(__ballot(start));
+    }
+    __syncthreads();
+    uint32_t seen = 0;
+    for (uint32_t c0 = first; c0 < last; c0 += 64) {
+        const uint32_t c = c0 + lane;
+        bool start = false;
+        uint2 f = make_uint2(0u, 0u);
+        if (c < last) {
+            f = feat_s[c];
+            start = c == first || f.x != feat_s[c - 1].x || f.y != feat_s[c - 1].y;
+        }
+        const uint64_t m = __ballot(start);
+        if (c < last) {
+            // runs started at or before this column, minus one
+            const uint32_t run = seen + (uint32_t)__popcll(m & ((2ull << lane) - 1ull)) - 1u;
+            const uint32_t it = runs <= kItemCap ? run : (uint32_t)((uint64_t)run * kItemCap / runs);
+            for (uint32_t b = 0; b < 8; ++b) {
+                atomicMin(&lo[it * 8 + b], byte_of(f, b));
+                atomicMax(&hi[it * 8 + b], byte_of(f, b));
+            }
+        }
+        seen += (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    const uint32_t ni = min(runs, kItemCap);
+    if (lane < ni) {
+        uint32_t v[4] = {0u, 0u, 0u, 0u}, width = 0;
+        for (uint32_t b = 0; b < 8; ++b) {
+            const uint32_t l = lo[lane * 8 + b], u = hi[lane * 8 + b];
+            v[b >> 2] |= l << (8 * (b & 3));
+            v[2 + (b >> 2)] |= u << (8 * (b & 3));
+            width += u - l;
+        }
+        items[(size_t)w * kItemCap + lane] = make_uint4(v[0], v[1], v[2], v[3]);
+        bias[(size_t)w * kItemCap + lane] = kItemBias - width;
+    }
+    if (lane == 0) count[w] = ni;
+}
+
+// Position of every count class (rank(prefix) * comp_count(n-h) + rank(suffix))
+// along a Hilbert curve through (p0, p1, p2, s0, s1, s2), 5 bits per axis.
+// The curve index is J. Skilling's transpose form ("Programming the Hilbert
+// curve", AIP Conf. Proc. 707, 2004): undo the excess work, Gray-encode, then
+// interleave the axes' bits from the top.  Host code.
+inline uint64_t hilbert_index6(uint32_t X[6])
+{
+    constexpr int D = 6, B = 5;
+    const uint32_t M = 1u << (B - 1);
+    for (uint32_t Q = M; Q > 1; Q >>= 1) {
+        const uint32_t P = Q - 1;
+        for (int i = 0; i < D; ++i) {
+            if (X[i] & Q) {
+                X[0] ^= P;
+            } else {
+                const uint32_t t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    for (int i = 1; i < D; ++i) X[i] ^= X[i - 1];
+    uint32_t t = 0;
+    for (uint32_t Q = M; Q > 1; Q >>= 1)
+        if (X[D - 1] & Q) t ^= Q - 1;
+    for (int i = 0; i < D; ++i) X[i] ^= t;
+    uint64_t key = 0;
+    for (int b = B - 1; b >= 0; --b)
+        for (int i = 0; i < D; ++i) key = (key << 1) | ((X[i] >> b) & 1u);
+    return key;
+}
+
+// Compositions of `len` into (c0, c1, c2, c3) in the lexicographic order
+// comp_rank numbers them; three counts per composition.
+inline void compositions(uint32_t len, std::vector<uint32_t> &out)
+{
+    out.clear();
+    for (uint32_t a = 0; a <= len; ++a)
+        for (uint32_t b = 0; a + b <= len; ++b)
+            for (uint32_t c = 0; a + b + c <= len; ++c) {
+                out.push_back(a);
+                out.push_back(b);
+                out.push_back(c);
+            }
+}
+
+inline void class_order(uint32_t n, std::vector<uint32_t> &order)
+{
+    const uint32_t h = n / 2;
+    std::vector<uint32_t> cp, cs;
+    compositions(h, cp);
+    compositions(n - h, cs);
+    const uint32_t np = (uint32_t)cp.size() / 3, ns = (uint32_t)cs.size() / 3;
+    std::vector<std::pair<uint64_t, uint32_t>> keyed((size_t)np * ns);
+    for (uint32_t a = 0; a < np; ++a)
+        for (uint32_t b = 0; b < ns; ++b) {
+            uint32_t X[6] = {cp[3 * a], cp[3 * a + 1], cp[3 * a + 2], cs[3 * b], cs[3 * b + 1], cs[3 * b + 2]};
+            keyed[(size_t)a * ns + b] = {hilbert_index6(X), a * ns + b};
+        }
+    std::sort(keyed.begin(), keyed.end());
+    order.assign(keyed.size(), 0u);
+    for (uint32_t r = 0; r < (uint32_t)keyed.size(); ++r) order[keyed[r].second] = r;
 }
 
 } // namespace igc

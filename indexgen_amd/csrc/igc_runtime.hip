@@ -312,8 +312,18 @@ static int setup_filter(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_
     if ((rc = ctx->nkey.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
     if ((rc = ctx->perm.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
     if ((rc = ctx->packed_s.ensure(word_bytes(n) * (size_t)N))) return rc;
-    if ((rc = ctx->fbox.ensure(sizeof(uint4) * (size_t)nwin))) return rc;
-    if ((rc = ctx->fthr.ensure(sizeof(uint32_t) * (size_t)nwin))) return rc;
+    if ((rc = ctx->fitems.ensure(sizeof(uint4) * kItemCap * (size_t)nwin))) return rc;
+    if ((rc = ctx->fbias.ensure(sizeof(uint32_t) * kItemCap * (size_t)nwin))) return rc;
+    if ((rc = ctx->fcount.ensure(sizeof(uint32_t) * (size_t)nwin))) return rc;
+    if (ctx->corder_n != n) { // class order along the curve, once per string length
+        std::vector<uint32_t> order;
+        class_order(n, order);
+        if ((rc = ctx->corder.ensure(sizeof(uint32_t) * order.size()))) return rc;
+        HIP_TRY(hipMemcpyAsync(ctx->corder.p, order.data(), sizeof(uint32_t) * order.size(), hipMemcpyHostToDevice,
+                               s));
+        HIP_TRY(hipStreamSynchronize(s)); // `order` is pageable and about to go
+        ctx->corder_n = n;
+    }
     if ((rc = ctx->rcount.ensure(sizeof(uint32_t) * (size_t)B))) return rc;
     if ((rc = ctx->roffset.ensure(sizeof(uint64_t) * ((size_t)B + 1)))) return rc;
     if ((rc = ctx->bsum.ensure(sizeof(uint64_t) * (size_t)nsb))) return rc;
@@ -321,8 +331,8 @@ static int setup_filter(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_
     uint64_t *off = ctx->roffset.as<uint64_t>();
     const uint32_t g = (N + 255) / 256;
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(B, 256, 4096)), dim3(256), 0, s, hist, (uint64_t)B, 0u);
-    hipLaunchKernelGGL(k_node_keys, dim3(g), dim3(256), 0, s, d_syms, N, n, ctx->feat.as<uint2>(),
-                       ctx->nkey.as<uint32_t>(), hist);
+    hipLaunchKernelGGL(k_node_keys, dim3(g), dim3(256), 0, s, d_syms, N, n, ctx->corder.as<uint32_t>(),
+                       ctx->feat.as<uint2>(), ctx->nkey.as<uint32_t>(), hist);
     hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(kScanThreads), 0, s, hist, B, ctx->bsum.as<uint64_t>());
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, s, ctx->bsum.as<uint64_t>(), nsb);
     hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(kScanThreads), 0, s, hist, B, ctx->bsum.as<uint64_t>(), off);
@@ -336,14 +346,16 @@ static int setup_filter(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_
         hipLaunchKernelGGL(k_node_permute<uint32_t>, dim3(g), dim3(256), 0, s, ctx->perm.as<uint32_t>(), N,
                            ctx->packed.as<uint32_t>(), ctx->feat.as<uint2>(), ctx->packed_s.as<uint32_t>(),
                            ctx->feat_s.as<uint2>());
-    hipLaunchKernelGGL(k_node_boxes, dim3((nwin + 255) / 256), dim3(256), 0, s, ctx->feat_s.as<uint2>(), N, window,
-                       (uint32_t)K, ctx->fbox.as<uint4>(), ctx->fthr.as<uint32_t>());
+    hipLaunchKernelGGL(k_node_items, dim3(nwin), dim3(64), 0, s, ctx->feat_s.as<uint2>(), N, window,
+                       ctx->fitems.as<uint4>(), ctx->fbias.as<uint32_t>(), ctx->fcount.as<uint32_t>());
     HIP_TRY(hipGetLastError());
     f.feat = ctx->feat_s.as<uint2>();
-    f.box = ctx->fbox.as<uint4>();
-    f.thr = ctx->fthr.as<uint32_t>();
+    f.items = ctx->fitems.as<uint4>();
+    f.bias = ctx->fbias.as<uint32_t>();
+    f.count = ctx->fcount.as<uint32_t>();
     f.perm = ctx->perm.as<uint32_t>();
     f.window = window;
+    f.limit = kItemBias + 4u * (uint32_t)K;
     return IGC_OK;
 }
 
@@ -491,7 +503,7 @@ IGC_API int igc_ctx_destroy(igc_ctx *ctx)
         (void)hipStreamSynchronize(ctx->stream);
         for (DevBuf *b : {&ctx->syms, &ctx->packed, &ctx->edges, &ctx->misc, &ctx->row_ptr, &ctx->col, &ctx->keys[0],
                           &ctx->keys[1], &ctx->rcount, &ctx->roffset, &ctx->bsum, &ctx->sets, &ctx->bounds,
-                          &ctx->feat, &ctx->feat_s, &ctx->nkey, &ctx->perm, &ctx->packed_s, &ctx->fbox, &ctx->fthr})
+                          &ctx->feat, &ctx->feat_s, &ctx->nkey, &ctx->perm, &ctx->packed_s, &ctx->fitems, &ctx->fbias, &ctx->fcount, &ctx->corder})
             b->release();
         for (HostBuf *b : {&ctx->host_rp, &ctx->host_col, &ctx->stage}) b->release();
         kmeans_release(ctx);
