@@ -287,8 +287,8 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "k_pairs_traffic.json")
     if os.path.exists(tpath):
         rec = json.load(open(tpath)).get(args.workload)
-        if rec and (rec.get("slots"), rec.get("q"), rec.get("lane_cols")) == (plan["slots"], plan["q"],
-                                                                             plan["lane_cols"]):
+        if rec and (rec.get("slots"), rec.get("q"), rec.get("lane_cols"), rec.get("row_filter")) == (
+                plan["slots"], plan["q"], plan["lane_cols"], plan["row_filter"]):
             # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE passes
             traffic = int(rec["fetch_size_kib"] * 1024 * rec["fetch_scale"]
                           + rec["write_size_kib"] * 1024 * rec["write_scale"])

@@ -105,12 +105,19 @@ IGC_API int igc_ctx_destroy(igc_ctx *ctx);
  * restores the context's own stream. */
 IGC_API int igc_ctx_set_stream(igc_ctx *ctx, void *hip_stream);
 IGC_API void *igc_ctx_stream(igc_ctx *ctx);
-/* Exact row skipping in single-set builds of 4096 or more strings (default on):
- * candidates are ordered by the symbol counts of their two halves, and a row is
- * skipped for a wave's column window when those counts prove no column of the
- * window is within the threshold.  The graph is the same either way.  No
- * counterpart in the reference (an internal speed-up of FillAdjListGPU). */
-IGC_API int igc_ctx_set_prefilter(igc_ctx *ctx, int enable);
+/* Exact row skipping (single sets and batched sets): candidates are ordered by
+ * the symbol counts of their two halves, and a row is skipped for a wave's
+ * column window when those counts prove no column of the window is within the
+ * threshold.  The graph is the same either way.  IGC_PREFILTER_AUTO (default)
+ * sorts sets of at least IGC_PREFILTER_MIN_N strings, where the rows saved
+ * outweigh the sort; IGC_PREFILTER_ALWAYS sorts every set with a nontrivial
+ * threshold.  No counterpart in the reference (an internal speed-up of
+ * FillAdjListGPU). */
+#define IGC_PREFILTER_OFF 0
+#define IGC_PREFILTER_AUTO 1
+#define IGC_PREFILTER_ALWAYS 2
+#define IGC_PREFILTER_MIN_N 16384u
+IGC_API int igc_ctx_set_prefilter(igc_ctx *ctx, int mode);
 IGC_API int igc_last_timing(igc_ctx *ctx, igc_timing *out);
 
 /* ---- single candidate set ---- */
@@ -168,6 +175,8 @@ typedef struct igc_pair_plan {
     uint32_t tile_cols; /* 4 waves x 64 lanes x slots x lane_cols */
     uint32_t tile_rows;
     uint64_t tiles;     /* workgroups of the launch (upper triangle) */
+    int row_filter;     /* 1: candidates sorted for exact row skipping (slots is then 1) */
+    int pad;
 } igc_pair_plan;
 IGC_API int igc_pair_plan_of(igc_ctx *ctx, uint32_t n, uint32_t edit_dist, uint32_t N, igc_pair_plan *out);
 

@@ -36,6 +36,8 @@ IGC_ERR_IO = 5
 IGC_ERR_BUSY = 6
 IGC_MAX_LEN = 32
 IGC_UPPER_TRIANGLE = 1  # flag: CSR rows list only neighbours above the row
+IGC_PREFILTER_OFF, IGC_PREFILTER_AUTO, IGC_PREFILTER_ALWAYS = 0, 1, 2
+IGC_PREFILTER_MIN_N = 16384  # smallest set the automatic mode sorts
 IGC_ON_HOST = 0
 IGC_ON_DEVICE = 1
 
@@ -139,7 +141,7 @@ class _TuneEntry(ctypes.Structure):
 class _PairPlan(ctypes.Structure):
     _fields_ = [("predicate", ctypes.c_int), ("k", ctypes.c_int), ("lane_cols", ctypes.c_int), ("slots", ctypes.c_int),
                 ("q", ctypes.c_uint32), ("tile_cols", ctypes.c_uint32), ("tile_rows", ctypes.c_uint32),
-                ("tiles", ctypes.c_uint64)]
+                ("tiles", ctypes.c_uint64), ("row_filter", ctypes.c_int), ("pad", ctypes.c_int)]
 
 
 PREDICATES = {0: "landau_vishkin", 1: "myers", 2: "all_pairs"}
@@ -471,16 +473,20 @@ class Context:
         _check(lib().igc_assign_fp32(self._h, s.ctypes.data, N, n, C.ctypes.data, C.shape[0], out.ctypes.data))
         return out
 
-    def set_prefilter(self, enable: bool) -> None:
-        """Exact row skipping by half-string symbol counts (default on; same graph)."""
-        _check(lib().igc_ctx_set_prefilter(self._h, 1 if enable else 0))
+    def set_prefilter(self, mode) -> None:
+        """Exact row skipping by half-string symbol counts (same graph either way):
+        False/IGC_PREFILTER_OFF, True/IGC_PREFILTER_AUTO (default: sets of at least
+        IGC_PREFILTER_MIN_N strings) or IGC_PREFILTER_ALWAYS."""
+        m = int(mode) if not isinstance(mode, bool) else (IGC_PREFILTER_AUTO if mode else IGC_PREFILTER_OFF)
+        _check(lib().igc_ctx_set_prefilter(self._h, m))
 
     def pair_plan(self, n: int, min_ed: int, N: int) -> dict:
         """The pair launch this context makes for N strings at (n, min_ed)."""
         p = _PairPlan()
         _check(lib().igc_pair_plan_of(self._h, int(n), int(min_ed), int(N), ctypes.byref(p)))
         return {"predicate": PREDICATES[p.predicate], "k": p.k, "lane_cols": p.lane_cols, "slots": p.slots,
-                "q": p.q, "tile_cols": p.tile_cols, "tile_rows": p.tile_rows, "tiles": int(p.tiles)}
+                "q": p.q, "tile_cols": p.tile_cols, "tile_rows": p.tile_rows, "tiles": int(p.tiles),
+                "row_filter": bool(p.row_filter)}
 
     def valu_probe(self) -> dict:
         """Wave-level VALU instructions per second for each int32 op type of the pair kernel."""

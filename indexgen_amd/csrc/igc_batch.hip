@@ -9,7 +9,10 @@
 // and K are uniform per tile and dispatched at run time), sets with n > 16 get
 // one pair launch each into the same edge list, and the unchanged CSR assembly
 // runs once over the union.  Set s's CSR is the row slice node_off[s].. with
-// ids shifted back.
+// ids shifted back.  Sets large enough for the exact row filter
+// (igc_prefilter.hpp) are sorted by count class first, each into its own
+// slices of the sorted-word, count, permutation and window-item arrays; their
+// tiles read the sorted words and emit ids through the permutation.
 //
 // This replaces the reference's one process + one GPU context + one launch per
 // cluster (SolveIndependentSet, src/SparseMat.cpp:784-851, called from the
@@ -39,6 +42,19 @@ struct SetDesc {
     uint32_t n;          // string length
     int32_t K;           // Levenshtein threshold (edge iff ED <= K); kAllPairs = every pair
     uint32_t pred;       // BatchPred of the persistent kernel; kBatchNone = not in it
+    uint32_t filt;       // 1: row filter (sorted words at word_off of the sorted array)
+    uint32_t wbase;      // first window of the set in the item arrays
+    uint32_t window;     // columns per window
+    uint32_t pad;
+};
+
+// Device arrays of the row filter shared by all filtered sets of a batch.
+struct FilterBase {
+    const uint2 *feat;     // per node (node_off + sorted position)
+    const uint4 *items;    // per window (wbase + w) * kItemCap
+    const uint32_t *bias;
+    const uint32_t *count; // per window
+    const uint32_t *perm;  // per node: sorted position -> local id
 };
 
 static __device__ __forceinline__ uint32_t find_set_by(const SetDesc *__restrict__ sets, uint32_t nsets, uint64_t key,
@@ -73,26 +89,30 @@ static __global__ void k_pack_batch(const uint8_t *__restrict__ syms, const SetD
 }
 
 template <int CPL>
-__device__ __forceinline__ void batch_tile(const SetDesc &d, const uint32_t *__restrict__ packed, uint32_t rb,
-                                           uint32_t cb, uint64_t *st, uint32_t lane, uint32_t wave,
-                                           uint64_t *__restrict__ edges, uint64_t cap,
+__device__ __forceinline__ void batch_tile(const SetDesc &d, const uint32_t *__restrict__ packed,
+                                           const FilterBase &fb, uint32_t rb, uint32_t cb, uint64_t *st,
+                                           uint32_t lane, uint32_t wave, uint64_t *__restrict__ edges, uint64_t cap,
                                            unsigned long long *__restrict__ counter)
 {
-    const RowFilter none{};
+    RowFilter flt{};
+    if (d.filt)
+        flt = filter_view(fb.feat + d.node_off, fb.items + (size_t)d.wbase * kItemCap,
+                           fb.bias + (size_t)d.wbase * kItemCap, fb.count + d.wbase, fb.perm + d.node_off, d.window,
+                           d.K);
     if (d.pred == kBatchMyers) {
-        wave_tile<PolMyers<uint32_t>, CPL, true>(packed, d.G, rb, cb, d.node_off, d.n, d.K, none, st, lane, wave, edges,
+        wave_tile<PolMyers<uint32_t>, CPL, true>(packed, d.G, rb, cb, d.node_off, d.n, d.K, flt, st, lane, wave, edges,
                                                  cap, counter);
         return;
     }
     if (d.pred == kBatchAll) {
-        wave_tile<PolAll<uint32_t>, CPL, true>(packed, d.G, rb, cb, d.node_off, d.n, d.K, none, st, lane, wave, edges,
+        wave_tile<PolAll<uint32_t>, CPL, true>(packed, d.G, rb, cb, d.node_off, d.n, d.K, flt, st, lane, wave, edges,
                                                cap, counter);
         return;
     }
     switch (d.K) {
 #define IGC_BCASE(k)                                                                                          \
     case k:                                                                                                   \
-        wave_tile<PolLV<k, uint32_t>, CPL, true>(packed, d.G, rb, cb, d.node_off, d.n, d.K, none, st, lane, wave, \
+        wave_tile<PolLV<k, uint32_t>, CPL, true>(packed, d.G, rb, cb, d.node_off, d.n, d.K, flt, st, lane, wave, \
                                                  edges, cap, counter);                                        \
         break;
         IGC_BCASE(0) IGC_BCASE(1) IGC_BCASE(2) IGC_BCASE(3) IGC_BCASE(4) IGC_BCASE(5) IGC_BCASE(6)
@@ -108,6 +128,7 @@ __device__ __forceinline__ void batch_tile(const SetDesc &d, const uint32_t *__r
 // first (host side), so the tail is made of cheap tiles.
 template <int CPL>
 __global__ __launch_bounds__(256) void k_pairs_batch(const uint8_t *__restrict__ packed,
+                                                     const uint8_t *__restrict__ packed_s, FilterBase fb,
                                                      const SetDesc *__restrict__ sets, uint32_t nsets,
                                                      uint64_t total_tiles, unsigned long long *__restrict__ work,
                                                      uint64_t *__restrict__ edges, uint64_t cap,
@@ -127,8 +148,8 @@ __global__ __launch_bounds__(256) void k_pairs_batch(const uint8_t *__restrict__
         const SetDesc d = sets[s];
         uint32_t rb, cb;
         decode_tile(t - d.tile_begin, d.G, rb, cb);
-        batch_tile<CPL>(d, reinterpret_cast<const uint32_t *>(packed + d.word_off), rb, cb, stage[wave], lane, wave,
-                        edges, cap, counter);
+        batch_tile<CPL>(d, reinterpret_cast<const uint32_t *>((d.filt ? packed_s : packed) + d.word_off), fb, rb, cb,
+                        stage[wave], lane, wave, edges, cap, counter);
     }
 }
 
@@ -147,6 +168,8 @@ struct BatchPlan {
     std::vector<uint32_t> order;   // desc index -> caller's set index
     std::vector<uint32_t> node_off_by_set;
     std::vector<uint32_t> extra;   // desc indices launched separately (n > 16)
+    std::vector<uint32_t> filtered;// desc indices with the row filter
+    uint64_t windows = 0;          // filter windows of all filtered sets
     uint64_t tiles = 0;            // tiles of the persistent kernel
     uint64_t pairs = 0;
     uint64_t sym_bytes = 0;
@@ -219,6 +242,13 @@ static int plan_batch(igc_ctx *ctx, const igc_set *sets, uint32_t count, int cpl
             else if (K[s] <= kBatchMaxK) d.pred = kBatchLV;
         }
         const bool in_kernel = d.pred != kBatchNone;
+        if (runnable && filter_pays(ctx->prefilter, x.N, x.n, K[s])) {
+            d.filt = 1;
+            d.wbase = (uint32_t)P.windows;
+            d.window = 64u * (uint32_t)cpl * (uint32_t)lane_cols(x.n);
+            P.windows += (x.N + d.window - 1) / d.window;
+            P.filtered.push_back((uint32_t)P.desc.size());
+        }
         d.tile_begin = P.tiles;
         if (in_kernel) P.tiles += tile_count(d.G);
         if (runnable && !in_kernel) P.extra.push_back((uint32_t)P.desc.size());
@@ -228,17 +258,43 @@ static int plan_batch(igc_ctx *ctx, const igc_set *sets, uint32_t count, int cpl
     P.by_node.resize(P.desc.size());
     for (size_t k = 0; k < P.desc.size(); ++k) P.by_node[P.order[k]] = P.desc[k];
     P.cap = std::max<uint64_t>((uint64_t)est_edges, 1);
-    (void)ctx;
+    if (P.windows > 0xFFFFFFFFull) return set_error(IGC_ERR_INVALID, "batch has too many filter windows");
     return IGC_OK;
 }
 
 template <int CPL>
-static void launch_batch_kernel(igc_ctx *ctx, const BatchPlan &P, const SetDesc *d_sets, unsigned long long *work,
-                                unsigned long long *counter, uint64_t cap)
+static void launch_batch_kernel(igc_ctx *ctx, const BatchPlan &P, const FilterBase &fb, const SetDesc *d_sets,
+                                unsigned long long *work, unsigned long long *counter, uint64_t cap)
 {
     const uint32_t grid = (uint32_t)std::min<uint64_t>(P.tiles, (uint64_t)ctx->cu_count * 8);
     hipLaunchKernelGGL((k_pairs_batch<CPL>), dim3(grid), dim3(256), 0, ctx->stream, ctx->packed.as<uint8_t>(),
-                       d_sets, (uint32_t)P.desc.size(), P.tiles, work, ctx->edges.as<uint64_t>(), cap, counter);
+                       ctx->packed_s.as<uint8_t>(), fb, d_sets, (uint32_t)P.desc.size(), P.tiles, work,
+                       ctx->edges.as<uint64_t>(), cap, counter);
+}
+
+// Buffers and scratch of the row filter for the batch's filtered sets, all
+// sized before the first set-up launch (no reallocation while kernels run).
+static int batch_filter_buffers(igc_ctx *ctx, const BatchPlan &P, FilterBase &fb)
+{
+    fb = FilterBase{};
+    if (P.filtered.empty()) return IGC_OK;
+    int rc;
+    if ((rc = ctx->feat_s.ensure(sizeof(uint2) * (size_t)P.total))) return rc;
+    if ((rc = ctx->perm.ensure(sizeof(uint32_t) * (size_t)P.total))) return rc;
+    if ((rc = ctx->packed_s.ensure(std::max<size_t>(P.word_bytes, 8)))) return rc;
+    if ((rc = ctx->fitems.ensure(sizeof(uint4) * kItemCap * (size_t)P.windows))) return rc;
+    if ((rc = ctx->fbias.ensure(sizeof(uint32_t) * kItemCap * (size_t)P.windows))) return rc;
+    if ((rc = ctx->fcount.ensure(sizeof(uint32_t) * (size_t)P.windows))) return rc;
+    uint32_t maxN[IGC_MAX_LEN + 1] = {};
+    for (uint32_t e : P.filtered) maxN[P.desc[e].n] = std::max(maxN[P.desc[e].n], P.desc[e].G.N);
+    for (uint32_t n = 1; n <= IGC_MAX_LEN; ++n)
+        if (maxN[n] && (rc = filter_scratch(ctx, maxN[n], n))) return rc;
+    fb.feat = ctx->feat_s.as<uint2>();
+    fb.items = ctx->fitems.as<uint4>();
+    fb.bias = ctx->fbias.as<uint32_t>();
+    fb.count = ctx->fcount.as<uint32_t>();
+    fb.perm = ctx->perm.as<uint32_t>();
+    return IGC_OK;
 }
 
 // Builds the union CSR of all sets in ctx->row_ptr / ctx->col; fills
@@ -262,6 +318,8 @@ static int batch_device_locked(igc_ctx *ctx, const uint8_t *d_syms, const BatchP
     HIP_TRY(hipMemcpyAsync(ctx->sets.p, P.desc.data(), sizeof(SetDesc) * nd, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(ctx->sets.as<SetDesc>() + nd, P.by_node.data(), sizeof(SetDesc) * nd, hipMemcpyHostToDevice,
                            s));
+    FilterBase fb;
+    if ((rc = batch_filter_buffers(ctx, P, fb))) return rc;
     uint32_t *misc = ctx->misc.as<uint32_t>();
     unsigned long long *counter = reinterpret_cast<unsigned long long *>(misc);
     unsigned long long *work = reinterpret_cast<unsigned long long *>(misc + 6);
@@ -274,19 +332,35 @@ static int batch_device_locked(igc_ctx *ctx, const uint8_t *d_syms, const BatchP
             hipLaunchKernelGGL(k_pack_batch, dim3((P.total + 255) / 256), dim3(256), 0, s, d_syms, d_node_order, nd,
                                P.total, ctx->packed.as<uint8_t>(), misc + 2);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipEventRecord(ctx->ev[1], s));
         uint32_t launches = 2;
+        for (uint32_t e : P.filtered) {
+            const SetDesc &d = P.desc[e];
+            if ((rc = filter_set(ctx, d_syms + d.sym_off, d.G.N, d.n, d.window, ctx->packed.as<uint8_t>() + d.word_off,
+                                 ctx->packed_s.as<uint8_t>() + d.word_off, ctx->feat_s.as<uint2>() + d.node_off,
+                                 ctx->perm.as<uint32_t>() + d.node_off,
+                                 ctx->fitems.as<uint4>() + (size_t)d.wbase * kItemCap,
+                                 ctx->fbias.as<uint32_t>() + (size_t)d.wbase * kItemCap,
+                                 ctx->fcount.as<uint32_t>() + d.wbase)))
+                return rc;
+            launches += 8;
+        }
+        HIP_TRY(hipEventRecord(ctx->ev[1], s));
         if (P.tiles) {
-            if (cpl == 1) launch_batch_kernel<1>(ctx, P, d_tile_order, work, counter, cap);
-            else if (cpl == 2) launch_batch_kernel<2>(ctx, P, d_tile_order, work, counter, cap);
-            else launch_batch_kernel<4>(ctx, P, d_tile_order, work, counter, cap);
+            if (cpl == 1) launch_batch_kernel<1>(ctx, P, fb, d_tile_order, work, counter, cap);
+            else if (cpl == 2) launch_batch_kernel<2>(ctx, P, fb, d_tile_order, work, counter, cap);
+            else launch_batch_kernel<4>(ctx, P, fb, d_tile_order, work, counter, cap);
             HIP_TRY(hipGetLastError());
             launches++;
         }
         for (uint32_t e : P.extra) {
             const SetDesc &d = P.desc[e];
-            if ((rc = launch_pairs(d.n, d.K, cpl, true, s, tile_count(d.G), ctx->packed.as<uint8_t>() + d.word_off, d.G,
-                                   d.node_off, ctx->edges.as<uint64_t>(), cap, counter)))
+            const RowFilter f = d.filt ? filter_view(fb.feat + d.node_off, fb.items + (size_t)d.wbase * kItemCap,
+                                                     fb.bias + (size_t)d.wbase * kItemCap, fb.count + d.wbase,
+                                                     fb.perm + d.node_off, d.window, d.K)
+                                       : RowFilter{};
+            const uint8_t *words = (d.filt ? ctx->packed_s.as<uint8_t>() : ctx->packed.as<uint8_t>()) + d.word_off;
+            if ((rc = launch_pairs(d.n, d.K, cpl, true, s, tile_count(d.G), words, d.G, d.node_off,
+                                   ctx->edges.as<uint64_t>(), cap, counter, f)))
                 return rc;
             launches++;
         }
@@ -357,10 +431,17 @@ static int split_set(const uint64_t *rp, const uint32_t *col, uint32_t off, uint
     return IGC_OK;
 }
 
-static int batch_shape(const igc_set *sets, uint32_t count)
+// Columns per lane of a batch: one register (128-column filter windows) when
+// some set gets the row filter, else wider for large sets.
+static int batch_shape(const igc_ctx *ctx, const igc_set *sets, uint32_t count)
 {
     uint64_t maxN = 0;
-    for (uint32_t s = 0; s < count; ++s) maxN = std::max<uint64_t>(maxN, sets[s].N);
+    for (uint32_t s = 0; s < count; ++s) {
+        maxN = std::max<uint64_t>(maxN, sets[s].N);
+        if (sets[s].edit_dist > 0 && sets[s].n >= 1 && sets[s].n <= IGC_MAX_LEN &&
+            filter_pays(ctx->prefilter, sets[s].N, sets[s].n, effective_k(sets[s].n, sets[s].edit_dist)))
+            return 1;
+    }
     return maxN >= 32768 ? 4 : (maxN >= 4096 ? 2 : 1);
 }
 
@@ -388,7 +469,7 @@ int batch_build_host_locked(igc_ctx *ctx, const igc_set *sets, uint32_t count, i
 {
     for (uint32_t s = 0; s < count; ++s) std::memset(&outs[s], 0, sizeof(igc_csr));
     if (count == 0) return IGC_OK;
-    const int cpl = batch_shape(sets, count);
+    const int cpl = batch_shape(ctx, sets, count);
     BatchPlan P;
     int rc = plan_batch(ctx, sets, count, cpl, P);
     if (rc) return rc;
@@ -438,7 +519,7 @@ IGC_API int igc_graph_build_batch_device(igc_ctx *ctx, const igc_set *sets, uint
         return set_error(IGC_ERR_INVALID, "null argument or empty batch");
     std::lock_guard<std::mutex> lk(ctx->mu);
     HIP_TRY(hipSetDevice(ctx->device));
-    const int cpl = batch_shape(sets, count);
+    const int cpl = batch_shape(ctx, sets, count);
     BatchPlan P;
     int rc = plan_batch(ctx, sets, count, cpl, P);
     if (rc) return rc;

@@ -110,6 +110,16 @@ int launch_pack(hipStream_t s, const uint8_t *d_syms, uint32_t N, uint32_t n, vo
 int pred_variant(uint32_t n, int K);
 constexpr int kPredVariants = 2 * (kLvLaunchMaxK + 1) + 4;
 
+// Whether a set of N strings gets the row filter under prefilter `mode`: it
+// needs a nontrivial threshold, and automatically only sets of at least
+// IGC_PREFILTER_MIN_N strings (below it the sort's launches cost more than the
+// rows it saves).
+inline bool filter_pays(int mode, uint32_t N, uint32_t n, int K)
+{
+    if (mode == IGC_PREFILTER_OFF || N < 2 || K < 1 || pred_kind(n, K) == PredKind::ALL) return false;
+    return mode == IGC_PREFILTER_ALWAYS || N >= IGC_PREFILTER_MIN_N;
+}
+
 } // namespace igc
 
 struct igc_ctx {
@@ -122,9 +132,9 @@ struct igc_ctx {
     igc::DevBuf rcount, roffset; // per-tile digit counts and their scan
     // row filter: half counts (unsorted, sorted), bucket keys, permutation,
     // sorted packed words, window boxes and thresholds
-    igc::DevBuf feat, feat_s, nkey, perm, packed_s, fitems, fbias, fcount, corder;
-    uint32_t corder_n = 0; // string length corder was built for
-    bool prefilter = true; // exact row skipping in single-set builds
+    igc::DevBuf feat, feat_s, nkey, perm, packed_s, fitems, fbias, fcount;
+    igc::DevBuf corder[IGC_MAX_LEN + 1]; // class order along the curve, per string length
+    int prefilter = IGC_PREFILTER_AUTO; // exact row skipping (IGC_PREFILTER_*)
     igc::HostBuf host_rp, host_col, stage; // pinned: batch CSR copies, staged copies
     hipEvent_t ev[5] = {};
     igc_timing last{};
@@ -141,6 +151,10 @@ namespace igc {
 int launch_csr(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *counter, uint32_t *misc,
                uint32_t *launches, bool upper = false);
 Shape pick_shape(igc_ctx *ctx, uint32_t n, int K, uint32_t N);
+
+int filter_scratch(igc_ctx *ctx, uint32_t N, uint32_t n);
+int filter_set(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n, uint32_t window, const void *packed,
+               void *packed_s, uint2 *feat_s, uint32_t *perm, uint4 *items, uint32_t *bias, uint32_t *count);
 
 int kmeans_attach(igc_ctx *ctx);
 void kmeans_release(igc_ctx *ctx);

@@ -47,6 +47,20 @@ struct RowFilter {
     uint32_t limit;       // kItemBias + 4K: a row is kept when some item's SAD chain is at most this
 };
 
+__host__ __device__ inline RowFilter filter_view(const uint2 *feat, const uint4 *items, const uint32_t *bias,
+                                                 const uint32_t *count, const uint32_t *perm, uint32_t window, int K)
+{
+    RowFilter f{};
+    f.feat = feat;
+    f.items = items;
+    f.bias = bias;
+    f.count = count;
+    f.perm = perm;
+    f.window = window;
+    f.limit = kItemBias + 4u * (uint32_t)K;
+    return f;
+}
+
 __device__ __forceinline__ uint32_t half_counts(const uint8_t *s, uint32_t a, uint32_t b)
 {
     uint32_t c = 0;

@@ -125,6 +125,16 @@ Shape pick_shape(igc_ctx *ctx, uint32_t n, int K, uint32_t N)
     return t.cpl ? t : default_shape(N);
 }
 
+// Launch shape of a single-set build: with the row filter one register of
+// columns per lane, since the filter decides per window of 64 lanes' columns
+// (128 on 16-bit lanes) and wider windows skip far fewer rows.
+static Shape build_shape(igc_ctx *ctx, uint32_t n, int K, uint32_t N, bool filtered)
+{
+    Shape sh = pick_shape(ctx, n, K, N);
+    if (filtered) sh.cpl = 1;
+    return sh;
+}
+
 template <class Pol, int CPL, bool EMIT>
 static void launch_pairs_t(hipStream_t s, uint64_t tiles, const void *packed, const TileGeom &G, uint32_t node_off,
                            uint32_t n, int K, const RowFilter &f, uint64_t *edges, uint64_t cap,
@@ -291,71 +301,98 @@ int launch_csr(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *count
     return IGC_OK;
 }
 
-// Smallest set the row filter is set up for (below it the sort costs more than
-// the rows it saves).
-constexpr uint32_t kFilterMinN = 4096;
+// Class-order table of string length n on the device (built on first use).
+static int class_order_table(igc_ctx *ctx, uint32_t n, const uint32_t **out)
+{
+    DevBuf &t = ctx->corder[n];
+    if (!t.p) {
+        std::vector<uint32_t> order;
+        class_order(n, order);
+        int rc;
+        if ((rc = t.ensure(sizeof(uint32_t) * order.size()))) return rc;
+        HIP_TRY(hipMemcpyAsync(t.p, order.data(), sizeof(uint32_t) * order.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream)); // `order` is pageable and about to go
+    }
+    *out = t.as<uint32_t>();
+    return IGC_OK;
+}
 
-// Sorts the candidates by their half counts for the row filter (igc_prefilter.hpp)
-// and fills f; `window` is a wave's column span.  Asynchronous on ctx->stream.
-static int setup_filter(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n, int K, uint32_t window,
-                        RowFilter &f)
+// Scratch the per-set filter set-up needs for sets of up to N strings of length n.
+int filter_scratch(igc_ctx *ctx, uint32_t N, uint32_t n)
+{
+    const uint32_t h = n / 2;
+    const uint32_t B = comp_count(h) * comp_count(n - h); // at most 969^2 for n = 32
+    const uint32_t nsb = (B + kScanTile - 1) / kScanTile;
+    int rc;
+    if ((rc = ctx->feat.ensure(sizeof(uint2) * (size_t)N))) return rc;
+    if ((rc = ctx->nkey.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
+    if ((rc = ctx->rcount.ensure(sizeof(uint32_t) * (size_t)B))) return rc;
+    if ((rc = ctx->roffset.ensure(sizeof(uint64_t) * ((size_t)B + 1)))) return rc;
+    if ((rc = ctx->bsum.ensure(sizeof(uint64_t) * (size_t)nsb))) return rc;
+    const uint32_t *t;
+    return class_order_table(ctx, n, &t);
+}
+
+// Row-filter set-up of one set (igc_prefilter.hpp): sorts its N candidates
+// (symbols d_syms, packed words `packed`) by count class and writes the sorted
+// words to packed_s, the sorted counts to feat_s, sorted position -> local id
+// to perm, and the items of its windows of `window` columns to items, bias
+// and count.  Scratch from filter_scratch; asynchronous on ctx->stream.
+int filter_set(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n, uint32_t window, const void *packed,
+               void *packed_s, uint2 *feat_s, uint32_t *perm, uint4 *items, uint32_t *bias, uint32_t *count)
 {
     hipStream_t s = ctx->stream;
     const uint32_t h = n / 2;
-    const uint64_t B64 = (uint64_t)comp_count(h) * comp_count(n - h);
-    const uint32_t B = (uint32_t)B64; // at most 969^2 for n = 32
+    const uint32_t B = comp_count(h) * comp_count(n - h);
     const uint32_t nwin = (N + window - 1) / window;
-    const uint32_t nsb = (uint32_t)(((uint64_t)B + kScanTile - 1) / kScanTile);
+    const uint32_t nsb = (B + kScanTile - 1) / kScanTile;
+    const uint32_t *order;
     int rc;
-    if ((rc = ctx->feat.ensure(sizeof(uint2) * (size_t)N))) return rc;
+    if ((rc = class_order_table(ctx, n, &order))) return rc;
+    uint32_t *hist = ctx->rcount.as<uint32_t>();
+    uint64_t *off = ctx->roffset.as<uint64_t>();
+    const uint32_t g = (N + 255) / 256;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(B, 256, 4096)), dim3(256), 0, s, hist, (uint64_t)B, 0u);
+    hipLaunchKernelGGL(k_node_keys, dim3(g), dim3(256), 0, s, d_syms, N, n, order, ctx->feat.as<uint2>(),
+                       ctx->nkey.as<uint32_t>(), hist);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(kScanThreads), 0, s, hist, B, ctx->bsum.as<uint64_t>());
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, s, ctx->bsum.as<uint64_t>(), nsb);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(kScanThreads), 0, s, hist, B, ctx->bsum.as<uint64_t>(), off);
+    hipLaunchKernelGGL(k_node_scatter, dim3(g), dim3(256), 0, s, ctx->nkey.as<uint32_t>(), N, off, hist, perm);
+    if (wide_words(n))
+        hipLaunchKernelGGL(k_node_permute<uint64_t>, dim3(g), dim3(256), 0, s, perm, N,
+                           static_cast<const uint64_t *>(packed), ctx->feat.as<uint2>(),
+                           static_cast<uint64_t *>(packed_s), feat_s);
+    else
+        hipLaunchKernelGGL(k_node_permute<uint32_t>, dim3(g), dim3(256), 0, s, perm, N,
+                           static_cast<const uint32_t *>(packed), ctx->feat.as<uint2>(),
+                           static_cast<uint32_t *>(packed_s), feat_s);
+    hipLaunchKernelGGL(k_node_items, dim3(nwin), dim3(64), 0, s, feat_s, N, window, items, bias, count);
+    HIP_TRY(hipGetLastError());
+    return IGC_OK;
+}
+
+// Row filter of a single set: buffers, set-up, and the kernel's view in f;
+// `window` is a wave's column span.
+static int setup_filter(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n, int K, uint32_t window,
+                        RowFilter &f)
+{
+    const uint32_t nwin = (N + window - 1) / window;
+    int rc;
+    if ((rc = filter_scratch(ctx, N, n))) return rc;
     if ((rc = ctx->feat_s.ensure(sizeof(uint2) * (size_t)N))) return rc;
-    if ((rc = ctx->nkey.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
     if ((rc = ctx->perm.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
     if ((rc = ctx->packed_s.ensure(word_bytes(n) * (size_t)N))) return rc;
     if ((rc = ctx->fitems.ensure(sizeof(uint4) * kItemCap * (size_t)nwin))) return rc;
     if ((rc = ctx->fbias.ensure(sizeof(uint32_t) * kItemCap * (size_t)nwin))) return rc;
     if ((rc = ctx->fcount.ensure(sizeof(uint32_t) * (size_t)nwin))) return rc;
-    if (ctx->corder_n != n) { // class order along the curve, once per string length
-        std::vector<uint32_t> order;
-        class_order(n, order);
-        if ((rc = ctx->corder.ensure(sizeof(uint32_t) * order.size()))) return rc;
-        HIP_TRY(hipMemcpyAsync(ctx->corder.p, order.data(), sizeof(uint32_t) * order.size(), hipMemcpyHostToDevice,
-                               s));
-        HIP_TRY(hipStreamSynchronize(s)); // `order` is pageable and about to go
-        ctx->corder_n = n;
-    }
-    if ((rc = ctx->rcount.ensure(sizeof(uint32_t) * (size_t)B))) return rc;
-    if ((rc = ctx->roffset.ensure(sizeof(uint64_t) * ((size_t)B + 1)))) return rc;
-    if ((rc = ctx->bsum.ensure(sizeof(uint64_t) * (size_t)nsb))) return rc;
-    uint32_t *hist = ctx->rcount.as<uint32_t>();
-    uint64_t *off = ctx->roffset.as<uint64_t>();
-    const uint32_t g = (N + 255) / 256;
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(B, 256, 4096)), dim3(256), 0, s, hist, (uint64_t)B, 0u);
-    hipLaunchKernelGGL(k_node_keys, dim3(g), dim3(256), 0, s, d_syms, N, n, ctx->corder.as<uint32_t>(),
-                       ctx->feat.as<uint2>(), ctx->nkey.as<uint32_t>(), hist);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(kScanThreads), 0, s, hist, B, ctx->bsum.as<uint64_t>());
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, s, ctx->bsum.as<uint64_t>(), nsb);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(kScanThreads), 0, s, hist, B, ctx->bsum.as<uint64_t>(), off);
-    hipLaunchKernelGGL(k_node_scatter, dim3(g), dim3(256), 0, s, ctx->nkey.as<uint32_t>(), N, off, hist,
-                       ctx->perm.as<uint32_t>());
-    if (wide_words(n))
-        hipLaunchKernelGGL(k_node_permute<uint64_t>, dim3(g), dim3(256), 0, s, ctx->perm.as<uint32_t>(), N,
-                           ctx->packed.as<uint64_t>(), ctx->feat.as<uint2>(), ctx->packed_s.as<uint64_t>(),
-                           ctx->feat_s.as<uint2>());
-    else
-        hipLaunchKernelGGL(k_node_permute<uint32_t>, dim3(g), dim3(256), 0, s, ctx->perm.as<uint32_t>(), N,
-                           ctx->packed.as<uint32_t>(), ctx->feat.as<uint2>(), ctx->packed_s.as<uint32_t>(),
-                           ctx->feat_s.as<uint2>());
-    hipLaunchKernelGGL(k_node_items, dim3(nwin), dim3(64), 0, s, ctx->feat_s.as<uint2>(), N, window,
-                       ctx->fitems.as<uint4>(), ctx->fbias.as<uint32_t>(), ctx->fcount.as<uint32_t>());
-    HIP_TRY(hipGetLastError());
-    f.feat = ctx->feat_s.as<uint2>();
-    f.items = ctx->fitems.as<uint4>();
-    f.bias = ctx->fbias.as<uint32_t>();
-    f.count = ctx->fcount.as<uint32_t>();
-    f.perm = ctx->perm.as<uint32_t>();
-    f.window = window;
-    f.limit = kItemBias + 4u * (uint32_t)K;
+    if ((rc = filter_set(ctx, d_syms, N, n, window, ctx->packed.p, ctx->packed_s.p, ctx->feat_s.as<uint2>(),
+                         ctx->perm.as<uint32_t>(), ctx->fitems.as<uint4>(), ctx->fbias.as<uint32_t>(),
+                         ctx->fcount.as<uint32_t>())))
+        return rc;
+    f = filter_view(ctx->feat_s.as<uint2>(), ctx->fitems.as<uint4>(), ctx->fbias.as<uint32_t>(),
+                    ctx->fcount.as<uint32_t>(), ctx->perm.as<uint32_t>(), window, K);
     return IGC_OK;
 }
 
@@ -381,13 +418,13 @@ static int build_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, 
         return IGC_OK;
     }
     const int K = effective_k(n, edit_dist);
-    const Shape sh = pick_shape(ctx, n, K, N);
+    // exact row skipping by half counts: worth its sort for large sets with a
+    // nontrivial threshold
+    const bool use_filter = filter_pays(ctx->prefilter, N, n, K);
+    const Shape sh = build_shape(ctx, n, K, N, use_filter);
     const int cpl = sh.cpl;
     const TileGeom G = choose_geometry(N, sh.cpl * lane_cols(n), sh.q);
     const uint64_t tiles = tile_count(G);
-    // exact row skipping by half counts: worth its sort for large sets with a
-    // nontrivial threshold
-    const bool use_filter = ctx->prefilter && N >= kFilterMinN && K >= 1 && pred_kind(n, K) != PredKind::ALL;
 
     int rc;
     if ((rc = ctx->packed.ensure(word_bytes(n) * (size_t)N))) return rc;
@@ -503,8 +540,9 @@ IGC_API int igc_ctx_destroy(igc_ctx *ctx)
         (void)hipStreamSynchronize(ctx->stream);
         for (DevBuf *b : {&ctx->syms, &ctx->packed, &ctx->edges, &ctx->misc, &ctx->row_ptr, &ctx->col, &ctx->keys[0],
                           &ctx->keys[1], &ctx->rcount, &ctx->roffset, &ctx->bsum, &ctx->sets, &ctx->bounds,
-                          &ctx->feat, &ctx->feat_s, &ctx->nkey, &ctx->perm, &ctx->packed_s, &ctx->fitems, &ctx->fbias, &ctx->fcount, &ctx->corder})
+                          &ctx->feat, &ctx->feat_s, &ctx->nkey, &ctx->perm, &ctx->packed_s, &ctx->fitems, &ctx->fbias, &ctx->fcount})
             b->release();
+        for (DevBuf &b : ctx->corder) b.release();
         for (HostBuf *b : {&ctx->host_rp, &ctx->host_col, &ctx->stage}) b->release();
         kmeans_release(ctx);
         for (auto &e : ctx->ev)
@@ -525,11 +563,13 @@ IGC_API int igc_ctx_set_stream(igc_ctx *ctx, void *hip_stream)
 
 IGC_API void *igc_ctx_stream(igc_ctx *ctx) { return ctx ? static_cast<void *>(ctx->stream) : nullptr; }
 
-IGC_API int igc_ctx_set_prefilter(igc_ctx *ctx, int enable)
+IGC_API int igc_ctx_set_prefilter(igc_ctx *ctx, int mode)
 {
     if (!ctx) return set_error(IGC_ERR_INVALID, "null context");
+    if (mode < IGC_PREFILTER_OFF || mode > IGC_PREFILTER_ALWAYS)
+        return set_error(IGC_ERR_INVALID, "prefilter mode %d is not 0, 1 or 2", mode);
     std::lock_guard<std::mutex> lk(ctx->mu);
-    ctx->prefilter = enable != 0;
+    ctx->prefilter = mode;
     return IGC_OK;
 }
 
@@ -748,7 +788,8 @@ IGC_API int igc_pair_plan_of(igc_ctx *ctx, uint32_t n, uint32_t edit_dist, uint3
     if (edit_dist == 0) return set_error(IGC_ERR_INVALID, "editDist 0 launches no pair kernel");
     std::lock_guard<std::mutex> lk(ctx->mu);
     const int K = effective_k(n, edit_dist);
-    const Shape sh = pick_shape(ctx, n, K, N);
+    const bool filtered = filter_pays(ctx->prefilter, N, n, K);
+    const Shape sh = build_shape(ctx, n, K, N, filtered);
     const TileGeom G = choose_geometry(N, sh.cpl * lane_cols(n), sh.q);
     const PredKind pk = pred_kind(n, K);
     out->predicate = pk == PredKind::LV ? IGC_PRED_LANDAU_VISHKIN : (pk == PredKind::MYERS ? IGC_PRED_MYERS : IGC_PRED_ALL_PAIRS);
@@ -759,6 +800,8 @@ IGC_API int igc_pair_plan_of(igc_ctx *ctx, uint32_t n, uint32_t edit_dist, uint3
     out->tile_cols = G.W;
     out->tile_rows = G.R;
     out->tiles = tile_count(G);
+    out->row_filter = filtered ? 1 : 0;
+    out->pad = 0;
     return IGC_OK;
 }
 

@@ -62,14 +62,11 @@ def test_random_sets_every_length_and_threshold(gpu_ctx, n):
         assert np.array_equal(adj.col, col), (n, ed)
 
 
-@pytest.mark.parametrize("n", [5, 8, 11, 13, 16, 17, 24, 32])
-def test_prefilter_sets_match_oracle(gpu_ctx, n):
-    """Sets large enough for the row filter (sorted by half counts, rows skipped
-    per column window): same CSR as the oracle and as the unfiltered build."""
-    rng = np.random.default_rng(3000 + n)
-    N = 6000
-    syms = rng.integers(0, 4, size=(N, n), dtype=np.uint8)
-    for r in range(0, N // 3):  # near copies (substitutions, indels) so thresholds see edges
+def _near_copies(rng, syms, rows):
+    """Rows 0..rows-1 become copies of random rows with 1-3 substitutions or
+    indel pairs, so thresholds see edges."""
+    N, n = syms.shape
+    for r in range(rows):
         src = list(syms[int(rng.integers(0, N))])
         for _ in range(int(rng.integers(1, 4))):
             p = int(rng.integers(0, n))
@@ -79,23 +76,53 @@ def test_prefilter_sets_match_oracle(gpu_ctx, n):
                 del src[p]
                 src.insert(int(rng.integers(0, n)), int(rng.integers(0, 4)))
         syms[r] = src
+
+
+@pytest.mark.parametrize("n", [5, 8, 11, 13, 16, 17, 24, 32])
+def test_prefilter_sets_match_oracle(gpu_ctx, n):
+    """Sets sorted for the row filter (forced below its automatic size; rows
+    skipped per column window): same CSR as the oracle and as the unfiltered
+    build."""
+    rng = np.random.default_rng(3000 + n)
+    N = 6000
+    syms = rng.integers(0, 4, size=(N, n), dtype=np.uint8)
+    _near_copies(rng, syms, N // 3)
     syms[N // 3: N // 3 + 50] = 0  # one-symbol strings: all counts in one byte
-    for ed in sorted({1, 2, 3, 4, 5, n // 2, n - 1, n, n + 1}):
-        if ed < 1:
-            continue
-        banner(f"prefilter n={n} min_ed={ed} N={N}")
-        rp, col = O.graph_csr(syms, ed)
-        adj = gpu_ctx.graph_build(syms, ed)
-        assert np.array_equal(adj.row_ptr, rp), (n, ed)
-        assert np.array_equal(adj.col, col), (n, ed)
-        assert gpu_ctx.graph_count(syms, ed) == col.size // 2
-    gpu_ctx.set_prefilter(False)
+    gpu_ctx.set_prefilter(igc.IGC_PREFILTER_ALWAYS)
     try:
+        for ed in sorted({1, 2, 3, 4, 5, n // 2, n - 1, n, n + 1}):
+            if ed < 1:
+                continue
+            banner(f"prefilter n={n} min_ed={ed} N={N}")
+            rp, col = O.graph_csr(syms, ed)
+            adj = gpu_ctx.graph_build(syms, ed)
+            assert np.array_equal(adj.row_ptr, rp), (n, ed)
+            assert np.array_equal(adj.col, col), (n, ed)
+            assert gpu_ctx.graph_count(syms, ed) == col.size // 2
+        gpu_ctx.set_prefilter(False)
         adj_off = gpu_ctx.graph_build(syms, 4)
     finally:
         gpu_ctx.set_prefilter(True)
     rp, col = O.graph_csr(syms, 4)
     assert np.array_equal(adj_off.col, col)
+
+
+def test_prefilter_modes(gpu_ctx):
+    """The automatic mode sorts only sets of at least IGC_PREFILTER_MIN_N
+    strings; bad modes are rejected; every mode gives the same graph."""
+    assert igc.IGC_PREFILTER_MIN_N == 16384
+    with pytest.raises(igc.IGCError):
+        gpu_ctx.set_prefilter(3)
+    rng = np.random.default_rng(65)
+    syms = rng.integers(0, 4, size=(igc.IGC_PREFILTER_MIN_N, 11), dtype=np.uint8)
+    _near_copies(rng, syms, 2000)
+    graphs = []
+    for mode in (igc.IGC_PREFILTER_OFF, igc.IGC_PREFILTER_ALWAYS, igc.IGC_PREFILTER_AUTO):
+        gpu_ctx.set_prefilter(mode)
+        graphs.append(gpu_ctx.graph_build(syms, 3))
+    gpu_ctx.set_prefilter(True)
+    for g in graphs[1:]:
+        assert np.array_equal(g.row_ptr, graphs[0].row_ptr) and np.array_equal(g.col, graphs[0].col)
 
 
 def test_edge_cases(gpu_ctx):
@@ -233,6 +260,38 @@ def test_batch_matches_single_and_oracle(gpu_ctx):
             assert np.array_equal(adj.col, col)
 
 
+def test_batch_prefilter_sets_match_oracle(gpu_ctx):
+    """Sets forced through the row filter are sorted and row-filtered inside the batch
+    (Landau-Vishkin and Myers sets in the persistent kernel, n > 16 sets in
+    their own launches): the same CSRs as the oracle, and as the batch with the
+    filter switched off."""
+    rng = np.random.default_rng(64)
+    sets = []
+    for n, N, ed in [(13, 6000, 4), (16, 4096, 5), (10, 5000, 3), (8, 4500, 2), (11, 5000, 9), (24, 4200, 6),
+                     (32, 4100, 9), (12, 300, 3), (14, 4300, 15)]:
+        syms = rng.integers(0, 4, size=(N, n), dtype=np.uint8)
+        _near_copies(rng, syms, N // 3)
+        syms[N // 2: N // 2 + 20] = 1  # one-symbol strings: all counts in one byte
+        sets.append((syms, ed))
+    banner(f"prefiltered batch of {len(sets)} sets")
+    gpu_ctx.set_prefilter(igc.IGC_PREFILTER_ALWAYS)
+    try:
+        outs = gpu_ctx.graph_build_batch(sets)
+    finally:
+        gpu_ctx.set_prefilter(True)
+    for (syms, ed), adj in zip(sets, outs):
+        rp, col = O.graph_csr(syms, ed)
+        assert np.array_equal(adj.row_ptr, rp), (syms.shape, ed)
+        assert np.array_equal(adj.col, col), (syms.shape, ed)
+    gpu_ctx.set_prefilter(False)
+    try:
+        outs_off = gpu_ctx.graph_build_batch(sets)
+    finally:
+        gpu_ctx.set_prefilter(True)
+    for a, b in zip(outs, outs_off):
+        assert np.array_equal(a.row_ptr, b.row_ptr) and np.array_equal(a.col, b.col)
+
+
 def test_batched_config_sets_alone_and_in_the_mix(gpu_ctx):
     # a scaled-down batched workload: each set's graph is the same alone and in the batch
     sets = W.batched(seed=3, sets=24, scale=0.05)
@@ -296,7 +355,11 @@ def test_upper_triangle_csr(gpu_ctx):
         syms = rng.integers(0, 4, size=(N, n), dtype=np.uint8)
         syms[: N // 4] = syms[N // 4: 2 * (N // 4)]
         full = gpu_ctx.graph_build(syms, ed)
-        up = gpu_ctx.graph_build(syms, ed, igc.IGC_UPPER_TRIANGLE)
+        gpu_ctx.set_prefilter(igc.IGC_PREFILTER_ALWAYS if N == 6000 else igc.IGC_PREFILTER_AUTO)
+        try:
+            up = gpu_ctx.graph_build(syms, ed, igc.IGC_UPPER_TRIANGLE)
+        finally:
+            gpu_ctx.set_prefilter(True)
         assert up.n_edges == full.n_edges and up.col.size == full.n_edges
         for x in range(0, N, max(1, N // 200)):
             row = full.col[full.row_ptr[x]:full.row_ptr[x + 1]]
@@ -408,6 +471,9 @@ def test_autotuned_shapes_give_identical_graphs(gpu_ctx):
     assert plan["predicate"] == "landau_vishkin" and plan["k"] == 3 and plan["lane_cols"] == 2
     assert (plan["slots"], plan["q"]) == (pick["cpl"], pick["q"])
     assert plan["tile_cols"] == 256 * plan["slots"] * 2 and plan["tile_rows"] == plan["tile_cols"] // plan["q"]
+    assert not plan["row_filter"]
+    big = gpu_ctx.pair_plan(12, 4, 200000)  # filtered: one register of columns per lane
+    assert big["row_filter"] and big["slots"] == 1
     assert gpu_ctx.pair_plan(24, 20, 100)["predicate"] == "myers"
     assert gpu_ctx.pair_plan(24, 40, 100)["predicate"] == "all_pairs"
     adj = gpu_ctx.graph_build(syms, 4)  # built with the tuned shape
