@@ -1,18 +1,19 @@
-// igc_batch.hip -- many uneven candidate sets in one persistent launch.
+// igc_batch.hip -- many uneven candidate sets in a few launches.
 //
 // The sets are treated as one disjoint union of graphs on a concatenated node
 // space (set s owns global ids node_off[s] .. node_off[s]+N_s-1).  One pack
 // kernel packs every set (per-set length n, each set's words at its own byte
-// offset: 32-bit words for n <= 16, 64-bit above), one persistent pair kernel
-// pulls tiles of all sets with n <= 16 from a global work counter (the set of a
-// tile is found by binary search over per-set tile prefix sums; the predicate
-// and K are uniform per tile and dispatched at run time), sets with n > 16 get
-// one pair launch each into the same edge list, and the unchanged CSR assembly
-// runs once over the union.  Set s's CSR is the row slice node_off[s].. with
-// ids shifted back.  Sets large enough for the exact row filter
-// (igc_prefilter.hpp) are sorted by count class first, each into its own
-// slices of the sorted-word, count, permutation and window-item arrays; their
-// tiles read the sorted words and emit ids through the permutation.
+// offset: 32-bit words for n <= 16, 64-bit above).  Sets with the same pair
+// predicate (Landau-Vishkin at K, Myers, every pair; on 32- or 64-bit words)
+// form a group, and each group is one launch of the single-set tile code over
+// the concatenated tile lists of its sets (workgroup -> set by binary search
+// over per-set tile prefix sums, largest sets first so the tail is cheap),
+// all into one edge list; the unchanged CSR assembly then runs once over the
+// union.  Set s's CSR is the row slice node_off[s].. with ids shifted back.
+// Sets large enough for the exact row filter (igc_prefilter.hpp) are sorted
+// by count class first, each into its own slices of the sorted-word, count,
+// permutation and window-item arrays; their tiles read the sorted words and
+// emit ids through the permutation.
 //
 // This replaces the reference's one process + one GPU context + one launch per
 // cluster (SolveIndependentSet, src/SparseMat.cpp:784-851, called from the
@@ -27,25 +28,17 @@
 
 namespace igc {
 
-// Predicates the persistent kernel dispatches at run time (sets with n <= 16,
-// two pairs per register): Landau-Vishkin for K <= kBatchMaxK (pred_kind never
-// picks it above 9 for n <= 16), Myers, every pair.
-constexpr int kBatchMaxK = 9;
-enum BatchPred : uint32_t { kBatchLV = 0, kBatchMyers = 1, kBatchAll = 2, kBatchNone = 3 };
-
 struct SetDesc {
     TileGeom G;          // per-set geometry (G.N = N_s)
-    uint64_t tile_begin; // first global tile of the set
+    uint64_t tile_begin; // first tile of the set within its group's launch
     uint64_t sym_off;    // byte offset of the set's symbols in the concatenated input
     uint64_t word_off;   // byte offset of the set's packed words (8-byte aligned)
     uint32_t node_off;   // first global node id
     uint32_t n;          // string length
     int32_t K;           // Levenshtein threshold (edge iff ED <= K); kAllPairs = every pair
-    uint32_t pred;       // BatchPred of the persistent kernel; kBatchNone = not in it
     uint32_t filt;       // 1: row filter (sorted words at word_off of the sorted array)
     uint32_t wbase;      // first window of the set in the item arrays
     uint32_t window;     // columns per window
-    uint32_t pad;
 };
 
 // Device arrays of the row filter shared by all filtered sets of a batch.
@@ -88,69 +81,32 @@ static __global__ void k_pack_batch(const uint8_t *__restrict__ syms, const SetD
     if (or_all > 3u) atomicOr(bad, 1u);
 }
 
-template <int CPL>
-__device__ __forceinline__ void batch_tile(const SetDesc &d, const uint32_t *__restrict__ packed,
-                                           const FilterBase &fb, uint32_t rb, uint32_t cb, uint64_t *st,
-                                           uint32_t lane, uint32_t wave, uint64_t *__restrict__ edges, uint64_t cap,
-                                           unsigned long long *__restrict__ counter)
+// One group: workgroup tile0 + blockIdx.x is a tile of the set whose tile
+// range holds it; the set's predicate is the group's (Pol), one register of
+// columns per lane.
+template <class Pol>
+__global__ __launch_bounds__(256) void k_pairs_sets(const uint8_t *__restrict__ packed,
+                                                    const uint8_t *__restrict__ packed_s, FilterBase fb,
+                                                    const SetDesc *__restrict__ sets, uint32_t nsets, uint64_t tile0,
+                                                    uint64_t *__restrict__ edges, uint64_t cap,
+                                                    unsigned long long *__restrict__ counter)
 {
+    __shared__ uint64_t stage[kWaves][kStage];
+    const uint32_t lane = lane_id();
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint64_t t = tile0 + blockIdx.x;
+    const uint32_t s = find_set_by(sets, nsets, t, false);
+    const SetDesc d = sets[s];
+    uint32_t rb, cb;
+    decode_tile(t - d.tile_begin, d.G, rb, cb);
     RowFilter flt{};
     if (d.filt)
         flt = filter_view(fb.feat + d.node_off, fb.items + (size_t)d.wbase * kItemCap,
-                           fb.bias + (size_t)d.wbase * kItemCap, fb.count + d.wbase, fb.perm + d.node_off, d.window,
-                           d.K);
-    if (d.pred == kBatchMyers) {
-        wave_tile<PolMyers<uint32_t>, CPL, true>(packed, d.G, rb, cb, d.node_off, d.n, d.K, flt, st, lane, wave, edges,
-                                                 cap, counter);
-        return;
-    }
-    if (d.pred == kBatchAll) {
-        wave_tile<PolAll<uint32_t>, CPL, true>(packed, d.G, rb, cb, d.node_off, d.n, d.K, flt, st, lane, wave, edges,
-                                               cap, counter);
-        return;
-    }
-    switch (d.K) {
-#define IGC_BCASE(k)                                                                                          \
-    case k:                                                                                                   \
-        wave_tile<PolLV<k, uint32_t>, CPL, true>(packed, d.G, rb, cb, d.node_off, d.n, d.K, flt, st, lane, wave, \
-                                                 edges, cap, counter);                                        \
-        break;
-        IGC_BCASE(0) IGC_BCASE(1) IGC_BCASE(2) IGC_BCASE(3) IGC_BCASE(4) IGC_BCASE(5) IGC_BCASE(6)
-        IGC_BCASE(7) IGC_BCASE(8) IGC_BCASE(9)
-#undef IGC_BCASE
-    default:
-        break;
-    }
-}
-
-// Persistent: each workgroup loops, taking the next tile from the work
-// counter until all `total_tiles` are done.  Tiles are ordered heaviest set
-// first (host side), so the tail is made of cheap tiles.
-template <int CPL>
-__global__ __launch_bounds__(256) void k_pairs_batch(const uint8_t *__restrict__ packed,
-                                                     const uint8_t *__restrict__ packed_s, FilterBase fb,
-                                                     const SetDesc *__restrict__ sets, uint32_t nsets,
-                                                     uint64_t total_tiles, unsigned long long *__restrict__ work,
-                                                     uint64_t *__restrict__ edges, uint64_t cap,
-                                                     unsigned long long *__restrict__ counter)
-{
-    __shared__ uint64_t stage[kWaves][kStage];
-    __shared__ unsigned long long next;
-    const uint32_t lane = lane_id();
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    for (;;) {
-        if (threadIdx.x == 0) next = atomicAdd(work, 1ull);
-        __syncthreads();
-        const uint64_t t = uniform64(next);
-        __syncthreads();
-        if (t >= total_tiles) break;
-        const uint32_t s = find_set_by(sets, nsets, t, false);
-        const SetDesc d = sets[s];
-        uint32_t rb, cb;
-        decode_tile(t - d.tile_begin, d.G, rb, cb);
-        batch_tile<CPL>(d, reinterpret_cast<const uint32_t *>((d.filt ? packed_s : packed) + d.word_off), fb, rb, cb,
-                        stage[wave], lane, wave, edges, cap, counter);
-    }
+                          fb.bias + (size_t)d.wbase * kItemCap, fb.count + d.wbase, fb.perm + d.node_off, d.window,
+                          d.K);
+    const auto *words = reinterpret_cast<const typename Pol::Word *>((d.filt ? packed_s : packed) + d.word_off);
+    wave_tile<Pol, 1, true>(words, d.G, rb, cb, d.node_off, d.n, d.K, flt, stage[wave], lane, wave, edges, cap,
+                            counter);
 }
 
 // Gathers row_ptr at the set boundaries so the host can split the union CSR.
@@ -162,15 +118,24 @@ static __global__ void k_gather_bounds(const uint64_t *__restrict__ row_ptr, con
     if (s == nsets) out[s] = row_ptr[total];
 }
 
+// Sets of one pair predicate, launched together.
+struct BatchGroup {
+    PredKind pk;
+    int K;             // Landau-Vishkin threshold (pk == LV)
+    bool wide;         // 64-bit words (n > 16)
+    uint32_t first;    // first descriptor of the group in BatchPlan::desc
+    uint32_t count;    // descriptors in the group
+    uint64_t tiles;    // tiles of the group's launch
+};
+
 struct BatchPlan {
-    std::vector<SetDesc> desc;     // in processing order (heaviest first); tile_begin ascending
-    std::vector<SetDesc> by_node;  // the same descriptors in caller order; node_off ascending
+    std::vector<SetDesc> desc;     // by group, largest sets first within a group; tile_begin per group
+    std::vector<SetDesc> by_node;  // every set's descriptor in caller order; node_off ascending
+    std::vector<BatchGroup> groups;
     std::vector<uint32_t> order;   // desc index -> caller's set index
     std::vector<uint32_t> node_off_by_set;
-    std::vector<uint32_t> extra;   // desc indices launched separately (n > 16)
     std::vector<uint32_t> filtered;// desc indices with the row filter
     uint64_t windows = 0;          // filter windows of all filtered sets
-    uint64_t tiles = 0;            // tiles of the persistent kernel
     uint64_t pairs = 0;
     uint64_t sym_bytes = 0;
     uint64_t word_bytes = 0;       // packed words of all sets
@@ -178,12 +143,12 @@ struct BatchPlan {
     uint64_t cap = 0;
 };
 
-static int plan_batch(igc_ctx *ctx, const igc_set *sets, uint32_t count, int cpl, BatchPlan &P)
+// Plans a batch: per-set geometry (one register of columns per lane, rows per
+// tile by set size), groups by predicate, filter windows, edge estimate.
+static int plan_batch(igc_ctx *ctx, const igc_set *sets, uint32_t count, BatchPlan &P)
 {
     P = BatchPlan();
     P.node_off_by_set.assign(count + 1, 0);
-    std::vector<uint32_t> idx(count);
-    std::iota(idx.begin(), idx.end(), 0u);
     std::vector<int> K(count);
     uint64_t total = 0, symb = 0, wordb = 0;
     std::vector<uint64_t> sym_off(count), word_off(count);
@@ -206,70 +171,114 @@ static int plan_batch(igc_ctx *ctx, const igc_set *sets, uint32_t count, int cpl
     P.total = (uint32_t)total;
     P.sym_bytes = symb;
     P.word_bytes = wordb;
-    // heaviest tiles first: costlier predicate per pair first, then larger sets
-    auto cost = [&](uint32_t s) -> uint32_t {
-        if (K[s] < 0) return 0;
-        const bool wide = wide_words(sets[s].n);
-        switch (pred_kind(sets[s].n, K[s])) {
-        case PredKind::LV: return lv_cost2(K[s], wide);
-        case PredKind::MYERS: return myers_cost2(sets[s].n, wide);
-        default: return 1;
-        }
+    P.by_node.resize(count);
+    // group key: predicate kind, K for Landau-Vishkin, word width
+    auto key = [&](uint32_t s) -> int {
+        const PredKind pk = pred_kind(sets[s].n, K[s]);
+        const int k = pk == PredKind::LV ? K[s] : 0;
+        return ((int)pk * 64 + k) * 2 + (wide_words(sets[s].n) ? 1 : 0);
     };
-    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
-        const uint32_t ca = cost(a), cb = cost(b);
-        if (ca != cb) return ca > cb;
-        return sets[a].N > sets[b].N;
-    });
-    double est_edges = 0;
-    for (uint32_t s : idx) {
-        const igc_set &x = sets[s];
+    std::vector<uint32_t> run; // runnable sets, by group then size (largest first)
+    for (uint32_t s = 0; s < count; ++s) {
         SetDesc d{};
-        const Shape sh{cpl, 1};
-        d.G = choose_geometry(x.N, sh.cpl * lane_cols(x.n), x.N >= 8192 ? 1u : (x.N >= 2048 ? 2u : 4u));
+        const igc_set &x = sets[s];
+        d.G = choose_geometry(x.N, lane_cols(x.n), x.N >= 8192 ? 1u : (x.N >= 2048 ? 2u : 4u));
         d.sym_off = sym_off[s];
         d.word_off = word_off[s];
         d.node_off = P.node_off_by_set[s];
         d.n = x.n;
         d.K = K[s];
-        d.pred = kBatchNone;
-        if (K[s] >= 0 && x.N >= 2) est_edges += (double)initial_edge_cap(x.N, x.n, K[s]);
-        const bool runnable = K[s] >= 0 && x.N >= 2;
-        if (runnable && !wide_words(x.n)) {
+        P.by_node[s] = d;
+        if (K[s] >= 0 && x.N >= 2) run.push_back(s);
+    }
+    std::stable_sort(run.begin(), run.end(), [&](uint32_t a, uint32_t b) {
+        const int ka = key(a), kb = key(b);
+        if (ka != kb) return ka < kb;
+        return sets[a].N > sets[b].N;
+    });
+    double est_edges = 0;
+    for (size_t r = 0; r < run.size(); ++r) {
+        const uint32_t s = run[r];
+        const igc_set &x = sets[s];
+        SetDesc d = P.by_node[s];
+        if (r == 0 || key(s) != key(run[r - 1])) {
             const PredKind pk = pred_kind(x.n, K[s]);
-            if (pk == PredKind::ALL) d.pred = kBatchAll;
-            else if (pk == PredKind::MYERS) d.pred = kBatchMyers;
-            else if (K[s] <= kBatchMaxK) d.pred = kBatchLV;
+            P.groups.push_back(BatchGroup{pk, pk == PredKind::LV ? K[s] : 0, wide_words(x.n), (uint32_t)P.desc.size(), 0, 0});
         }
-        const bool in_kernel = d.pred != kBatchNone;
-        if (runnable && filter_pays(ctx->prefilter, x.N, x.n, K[s])) {
+        BatchGroup &g = P.groups.back();
+        if (filter_pays(ctx->prefilter, x.N, x.n, K[s])) {
             d.filt = 1;
             d.wbase = (uint32_t)P.windows;
-            d.window = 64u * (uint32_t)cpl * (uint32_t)lane_cols(x.n);
+            d.window = 64u * (uint32_t)lane_cols(x.n);
             P.windows += (x.N + d.window - 1) / d.window;
             P.filtered.push_back((uint32_t)P.desc.size());
         }
-        d.tile_begin = P.tiles;
-        if (in_kernel) P.tiles += tile_count(d.G);
-        if (runnable && !in_kernel) P.extra.push_back((uint32_t)P.desc.size());
+        d.tile_begin = g.tiles;
+        g.tiles += tile_count(d.G);
+        g.count++;
+        est_edges += (double)initial_edge_cap(x.N, x.n, K[s]);
+        P.by_node[s] = d;
         P.order.push_back(s);
         P.desc.push_back(d);
     }
-    P.by_node.resize(P.desc.size());
-    for (size_t k = 0; k < P.desc.size(); ++k) P.by_node[P.order[k]] = P.desc[k];
     P.cap = std::max<uint64_t>((uint64_t)est_edges, 1);
     if (P.windows > 0xFFFFFFFFull) return set_error(IGC_ERR_INVALID, "batch has too many filter windows");
     return IGC_OK;
 }
 
-template <int CPL>
-static void launch_batch_kernel(igc_ctx *ctx, const BatchPlan &P, const FilterBase &fb, const SetDesc *d_sets,
-                                unsigned long long *work, unsigned long long *counter, uint64_t cap)
+template <class Pol>
+static void launch_sets_t(hipStream_t s, const uint8_t *packed, const uint8_t *packed_s, const FilterBase &fb,
+                          const SetDesc *d_sets, uint32_t nsets, uint64_t tile0, uint32_t ntiles, uint64_t *edges,
+                          uint64_t cap, unsigned long long *counter)
 {
-    const uint32_t grid = (uint32_t)std::min<uint64_t>(P.tiles, (uint64_t)ctx->cu_count * 8);
-    hipLaunchKernelGGL((k_pairs_batch<CPL>), dim3(grid), dim3(256), 0, ctx->stream, ctx->packed.as<uint8_t>(),
-                       ctx->packed_s.as<uint8_t>(), fb, d_sets, (uint32_t)P.desc.size(), P.tiles, work,
-                       ctx->edges.as<uint64_t>(), cap, counter);
+    hipLaunchKernelGGL((k_pairs_sets<Pol>), dim3(ntiles), dim3(256), 0, s, packed, packed_s, fb, d_sets, nsets, tile0,
+                       edges, cap, counter);
+}
+
+template <class W>
+static int launch_sets_w(const BatchGroup &g, hipStream_t s, const uint8_t *packed, const uint8_t *packed_s,
+                         const FilterBase &fb, const SetDesc *d_sets, uint64_t tile0, uint32_t ntiles,
+                         uint64_t *edges, uint64_t cap, unsigned long long *counter)
+{
+    if (g.pk == PredKind::MYERS) {
+        launch_sets_t<PolMyers<W>>(s, packed, packed_s, fb, d_sets, g.count, tile0, ntiles, edges, cap, counter);
+        return IGC_OK;
+    }
+    if (g.pk == PredKind::ALL) {
+        launch_sets_t<PolAll<W>>(s, packed, packed_s, fb, d_sets, g.count, tile0, ntiles, edges, cap, counter);
+        return IGC_OK;
+    }
+    switch (g.K) {
+#define IGC_GCASE(k)                                                                                               \
+    case k:                                                                                                        \
+        launch_sets_t<PolLV<k, W>>(s, packed, packed_s, fb, d_sets, g.count, tile0, ntiles, edges, cap, counter);  \
+        return IGC_OK;
+        IGC_GCASE(0) IGC_GCASE(1) IGC_GCASE(2) IGC_GCASE(3) IGC_GCASE(4) IGC_GCASE(5) IGC_GCASE(6) IGC_GCASE(7)
+        IGC_GCASE(8) IGC_GCASE(9) IGC_GCASE(10) IGC_GCASE(11) IGC_GCASE(12) IGC_GCASE(13)
+#undef IGC_GCASE
+    default:
+        return set_error(IGC_ERR_INVALID, "no Landau-Vishkin kernel for K=%d", g.K);
+    }
+}
+
+// The group's tiles in launches of at most 2^31 - 1 workgroups.
+static int launch_group(igc_ctx *ctx, const BatchGroup &g, const FilterBase &fb, const SetDesc *d_group,
+                        uint64_t *edges, uint64_t cap, unsigned long long *counter, uint32_t *launches)
+{
+    constexpr uint64_t kMaxGrid = 0x7FFFFFFFull;
+    for (uint64_t t0 = 0; t0 < g.tiles; t0 += kMaxGrid) {
+        const uint32_t nt = (uint32_t)std::min<uint64_t>(g.tiles - t0, kMaxGrid);
+        const int rc = g.wide ? launch_sets_w<uint64_t>(g, ctx->stream, ctx->packed.as<uint8_t>(),
+                                                        ctx->packed_s.as<uint8_t>(), fb, d_group, t0, nt, edges, cap,
+                                                        counter)
+                              : launch_sets_w<uint32_t>(g, ctx->stream, ctx->packed.as<uint8_t>(),
+                                                        ctx->packed_s.as<uint8_t>(), fb, d_group, t0, nt, edges, cap,
+                                                        counter);
+        if (rc) return rc;
+        HIP_TRY(hipGetLastError());
+        ++*launches;
+    }
+    return IGC_OK;
 }
 
 // Buffers and scratch of the row filter for the batch's filtered sets, all
@@ -300,36 +309,36 @@ static int batch_filter_buffers(igc_ctx *ctx, const BatchPlan &P, FilterBase &fb
 // Builds the union CSR of all sets in ctx->row_ptr / ctx->col; fills
 // bounds_out[0..count] (row_ptr at each set's first row, in caller order) and
 // the union edge count.  d_syms: concatenated symbols already on the device.
-static int batch_device_locked(igc_ctx *ctx, const uint8_t *d_syms, const BatchPlan &Pin, int cpl,
+static int batch_device_locked(igc_ctx *ctx, const uint8_t *d_syms, const BatchPlan &Pin,
                                std::vector<uint64_t> &bounds_out, uint64_t *n_edges)
 {
     BatchPlan P = Pin;
     hipStream_t s = ctx->stream;
     igc_timing tm{};
     tm.pairs = P.pairs;
-    const uint32_t nd = (uint32_t)P.desc.size();
+    const uint32_t nr = (uint32_t)P.desc.size();    // runnable sets, by group
+    const uint32_t nset = (uint32_t)P.by_node.size(); // every set, caller order
     int rc;
     if ((rc = ctx->packed.ensure(std::max<size_t>(P.word_bytes, 8)))) return rc;
     if ((rc = ctx->misc.ensure(sizeof(uint32_t) * kMiscWords))) return rc;
-    if ((rc = ctx->sets.ensure(2 * sizeof(SetDesc) * std::max<size_t>(nd, 1)))) return rc;
-    if ((rc = ctx->bounds.ensure(sizeof(uint64_t) * ((size_t)nd + 1)))) return rc;
-    const SetDesc *d_tile_order = ctx->sets.as<SetDesc>();
-    const SetDesc *d_node_order = ctx->sets.as<SetDesc>() + nd;
-    HIP_TRY(hipMemcpyAsync(ctx->sets.p, P.desc.data(), sizeof(SetDesc) * nd, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(ctx->sets.as<SetDesc>() + nd, P.by_node.data(), sizeof(SetDesc) * nd, hipMemcpyHostToDevice,
-                           s));
+    if ((rc = ctx->sets.ensure(sizeof(SetDesc) * std::max<size_t>((size_t)nr + nset, 1)))) return rc;
+    if ((rc = ctx->bounds.ensure(sizeof(uint64_t) * ((size_t)nset + 1)))) return rc;
+    const SetDesc *d_groups = ctx->sets.as<SetDesc>();
+    const SetDesc *d_node_order = ctx->sets.as<SetDesc>() + nr;
+    if (nr) HIP_TRY(hipMemcpyAsync(ctx->sets.p, P.desc.data(), sizeof(SetDesc) * nr, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(ctx->sets.as<SetDesc>() + nr, P.by_node.data(), sizeof(SetDesc) * nset,
+                           hipMemcpyHostToDevice, s));
     FilterBase fb;
     if ((rc = batch_filter_buffers(ctx, P, fb))) return rc;
     uint32_t *misc = ctx->misc.as<uint32_t>();
     unsigned long long *counter = reinterpret_cast<unsigned long long *>(misc);
-    unsigned long long *work = reinterpret_cast<unsigned long long *>(misc + 6);
     uint64_t cap = P.cap;
     for (int attempt = 0; attempt < 3; ++attempt) {
         if ((rc = ctx->edges.ensure(sizeof(uint64_t) * (size_t)cap))) return rc;
         HIP_TRY(hipEventRecord(ctx->ev[0], s));
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, misc, (uint64_t)kMiscWords, 0u);
         if (P.total)
-            hipLaunchKernelGGL(k_pack_batch, dim3((P.total + 255) / 256), dim3(256), 0, s, d_syms, d_node_order, nd,
+            hipLaunchKernelGGL(k_pack_batch, dim3((P.total + 255) / 256), dim3(256), 0, s, d_syms, d_node_order, nset,
                                P.total, ctx->packed.as<uint8_t>(), misc + 2);
         HIP_TRY(hipGetLastError());
         uint32_t launches = 2;
@@ -345,25 +354,9 @@ static int batch_device_locked(igc_ctx *ctx, const uint8_t *d_syms, const BatchP
             launches += 8;
         }
         HIP_TRY(hipEventRecord(ctx->ev[1], s));
-        if (P.tiles) {
-            if (cpl == 1) launch_batch_kernel<1>(ctx, P, fb, d_tile_order, work, counter, cap);
-            else if (cpl == 2) launch_batch_kernel<2>(ctx, P, fb, d_tile_order, work, counter, cap);
-            else launch_batch_kernel<4>(ctx, P, fb, d_tile_order, work, counter, cap);
-            HIP_TRY(hipGetLastError());
-            launches++;
-        }
-        for (uint32_t e : P.extra) {
-            const SetDesc &d = P.desc[e];
-            const RowFilter f = d.filt ? filter_view(fb.feat + d.node_off, fb.items + (size_t)d.wbase * kItemCap,
-                                                     fb.bias + (size_t)d.wbase * kItemCap, fb.count + d.wbase,
-                                                     fb.perm + d.node_off, d.window, d.K)
-                                       : RowFilter{};
-            const uint8_t *words = (d.filt ? ctx->packed_s.as<uint8_t>() : ctx->packed.as<uint8_t>()) + d.word_off;
-            if ((rc = launch_pairs(d.n, d.K, cpl, true, s, tile_count(d.G), words, d.G, d.node_off,
-                                   ctx->edges.as<uint64_t>(), cap, counter, f)))
+        for (const BatchGroup &g : P.groups)
+            if ((rc = launch_group(ctx, g, fb, d_groups + g.first, ctx->edges.as<uint64_t>(), cap, counter, &launches)))
                 return rc;
-            launches++;
-        }
         HIP_TRY(hipEventRecord(ctx->ev[2], s));
         uint32_t host_misc[kMiscWords];
         HIP_TRY(hipMemcpyAsync(host_misc, misc, sizeof host_misc, hipMemcpyDeviceToHost, s));
@@ -375,12 +368,12 @@ static int batch_device_locked(igc_ctx *ctx, const uint8_t *d_syms, const BatchP
             continue;
         }
         if ((rc = launch_csr(ctx, P.total, E, counter, misc, &launches))) return rc;
-        hipLaunchKernelGGL(k_gather_bounds, dim3((nd + 1 + 255) / 256), dim3(256), 0, s, ctx->row_ptr.as<uint64_t>(),
-                           d_node_order, nd, P.total, ctx->bounds.as<uint64_t>());
+        hipLaunchKernelGGL(k_gather_bounds, dim3((nset + 1 + 255) / 256), dim3(256), 0, s, ctx->row_ptr.as<uint64_t>(),
+                           d_node_order, nset, P.total, ctx->bounds.as<uint64_t>());
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(ctx->ev[3], s));
-        std::vector<uint64_t> b(nd + 1);
-        HIP_TRY(hipMemcpyAsync(b.data(), ctx->bounds.p, sizeof(uint64_t) * (nd + 1), hipMemcpyDeviceToHost, s));
+        std::vector<uint64_t> b(nset + 1);
+        HIP_TRY(hipMemcpyAsync(b.data(), ctx->bounds.p, sizeof(uint64_t) * (nset + 1), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         HIP_TRY(hipEventElapsedTime(&tm.pack_ms, ctx->ev[0], ctx->ev[1]));
         HIP_TRY(hipEventElapsedTime(&tm.pairs_ms, ctx->ev[1], ctx->ev[2]));
@@ -431,20 +424,6 @@ static int split_set(const uint64_t *rp, const uint32_t *col, uint32_t off, uint
     return IGC_OK;
 }
 
-// Columns per lane of a batch: one register (128-column filter windows) when
-// some set gets the row filter, else wider for large sets.
-static int batch_shape(const igc_ctx *ctx, const igc_set *sets, uint32_t count)
-{
-    uint64_t maxN = 0;
-    for (uint32_t s = 0; s < count; ++s) {
-        maxN = std::max<uint64_t>(maxN, sets[s].N);
-        if (sets[s].edit_dist > 0 && sets[s].n >= 1 && sets[s].n <= IGC_MAX_LEN &&
-            filter_pays(ctx->prefilter, sets[s].N, sets[s].n, effective_k(sets[s].n, sets[s].edit_dist)))
-            return 1;
-    }
-    return maxN >= 32768 ? 4 : (maxN >= 4096 ? 2 : 1);
-}
-
 } // namespace igc
 
 using namespace igc;
@@ -469,9 +448,8 @@ int batch_build_host_locked(igc_ctx *ctx, const igc_set *sets, uint32_t count, i
 {
     for (uint32_t s = 0; s < count; ++s) std::memset(&outs[s], 0, sizeof(igc_csr));
     if (count == 0) return IGC_OK;
-    const int cpl = batch_shape(ctx, sets, count);
     BatchPlan P;
-    int rc = plan_batch(ctx, sets, count, cpl, P);
+    int rc = plan_batch(ctx, sets, count, P);
     if (rc) return rc;
     if ((rc = ctx->syms.ensure(std::max<size_t>(P.sym_bytes, 1)))) return rc;
     {
@@ -486,7 +464,7 @@ int batch_build_host_locked(igc_ctx *ctx, const igc_set *sets, uint32_t count, i
     }
     std::vector<uint64_t> bounds;
     uint64_t E = 0;
-    if ((rc = batch_device_locked(ctx, ctx->syms.as<uint8_t>(), P, cpl, bounds, &E))) return rc;
+    if ((rc = batch_device_locked(ctx, ctx->syms.as<uint8_t>(), P, bounds, &E))) return rc;
     // union CSR -> pinned staging -> per-set host CSRs
     const uint64_t ents = 2 * E;
     if ((rc = ctx->host_rp.ensure(sizeof(uint64_t) * ((size_t)P.total + 1)))) return rc;
@@ -519,9 +497,8 @@ IGC_API int igc_graph_build_batch_device(igc_ctx *ctx, const igc_set *sets, uint
         return set_error(IGC_ERR_INVALID, "null argument or empty batch");
     std::lock_guard<std::mutex> lk(ctx->mu);
     HIP_TRY(hipSetDevice(ctx->device));
-    const int cpl = batch_shape(ctx, sets, count);
     BatchPlan P;
-    int rc = plan_batch(ctx, sets, count, cpl, P);
+    int rc = plan_batch(ctx, sets, count, P);
     if (rc) return rc;
     if ((rc = ctx->syms.ensure(std::max<size_t>(P.sym_bytes, 1)))) return rc;
     {
@@ -536,7 +513,7 @@ IGC_API int igc_graph_build_batch_device(igc_ctx *ctx, const igc_set *sets, uint
     }
     std::vector<uint64_t> bounds;
     uint64_t E = 0;
-    if ((rc = batch_device_locked(ctx, ctx->syms.as<uint8_t>(), P, cpl, bounds, &E))) return rc;
+    if ((rc = batch_device_locked(ctx, ctx->syms.as<uint8_t>(), P, bounds, &E))) return rc;
     for (uint32_t s = 0; s <= count; ++s) node_off[s] = P.node_off_by_set[s];
     for (uint32_t s = 0; s < count; ++s) set_edges[s] = (bounds[s + 1] - bounds[s]) / 2;
     union_out->n_nodes = P.total;

@@ -256,6 +256,14 @@ __device__ __forceinline__ void wave_tile(const typename Pol::Word *__restrict__
     typename Pol::Col col[CPL];
 #pragma unroll
     for (int c = 0; c < CPL; ++c) col[c].load(packed, colw + 64u * P * c + lane, N);
+    // caller ids of this lane's columns (fixed for the tile), so emitting an
+    // edge needs no load
+    uint32_t ocol[SL];
+#pragma unroll
+    for (int sl = 0; sl < SL; ++sl) {
+        const uint32_t j = colw + 64u * sl + lane;
+        ocol[sl] = (EMIT && flt.perm && j < N) ? flt.perm[j] : j;
+    }
     uint32_t staged = 0;
     uint64_t counted = 0;
     const uint4 *items = nullptr;
@@ -269,12 +277,17 @@ __device__ __forceinline__ void wave_tile(const typename Pol::Word *__restrict__
     }
 
     // rows in chunks of 64: with a filter, lane l tests row r0 + l against the
-    // window's items and one ballot gives the chunk's rows that can conflict
+    // window's items and one ballot gives the chunk's rows that can conflict.
+    // Each chunk's counts are loaded one chunk ahead, into registers that stay
+    // live through the row loop (a load still in flight there would make the
+    // row loop wait for every outstanding store as well).
+    uint2 fnext = make_uint2(0u, 0u);
+    if (flt.items && lane < rows) fnext = flt.feat[row0 + lane];
     for (uint32_t r0 = 0; r0 < rows; r0 += 64) {
         const uint32_t left = rows - r0;
         uint64_t todo = left >= 64 ? ~0ull : ((1ull << left) - 1ull);
         if (flt.items) {
-            const uint2 f = lane < left ? flt.feat[row0 + r0 + lane] : make_uint2(0u, 0u);
+            const uint2 f = fnext;
             uint32_t best = 0xFFFFFFFFu;
             // the window's items are wave-uniform and read-only: scalar loads
             using CU32 = const __attribute__((address_space(4))) uint32_t;
@@ -287,6 +300,8 @@ __device__ __forceinline__ void wave_tile(const typename Pol::Word *__restrict__
                 best = min(best, v);
             }
             todo &= __ballot(best <= flt.limit);
+            const uint32_t nl = r0 + 64u + lane;
+            fnext = nl < rows ? flt.feat[row0 + nl] : make_uint2(0u, 0u);
         }
         // the next row's word is loaded one row ahead (scalar load latency)
         using Word = typename Pol::Word;
@@ -310,6 +325,9 @@ __device__ __forceinline__ void wave_tile(const typename Pol::Word *__restrict__
                 }
             }
             if (any) {
+                // caller id of the row: a scalar load (uniform, read-only)
+                using CU32 = const __attribute__((address_space(4))) uint32_t;
+                const uint32_t oi = (EMIT && flt.perm) ? ((CU32 *)flt.perm)[i] : i;
 #pragma unroll
                 for (int sl = 0; sl < SL; ++sl) {
                     if (!mk[sl]) continue;
@@ -323,11 +341,8 @@ __device__ __forceinline__ void wave_tile(const typename Pol::Word *__restrict__
                             flush_stage(st, staged, lane, edges, cap, counter);
                             staged = 0;
                         }
-                        if (mine) {
-                            const uint32_t oi = flt.perm ? flt.perm[i] : i;
-                            const uint32_t oj = flt.perm ? flt.perm[j] : j;
-                            st[staged + rank_in_mask(m2)] = ((uint64_t)(node_off + oi) << 32) | (node_off + oj);
-                        }
+                        if (mine)
+                            st[staged + rank_in_mask(m2)] = ((uint64_t)(node_off + oi) << 32) | (node_off + ocol[sl]);
                         staged += cnt;
                     } else {
                         counted += cnt;
