@@ -104,7 +104,7 @@ def cpu_baseline(syms: np.ndarray, min_ed: int, target_s: float = 12.0):
 
 def batched_block(ctx, torch, steps: int = 3, seed: int = 3, sets_n: int = 64, scale: float = 1.0,
                   label: str = "batched"):
-    """Uneven sets: one persistent launch for all of them against the plain form
+    """Uneven sets: one batched build for all of them against the plain form
     (one host-API build per set: copy in, launch, synchronise, copy the CSR out --
     what a per-cluster FillAdjListGPU call does, minus the process start)."""
     from indexgen_amd import workloads as W

@@ -209,8 +209,9 @@ typedef struct igc_set {
 } igc_set;
 
 /* Builds the conflict graphs of `count` independent, unevenly sized sets (each
- * with its own n and edit_dist) in one persistent kernel launch plus one CSR
- * assembly over their disjoint union; outs[s] is set s's host CSR, identical
+ * with its own n and edit_dist) in one pair launch per predicate (sets of the
+ * same predicate share a launch) plus one CSR assembly over their disjoint
+ * union; outs[s] is set s's host CSR, identical
  * to what igc_graph_build returns for that set alone.  Replaces one
  * FillAdjListGPU process per cluster in the reference's clustered loop
  * (SolveIndependentSet, src/SparseMat.cpp:784-851, called at :985-1020). */
@@ -318,7 +319,7 @@ typedef struct igc_server_stats {
  * the launches in flight, `pipelines` GPU pipelines (1 or 2, each with its own
  * stream) and `host_threads` greedy workers.  Jobs submitted while others run
  * are admitted FIFO as soon as a pipeline is free and the budget allows, and
- * all jobs admitted together run in one persistent launch. */
+ * all jobs admitted together run as one batched build. */
 IGC_API int igc_server_create(int device, uint64_t hbm_budget_bytes, uint32_t pipelines, uint32_t host_threads,
                               igc_server **out);
 /* Queues a codebook job (the candidates are copied); returns at once. */

@@ -75,7 +75,7 @@ struct HostBuf {
 };
 
 // misc buffer layout (uint32 words): [0..1] edge counter (u64), [2] bad symbol
-// flag, [6..7] batch work counter (u64)
+// flag; the rest is spare
 constexpr size_t kMiscWords = 8;
 
 // Tile shape of the single-set kernel: column slots per lane (each slot one

@@ -3,8 +3,8 @@
 // (GenerateCodebookAdj, src/SparseMat.cpp:918-1095) with its pieces replaced:
 //   cluster step      FP32 K-means on one-hot features, Lloyd iterations
 //                     replayed from a captured hipGraph (igc_kmeans.hip)
-//   per-cluster solve all clusters' conflict graphs in ONE persistent launch
-//                     (igc_batch.hip), then the greedy per cluster on host
+//   per-cluster solve all clusters' conflict graphs in one batched build (one
+//                     launch per pair predicate, igc_batch.hip), then the greedy per cluster on host
 //                     threads (the reference runs one Python process + GPU
 //                     context per cluster, :985-1020 -> :784-851)
 //   combine           next candidates = cluster codebooks concatenated in

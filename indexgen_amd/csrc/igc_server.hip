@@ -5,7 +5,7 @@
 // one batch's pair kernel overlaps the previous batch's CSR assembly and
 // copies) takes, as soon as it is free, every queued job that fits the HBM
 // budget -- the budget is shared by the batches in flight -- and builds their
-// conflict graphs in ONE persistent launch (igc_batch.hip).  The per-job CSRs
+// conflict graphs in one batched build (igc_batch.hip).  The per-job CSRs
 // go to a pool of host threads running the greedy independent set, while the
 // pipeline already serves the next batch.  A job's latency is its completion
 // time minus its submission time.

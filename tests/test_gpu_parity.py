@@ -240,8 +240,8 @@ def test_batch_matches_single_and_oracle(gpu_ctx):
             src[int(rng.integers(0, n))] = int(rng.integers(0, 4))
             syms[r] = src
         sets.append((syms, ed))
-    # large thresholds run Myers inside the persistent kernel; n > 16 sets take
-    # the separate-launch path on 64-bit words
+    # large thresholds make Myers groups; n > 16 sets make groups on 64-bit
+    # words
     sets.append((rng.integers(0, 4, size=(400, 16), dtype=np.uint8), 11))
     sets.append((rng.integers(0, 4, size=(300, 14), dtype=np.uint8), 13))
     sets.append((rng.integers(0, 4, size=(500, 32), dtype=np.uint8), 14))
@@ -262,9 +262,8 @@ def test_batch_matches_single_and_oracle(gpu_ctx):
 
 def test_batch_prefilter_sets_match_oracle(gpu_ctx):
     """Sets forced through the row filter are sorted and row-filtered inside the batch
-    (Landau-Vishkin and Myers sets in the persistent kernel, n > 16 sets in
-    their own launches): the same CSRs as the oracle, and as the batch with the
-    filter switched off."""
+    (Landau-Vishkin and Myers groups, on 32- and 64-bit words): the same CSRs as
+    the oracle, and as the batch with the filter switched off."""
     rng = np.random.default_rng(64)
     sets = []
     for n, N, ed in [(13, 6000, 4), (16, 4096, 5), (10, 5000, 3), (8, 4500, 2), (11, 5000, 9), (24, 4200, 6),
