@@ -118,6 +118,15 @@ IGC_API void *igc_ctx_stream(igc_ctx *ctx);
 #define IGC_PREFILTER_ALWAYS 2
 #define IGC_PREFILTER_MIN_N 16384u
 IGC_API int igc_ctx_set_prefilter(igc_ctx *ctx, int mode);
+/* How the CSR is assembled (same CSR either way).  IGC_CSR_AUTO (default): a
+ * split pass on the row's high bits and 32-bit radix passes when the row's
+ * low bits and the neighbour fit 32 bits with a split of at most 9 bits
+ * (single sets up to 2^20 strings; batches up to 2^32 / largest set nodes),
+ * else 64-bit keys; IGC_CSR_WIDE_KEYS: always 64-bit keys.  No counterpart in
+ * the reference (its host AdjList is filled pair by pair). */
+#define IGC_CSR_AUTO 0
+#define IGC_CSR_WIDE_KEYS 1
+IGC_API int igc_ctx_set_csr_form(igc_ctx *ctx, int form);
 IGC_API int igc_last_timing(igc_ctx *ctx, igc_timing *out);
 
 /* ---- single candidate set ---- */

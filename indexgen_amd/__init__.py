@@ -38,6 +38,7 @@ IGC_MAX_LEN = 32
 IGC_UPPER_TRIANGLE = 1  # flag: CSR rows list only neighbours above the row
 IGC_PREFILTER_OFF, IGC_PREFILTER_AUTO, IGC_PREFILTER_ALWAYS = 0, 1, 2
 IGC_PREFILTER_MIN_N = 16384  # smallest set the automatic mode sorts
+IGC_CSR_AUTO, IGC_CSR_WIDE_KEYS = 0, 1
 IGC_ON_HOST = 0
 IGC_ON_DEVICE = 1
 
@@ -212,6 +213,7 @@ def lib() -> ctypes.CDLL:
         "igc_tuned_shape": (i, [vp, u32, u32, u32, ctypes.POINTER(i), ctypes.POINTER(u32)]),
         "igc_pair_plan_of": (i, [vp, u32, u32, u32, ctypes.POINTER(_PairPlan)]),
         "igc_ctx_set_prefilter": (i, [vp, i]),
+        "igc_ctx_set_csr_form": (i, [vp, i]),
         "igc_assign_fp32": (i, [vp, vp, u32, u32, vp, u32, vp]),
         "igc_valu_probe": (i, [vp, vp]),
         "igc_last_error": (ctypes.c_char_p, []),
@@ -479,6 +481,11 @@ class Context:
         IGC_PREFILTER_MIN_N strings) or IGC_PREFILTER_ALWAYS."""
         m = int(mode) if not isinstance(mode, bool) else (IGC_PREFILTER_AUTO if mode else IGC_PREFILTER_OFF)
         _check(lib().igc_ctx_set_prefilter(self._h, m))
+
+    def set_csr_form(self, form: int) -> None:
+        """CSR assembly: IGC_CSR_AUTO (split pass + 32-bit radix passes where they
+        fit) or IGC_CSR_WIDE_KEYS (64-bit keys); the same CSR either way."""
+        _check(lib().igc_ctx_set_csr_form(self._h, int(form)))
 
     def pair_plan(self, n: int, min_ed: int, N: int) -> dict:
         """The pair launch this context makes for N strings at (n, min_ed)."""

@@ -65,7 +65,7 @@ static __device__ __forceinline__ uint32_t find_set_by(const SetDesc *__restrict
 
 static __global__ void k_pack_batch(const uint8_t *__restrict__ syms, const SetDesc *__restrict__ sets,
                                     uint32_t nsets, uint32_t total, uint8_t *__restrict__ packed,
-                                    uint32_t *__restrict__ bad)
+                                    uint32_t *__restrict__ nbase, uint32_t *__restrict__ bad)
 {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= total) return;
@@ -73,6 +73,7 @@ static __global__ void k_pack_batch(const uint8_t *__restrict__ syms, const SetD
     const SetDesc &d = sets[s];
     const uint32_t n = d.n;
     const uint32_t i = g - d.node_off;
+    nbase[g] = d.node_off; // neighbours are stored relative to it in the CSR sort
     const uint8_t *p = syms + d.sym_off + (size_t)i * n;
     uint32_t or_all = 0;
     for (uint32_t q = 0; q < n; ++q) or_all |= p[q];
@@ -140,6 +141,7 @@ struct BatchPlan {
     uint64_t sym_bytes = 0;
     uint64_t word_bytes = 0;       // packed words of all sets
     uint32_t total = 0;
+    uint32_t max_set = 0;          // largest set (strings)
     uint64_t cap = 0;
 };
 
@@ -162,6 +164,7 @@ static int plan_batch(igc_ctx *ctx, const igc_set *sets, uint32_t count, BatchPl
         sym_off[s] = symb;
         word_off[s] = wordb;
         total += x.N;
+        P.max_set = std::max(P.max_set, x.N);
         symb += (uint64_t)x.N * x.n;
         wordb += ((uint64_t)x.N * word_bytes(x.n) + 7) & ~7ull;
         P.pairs += (uint64_t)x.N * (x.N > 0 ? x.N - 1 : 0) / 2;
@@ -323,6 +326,7 @@ static int batch_device_locked(igc_ctx *ctx, const uint8_t *d_syms, const BatchP
     if ((rc = ctx->misc.ensure(sizeof(uint32_t) * kMiscWords))) return rc;
     if ((rc = ctx->sets.ensure(sizeof(SetDesc) * std::max<size_t>((size_t)nr + nset, 1)))) return rc;
     if ((rc = ctx->bounds.ensure(sizeof(uint64_t) * ((size_t)nset + 1)))) return rc;
+    if ((rc = ctx->nbase.ensure(sizeof(uint32_t) * std::max<size_t>(P.total, 1)))) return rc;
     const SetDesc *d_groups = ctx->sets.as<SetDesc>();
     const SetDesc *d_node_order = ctx->sets.as<SetDesc>() + nr;
     if (nr) HIP_TRY(hipMemcpyAsync(ctx->sets.p, P.desc.data(), sizeof(SetDesc) * nr, hipMemcpyHostToDevice, s));
@@ -339,7 +343,7 @@ static int batch_device_locked(igc_ctx *ctx, const uint8_t *d_syms, const BatchP
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, misc, (uint64_t)kMiscWords, 0u);
         if (P.total)
             hipLaunchKernelGGL(k_pack_batch, dim3((P.total + 255) / 256), dim3(256), 0, s, d_syms, d_node_order, nset,
-                               P.total, ctx->packed.as<uint8_t>(), misc + 2);
+                               P.total, ctx->packed.as<uint8_t>(), ctx->nbase.as<uint32_t>(), misc + 2);
         HIP_TRY(hipGetLastError());
         uint32_t launches = 2;
         for (uint32_t e : P.filtered) {
@@ -367,7 +371,8 @@ static int batch_device_locked(igc_ctx *ctx, const uint8_t *d_syms, const BatchP
             cap = E + E / 64 + 1024; // exact regrow, then replay the pairs
             continue;
         }
-        if ((rc = launch_csr(ctx, P.total, E, counter, misc, &launches))) return rc;
+        if ((rc = launch_csr(ctx, P.total, E, counter, misc, &launches, false, P.max_set, ctx->nbase.as<uint32_t>())))
+            return rc;
         hipLaunchKernelGGL(k_gather_bounds, dim3((nset + 1 + 255) / 256), dim3(256), 0, s, ctx->row_ptr.as<uint64_t>(),
                            d_node_order, nset, P.total, ctx->bounds.as<uint64_t>());
         HIP_TRY(hipGetLastError());

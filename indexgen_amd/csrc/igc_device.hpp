@@ -75,7 +75,7 @@ struct HostBuf {
 };
 
 // misc buffer layout (uint32 words): [0..1] edge counter (u64), [2] bad symbol
-// flag; the rest is spare
+// flag, [4] tiles of the segmented CSR form; the rest is spare
 constexpr size_t kMiscWords = 8;
 
 // Tile shape of the single-set kernel: column slots per lane (each slot one
@@ -130,6 +130,8 @@ struct igc_ctx {
     igc::DevBuf syms, packed, edges, misc, row_ptr, col, bsum, sets, bounds;
     igc::DevBuf keys[2];         // radix-sort ping-pong buffers (2 entries per edge)
     igc::DevBuf rcount, roffset; // per-tile digit counts and their scan
+    igc::DevBuf segmeta, nbase;  // segment table of the 32-bit CSR form; batch node -> first node of its set
+    int csr_form = IGC_CSR_AUTO; // IGC_CSR_*
     // row filter: half counts (unsorted, sorted), bucket keys, permutation,
     // sorted packed words, window boxes and thresholds
     igc::DevBuf feat, feat_s, nkey, perm, packed_s, fitems, fbias, fcount;
@@ -149,7 +151,7 @@ namespace igc {
 // Builds the CSR of the edge list in ctx->edges (count in counter, capacity cap)
 // over N nodes into ctx->row_ptr / ctx->col; asynchronous on ctx->stream.
 int launch_csr(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *counter, uint32_t *misc,
-               uint32_t *launches, bool upper = false);
+               uint32_t *launches, bool upper = false, uint32_t maxset = 0, const uint32_t *base = nullptr);
 Shape pick_shape(igc_ctx *ctx, uint32_t n, int K, uint32_t N);
 
 int filter_scratch(igc_ctx *ctx, uint32_t N, uint32_t n);

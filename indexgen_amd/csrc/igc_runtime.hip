@@ -244,14 +244,113 @@ static int check_args(igc_ctx *ctx, uint32_t N, uint32_t n)
 
 // CSR of the edge list by LSD radix sort of the 2E directed entries (igc_csr.hpp),
 // sized by N and cap so no host round trip is needed before the final count.
-int launch_csr(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *counter, uint32_t *misc,
-               uint32_t *launches, bool upper)
+static uint32_t bits_for(uint64_t n) // smallest b with n <= 2^b (at least 1)
 {
-    (void)misc;
+    uint32_t b = 1;
+    while (b < 63 && ((uint64_t)1 << b) < n) ++b;
+    return b;
+}
+
+static void scan_counts(hipStream_t s, const uint32_t *counts, uint32_t L, uint64_t *bsum, uint64_t *offsets)
+{
+    const uint32_t nsb = (uint32_t)(((uint64_t)L + kScanTile - 1) / kScanTile);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(kScanThreads), 0, s, counts, L, bsum);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, s, bsum, nsb);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(kScanThreads), 0, s, counts, L, bsum, offsets);
+}
+
+// Segmented CSR (igc_csr.hpp): the split pass over the edge list, then 32-bit
+// passes over the low key bits.  rowbits + dbits <= 32 and the split is at
+// most kRadixMaxBits bits (the caller checks).
+static int launch_csr_seg(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *counter, uint32_t *misc,
+                          uint32_t *launches, bool upper, uint32_t dbits, uint32_t rowbits, uint32_t hb,
+                          const uint32_t *base)
+{
     hipStream_t s = ctx->stream;
     int rc;
-    uint32_t bsh = 1;
-    while (bsh < 32 && ((uint64_t)1 << bsh) < (uint64_t)N) ++bsh; // node ids < 2^bsh
+    const uint64_t items_cap = (upper ? 1 : 2) * std::max<uint64_t>(cap, 1);
+    const uint32_t nseg = 1u << hb;
+    // split pass: 64-bit keys from the edge list, digit = the row's high bits
+    const uint64_t tiles64 = (items_cap + kRadixTile - 1) / kRadixTile;
+    const uint64_t ub = (items_cap + kSegTile - 1) / kSegTile + nseg; // tiles of all segments, at most
+    const uint32_t lb = rowbits + dbits;                                // low key bits
+    const uint32_t passes = std::max<uint32_t>(1, (lb + kSegMaxBits - 1) / kSegMaxBits);
+    const uint32_t bits = (lb + passes - 1) / passes; // at most kSegMaxBits
+    if ((tiles64 << hb) > 0xFFFFFFFFull || (ub << bits) > 0xFFFFFFFFull)
+        return set_error(IGC_ERR_INVALID, "edge buffer of %llu entries is too large", (unsigned long long)cap);
+    const uint32_t L64 = (uint32_t)(tiles64 << hb);
+    const uint32_t L32 = (uint32_t)(ub << bits);
+    const uint32_t Lmax = std::max(L64, L32);
+    const size_t slots = (size_t)ub * kSegTile; // padded 32-bit key slots
+    if ((rc = ctx->row_ptr.ensure(sizeof(uint64_t) * ((size_t)N + 1)))) return rc;
+    if ((rc = ctx->col.ensure(sizeof(uint32_t) * items_cap))) return rc;
+    if ((rc = ctx->keys[0].ensure(sizeof(uint32_t) * slots))) return rc;
+    if ((rc = ctx->keys[1].ensure(sizeof(uint32_t) * std::max<size_t>(slots, items_cap)))) return rc;
+    if ((rc = ctx->rcount.ensure(sizeof(uint32_t) * (size_t)Lmax))) return rc;
+    if ((rc = ctx->roffset.ensure(sizeof(uint64_t) * ((size_t)Lmax + 1)))) return rc;
+    if ((rc = ctx->bsum.ensure(sizeof(uint64_t) * (((size_t)Lmax + kScanTile - 1) / kScanTile)))) return rc;
+    if ((rc = ctx->segmeta.ensure(sizeof(SegMeta) * nseg))) return rc;
+    uint32_t *counts = ctx->rcount.as<uint32_t>();
+    uint64_t *offsets = ctx->roffset.as<uint64_t>();
+    SegMeta *meta = ctx->segmeta.as<SegMeta>();
+    uint32_t *total = misc + 4; // tiles of all segments, on the device
+    RadixPass rp{};
+    rp.edges = ctx->edges.as<uint64_t>();
+    rp.cnt = counter;
+    rp.cap = cap;
+    rp.bsh = rowbits + hb;
+    rp.dbits = dbits;
+    rp.base = base;
+    rp.shift = lb;
+    rp.bits = hb;
+    rp.from_edges = 1;
+    rp.upper = upper ? 1u : 0u;
+    const uint32_t t64 = (uint32_t)tiles64;
+    hipLaunchKernelGGL(k_radix_count, dim3(t64), dim3(kRadixThreads), 0, s, rp, t64, counts);
+    scan_counts(s, counts, L64, ctx->bsum.as<uint64_t>(), offsets);
+    hipLaunchKernelGGL(k_seg_meta, dim3(1), dim3(512), 0, s, offsets, t64, nseg, meta, total);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(t64), dim3(kRadixThreads), 0, s, rp, t64, offsets, 0u, nullptr, nullptr,
+                       nullptr, meta, ctx->keys[0].as<uint32_t>());
+    *launches += 6;
+    // 32-bit passes, least significant digit first; the last writes col and src
+    for (uint32_t pass = 0; pass < passes; ++pass) {
+        const uint32_t shift = pass * bits;
+        const uint32_t pb = std::min(bits, lb - shift);
+        const uint32_t L = (uint32_t)(ub << pb);
+        const bool last = pass + 1 == passes;
+        uint32_t *in = ctx->keys[pass & 1].as<uint32_t>();
+        uint32_t *out = ctx->keys[(pass + 1) & 1].as<uint32_t>(); // keys, or src on the last pass
+        hipLaunchKernelGGL(k_seg_count, dim3((uint32_t)ub), dim3(kSegThreads), 0, s, in, meta, nseg, total, shift, pb,
+                           counts);
+        scan_counts(s, counts, L, ctx->bsum.as<uint64_t>(), offsets);
+        hipLaunchKernelGGL(k_seg_scatter, dim3((uint32_t)ub), dim3(kSegThreads), 0, s, in, meta, nseg, total, shift,
+                           pb, offsets, (uint32_t)last, out, dbits, rowbits, base, ctx->col.as<uint32_t>(), out);
+        *launches += 5;
+        if (last)
+            hipLaunchKernelGGL(k_csr_row_ptr, dim3(grid_for((uint64_t)N + 1, 256, 8192)), dim3(256), 0, s, out,
+                               counter, cap, upper ? 1u : 2u, N, ctx->row_ptr.as<uint64_t>());
+    }
+    HIP_TRY(hipGetLastError());
+    *launches += 1;
+    return IGC_OK;
+}
+
+// CSR of the edge list by LSD radix sort of the 2E directed entries (igc_csr.hpp),
+// sized by N and cap so no host round trip is needed before the final count.
+// maxset: largest set of a batch (neighbours are stored relative to base[row],
+// the first node of the row's set; 0 and null for a single set).  Large graphs
+// take the segmented 32-bit form when the row's high part fits one split pass.
+int launch_csr(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *counter, uint32_t *misc,
+               uint32_t *launches, bool upper, uint32_t maxset, const uint32_t *base)
+{
+    hipStream_t s = ctx->stream;
+    int rc;
+    const uint32_t bsh = bits_for(N); // node ids < 2^bsh
+    const uint32_t dbits = base ? bits_for(std::max<uint32_t>(maxset, 2)) : bsh;
+    const uint32_t rowbits = std::min(bsh, 32u - std::min(dbits, 31u));
+    const uint32_t hb = bsh - rowbits;
+    if (ctx->csr_form == IGC_CSR_AUTO && dbits < 32 && hb <= kRadixMaxBits)
+        return launch_csr_seg(ctx, N, cap, counter, misc, launches, upper, dbits, rowbits, hb, base);
     // fewest passes of at most kRadixMaxBits-bit digits, then the narrowest digit
     const uint32_t passes = (2 * bsh + kRadixMaxBits - 1) / kRadixMaxBits;
     const uint32_t bits = (2 * bsh + passes - 1) / passes;
@@ -272,12 +371,14 @@ int launch_csr(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *count
     uint32_t *counts = ctx->rcount.as<uint32_t>();
     uint64_t *offsets = ctx->roffset.as<uint64_t>();
     for (uint32_t pass = 0; pass < passes; ++pass) {
-        RadixPass rp;
+        RadixPass rp{};
         rp.keys = pass ? ctx->keys[(pass - 1) & 1].as<uint64_t>() : nullptr;
         rp.edges = ctx->edges.as<uint64_t>();
         rp.cnt = counter;
         rp.cap = cap;
         rp.bsh = bsh;
+        rp.dbits = bsh;
+        rp.base = nullptr;
         rp.shift = pass * bits;
         rp.bits = std::min(bits, 2 * bsh - rp.shift);
         rp.from_edges = pass == 0;
@@ -290,7 +391,8 @@ int launch_csr(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *count
                            offsets);
         hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kRadixThreads), 0, s, rp, tiles, offsets,
                            (uint32_t)last, last ? nullptr : ctx->keys[pass & 1].as<uint64_t>(),
-                           ctx->col.as<uint32_t>(), ctx->keys[pass & 1].as<uint32_t>() /* src, last pass */);
+                           ctx->col.as<uint32_t>(), ctx->keys[pass & 1].as<uint32_t>() /* src, last pass */,
+                           nullptr, nullptr);
         *launches += 5;
     }
     hipLaunchKernelGGL(k_csr_row_ptr, dim3(grid_for((uint64_t)N + 1, 256, 8192)), dim3(256), 0, s,
@@ -539,7 +641,8 @@ IGC_API int igc_ctx_destroy(igc_ctx *ctx)
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
         for (DevBuf *b : {&ctx->syms, &ctx->packed, &ctx->edges, &ctx->misc, &ctx->row_ptr, &ctx->col, &ctx->keys[0],
-                          &ctx->keys[1], &ctx->rcount, &ctx->roffset, &ctx->bsum, &ctx->sets, &ctx->bounds,
+                          &ctx->keys[1], &ctx->rcount, &ctx->roffset, &ctx->segmeta, &ctx->nbase, &ctx->bsum,
+                          &ctx->sets, &ctx->bounds,
                           &ctx->feat, &ctx->feat_s, &ctx->nkey, &ctx->perm, &ctx->packed_s, &ctx->fitems, &ctx->fbias, &ctx->fcount})
             b->release();
         for (DevBuf &b : ctx->corder) b.release();
@@ -570,6 +673,16 @@ IGC_API int igc_ctx_set_prefilter(igc_ctx *ctx, int mode)
         return set_error(IGC_ERR_INVALID, "prefilter mode %d is not 0, 1 or 2", mode);
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->prefilter = mode;
+    return IGC_OK;
+}
+
+IGC_API int igc_ctx_set_csr_form(igc_ctx *ctx, int form)
+{
+    if (!ctx) return set_error(IGC_ERR_INVALID, "null context");
+    if (form != IGC_CSR_AUTO && form != IGC_CSR_WIDE_KEYS)
+        return set_error(IGC_ERR_INVALID, "CSR form %d is not 0 or 1", form);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->csr_form = form;
     return IGC_OK;
 }
 
