@@ -27,8 +27,8 @@
 
 namespace igc {
 
-constexpr int kRadixThreads = 256;
-constexpr int kRadixRounds = 16; // entries per thread and tile
+constexpr int kRadixThreads = 512;
+constexpr int kRadixRounds = 8; // entries per thread and tile
 constexpr uint32_t kRadixTile = kRadixThreads * kRadixRounds;
 constexpr uint32_t kRadixMaxBits = 9; // digit width of a pass: 1..9 bits
 constexpr uint32_t kRadixMaxDigits = 1u << kRadixMaxBits;
@@ -323,8 +323,8 @@ static __global__ __launch_bounds__(kRadixThreads) void k_radix_count(RadixPass 
 }
 
 // Stable scatter of tile b; offsets[d * tiles + b] is where the tile's first
-// entry with digit d goes.  Wave w owns the tile's entries w*1024 .. w*1024+1023
-// (16 rounds of 64), so input order within the tile is wave-major.  Each wave
+// entry with digit d goes.  Wave w owns the tile's entries w*512 .. w*512+511
+// (8 rounds of 64), so input order within the tile is wave-major.  Each wave
 // counts its digits into LDS; one scan over the tile's digits gives every wave
 // its running position per digit; the waves then rank their entries round by
 // round with wave-level operations only (one ballot per digit bit gives each
