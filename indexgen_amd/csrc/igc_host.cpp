@@ -54,10 +54,11 @@ void reorder_threads(igc_csr &csr, uint32_t T)
 //     swap-remove that fills the hole with the bucket's last node (DeleteRow,
 //     :139-154) and an append to the lower bucket.
 // Layout for speed, not for semantics: a node's degree and bucket position
-// share one 8-byte record, the deleted flags are a byte array (the most
-// frequent test), buckets live in one flat array (bucket d never holds more
-// nodes than there are nodes of initial degree >= d, since degrees only fall),
-// and on large graphs the records of the next neighbours are prefetched.  The
+// share one 8-byte record, and a deleted node's degree field holds kDead, so
+// the most frequent test (deleted or degree 0) reads that record only; buckets
+// live in one flat array (bucket d never holds more nodes than there are nodes
+// of initial degree >= d, since degrees only fall), and the records of the
+// next neighbours are prefetched.  The
 // visiting order is exactly the reference's.
 class Greedy {
   public:
@@ -68,7 +69,6 @@ class Greedy {
         uint32_t maxd = 0;
         try {
             st_.resize(N_);
-            dead_.assign(N_, 0);
             for (uint32_t v = 0; v < N_; ++v) {
                 st_[v].deg = (uint32_t)(rp_[v + 1] - rp_[v]);
                 maxd = std::max(maxd, st_[v].deg);
@@ -90,8 +90,6 @@ class Greedy {
         } catch (const std::bad_alloc &) {
             return set_error(IGC_ERR_OOM, "greedy: host allocation failed");
         }
-        // prefetching only pays once the node records no longer fit in cache
-        const bool pf = N_ > (1u << 18);
         tracker_ = nb_;
         for (uint32_t v = 0; v < N_; ++v) {
             push(st_[v].deg, v);
@@ -107,24 +105,23 @@ class Greedy {
             todel.clear();
             const uint64_t v0 = rp_[v], v1 = rp_[v + 1];
             for (uint64_t q = v0; q < v1; ++q)
-                if (!dead_[col_[q]]) todel.push_back(col_[q]);
+                if (st_[col_[q]].deg != kDead) todel.push_back(col_[q]);
             todel.push_back(v);
             for (const uint32_t x : todel) {
-                if (dead_[x]) continue;
+                if (st_[x].deg == kDead) continue;
                 const uint64_t x0 = rp_[x], x1 = rp_[x + 1];
                 for (uint64_t q = x0; q < x1; ++q) {
-                    if (pf && q + kAhead < x1) __builtin_prefetch(&st_[col_[q + kAhead]]);
+                    if (q + kAhead < x1) __builtin_prefetch(&st_[col_[q + kAhead]]);
                     const uint32_t u = col_[q];
-                    if (dead_[u]) continue;
                     const uint32_t du = st_[u].deg;
-                    if (du == 0) continue;
+                    if (du - 1u >= kDead - 1u) continue; // deleted, or degree 0
                     remove(du, u);
                     st_[u].deg = du - 1;
                     push(du - 1, u);
                     if (du - 1 < tracker_) tracker_ = du - 1;
                 }
                 remove(st_[x].deg, x);
-                dead_[x] = 1;
+                st_[x].deg = kDead;
                 --active;
             }
         }
@@ -134,6 +131,7 @@ class Greedy {
 
   private:
     static constexpr uint64_t kAhead = 8;
+    static constexpr uint32_t kDead = 0xFFFFFFFFu; // degree field of a deleted node
     struct Node {
         uint32_t deg;
         uint32_t pos;
@@ -165,7 +163,6 @@ class Greedy {
     const uint32_t *col_;
     uint32_t nb_ = 0, tracker_ = 0;
     std::vector<Node> st_;
-    std::vector<uint8_t> dead_;
     std::vector<Bucket> bk_;
     std::unique_ptr<uint32_t[]> store_;
 };
