@@ -144,6 +144,31 @@ def batched_block(ctx, torch, steps: int = 3, seed: int = 3, sets_n: int = 64, s
     }
 
 
+def refine_block(ctx, host_threads: int = 16, seed: int = 4):
+    """The cluster-refinement workload: 500,000 strings of length 16 with a
+    Varshamov-Tenengolts-style syndrome (sum i*x_i = 0 mod 17) passing the run
+    and GC filters, editDist 5; per round FP32 K-means (k = 256, 10 Lloyd
+    iterations) on one-hot features, every cluster's graph in one batched build,
+    the greedy per cluster on host threads; at most 5 rounds."""
+    from indexgen_amd import workloads as W
+    syms = W.vt_refine(seed=seed)
+    t0 = time.perf_counter()
+    order, st = ctx.refine(syms, 5, k=256, kmeans_iters=10, max_rounds=5, converge_n=3, seed=1,
+                           host_threads=host_threads)
+    wall = time.perf_counter() - t0
+    rounds = st["rounds"]
+    pairs = sum(r["pairs"] for r in rounds)
+    dev_pairs_s = sum(r["pairs_ms"] for r in rounds) / 1e3
+    return {
+        "workload": f"{syms.shape[0]} strings n=16 (VT syndrome mod 17, run/GC filtered, seed {seed}), editDist 5, "
+                    f"k=256, 10 Lloyd iterations, up to 5 rounds",
+        "rounds": len(rounds), "converged": st["converged"], "codewords": int(order.size),
+        "wall_s": round(wall, 3), "pairs": pairs,
+        "pair_kernels_gpairs_s": round(pairs / dev_pairs_s / 1e9, 1) if dev_pairs_s > 0 else None,
+        "per_round": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in rounds],
+    }
+
+
 def serving_block(jobs: int = 200, pipelines: int = 2, host_threads: int = 16, seed: int = 5):
     """The codebook-request stream (Poisson 20 jobs/s, n in 11..16, editDist in
     3..5, sizes lognormal(ln 30000, 1.2) clipped to [500, 400000]) replayed in
@@ -197,6 +222,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the uneven-sets comparison block")
     ap.add_argument("--no-autotune", action="store_true", help="use the default tile shape")
     ap.add_argument("--serving-jobs", type=int, default=200, help="jobs of the serving stream to replay (0: skip)")
+    ap.add_argument("--no-refine", action="store_true", help="skip the cluster-refinement block")
     ap.add_argument("--valu-probe", action="store_true", help="only measure the int32 VALU issue rates")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (scales rocprof FETCH_SIZE/WRITE_SIZE)")
@@ -354,6 +380,8 @@ def main():
         result["batched"] = batched_block(ctx, torch)
         result["many_small_sets"] = batched_block(ctx, torch, steps=5, seed=4, sets_n=256, scale=0.08,
                                                   label="per-cluster sets")
+    if not args.no_refine:
+        result["refine"] = refine_block(ctx, host_threads=host_threads())
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(syms, min_ed, args.cpu_seconds)
     print(json.dumps(result), flush=True)
