@@ -245,7 +245,7 @@ def main():
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the conflict-graph build here runs on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not run). Use --gpus 1.", file=sys.stderr)
+              "(sharding the pair triangle over GPUs is designed but not built). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch
