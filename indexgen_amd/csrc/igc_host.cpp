@@ -96,25 +96,42 @@ class Greedy {
             if (st_[v].deg < tracker_) tracker_ = st_[v].deg;
         }
         uint32_t active = N_, picked = 0;
-        std::vector<uint32_t> todel;
-        todel.reserve(maxd + 1);
+        try {
+            ball_.resize((size_t)maxd + 2);
+            live_.resize((size_t)maxd + 1);
+        } catch (const std::bad_alloc &) {
+            return set_error(IGC_ERR_OOM, "greedy: host allocation failed");
+        }
         while (active > 0) {
             const Bucket &b = bk_[tracker_];
             const uint32_t v = store_[b.off + b.len - 1];
             order[picked++] = v;
-            todel.clear();
+            // the ball: v's active neighbours in list order, then v (compacted
+            // without branches: about half the tests fail, unpredictably)
             const uint64_t v0 = rp_[v], v1 = rp_[v + 1];
-            for (uint64_t q = v0; q < v1; ++q)
-                if (st_[col_[q]].deg != kDead) todel.push_back(col_[q]);
-            todel.push_back(v);
-            for (const uint32_t x : todel) {
-                if (st_[x].deg == kDead) continue;
+            uint32_t nb = 0;
+            for (uint64_t q = v0; q < v1; ++q) {
+                const uint32_t u = col_[q];
+                ball_[nb] = u;
+                nb += st_[u].deg != kDead ? 1u : 0u;
+            }
+            ball_[nb++] = v;
+            for (uint32_t bi = 0; bi < nb; ++bi) {
+                const uint32_t x = ball_[bi];
                 const uint64_t x0 = rp_[x], x1 = rp_[x + 1];
+                // x's neighbours that move (active, degree > 0), compacted the same
+                // way; a move changes only the moved node's degree, so every test
+                // is already fixed before the first move
+                uint32_t m = 0;
                 for (uint64_t q = x0; q < x1; ++q) {
-                    if (q + kAhead < x1) __builtin_prefetch(&st_[col_[q + kAhead]]);
                     const uint32_t u = col_[q];
+                    live_[m] = u;
+                    m += st_[u].deg - 1u < kDead - 1u ? 1u : 0u;
+                }
+                for (uint32_t e = 0; e < m; ++e) {
+                    if (e + kAhead < m) __builtin_prefetch(&st_[live_[e + kAhead]]);
+                    const uint32_t u = live_[e];
                     const uint32_t du = st_[u].deg;
-                    if (du - 1u >= kDead - 1u) continue; // deleted, or degree 0
                     remove(du, u);
                     st_[u].deg = du - 1;
                     push(du - 1, u);
@@ -132,6 +149,7 @@ class Greedy {
   private:
     static constexpr uint64_t kAhead = 8;
     static constexpr uint32_t kDead = 0xFFFFFFFFu; // degree field of a deleted node
+    std::vector<uint32_t> ball_, live_;            // the current ball; a node's moving neighbours
     struct Node {
         uint32_t deg;
         uint32_t pos;
