@@ -54,23 +54,18 @@ __device__ __forceinline__ uint64_t radix_items(const RadixPass &p)
     return (p.upper ? 1 : 2) * (e < p.cap ? e : p.cap);
 }
 
-// key of directed entry (row r, neighbour c): r << dbits | (c - base[r])
-__device__ __forceinline__ uint64_t entry_key(const RadixPass &p, uint64_t r, uint64_t c)
-{
-    return (r << p.dbits) | (p.base ? c - p.base[r] : c);
-}
-
+// Key of directed entry t: row << dbits | (neighbour - base), where base is
+// the first node of the edge's set (both ends of an edge lie in one set, so
+// the lookup is by the edge's first id: both directions of an edge, in
+// neighbouring lanes, read the same word).
 __device__ __forceinline__ uint64_t radix_key(const RadixPass &p, uint64_t t)
 {
     if (p.from_edges) {
-        if (p.upper) {
-            const uint64_t v = p.edges[t];
-            const uint64_t i = v >> 32, j = (uint32_t)v;
-            return i < j ? entry_key(p, i, j) : entry_key(p, j, i);
-        }
-        const uint64_t v = p.edges[t >> 1];
+        const uint64_t v = p.edges[p.upper ? t : t >> 1];
         const uint64_t i = v >> 32, j = (uint32_t)v;
-        return (t & 1) ? entry_key(p, j, i) : entry_key(p, i, j);
+        const uint64_t b = p.base ? p.base[i] : 0;
+        const bool flip = p.upper ? j < i : (t & 1) != 0; // row = j
+        return flip ? ((j << p.dbits) | (i - b)) : ((i << p.dbits) | (j - b));
     }
     return p.keys[t];
 }
