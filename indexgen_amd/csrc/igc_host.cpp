@@ -129,7 +129,12 @@ class Greedy {
                     m += st_[u].deg - 1u < kDead - 1u ? 1u : 0u;
                 }
                 for (uint32_t e = 0; e < m; ++e) {
+                    // node records kAhead moves ahead, their bucket slots half as far
                     if (e + kAhead < m) __builtin_prefetch(&st_[live_[e + kAhead]]);
+                    if (e + kAhead / 2 < m) {
+                        const Node nw = st_[live_[e + kAhead / 2]];
+                        if (nw.deg < nb_) __builtin_prefetch(&store_[bk_[nw.deg].off + nw.pos]);
+                    }
                     const uint32_t u = live_[e];
                     const uint32_t du = st_[u].deg;
                     remove(du, u);
