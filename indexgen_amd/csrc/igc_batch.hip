@@ -284,8 +284,8 @@ static int launch_group(igc_ctx *ctx, const BatchGroup &g, const FilterBase &fb,
     return IGC_OK;
 }
 
-// Buffers and scratch of the row filter for the batch's filtered sets, all
-// sized before the first set-up launch (no reallocation while kernels run).
+// Output buffers of the row filter for the batch's filtered sets, all sized
+// before the first set-up launch (no reallocation while kernels run).
 static int batch_filter_buffers(igc_ctx *ctx, const BatchPlan &P, FilterBase &fb)
 {
     fb = FilterBase{};
@@ -297,10 +297,6 @@ static int batch_filter_buffers(igc_ctx *ctx, const BatchPlan &P, FilterBase &fb
     if ((rc = ctx->fitems.ensure(sizeof(uint4) * kItemCap * (size_t)P.windows))) return rc;
     if ((rc = ctx->fbias.ensure(sizeof(uint32_t) * kItemCap * (size_t)P.windows))) return rc;
     if ((rc = ctx->fcount.ensure(sizeof(uint32_t) * (size_t)P.windows))) return rc;
-    uint32_t maxN[IGC_MAX_LEN + 1] = {};
-    for (uint32_t e : P.filtered) maxN[P.desc[e].n] = std::max(maxN[P.desc[e].n], P.desc[e].G.N);
-    for (uint32_t n = 1; n <= IGC_MAX_LEN; ++n)
-        if (maxN[n] && (rc = filter_scratch(ctx, maxN[n], n))) return rc;
     fb.feat = ctx->feat_s.as<uint2>();
     fb.items = ctx->fitems.as<uint4>();
     fb.bias = ctx->fbias.as<uint32_t>();
@@ -309,9 +305,6 @@ static int batch_filter_buffers(igc_ctx *ctx, const BatchPlan &P, FilterBase &fb
     return IGC_OK;
 }
 
-// Builds the union CSR of all sets in ctx->row_ptr / ctx->col; fills
-// bounds_out[0..count] (row_ptr at each set's first row, in caller order) and
-// the union edge count.  d_syms: concatenated symbols already on the device.
 static int batch_device_locked(igc_ctx *ctx, const uint8_t *d_syms, const BatchPlan &Pin,
                                std::vector<uint64_t> &bounds_out, uint64_t *n_edges)
 {
@@ -346,14 +339,22 @@ static int batch_device_locked(igc_ctx *ctx, const uint8_t *d_syms, const BatchP
                                P.total, ctx->packed.as<uint8_t>(), ctx->nbase.as<uint32_t>(), misc + 2);
         HIP_TRY(hipGetLastError());
         uint32_t launches = 2;
-        for (uint32_t e : P.filtered) {
-            const SetDesc &d = P.desc[e];
-            if ((rc = filter_set(ctx, d_syms + d.sym_off, d.G.N, d.n, d.window, ctx->packed.as<uint8_t>() + d.word_off,
-                                 ctx->packed_s.as<uint8_t>() + d.word_off, ctx->feat_s.as<uint2>() + d.node_off,
-                                 ctx->perm.as<uint32_t>() + d.node_off,
-                                 ctx->fitems.as<uint4>() + (size_t)d.wbase * kItemCap,
-                                 ctx->fbias.as<uint32_t>() + (size_t)d.wbase * kItemCap,
-                                 ctx->fcount.as<uint32_t>() + d.wbase)))
+        if (!P.filtered.empty()) {
+            std::vector<FilterSet> fs(P.filtered.size());
+            for (size_t q = 0; q < P.filtered.size(); ++q) {
+                const SetDesc &d = P.desc[P.filtered[q]];
+                fs[q] = FilterSet{};
+                fs[q].sym_off = d.sym_off;
+                fs[q].word_off = d.word_off;
+                fs[q].node_off = d.node_off;
+                fs[q].N = d.G.N;
+                fs[q].n = d.n;
+                fs[q].wbase = d.wbase;
+                fs[q].window = d.window;
+            }
+            if ((rc = filter_sets(ctx, d_syms, ctx->packed.p, ctx->packed_s.p, ctx->feat_s.as<uint2>(),
+                                  ctx->perm.as<uint32_t>(), ctx->fitems.as<uint4>(), ctx->fbias.as<uint32_t>(),
+                                  ctx->fcount.as<uint32_t>(), fs)))
                 return rc;
             launches += 8;
         }

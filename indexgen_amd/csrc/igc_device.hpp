@@ -134,7 +134,8 @@ struct igc_ctx {
     int csr_form = IGC_CSR_AUTO; // IGC_CSR_*
     // row filter: half counts (unsorted, sorted), bucket keys, permutation,
     // sorted packed words, window boxes and thresholds
-    igc::DevBuf feat, feat_s, nkey, perm, packed_s, fitems, fbias, fcount;
+    igc::DevBuf feat, feat_s, nkey, perm, packed_s, fitems, fbias, fcount, fsets;
+    std::vector<igc::FilterSet> fs_host; // the last filter set-up's table (read by its copy)
     igc::DevBuf corder[IGC_MAX_LEN + 1]; // class order along the curve, per string length
     int prefilter = IGC_PREFILTER_AUTO; // exact row skipping (IGC_PREFILTER_*)
     igc::HostBuf host_rp, host_col, stage; // pinned: batch CSR copies, staged copies
@@ -154,9 +155,8 @@ int launch_csr(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *count
                uint32_t *launches, bool upper = false, uint32_t maxset = 0, const uint32_t *base = nullptr);
 Shape pick_shape(igc_ctx *ctx, uint32_t n, int K, uint32_t N);
 
-int filter_scratch(igc_ctx *ctx, uint32_t N, uint32_t n);
-int filter_set(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n, uint32_t window, const void *packed,
-               void *packed_s, uint2 *feat_s, uint32_t *perm, uint4 *items, uint32_t *bias, uint32_t *count);
+int filter_sets(igc_ctx *ctx, const uint8_t *d_syms, const void *packed, void *packed_s, uint2 *feat_s,
+                uint32_t *perm, uint4 *items, uint32_t *bias, uint32_t *count, std::vector<igc::FilterSet> &fs);
 
 int kmeans_attach(igc_ctx *ctx);
 void kmeans_release(igc_ctx *ctx);

@@ -84,45 +84,87 @@ __device__ __forceinline__ uint32_t comp_rank(uint32_t packed, uint32_t len)
 
 __host__ __device__ inline uint32_t comp_count(uint32_t len) { return (len + 3) * (len + 2) * (len + 1) / 6; }
 
-// feat[i], bucket key[i] = the position of i's count class along the curve
+// One set of a filter set-up; several sets (a batch) share one set of launches.
+struct FilterSet {
+    uint64_t sym_off;      // the set's symbols (byte offset)
+    uint64_t word_off;     // the set's packed words (byte offset; 64-bit words for n > 16)
+    const uint32_t *order; // class order of the set's string length
+    uint32_t node_off;     // first node of the set (sorted counts and permutation are per node)
+    uint32_t N, n;
+    uint32_t fbase;        // first position of the set among all sets' strings
+    uint32_t kbase;        // first bucket of the set
+    uint32_t wbase;        // first window of the set
+    uint32_t window;       // columns per window
+    uint32_t pad;
+};
+
+// Set holding position `pos` of the field (fbase or wbase) `by_window` picks.
+__device__ __forceinline__ uint32_t filter_set_of(const FilterSet *__restrict__ fs, uint32_t nf, uint32_t pos,
+                                                  bool by_window)
+{
+    uint32_t lo = 0, hi = nf - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if ((by_window ? fs[mid].wbase : fs[mid].fbase) <= pos) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// For string g of all sets: its half counts feat[g], its bucket key[g] = the
+// set's first bucket + the position of its count class along the curve
 // (order[rank(prefix) * comp_count(n-h) + rank(suffix)]), and the bucket
 // histogram.
-static __global__ void k_node_keys(const uint8_t *__restrict__ syms, uint32_t N, uint32_t n,
-                                   const uint32_t *__restrict__ order, uint2 *__restrict__ feat,
-                                   uint32_t *__restrict__ key, uint32_t *__restrict__ hist)
+static __global__ void k_fs_keys(const uint8_t *__restrict__ syms, const FilterSet *__restrict__ fs, uint32_t nf,
+                                 uint32_t total, uint2 *__restrict__ feat, uint32_t *__restrict__ key,
+                                 uint32_t *__restrict__ hist)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const uint32_t h = n / 2;
-    const uint8_t *s = syms + (size_t)i * n;
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const FilterSet &f = fs[filter_set_of(fs, nf, g, false)];
+    const uint32_t n = f.n, h = n / 2;
+    const uint8_t *s = syms + f.sym_off + (size_t)(g - f.fbase) * n;
     const uint32_t p = half_counts(s, 0, h), q = half_counts(s, h, n);
-    feat[i] = make_uint2(p, q);
-    const uint32_t k = order[comp_rank(p, h) * comp_count(n - h) + comp_rank(q, n - h)];
-    key[i] = k;
+    feat[g] = make_uint2(p, q);
+    const uint32_t k = f.kbase + f.order[comp_rank(p, h) * comp_count(n - h) + comp_rank(q, n - h)];
+    key[g] = k;
     atomicAdd(&hist[k], 1u);
 }
 
 // Counting-sort scatter: positions of a bucket are handed out from its end
-// (any order inside a bucket will do).
-static __global__ void k_node_scatter(const uint32_t *__restrict__ key, uint32_t N, const uint64_t *__restrict__ off,
-                                      uint32_t *__restrict__ hist, uint32_t *__restrict__ perm)
+// (any order inside a bucket will do).  A set's buckets follow the previous
+// sets', so its sorted positions are fbase .. fbase+N-1; perm[node_off + k] =
+// the set-local string at sorted position k.
+static __global__ void k_fs_scatter(const uint32_t *__restrict__ key, const FilterSet *__restrict__ fs, uint32_t nf,
+                                    uint32_t total, const uint64_t *__restrict__ off, uint32_t *__restrict__ hist,
+                                    uint32_t *__restrict__ perm)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const uint32_t k = key[i];
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const FilterSet &f = fs[filter_set_of(fs, nf, g, false)];
+    const uint32_t k = key[g];
     const uint32_t r = atomicSub(&hist[k], 1u) - 1u;
-    perm[off[k] + r] = i;
+    perm[f.node_off + (uint32_t)(off[k] + r - f.fbase)] = g - f.fbase;
 }
 
-template <class W>
-__global__ void k_node_permute(const uint32_t *__restrict__ perm, uint32_t N, const W *__restrict__ packed,
-                               const uint2 *__restrict__ feat, W *__restrict__ packed_s, uint2 *__restrict__ feat_s)
+// Sorted words and counts of every set, position by position.
+static __global__ void k_fs_permute(const FilterSet *__restrict__ fs, uint32_t nf, uint32_t total,
+                                    const uint32_t *__restrict__ perm, const uint8_t *__restrict__ packed,
+                                    const uint2 *__restrict__ feat, uint8_t *__restrict__ packed_s,
+                                    uint2 *__restrict__ feat_s)
 {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= N) return;
-    const uint32_t i = perm[k];
-    packed_s[k] = packed[i];
-    feat_s[k] = feat[i];
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const FilterSet &f = fs[filter_set_of(fs, nf, g, false)];
+    const uint32_t k = g - f.fbase;
+    const uint32_t i = perm[f.node_off + k];
+    if (f.n > 16)
+        reinterpret_cast<uint64_t *>(packed_s + f.word_off)[k] =
+            reinterpret_cast<const uint64_t *>(packed + f.word_off)[i];
+    else
+        reinterpret_cast<uint32_t *>(packed_s + f.word_off)[k] =
+            reinterpret_cast<const uint32_t *>(packed + f.word_off)[i];
+    feat_s[f.node_off + k] = feat[f.fbase + i];
 }
 
 __device__ __forceinline__ uint32_t byte_of(const uint2 f, uint32_t b)
@@ -130,18 +172,22 @@ __device__ __forceinline__ uint32_t byte_of(const uint2 f, uint32_t b)
     return ((b < 4 ? f.x : f.y) >> (8 * (b & 3))) & 0xFFu;
 }
 
-// Items of every window of `window` sorted candidates: one workgroup of 64
-// per window.  A column starts a run when its class differs from the column
+// Items of every window of every set (a set's windows hold `window` of its
+// sorted candidates): one workgroup of 64 per window.  A column starts a run when its class differs from the column
 // before it; run r goes to item r (at most kItemCap runs) or to item
 // r * kItemCap / runs, and each item's byte-wise min and max are gathered in
 // LDS.
-static __global__ __launch_bounds__(64) void k_node_items(const uint2 *__restrict__ feat_s, uint32_t N,
-                                                          uint32_t window, uint4 *__restrict__ items,
-                                                          uint32_t *__restrict__ bias, uint32_t *__restrict__ count)
+static __global__ __launch_bounds__(64) void k_fs_items(const FilterSet *__restrict__ fs, uint32_t nf,
+                                                        const uint2 *__restrict__ feat_s_all,
+                                                        uint4 *__restrict__ items, uint32_t *__restrict__ bias,
+                                                        uint32_t *__restrict__ count)
 {
     __shared__ uint32_t lo[kItemCap * 8], hi[kItemCap * 8];
-    const uint32_t w = blockIdx.x, lane = threadIdx.x;
-    const uint32_t first = w * window;
+    const uint32_t w = blockIdx.x, lane = threadIdx.x; // w: window of all sets
+    const FilterSet &f = fs[filter_set_of(fs, nf, w, true)];
+    const uint2 *__restrict__ feat_s = feat_s_all + f.node_off;
+    const uint32_t N = f.N, window = f.window;
+    const uint32_t first = (w - f.wbase) * window;
     if (first >= N) return;
     const uint32_t last = min(first + window, N);
     for (uint32_t t = lane; t < kItemCap * 8; t += 64) {

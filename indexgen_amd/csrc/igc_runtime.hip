@@ -420,57 +420,54 @@ static int class_order_table(igc_ctx *ctx, uint32_t n, const uint32_t **out)
     return IGC_OK;
 }
 
-// Scratch the per-set filter set-up needs for sets of up to N strings of length n.
-int filter_scratch(igc_ctx *ctx, uint32_t N, uint32_t n)
+// Row-filter set-up of several sets in one set of launches (igc_prefilter.hpp):
+// each set's candidates (symbols at d_syms + sym_off, words at packed +
+// word_off) are sorted by count class; the sorted words go to packed_s +
+// word_off, the sorted counts to feat_s[node_off..], sorted position -> local id
+// to perm[node_off..], and the items of the set's windows (wbase.., `window`
+// columns each) to items, bias and count.  fbase and kbase are filled in here.
+// Asynchronous on ctx->stream; the table stays in ctx until the next set-up.
+int filter_sets(igc_ctx *ctx, const uint8_t *d_syms, const void *packed, void *packed_s, uint2 *feat_s,
+                uint32_t *perm, uint4 *items, uint32_t *bias, uint32_t *count, std::vector<FilterSet> &fs)
 {
-    const uint32_t h = n / 2;
-    const uint32_t B = comp_count(h) * comp_count(n - h); // at most 969^2 for n = 32
-    const uint32_t nsb = (B + kScanTile - 1) / kScanTile;
+    hipStream_t s = ctx->stream;
+    const uint32_t nf = (uint32_t)fs.size();
+    if (nf == 0) return IGC_OK;
+    uint64_t F = 0, B = 0, W = 0;
     int rc;
-    if ((rc = ctx->feat.ensure(sizeof(uint2) * (size_t)N))) return rc;
-    if ((rc = ctx->nkey.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
+    for (FilterSet &f : fs) {
+        const uint32_t h = f.n / 2;
+        if ((rc = class_order_table(ctx, f.n, &f.order))) return rc;
+        f.fbase = (uint32_t)F;
+        f.kbase = (uint32_t)B;
+        F += f.N;
+        B += (uint64_t)comp_count(h) * comp_count(f.n - h);
+        W = std::max<uint64_t>(W, (uint64_t)f.wbase + (f.N + f.window - 1) / f.window);
+    }
+    if (F > 0xFFFFFFFFull || B > 0xFFFFFFFFull) return set_error(IGC_ERR_INVALID, "filter set-up too large");
+    const uint32_t nsb = (uint32_t)((B + kScanTile - 1) / kScanTile);
+    if ((rc = ctx->feat.ensure(sizeof(uint2) * (size_t)F))) return rc;
+    if ((rc = ctx->nkey.ensure(sizeof(uint32_t) * (size_t)F))) return rc;
     if ((rc = ctx->rcount.ensure(sizeof(uint32_t) * (size_t)B))) return rc;
     if ((rc = ctx->roffset.ensure(sizeof(uint64_t) * ((size_t)B + 1)))) return rc;
     if ((rc = ctx->bsum.ensure(sizeof(uint64_t) * (size_t)nsb))) return rc;
-    const uint32_t *t;
-    return class_order_table(ctx, n, &t);
-}
-
-// Row-filter set-up of one set (igc_prefilter.hpp): sorts its N candidates
-// (symbols d_syms, packed words `packed`) by count class and writes the sorted
-// words to packed_s, the sorted counts to feat_s, sorted position -> local id
-// to perm, and the items of its windows of `window` columns to items, bias
-// and count.  Scratch from filter_scratch; asynchronous on ctx->stream.
-int filter_set(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n, uint32_t window, const void *packed,
-               void *packed_s, uint2 *feat_s, uint32_t *perm, uint4 *items, uint32_t *bias, uint32_t *count)
-{
-    hipStream_t s = ctx->stream;
-    const uint32_t h = n / 2;
-    const uint32_t B = comp_count(h) * comp_count(n - h);
-    const uint32_t nwin = (N + window - 1) / window;
-    const uint32_t nsb = (B + kScanTile - 1) / kScanTile;
-    const uint32_t *order;
-    int rc;
-    if ((rc = class_order_table(ctx, n, &order))) return rc;
+    if ((rc = ctx->fsets.ensure(sizeof(FilterSet) * nf))) return rc;
+    ctx->fs_host = fs; // the copy below reads it
+    HIP_TRY(hipMemcpyAsync(ctx->fsets.p, ctx->fs_host.data(), sizeof(FilterSet) * nf, hipMemcpyHostToDevice, s));
+    const FilterSet *d_fs = ctx->fsets.as<FilterSet>();
     uint32_t *hist = ctx->rcount.as<uint32_t>();
     uint64_t *off = ctx->roffset.as<uint64_t>();
-    const uint32_t g = (N + 255) / 256;
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(B, 256, 4096)), dim3(256), 0, s, hist, (uint64_t)B, 0u);
-    hipLaunchKernelGGL(k_node_keys, dim3(g), dim3(256), 0, s, d_syms, N, n, order, ctx->feat.as<uint2>(),
+    const uint32_t g = (uint32_t)((F + 255) / 256);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(B, 256, 4096)), dim3(256), 0, s, hist, B, 0u);
+    hipLaunchKernelGGL(k_fs_keys, dim3(g), dim3(256), 0, s, d_syms, d_fs, nf, (uint32_t)F, ctx->feat.as<uint2>(),
                        ctx->nkey.as<uint32_t>(), hist);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(kScanThreads), 0, s, hist, B, ctx->bsum.as<uint64_t>());
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, s, ctx->bsum.as<uint64_t>(), nsb);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(kScanThreads), 0, s, hist, B, ctx->bsum.as<uint64_t>(), off);
-    hipLaunchKernelGGL(k_node_scatter, dim3(g), dim3(256), 0, s, ctx->nkey.as<uint32_t>(), N, off, hist, perm);
-    if (wide_words(n))
-        hipLaunchKernelGGL(k_node_permute<uint64_t>, dim3(g), dim3(256), 0, s, perm, N,
-                           static_cast<const uint64_t *>(packed), ctx->feat.as<uint2>(),
-                           static_cast<uint64_t *>(packed_s), feat_s);
-    else
-        hipLaunchKernelGGL(k_node_permute<uint32_t>, dim3(g), dim3(256), 0, s, perm, N,
-                           static_cast<const uint32_t *>(packed), ctx->feat.as<uint2>(),
-                           static_cast<uint32_t *>(packed_s), feat_s);
-    hipLaunchKernelGGL(k_node_items, dim3(nwin), dim3(64), 0, s, feat_s, N, window, items, bias, count);
+    scan_counts(s, hist, (uint32_t)B, ctx->bsum.as<uint64_t>(), off);
+    hipLaunchKernelGGL(k_fs_scatter, dim3(g), dim3(256), 0, s, ctx->nkey.as<uint32_t>(), d_fs, nf, (uint32_t)F, off,
+                       hist, perm);
+    hipLaunchKernelGGL(k_fs_permute, dim3(g), dim3(256), 0, s, d_fs, nf, (uint32_t)F, perm,
+                       static_cast<const uint8_t *>(packed), ctx->feat.as<uint2>(), static_cast<uint8_t *>(packed_s),
+                       feat_s);
+    hipLaunchKernelGGL(k_fs_items, dim3((uint32_t)W), dim3(64), 0, s, d_fs, nf, feat_s, items, bias, count);
     HIP_TRY(hipGetLastError());
     return IGC_OK;
 }
@@ -482,16 +479,20 @@ static int setup_filter(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_
 {
     const uint32_t nwin = (N + window - 1) / window;
     int rc;
-    if ((rc = filter_scratch(ctx, N, n))) return rc;
     if ((rc = ctx->feat_s.ensure(sizeof(uint2) * (size_t)N))) return rc;
     if ((rc = ctx->perm.ensure(sizeof(uint32_t) * (size_t)N))) return rc;
     if ((rc = ctx->packed_s.ensure(word_bytes(n) * (size_t)N))) return rc;
     if ((rc = ctx->fitems.ensure(sizeof(uint4) * kItemCap * (size_t)nwin))) return rc;
     if ((rc = ctx->fbias.ensure(sizeof(uint32_t) * kItemCap * (size_t)nwin))) return rc;
     if ((rc = ctx->fcount.ensure(sizeof(uint32_t) * (size_t)nwin))) return rc;
-    if ((rc = filter_set(ctx, d_syms, N, n, window, ctx->packed.p, ctx->packed_s.p, ctx->feat_s.as<uint2>(),
-                         ctx->perm.as<uint32_t>(), ctx->fitems.as<uint4>(), ctx->fbias.as<uint32_t>(),
-                         ctx->fcount.as<uint32_t>())))
+    std::vector<FilterSet> fs(1);
+    fs[0] = FilterSet{};
+    fs[0].N = N;
+    fs[0].n = n;
+    fs[0].window = window;
+    if ((rc = filter_sets(ctx, d_syms, ctx->packed.p, ctx->packed_s.p, ctx->feat_s.as<uint2>(),
+                          ctx->perm.as<uint32_t>(), ctx->fitems.as<uint4>(), ctx->fbias.as<uint32_t>(),
+                          ctx->fcount.as<uint32_t>(), fs)))
         return rc;
     f = filter_view(ctx->feat_s.as<uint2>(), ctx->fitems.as<uint4>(), ctx->fbias.as<uint32_t>(),
                     ctx->fcount.as<uint32_t>(), ctx->perm.as<uint32_t>(), window, K);
@@ -643,7 +644,7 @@ IGC_API int igc_ctx_destroy(igc_ctx *ctx)
         for (DevBuf *b : {&ctx->syms, &ctx->packed, &ctx->edges, &ctx->misc, &ctx->row_ptr, &ctx->col, &ctx->keys[0],
                           &ctx->keys[1], &ctx->rcount, &ctx->roffset, &ctx->segmeta, &ctx->nbase, &ctx->bsum,
                           &ctx->sets, &ctx->bounds,
-                          &ctx->feat, &ctx->feat_s, &ctx->nkey, &ctx->perm, &ctx->packed_s, &ctx->fitems, &ctx->fbias, &ctx->fcount})
+                          &ctx->feat, &ctx->feat_s, &ctx->nkey, &ctx->perm, &ctx->packed_s, &ctx->fitems, &ctx->fbias, &ctx->fcount, &ctx->fsets})
             b->release();
         for (DevBuf &b : ctx->corder) b.release();
         for (HostBuf *b : {&ctx->host_rp, &ctx->host_col, &ctx->stage}) b->release();
