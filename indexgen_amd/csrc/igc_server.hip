@@ -10,9 +10,11 @@
 // pipeline already serves the next batch.  A job's latency is its completion
 // time minus its submission time.
 //
-// Admission estimate per job: symbols + packed words + row pointers + 48 B per
-// edge of the density prior's estimate (edge list 8, col 8, the radix sort's
-// two key buffers 32).  A job whose estimate alone exceeds the budget is refused with
+// Admission estimate per job: symbols + packed words + row pointers + the
+// per-string arrays of the row filter and the batch (counts 8 + 8, permutation
+// 4, bucket keys 4, set base 4, window items ~3, sorted words) + 48 B per edge
+// of the density prior's estimate (edge list 8, col 8, the radix sort's two
+// key buffers 32 in the 64-bit form; less in the 32-bit form).  A job whose estimate alone exceeds the budget is refused with
 // IGC_ERR_ADMISSION; otherwise FIFO order is kept (a job that does not fit
 // waits for memory; nothing behind it overtakes it).
 //
@@ -49,7 +51,7 @@ static uint64_t job_bytes(uint32_t N, uint32_t n, uint32_t ed)
 {
     const int K = ed == 0 ? -1 : effective_k(n, ed);
     const uint64_t edges = (K < 0 || N < 2) ? 0 : initial_edge_cap(N, n, K);
-    return (uint64_t)N * n + (uint64_t)N * word_bytes(n) + ((uint64_t)N + 1) * 8 + edges * 48;
+    return (uint64_t)N * n + 2ull * N * word_bytes(n) + ((uint64_t)N + 1) * 8 + (uint64_t)N * 32 + edges * 48;
 }
 
 } // namespace igc
