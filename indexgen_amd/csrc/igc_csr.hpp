@@ -207,6 +207,9 @@ static __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(
     const uint32_t tile_n = (uint32_t)(left < kSegTile ? left : kSegTile);
     const uint32_t nd = 1u << bits;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // this thread's digit offset, loaded now so its latency overlaps the counting
+    const uint64_t my_off =
+        threadIdx.x < nd ? offsets[(uint64_t)m.tfirst * nd + (uint64_t)threadIdx.x * m.ntiles + k] : 0ull;
     for (uint32_t d = threadIdx.x; d < kSegMaxDigits; d += kSegThreads)
         for (int w = 0; w < kWaves; ++w) wpos[w][d] = 0;
     __syncthreads();
@@ -227,7 +230,7 @@ static __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(
             c[q] = 0;
             if (d < kSegMaxDigits) {
                 for (int w = 0; w < kWaves; ++w) c[q] += wpos[w][d];
-                gbase[d] = d < nd ? offsets[(uint64_t)m.tfirst * nd + (uint64_t)d * m.ntiles + k] : 0ull;
+                gbase[d] = my_off; // d == threadIdx.x (one digit per thread)
             }
             tsum += c[q];
         }
@@ -348,6 +351,9 @@ static __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(RadixPas
     const uint32_t tile_n = (uint32_t)((n - start) < kRadixTile ? (n - start) : kRadixTile);
     const uint32_t nd = 1u << p.bits;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    static_assert(kDigitsPerThread == 1, "one digit per thread");
+    // this thread's digit offset, loaded now so its latency overlaps the counting
+    const uint64_t my_off = threadIdx.x < nd ? offsets[(uint64_t)threadIdx.x * tiles + b] : 0ull;
     for (uint32_t d = threadIdx.x; d < kRadixMaxDigits; d += kRadixThreads)
         for (int w = 0; w < kWaves; ++w) wpos[w][d] = 0;
     __syncthreads();
@@ -368,7 +374,7 @@ static __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(RadixPas
             const uint32_t d = threadIdx.x * kDigitsPerThread + k;
             c[k] = 0;
             for (int w = 0; w < kWaves; ++w) c[k] += wpos[w][d];
-            gbase[d] = d < nd ? offsets[(uint64_t)d * tiles + b] : 0ull;
+            gbase[d] = my_off; // d == threadIdx.x
             tsum += c[k];
         }
         uint32_t incl = tsum;
