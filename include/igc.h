@@ -54,10 +54,16 @@ extern "C" {
 #define IGC_ERR_IO 5        /* file could not be written/read */
 #define IGC_ERR_BUSY 6      /* result not ready yet (server poll) */
 
-/* longest candidate the kernels accept: the reference GPU solver's limit
- * (L <= 32, src/cuda_edit_distance.py:287).  Packed as two symbol bit planes
- * in a 32-bit word for n <= 16, in a 64-bit word above. */
-#define IGC_MAX_LEN 32
+/* longest candidate the kernels accept: the reference's binary GPU solver's
+ * limit (L <= 64, src/cuda_edit_distance_binary.py:272).  Strings up to
+ * IGC_MAX_QUATERNARY_LEN take symbols 0..3 (the quaternary solver's limit,
+ * L <= 32, src/cuda_edit_distance.py:287) and are packed as two symbol bit
+ * planes, in a 32-bit word for n <= 16 and a 64-bit word above; longer strings
+ * must be binary (symbols 0/1, the reference's isBinary path) and are packed
+ * one bit per symbol in a 64-bit word.  Binary strings of length <= 32 take the
+ * quaternary path, which gives the same distances. */
+#define IGC_MAX_LEN 64
+#define IGC_MAX_QUATERNARY_LEN 32
 
 /* ---- flags for igc_graph_build ---- */
 #define IGC_ORDER_ASCENDING 0u
@@ -131,12 +137,15 @@ IGC_API int igc_last_timing(igc_ctx *ctx, igc_timing *out);
 
 /* ---- single candidate set ---- */
 
-/* Builds the conflict graph of N candidates of length n (1 <= n <= 32) given
- * as an N x n row-major array of symbol codes 0..3 in host memory.  edit_dist
- * is the reference's minED: edge iff Levenshtein < edit_dist.  The CSR comes
- * back in host memory.
- * Replaces FillAdjListGPU (src/SparseMat.cpp:454-534) and the Python solver it
- * runs (src/gpu_graph_generator.py:17-80, src/cuda_edit_distance.py:278-447). */
+/* Builds the conflict graph of N candidates of length n (1 <= n <= 64) given
+ * as an N x n row-major array of symbol codes in host memory: 0..3 for
+ * n <= 32, 0/1 for n > 32 (IGC_ERR_INVALID otherwise).  edit_dist is the
+ * reference's minED: edge iff Levenshtein < edit_dist.  The CSR comes back in
+ * host memory.
+ * Replaces FillAdjListGPU (src/SparseMat.cpp:454-534) and the Python solvers
+ * it runs (src/gpu_graph_generator.py:17-80, src/cuda_edit_distance.py:278-447;
+ * with isBinary, src/gpu_graph_generator_binary.py:17-54,
+ * src/cuda_edit_distance_binary.py:285-389). */
 IGC_API int igc_graph_build(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint32_t n, uint32_t edit_dist,
                             uint32_t flags, igc_csr *out);
 

@@ -34,7 +34,8 @@ IGC_ERR_DEVICE = 3
 IGC_ERR_ADMISSION = 4
 IGC_ERR_IO = 5
 IGC_ERR_BUSY = 6
-IGC_MAX_LEN = 32
+IGC_MAX_LEN = 64  # strings longer than IGC_MAX_QUATERNARY_LEN must be binary (symbols 0/1)
+IGC_MAX_QUATERNARY_LEN = 32
 IGC_UPPER_TRIANGLE = 1  # flag: CSR rows list only neighbours above the row
 IGC_PREFILTER_OFF, IGC_PREFILTER_AUTO, IGC_PREFILTER_ALWAYS = 0, 1, 2
 IGC_PREFILTER_MIN_N = 16384  # smallest set the automatic mode sorts

@@ -77,6 +77,11 @@ static __global__ void k_pack_batch(const uint8_t *__restrict__ syms, const SetD
     const uint8_t *p = syms + d.sym_off + (size_t)i * n;
     uint32_t or_all = 0;
     for (uint32_t q = 0; q < n; ++q) or_all |= p[q];
+    if (n > IGC_MAX_QUATERNARY_LEN) {
+        reinterpret_cast<uint64_t *>(packed + d.word_off)[i] = pack_bin64(p, n);
+        if (or_all > 1u) atomicOr(bad, 2u);
+        return;
+    }
     if (n > 16) reinterpret_cast<uint64_t *>(packed + d.word_off)[i] = pack_word64(p, n);
     else reinterpret_cast<uint32_t *>(packed + d.word_off)[i] = pack_word(p, n);
     if (or_all > 3u) atomicOr(bad, 1u);
@@ -123,7 +128,7 @@ static __global__ void k_gather_bounds(const uint64_t *__restrict__ row_ptr, con
 struct BatchGroup {
     PredKind pk;
     int K;             // Landau-Vishkin threshold (pk == LV)
-    bool wide;         // 64-bit words (n > 16)
+    int wc;            // word class (word_class: 32-bit, 64-bit two-plane, 64-bit binary)
     uint32_t first;    // first descriptor of the group in BatchPlan::desc
     uint32_t count;    // descriptors in the group
     uint64_t tiles;    // tiles of the group's launch
@@ -175,11 +180,11 @@ static int plan_batch(igc_ctx *ctx, const igc_set *sets, uint32_t count, BatchPl
     P.sym_bytes = symb;
     P.word_bytes = wordb;
     P.by_node.resize(count);
-    // group key: predicate kind, K for Landau-Vishkin, word width
+    // group key: predicate kind, K for Landau-Vishkin, word class
     auto key = [&](uint32_t s) -> int {
         const PredKind pk = pred_kind(sets[s].n, K[s]);
         const int k = pk == PredKind::LV ? K[s] : 0;
-        return ((int)pk * 64 + k) * 2 + (wide_words(sets[s].n) ? 1 : 0);
+        return ((int)pk * 64 + k) * 3 + word_class(sets[s].n);
     };
     std::vector<uint32_t> run; // runnable sets, by group then size (largest first)
     for (uint32_t s = 0; s < count; ++s) {
@@ -206,7 +211,7 @@ static int plan_batch(igc_ctx *ctx, const igc_set *sets, uint32_t count, BatchPl
         SetDesc d = P.by_node[s];
         if (r == 0 || key(s) != key(run[r - 1])) {
             const PredKind pk = pred_kind(x.n, K[s]);
-            P.groups.push_back(BatchGroup{pk, pk == PredKind::LV ? K[s] : 0, wide_words(x.n), (uint32_t)P.desc.size(), 0, 0});
+            P.groups.push_back(BatchGroup{pk, pk == PredKind::LV ? K[s] : 0, word_class(x.n), (uint32_t)P.desc.size(), 0, 0});
         }
         BatchGroup &g = P.groups.back();
         if (filter_pays(ctx->prefilter, x.N, x.n, K[s])) {
@@ -238,23 +243,39 @@ static void launch_sets_t(hipStream_t s, const uint8_t *packed, const uint8_t *p
                        edges, cap, counter);
 }
 
-template <class W>
+// Policies of one word class: two-plane words of type W, or binary words (BIN).
+template <class W, bool BIN> struct SetPols {
+    template <int K> using LV = PolLV<K, W>;
+    using Myers = PolMyers<W>;
+    using All = PolAll<W>;
+};
+template <class W> struct SetPols<W, true> {
+    template <int K> using LV = PolLVBin<K>;
+    using Myers = PolMyersBin;
+    using All = PolAllBin;
+};
+
+template <class W, bool BIN = false>
 static int launch_sets_w(const BatchGroup &g, hipStream_t s, const uint8_t *packed, const uint8_t *packed_s,
                          const FilterBase &fb, const SetDesc *d_sets, uint64_t tile0, uint32_t ntiles,
                          uint64_t *edges, uint64_t cap, unsigned long long *counter)
 {
+    using Pols = SetPols<W, BIN>;
     if (g.pk == PredKind::MYERS) {
-        launch_sets_t<PolMyers<W>>(s, packed, packed_s, fb, d_sets, g.count, tile0, ntiles, edges, cap, counter);
+        launch_sets_t<typename Pols::Myers>(s, packed, packed_s, fb, d_sets, g.count, tile0, ntiles, edges, cap,
+                                            counter);
         return IGC_OK;
     }
     if (g.pk == PredKind::ALL) {
-        launch_sets_t<PolAll<W>>(s, packed, packed_s, fb, d_sets, g.count, tile0, ntiles, edges, cap, counter);
+        launch_sets_t<typename Pols::All>(s, packed, packed_s, fb, d_sets, g.count, tile0, ntiles, edges, cap,
+                                          counter);
         return IGC_OK;
     }
     switch (g.K) {
 #define IGC_GCASE(k)                                                                                               \
     case k:                                                                                                        \
-        launch_sets_t<PolLV<k, W>>(s, packed, packed_s, fb, d_sets, g.count, tile0, ntiles, edges, cap, counter);  \
+        launch_sets_t<typename Pols::template LV<k>>(s, packed, packed_s, fb, d_sets, g.count, tile0, ntiles,      \
+                                                     edges, cap, counter);                                         \
         return IGC_OK;
         IGC_GCASE(0) IGC_GCASE(1) IGC_GCASE(2) IGC_GCASE(3) IGC_GCASE(4) IGC_GCASE(5) IGC_GCASE(6) IGC_GCASE(7)
         IGC_GCASE(8) IGC_GCASE(9) IGC_GCASE(10) IGC_GCASE(11) IGC_GCASE(12) IGC_GCASE(13)
@@ -271,12 +292,11 @@ static int launch_group(igc_ctx *ctx, const BatchGroup &g, const FilterBase &fb,
     constexpr uint64_t kMaxGrid = 0x7FFFFFFFull;
     for (uint64_t t0 = 0; t0 < g.tiles; t0 += kMaxGrid) {
         const uint32_t nt = (uint32_t)std::min<uint64_t>(g.tiles - t0, kMaxGrid);
-        const int rc = g.wide ? launch_sets_w<uint64_t>(g, ctx->stream, ctx->packed.as<uint8_t>(),
-                                                        ctx->packed_s.as<uint8_t>(), fb, d_group, t0, nt, edges, cap,
-                                                        counter)
-                              : launch_sets_w<uint32_t>(g, ctx->stream, ctx->packed.as<uint8_t>(),
-                                                        ctx->packed_s.as<uint8_t>(), fb, d_group, t0, nt, edges, cap,
-                                                        counter);
+        const uint8_t *pk = ctx->packed.as<uint8_t>(), *pks = ctx->packed_s.as<uint8_t>();
+        const int rc =
+            g.wc == 2   ? launch_sets_w<uint64_t, true>(g, ctx->stream, pk, pks, fb, d_group, t0, nt, edges, cap, counter)
+            : g.wc == 1 ? launch_sets_w<uint64_t>(g, ctx->stream, pk, pks, fb, d_group, t0, nt, edges, cap, counter)
+                        : launch_sets_w<uint32_t>(g, ctx->stream, pk, pks, fb, d_group, t0, nt, edges, cap, counter);
         if (rc) return rc;
         HIP_TRY(hipGetLastError());
         ++*launches;
@@ -366,7 +386,7 @@ static int batch_device_locked(igc_ctx *ctx, const uint8_t *d_syms, const BatchP
         uint32_t host_misc[kMiscWords];
         HIP_TRY(hipMemcpyAsync(host_misc, misc, sizeof host_misc, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        if (host_misc[2]) return set_error(IGC_ERR_INVALID, "symbol code above 3 in input");
+        if (host_misc[2]) return bad_symbols(host_misc[2]);
         const uint64_t E = (uint64_t)host_misc[0] | ((uint64_t)host_misc[1] << 32);
         if (E > cap) {
             cap = E + E / 64 + 1024; // exact regrow, then replay the pairs

@@ -94,8 +94,12 @@ uint64_t initial_edge_cap(uint32_t N, uint32_t n, int K);
 int effective_k(uint32_t n, uint32_t edit_dist);
 uint32_t grid_for(uint64_t work, uint32_t per_block, uint32_t cap_blocks);
 
-// Packed words are 32-bit for n <= 16 and 64-bit for 17 <= n <= 32.
+// Packed words are 32-bit for n <= 16, 64-bit (two planes) for 17 <= n <= 32,
+// and 64-bit binary words (one plane, symbols 0/1) for 33 <= n <= 64.
 inline bool wide_words(uint32_t n) { return n > 16; }
+inline bool binary_words(uint32_t n) { return n > IGC_MAX_QUATERNARY_LEN; }
+// 0: 32-bit words, 1: 64-bit two-plane words, 2: 64-bit binary words
+inline int word_class(uint32_t n) { return n <= 16 ? 0 : (binary_words(n) ? 2 : 1); }
 inline size_t word_bytes(uint32_t n) { return wide_words(n) ? 8 : 4; }
 // Columns a pair test covers per register: two 16-bit lanes for n <= 16.
 inline int lane_cols(uint32_t n) { return wide_words(n) ? 1 : 2; }
@@ -108,16 +112,27 @@ int launch_pairs(uint32_t n, int K, int cpl, bool emit, hipStream_t s, uint64_t 
                  const RowFilter &f = RowFilter{});
 int launch_pack(hipStream_t s, const uint8_t *d_syms, uint32_t N, uint32_t n, void *packed, uint32_t *bad);
 int pred_variant(uint32_t n, int K);
-constexpr int kPredVariants = 2 * (kLvLaunchMaxK + 1) + 4;
+constexpr int kPredVariants = 3 * (kLvLaunchMaxK + 1) + 6;
 
 // Whether a set of N strings gets the row filter under prefilter `mode`: it
 // needs a nontrivial threshold, and automatically only sets of at least
 // IGC_PREFILTER_MIN_N strings (below it the sort's launches cost more than the
-// rows it saves).
+// rows it saves).  Binary lengths (n > 32) run unfiltered.
 inline bool filter_pays(int mode, uint32_t N, uint32_t n, int K)
 {
-    if (mode == IGC_PREFILTER_OFF || N < 2 || K < 1 || pred_kind(n, K) == PredKind::ALL) return false;
+    if (mode == IGC_PREFILTER_OFF || N < 2 || K < 1 || binary_words(n) || pred_kind(n, K) == PredKind::ALL)
+        return false;
     return mode == IGC_PREFILTER_ALWAYS || N >= IGC_PREFILTER_MIN_N;
+}
+
+// Error for a packing pass that flagged symbols: bit 0 a code above 3, bit 1 a
+// code above 1 in a binary string (n > 32).
+inline int bad_symbols(uint32_t flags)
+{
+    if (flags & 2u)
+        return set_error(IGC_ERR_INVALID, "symbol code above 1 in a binary string (strings longer than %d take "
+                                          "symbols 0/1 only)", IGC_MAX_QUATERNARY_LEN);
+    return set_error(IGC_ERR_INVALID, "symbol code above 3 in input");
 }
 
 } // namespace igc

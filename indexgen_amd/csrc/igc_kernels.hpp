@@ -5,6 +5,8 @@
 //              32-bit words for n <= 16 (bit p = sym[p] & 1, bit 16+p =
 //              sym[p] >> 1), 64-bit words for 17 <= n <= 32 (planes at bits 0
 //              and 32); all other bits 0, so a shift by d moves whole symbols.
+//              Binary strings of length 33 <= n <= 64 (symbols 0/1): one
+//              64-bit word, bit p = sym[p].
 //   edges[e]   (i << 32) | j for every conflicting pair i < j, in no order.
 //   row_ptr    uint64 [N+1];  col  uint32 [2|E|]  -- the symmetric CSR.
 //
@@ -57,9 +59,27 @@ __global__ void k_pack(const uint8_t *__restrict__ syms, uint32_t N, uint32_t n,
     if (or_all > 3u) atomicOr(bad, 1u);
 }
 
+// Binary strings: one bit per symbol; sets bit 1 of *bad when a symbol exceeds 1.
+static __global__ void k_pack_bin(const uint8_t *__restrict__ syms, uint32_t N, uint32_t n,
+                                  uint64_t *__restrict__ packed, uint32_t *__restrict__ bad)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const uint8_t *s = syms + (size_t)i * n;
+    uint64_t w = 0;
+    uint32_t or_all = 0;
+    for (uint32_t p = 0; p < n; ++p) {
+        const uint32_t c = s[p];
+        or_all |= c;
+        w |= (uint64_t)(c & 1u) << p;
+    }
+    packed[i] = w;
+    if (or_all > 1u) atomicOr(bad, 2u);
+}
+
 // Packed words of uniformly random strings of length n (SplitMix64 per index),
-// used to time tile shapes.
-template <class W>
+// used to time tile shapes.  BIN: binary words (n <= 64, one plane).
+template <class W, bool BIN = false>
 __global__ void k_random_packed(W *__restrict__ packed, uint32_t N, uint32_t n, uint64_t seed)
 {
     constexpr int H = WordTraits<W>::half;
@@ -69,9 +89,13 @@ __global__ void k_random_packed(W *__restrict__ packed, uint32_t N, uint32_t n, 
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     z ^= z >> 31;
-    const uint32_t mask = n >= 32 ? 0xFFFFFFFFu : ((1u << n) - 1u);
-    const uint32_t lo = (uint32_t)z & mask, hi = (uint32_t)(z >> 32) & mask;
-    packed[i] = (W)lo | ((W)hi << H);
+    if constexpr (BIN) {
+        packed[i] = (W)(n >= 64 ? z : (z & ((1ull << n) - 1ull)));
+    } else {
+        const uint32_t mask = n >= 32 ? 0xFFFFFFFFu : ((1u << n) - 1u);
+        const uint32_t lo = (uint32_t)z & mask, hi = (uint32_t)(z >> 32) & mask;
+        packed[i] = (W)lo | ((W)hi << H);
+    }
 }
 
 static __global__ void k_fill_u32(uint32_t *__restrict__ p, uint64_t count, uint32_t v)
@@ -147,6 +171,52 @@ struct PolAll {
     struct Row {};
     static __device__ __forceinline__ void row(Row &, W, uint32_t, int) {}
     static __device__ __forceinline__ uint32_t test(const Row &, const Col &, uint32_t, int) { return (1u << P) - 1u; }
+};
+
+// Binary words: one column per lane, the word itself.
+struct ColBin {
+    static constexpr int P = 1;
+    uint64_t w;
+    __device__ __forceinline__ void load(const uint64_t *__restrict__ packed, uint32_t j0, uint32_t N)
+    {
+        w = j0 < N ? packed[j0] : 0ull;
+    }
+};
+
+template <int K>
+struct PolLVBin {
+    using Word = uint64_t;
+    using Col = ColBin;
+    static constexpr int P = 1;
+    using Row = LvtRowBin<K>;
+    static __device__ __forceinline__ void row(Row &r, uint64_t a, uint32_t, int) { lvt_row_bin<K>(a, r); }
+    static __device__ __forceinline__ uint32_t test(const Row &r, const Col &c, uint32_t, int)
+    {
+        return lvt_test_bin<K>(r, c.w);
+    }
+};
+
+struct PolMyersBin {
+    using Word = uint64_t;
+    using Col = ColBin;
+    static constexpr int P = 1;
+    struct Row {
+        uint64_t t;
+    };
+    static __device__ __forceinline__ void row(Row &r, uint64_t a, uint32_t, int) { r.t = a; }
+    static __device__ __forceinline__ uint32_t test(const Row &r, const Col &c, uint32_t n, int k)
+    {
+        return myers_bin(r.t, c.w, n, k);
+    }
+};
+
+struct PolAllBin {
+    using Word = uint64_t;
+    using Col = ColBin;
+    static constexpr int P = 1;
+    struct Row {};
+    static __device__ __forceinline__ void row(Row &, uint64_t, uint32_t, int) {}
+    static __device__ __forceinline__ uint32_t test(const Row &, const Col &, uint32_t, int) { return 1u; }
 };
 
 // ---------------------------------------------------------------------------

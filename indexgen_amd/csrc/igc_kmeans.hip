@@ -284,7 +284,9 @@ static int kmeans_device_w(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint
 int kmeans_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n, uint32_t k, uint32_t iters,
                          uint64_t seed, bool use_graph, float *ms_out)
 {
-    if (n < 1 || n > IGC_MAX_LEN) return set_error(IGC_ERR_INVALID, "string length n=%u outside 1..%d", n, IGC_MAX_LEN);
+    if (n < 1 || n > IGC_MAX_QUATERNARY_LEN)
+        return set_error(IGC_ERR_INVALID, "string length n=%u outside 1..%d (K-means runs on quaternary strings)", n,
+                         IGC_MAX_QUATERNARY_LEN);
     if (wide_words(n)) return kmeans_device_w<uint64_t>(ctx, d_syms, N, n, k, iters, seed, use_graph, ms_out);
     return kmeans_device_w<uint32_t>(ctx, d_syms, N, n, k, iters, seed, use_graph, ms_out);
 }
@@ -342,7 +344,9 @@ IGC_API int igc_assign_fp32(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint3
                             uint32_t k, int32_t *out)
 {
     if (!ctx || (!syms && N) || !centroids || (!out && N)) return set_error(IGC_ERR_INVALID, "null argument");
-    if (n < 1 || n > IGC_MAX_LEN) return set_error(IGC_ERR_INVALID, "string length n=%u outside 1..%d", n, IGC_MAX_LEN);
+    if (n < 1 || n > IGC_MAX_QUATERNARY_LEN)
+        return set_error(IGC_ERR_INVALID, "string length n=%u outside 1..%d (K-means runs on quaternary strings)", n,
+                         IGC_MAX_QUATERNARY_LEN);
     if (k < 1) return set_error(IGC_ERR_INVALID, "k must be at least 1");
     if (N == 0) return IGC_OK;
     std::lock_guard<std::mutex> lk(ctx->mu);

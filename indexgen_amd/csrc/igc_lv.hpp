@@ -6,11 +6,14 @@
 //   32-bit word (n <= 16): bit p = sym[p] & 1, bit 16+p = sym[p] >> 1
 //   64-bit word (n <= 32): bit p = sym[p] & 1, bit 32+p = sym[p] >> 1
 // all other bits 0.  A shift by d moves whole symbols in both planes.
+// Binary strings (symbols 0/1) of length 33 <= n <= 64 use one plane:
+//   64-bit binary word: bit p = sym[p], all other bits 0.
 //
 // The tests run on a lane type V: u16x2 holds two pairs, one per 16-bit half
 // of a 32-bit register (n <= 16; packed 16-bit subtract, add and shift, bitwise
 // ops on the whole register), uint32_t holds one pair (n <= 32).  Both tests
-// give Levenshtein(A, B) <= K exactly:
+// give Levenshtein(A, B) <= K exactly (binary words run the same tests on
+// uint64_t lanes, one pair per register pair):
 //   lvt_test     Landau-Vishkin furthest-reaching diagonals on thermometer
 //                codes, O(K^2) instructions independent of n;
 //   myers_lanes  the bit-parallel Myers recurrence, O(n) instructions.
@@ -64,6 +67,13 @@ IGC_HD inline uint64_t pack_word64(const uint8_t *s, uint32_t n)
     return w;
 }
 
+IGC_HD inline uint64_t pack_bin64(const uint8_t *s, uint32_t n)
+{
+    uint64_t w = 0;
+    for (uint32_t p = 0; p < n; ++p) w |= (uint64_t)(s[p] & 1u) << p;
+    return w;
+}
+
 // ---------------------------------------------------------------------------
 // lane types
 
@@ -95,6 +105,7 @@ IGC_HD inline uint32_t lane_bits(u16x2 v, int h) { return (uint32_t)v[h]; }
 // lanes whose value is zero, as a bit mask
 IGC_HD inline uint32_t zero_lanes(uint32_t z) { return z == 0 ? 1u : 0u; }
 IGC_HD inline uint32_t zero_lanes(u16x2 z) { return (z[0] == 0 ? 1u : 0u) | (z[1] == 0 ? 2u : 0u); }
+IGC_HD inline uint32_t zero_lanes(uint64_t z) { return z == 0 ? 1u : 0u; }
 
 // Planes of one row word shifted by s (s >= 0: towards bit 0), one lane's width.
 IGC_HD inline void row_planes(uint32_t a, int s, u16x2 &lo, u16x2 &hi)
@@ -282,6 +293,64 @@ IGC_HD inline void text_planes(uint64_t a, uint32_t &lo, uint32_t &hi)
 }
 
 // ---------------------------------------------------------------------------
+// Binary words (one plane, n <= 64): the same two tests on uint64_t lanes.
+// Diagonal d's mismatch mask is (A >> d) ^ B (A << -d for d < 0); the
+// thermometer argument above holds unchanged on a 64-bit lane, and at n = 64
+// the lane's end is the strings' end.
+
+template <int K>
+struct LvtRowBin {
+    uint64_t w[LvShape<K>::WD];
+};
+
+template <int K>
+IGC_HD inline void lvt_row_bin(uint64_t a, LvtRowBin<K> &r)
+{
+    constexpr int C = LvShape<K>::C;
+IGC_UNROLL
+    for (int t = 0; t < LvShape<K>::WD; ++t) {
+        const int s = t - C;
+        r.w[t] = s >= 0 ? (a >> s) : (a << (-s));
+    }
+}
+
+// 1 when the pair (row, b) has Levenshtein <= K
+template <int K>
+IGC_HD inline uint32_t lvt_test_bin(const LvtRowBin<K> &r, uint64_t b)
+{
+    uint64_t m[LvShape<K>::WD];
+IGC_UNROLL
+    for (int t = 0; t < LvShape<K>::WD; ++t) {
+        m[t] = r.w[t] ^ b;
+#if defined(__HIP_DEVICE_COMPILE__)
+        asm volatile("" : "+v"(m[t]));
+#endif
+    }
+    return zero_lanes(lvt_residue<K, uint64_t>(m));
+}
+
+// Myers with text t (the row) and pattern p (the column): the match mask of
+// text symbol t[j] is p where t[j] = 1 and ~p where t[j] = 0.  Returns 1 when
+// the distance is <= K.
+IGC_HD inline uint32_t myers_bin(uint64_t t, uint64_t p, uint32_t n, int K)
+{
+    uint64_t vp = ~0ull, vn = 0ull;
+    for (uint32_t j = 0; j < n; ++j) {
+        const uint64_t s = 0ull - ((t >> j) & 1ull);
+        const uint64_t eq = ~(p ^ s);
+        const uint64_t d0 = (((eq & vp) + vp) ^ vp) | eq | vn;
+        const uint64_t hp = vn | ~(d0 | vp);
+        const uint64_t hn = vp & d0;
+        const uint64_t x = (hp << 1) | 1ull;
+        vn = x & d0;
+        vp = (hn << 1) | ~(x | d0);
+    }
+    const uint64_t mask = n >= 64 ? ~0ull : ((1ull << n) - 1ull);
+    const int dist = (int)n + __builtin_popcountll(vp & mask) - __builtin_popcountll(vn & mask);
+    return dist <= K ? 1u : 0u;
+}
+
+// ---------------------------------------------------------------------------
 // Which predicate a (length, threshold) pair uses: Landau-Vishkin while its
 // cost is below Myers', Myers otherwise, and no test at all when K >= n.
 // n <= 16 runs two pairs per register (u16x2), 17 <= n <= 32 one (uint32_t).
@@ -300,9 +369,13 @@ IGC_HD inline uint32_t lv_cost2(int K, bool wide)
 
 IGC_HD inline uint32_t myers_cost2(uint32_t n, bool wide) { return wide ? 2u * (15u * n + 10u) : 16u * n + 12u; }
 
+// Binary words run both tests on 64-bit lanes, where every operation costs
+// about twice its 32-bit form on both sides, so the comparison is the 32-bit
+// one at the binary length; Landau-Vishkin is capped at the instantiated K.
 IGC_HD inline PredKind pred_kind(uint32_t n, int K)
 {
     if (K >= kAllPairs || K >= (int)n) return PredKind::ALL;
+    if (n > 32) return (K <= kLvLaunchMaxK && lv_cost2(K, true) <= myers_cost2(n, true)) ? PredKind::LV : PredKind::MYERS;
     const bool wide = n > 16;
     if (K <= kLvMaxK && lv_cost2(K, wide) <= myers_cost2(n, wide)) return PredKind::LV;
     return PredKind::MYERS;

@@ -28,7 +28,9 @@ IGC_API int igc_refine(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint32_t n
                        uint32_t host_threads, uint32_t *out_idx, uint32_t *out_count, igc_refine_stats *stats)
 {
     if (!ctx || (!syms && N) || !out_idx || !out_count) return set_error(IGC_ERR_INVALID, "null argument");
-    if (n < 1 || n > IGC_MAX_LEN) return set_error(IGC_ERR_INVALID, "string length n=%u outside 1..%d", n, IGC_MAX_LEN);
+    if (n < 1 || n > IGC_MAX_QUATERNARY_LEN)
+        return set_error(IGC_ERR_INVALID, "string length n=%u outside 1..%d (K-means runs on quaternary strings)", n,
+                         IGC_MAX_QUATERNARY_LEN);
     if (k < 1) return set_error(IGC_ERR_INVALID, "k must be >= 1");
     if (converge_n < 1) converge_n = 1;
     if (host_threads < 1) host_threads = 1;

@@ -107,15 +107,15 @@ static int size_bucket(uint32_t N)
 }
 
 // Index of the kernel variant a (length, threshold) pair runs, for the table
-// of autotuned shapes: Landau-Vishkin on 32-bit words K, on 64-bit words
-// kLvLaunchMaxK+1+K, then Myers and every-pair on each width.
+// of autotuned shapes: Landau-Vishkin on word class c (word_class) at
+// c*(kLvLaunchMaxK+1)+K, then Myers and every-pair on each word class.
 int pred_variant(uint32_t n, int K)
 {
-    const int wide = wide_words(n) ? 1 : 0;
+    const int wc = word_class(n);
     switch (pred_kind(n, K)) {
-    case PredKind::LV: return wide * (kLvLaunchMaxK + 1) + K;
-    case PredKind::MYERS: return 2 * (kLvLaunchMaxK + 1) + wide;
-    default: return 2 * (kLvLaunchMaxK + 1) + 2 + wide;
+    case PredKind::LV: return wc * (kLvLaunchMaxK + 1) + K;
+    case PredKind::MYERS: return 3 * (kLvLaunchMaxK + 1) + wc;
+    default: return 3 * (kLvLaunchMaxK + 1) + 3 + wc;
     }
 }
 
@@ -144,7 +144,12 @@ static void launch_pairs_t(hipStream_t s, uint64_t tiles, const void *packed, co
                        static_cast<const typename Pol::Word *>(packed), G, node_off, n, K, f, edges, cap, counter);
 }
 
-template <class W, int CPL, bool EMIT>
+// Landau-Vishkin policy of word type W (uint32_t, uint64_t: two planes) or of
+// binary words (BIN)
+template <int K, class W, bool BIN> struct LvPolOf { using type = PolLV<K, W>; };
+template <int K, class W> struct LvPolOf<K, W, true> { using type = PolLVBin<K>; };
+
+template <class W, int CPL, bool EMIT, bool BIN = false>
 static int launch_pairs_lv(hipStream_t s, uint64_t tiles, const void *packed, const TileGeom &G, uint32_t node_off,
                            uint32_t n, int K, const RowFilter &f, uint64_t *edges, uint64_t cap,
                            unsigned long long *counter)
@@ -152,7 +157,8 @@ static int launch_pairs_lv(hipStream_t s, uint64_t tiles, const void *packed, co
     switch (K) {
 #define IGC_CASE(k)                                                                                           \
     case k:                                                                                                   \
-        launch_pairs_t<PolLV<k, W>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);  \
+        launch_pairs_t<typename LvPolOf<k, W, BIN>::type, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, f,  \
+                                                                     edges, cap, counter);                    \
         return IGC_OK;
         IGC_CASE(0) IGC_CASE(1) IGC_CASE(2) IGC_CASE(3) IGC_CASE(4) IGC_CASE(5) IGC_CASE(6) IGC_CASE(7)
         IGC_CASE(8) IGC_CASE(9) IGC_CASE(10) IGC_CASE(11) IGC_CASE(12) IGC_CASE(13)
@@ -168,17 +174,21 @@ static int launch_pairs_c(hipStream_t s, uint64_t tiles, const void *packed, con
                           uint32_t n, int K, const RowFilter &f, uint64_t *edges, uint64_t cap,
                           unsigned long long *counter)
 {
-    const bool wide = wide_words(n);
+    const int wc = word_class(n);
     switch (pred_kind(n, K)) {
     case PredKind::LV:
-        if (wide) return launch_pairs_lv<uint64_t, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
+        if (wc == 2)
+            return launch_pairs_lv<uint64_t, CPL, EMIT, true>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
+        if (wc == 1) return launch_pairs_lv<uint64_t, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
         return launch_pairs_lv<uint32_t, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
     case PredKind::MYERS:
-        if (wide) launch_pairs_t<PolMyers<uint64_t>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
+        if (wc == 2) launch_pairs_t<PolMyersBin, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
+        else if (wc == 1) launch_pairs_t<PolMyers<uint64_t>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
         else launch_pairs_t<PolMyers<uint32_t>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
         return IGC_OK;
     default:
-        if (wide) launch_pairs_t<PolAll<uint64_t>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
+        if (wc == 2) launch_pairs_t<PolAllBin, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
+        else if (wc == 1) launch_pairs_t<PolAll<uint64_t>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
         else launch_pairs_t<PolAll<uint32_t>, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
         return IGC_OK;
     }
@@ -204,11 +214,15 @@ int launch_pairs(uint32_t n, int K, int cpl, bool emit, hipStream_t s, uint64_t 
 }
 
 // Packs N strings of length n into words of the width of n (32-bit for
-// n <= 16, 64-bit above); sets *bad when a symbol code exceeds 3.
+// n <= 16, 64-bit above, binary words above 32); sets bit 0 of *bad when a
+// symbol code exceeds 3, bit 1 when a binary string holds a code above 1.
 int launch_pack(hipStream_t s, const uint8_t *d_syms, uint32_t N, uint32_t n, void *packed, uint32_t *bad)
 {
     if (N == 0) return IGC_OK;
-    if (wide_words(n))
+    if (binary_words(n))
+        hipLaunchKernelGGL(k_pack_bin, dim3((N + 255) / 256), dim3(256), 0, s, d_syms, N, n,
+                           static_cast<uint64_t *>(packed), bad);
+    else if (wide_words(n))
         hipLaunchKernelGGL(k_pack<uint64_t>, dim3((N + 255) / 256), dim3(256), 0, s, d_syms, N, n,
                            static_cast<uint64_t *>(packed), bad);
     else
@@ -555,7 +569,7 @@ static int build_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, 
         uint32_t host_misc[kMiscWords];
         HIP_TRY(hipMemcpyAsync(host_misc, misc, sizeof host_misc, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        if (host_misc[2]) return set_error(IGC_ERR_INVALID, "symbol code above 3 in input");
+        if (host_misc[2]) return bad_symbols(host_misc[2]);
         const uint64_t E = (uint64_t)host_misc[0] | ((uint64_t)host_misc[1] << 32);
         if (!count_only && E > cap) {
             cap = E + E / 64 + 1024; // exact regrow, then replay
@@ -832,7 +846,10 @@ IGC_API int igc_autotune(igc_ctx *ctx, uint32_t n, uint32_t edit_dist, uint32_t 
     const int K = effective_k(n, edit_dist);
     if ((rc = ctx->packed.ensure(word_bytes(n) * (size_t)N))) return rc;
     if ((rc = ctx->misc.ensure(sizeof(uint32_t) * kMiscWords))) return rc;
-    if (wide_words(n))
+    if (binary_words(n))
+        hipLaunchKernelGGL((k_random_packed<uint64_t, true>), dim3((N + 255) / 256), dim3(256), 0, s,
+                           ctx->packed.as<uint64_t>(), N, n, 0x5EEDull + N);
+    else if (wide_words(n))
         hipLaunchKernelGGL(k_random_packed<uint64_t>, dim3((N + 255) / 256), dim3(256), 0, s,
                            ctx->packed.as<uint64_t>(), N, n, 0x5EEDull + N);
     else

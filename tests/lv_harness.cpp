@@ -5,7 +5,7 @@
 // 16-bit lanes (n <= 16) or one pair on a 32-bit lane (wide, n <= 32).  For
 // 16-bit lanes every pair (A[r], B[r]) is run twice, once in each lane, with an
 // unrelated column string in the other lane; out[r] is 2 when the lanes
-// disagree.
+// disagree.  Binary strings (n <= 64) run on 64-bit lanes.
 #include "../indexgen_amd/csrc/igc_lv.hpp"
 
 #include <stddef.h>
@@ -79,6 +79,41 @@ extern "C" int myers_pairs(const uint8_t *A, const uint8_t *B, int M, int n, int
             out[r] = e0 == e1 ? (uint8_t)e0 : 2;
         }
     }
+    return 0;
+}
+
+namespace {
+struct LvBin {
+    template <int K>
+    static void run(const uint8_t *A, const uint8_t *B, int M, int n, uint8_t *out)
+    {
+        for (int r = 0; r < M; ++r) {
+            LvtRowBin<K> row;
+            lvt_row_bin<K>(pack_bin64(A + (size_t)r * n, n), row);
+            out[r] = (uint8_t)lvt_test_bin<K>(row, pack_bin64(B + (size_t)r * n, n));
+        }
+    }
+};
+} // namespace
+
+// binary strings (symbols 0/1, n <= 64) on 64-bit lanes: Landau-Vishkin
+extern "C" int lv_pairs_bin(const uint8_t *A, const uint8_t *B, int M, int n, int K, uint8_t *out)
+{
+    if (n > 64) return -1;
+    switch (K) {
+#define C_(k) case k: LvBin::run<k>(A, B, M, n, out); return 0;
+        C_(0) C_(1) C_(2) C_(3) C_(4) C_(5) C_(6) C_(7) C_(8) C_(9) C_(10) C_(11) C_(12) C_(13) C_(14) C_(15)
+#undef C_
+    }
+    return -1;
+}
+
+// ... and Myers (A is the text, B the pattern)
+extern "C" int myers_pairs_bin(const uint8_t *A, const uint8_t *B, int M, int n, int K, uint8_t *out)
+{
+    if (n > 64) return -1;
+    for (int r = 0; r < M; ++r)
+        out[r] = (uint8_t)myers_bin(pack_bin64(A + (size_t)r * n, n), pack_bin64(B + (size_t)r * n, n), (uint32_t)n, K);
     return 0;
 }
 

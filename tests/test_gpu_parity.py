@@ -163,7 +163,105 @@ def test_edge_cases(gpu_ctx):
         gpu_ctx.graph_build(np.array([[0, 4, 1]], np.uint8).repeat(3, 0), 2)
     assert e.value.code == igc.IGC_ERR_INVALID
     with pytest.raises(igc.IGCError):
-        gpu_ctx.graph_build(np.zeros((4, 33), np.uint8), 3)
+        gpu_ctx.graph_build(np.zeros((4, 65), np.uint8), 3)
+    # strings longer than 32 are binary: a symbol above 1 is refused
+    bad = np.zeros((40, 40), np.uint8)
+    bad[17, 5] = 2
+    with pytest.raises(igc.IGCError) as e:
+        gpu_ctx.graph_build(bad, 3)
+    assert e.value.code == igc.IGC_ERR_INVALID and "binary" in str(e.value)
+
+
+def _binary_set(rng, N, n, near):
+    """N random binary strings of length n; rows 0..near-1 are copies of random
+    rows with 1-6 bit flips or indel pairs."""
+    syms = rng.integers(0, 2, size=(N, n), dtype=np.uint8)
+    for r in range(near):
+        src = list(syms[int(rng.integers(0, N))])
+        for _ in range(int(rng.integers(1, 7))):
+            p = int(rng.integers(0, n))
+            if rng.integers(0, 2):
+                src[p] ^= 1
+            else:
+                del src[p]
+                src.insert(int(rng.integers(0, n)), int(rng.integers(0, 2)))
+        syms[r] = src
+    return syms
+
+
+@pytest.mark.parametrize("n", [33, 34, 40, 47, 48, 56, 63, 64])
+def test_binary_sets_match_oracle(gpu_ctx, n):
+    """Binary strings longer than 32 (the reference's isBinary solver, L <= 64):
+    Landau-Vishkin on 64-bit words up to K = 13, Myers above, every pair once
+    editDist > n; the same CSR as the oracle at every listed threshold."""
+    rng = np.random.default_rng(3300 + n)
+    N = 1500
+    syms = _binary_set(rng, N, n, N // 3)
+    syms[N // 2: N // 2 + 6] = 0  # all-zero and all-one strings
+    syms[N // 2 + 6: N // 2 + 12] = 1
+    for ed in (0, 1, 2, 3, 4, 5, 8, 11, 14, 15, 16, 20, 31, n - 1, n, n + 1):
+        banner(f"binary n={n} min_ed={ed} N={N}")
+        adj = gpu_ctx.graph_build(syms, ed)
+        rp, col = O.graph_csr(syms, ed)
+        assert np.array_equal(adj.row_ptr, rp), (n, ed)
+        assert np.array_equal(adj.col, col), (n, ed)
+        assert gpu_ctx.graph_count(syms, ed) == adj.n_edges
+
+
+def test_binary_ragged_sizes_and_thread_order(gpu_ctx):
+    rng = np.random.default_rng(77)
+    for N, n, ed in ((63, 64, 4), (64, 33, 3), (65, 50, 12), (257, 64, 17), (1025, 40, 6), (4097, 64, 9)):
+        banner(f"binary N={N} n={n} min_ed={ed}")
+        s = _binary_set(rng, N, n, N // 4)
+        adj = gpu_ctx.graph_build(s, ed)
+        rp, col = O.graph_csr(s, ed)
+        assert np.array_equal(adj.row_ptr, rp) and np.array_equal(adj.col, col), (N, n, ed)
+        adj4 = gpu_ctx.graph_build(s, ed, igc.order_threads(4))
+        rp4, col4 = O.graph_csr(s, ed, T=4)
+        assert np.array_equal(adj4.row_ptr, rp4) and np.array_equal(adj4.col, col4), (N, n, ed)
+        check_csr_wellformed(adj)
+
+
+def test_binary_larger_set_properties(gpu_ctx):
+    """A 60,000-string binary set at n = 64: the CSR is well formed, the
+    sampled rows equal the oracle's neighbour lists, and the greedy codebook
+    is independent and maximal under the oracle's predicate."""
+    rng = np.random.default_rng(640)
+    N, n, ed = 60000, 64, 9
+    syms = _binary_set(rng, N, n, N // 10)
+    banner(f"binary N={N} n={n} min_ed={ed}")
+    adj = gpu_ctx.graph_build(syms, ed)
+    check_csr_wellformed(adj)
+    for i in rng.integers(0, N, 12):
+        i = int(i)
+        got = adj.col[int(adj.row_ptr[i]): int(adj.row_ptr[i + 1])]
+        assert np.array_equal(got, O.row_neighbors(syms, ed, i)), i
+    order = igc.greedy_order(adj)
+    rp, col = O.graph_csr(syms[order[:1500]], ed)
+    assert rp[-1] == 0  # independent
+    assert gpu_ctx.graph_count(syms, ed) == adj.n_edges
+
+
+def test_binary_sets_in_a_batch(gpu_ctx):
+    """Binary sets (n > 32) get their own groups in a batch next to quaternary
+    sets; every CSR equals the oracle's and the single-set build's."""
+    rng = np.random.default_rng(4040)
+    sets = []
+    for n, N, ed in [(64, 1200, 5), (40, 900, 3), (33, 700, 14), (48, 500, 20), (64, 300, 70), (36, 2, 2),
+                     (50, 1, 3), (40, 0, 3), (13, 800, 4), (24, 600, 6), (64, 1000, 9)]:
+        sets.append((_binary_set(rng, N, n, N // 3) if n > 32 else rng.integers(0, 4, size=(N, n), dtype=np.uint8),
+                     ed))
+    banner(f"batch of {len(sets)} binary and quaternary sets")
+    outs = gpu_ctx.graph_build_batch(sets)
+    for (syms, ed), adj in zip(sets, outs):
+        rp, col = O.graph_csr(syms, ed) if syms.shape[0] else (np.zeros(1, np.uint64), np.zeros(0, np.uint32))
+        assert np.array_equal(adj.row_ptr, rp), (syms.shape, ed)
+        assert np.array_equal(adj.col, col), (syms.shape, ed)
+    bad = [(s.copy(), ed) for s, ed in sets]
+    bad[0][0][3, 7] = 3
+    with pytest.raises(igc.IGCError) as e:
+        gpu_ctx.graph_build_batch(bad)
+    assert "binary" in str(e.value)
 
 
 def test_dense_rows_use_every_sort_path(gpu_ctx):

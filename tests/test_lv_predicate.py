@@ -2,7 +2,8 @@
 host with g++, against the oracle: Landau-Vishkin on thermometer codes and the
 Myers recurrence, each on two-pairs-per-register 16-bit lanes (n <= 16) and on
 32-bit lanes (n <= 32), exhaustive over all pairs of short strings and every
-threshold, and on random and near pairs up to n = 32.  Also the dispatch rule:
+threshold, and on random and near pairs up to n = 32; the binary forms on
+64-bit lanes (symbols 0/1, n <= 64) the same way.  Also the dispatch rule:
 which predicate runs for (n, K)."""
 import ctypes
 import itertools
@@ -26,6 +27,9 @@ def harness(tmp_path_factory):
     for f in (L.lv_pairs, L.myers_pairs):
         f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                       ctypes.c_void_p]
+        f.restype = ctypes.c_int
+    for f in (L.lv_pairs_bin, L.myers_pairs_bin):
+        f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
         f.restype = ctypes.c_int
     L.pred_kind_of.argtypes = [ctypes.c_int, ctypes.c_int]
     L.pred_kind_of.restype = ctypes.c_int
@@ -160,3 +164,100 @@ def test_dispatch_rule(harness):
     assert harness.pred_kind_of(13, 3) == 0
     assert harness.pred_kind_of(32, 3) == 0
     assert harness.pred_kind_of(16, 14) == 1
+
+
+def _bin_call(fn, A, B, K):
+    A = np.ascontiguousarray(A, np.uint8)
+    B = np.ascontiguousarray(B, np.uint8)
+    out = np.zeros(A.shape[0], np.uint8)
+    assert fn(A.ctypes.data, B.ctypes.data, A.shape[0], A.shape[1], K, out.ctypes.data) == 0
+    return out
+
+
+def _bin_near(rng, a, edits):
+    b = list(a)
+    n = len(a)
+    for _ in range(edits):
+        op = int(rng.integers(0, 3))
+        p = int(rng.integers(0, n))
+        if op == 0:
+            b[p] ^= 1
+        elif op == 1:
+            del b[p]
+            b.append(int(rng.integers(0, 2)))
+        else:
+            b.insert(p, int(rng.integers(0, 2)))
+            b.pop()
+    return b
+
+
+def _bin_pairs(n, M, seed=0):
+    rng = np.random.default_rng(500 + n + seed)
+    A = rng.integers(0, 2, size=(M, n), dtype=np.uint8)
+    B = rng.integers(0, 2, size=(M, n), dtype=np.uint8)
+    for r in range(0, M, 2):
+        B[r] = _bin_near(rng, A[r], int(rng.integers(1, 18)))
+    A[1:200:10] = 0
+    B[3:300:10] = 1
+    A[5:400:20] = 1
+    return A, B
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
+def test_binary_exhaustive(harness, n):
+    S = np.array(list(itertools.product(range(2), repeat=n)), np.uint8)
+    N = S.shape[0]
+    A = np.repeat(S, N, axis=0)
+    B = np.tile(S, (N, 1))
+    for K in range(0, min(n + 2, 16)):
+        want = O.conflict_pairs(A, B, K + 1)
+        assert np.array_equal(_bin_call(harness.lv_pairs_bin, A, B, K), want), ("lv", n, K)
+        assert np.array_equal(_bin_call(harness.myers_pairs_bin, A, B, K), want), ("myers", n, K)
+
+
+@pytest.mark.parametrize("n", [17, 31, 32, 33, 34, 40, 47, 48, 57, 63, 64])
+def test_binary_random_and_near_pairs(harness, n):
+    A, B = _bin_pairs(n, 6000)
+    for K in range(0, n + 1):
+        want = O.conflict_pairs(A, B, K + 1)
+        got = _bin_call(harness.myers_pairs_bin, A, B, K)
+        assert np.array_equal(got, want), ("myers", n, K, int((got != want).sum()))
+        if K <= 15:
+            got = _bin_call(harness.lv_pairs_bin, A, B, K)
+            assert np.array_equal(got, want), ("lv", n, K, int((got != want).sum()))
+
+
+def test_oracle_pins_binary_distances():
+    """The oracle's distance and banded predicate on binary strings up to
+    n = 64 against the textbook dynamic program."""
+    rng = np.random.default_rng(11)
+    for n in (33, 48, 64):
+        for _ in range(40):
+            a = rng.integers(0, 2, n).astype(np.uint8)
+            b = np.array(_bin_near(rng, list(a), int(rng.integers(0, 20))), np.uint8)
+            D = np.arange(n + 1)
+            for i in range(1, n + 1):
+                prev, D = D, np.empty(n + 1, np.int64)
+                D[0] = i
+                for j in range(1, n + 1):
+                    D[j] = min(prev[j] + 1, D[j - 1] + 1, prev[j - 1] + (a[i - 1] != b[j - 1]))
+            assert O.levenshtein(a, b) == D[n]
+            for K in (0, 1, 3, int(D[n]) - 1, int(D[n]), int(D[n]) + 1):
+                if K < 0:
+                    continue
+                assert O.conflict(a, b, K + 1) == (D[n] <= K), (n, K, int(D[n]))
+
+
+def test_binary_dispatch_rule(harness):
+    """Binary lengths: Landau-Vishkin up to the instantiated K, Myers above,
+    every pair once K >= n."""
+    kmax = harness.lv_launch_max_k()
+    for n in range(33, 65):
+        for K in range(0, 70):
+            k = harness.pred_kind_of(n, K)
+            if K >= n:
+                assert k == 2
+            elif K <= kmax:
+                assert k == 0, (n, K)
+            else:
+                assert k == 1, (n, K)
