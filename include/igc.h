@@ -124,6 +124,14 @@ IGC_API void *igc_ctx_stream(igc_ctx *ctx);
 #define IGC_PREFILTER_ALWAYS 2
 #define IGC_PREFILTER_MIN_N 16384u
 IGC_API int igc_ctx_set_prefilter(igc_ctx *ctx, int mode);
+/* Pair predicate of the context's kernels: IGC_PRED_AUTO (default) picks the
+ * cheaper of Landau-Vishkin and Myers for (n, editDist); IGC_PRED_LANDAU_VISHKIN
+ * forces Landau-Vishkin wherever it is instantiated (editDist - 1 <= 13) and
+ * IGC_PRED_MYERS forces Myers; editDist > n always takes every pair.  The graph
+ * is the same either way (exact predicates); for measurement and tests.  No
+ * counterpart in the reference. */
+#define IGC_PRED_AUTO (-1)
+IGC_API int igc_ctx_set_predicate(igc_ctx *ctx, int predicate);
 /* How the CSR is assembled (same CSR either way).  IGC_CSR_AUTO (default): a
  * split pass on the row's high bits and 32-bit radix passes when the row's
  * low bits and the neighbour fit 32 bits with a split of at most 9 bits

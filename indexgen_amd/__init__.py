@@ -147,6 +147,7 @@ class _PairPlan(ctypes.Structure):
 
 
 PREDICATES = {0: "landau_vishkin", 1: "myers", 2: "all_pairs"}
+IGC_PRED_AUTO, IGC_PRED_LANDAU_VISHKIN, IGC_PRED_MYERS = -1, 0, 1
 
 
 class _CodebookInfo(ctypes.Structure):
@@ -214,6 +215,7 @@ def lib() -> ctypes.CDLL:
         "igc_tuned_shape": (i, [vp, u32, u32, u32, ctypes.POINTER(i), ctypes.POINTER(u32)]),
         "igc_pair_plan_of": (i, [vp, u32, u32, u32, ctypes.POINTER(_PairPlan)]),
         "igc_ctx_set_prefilter": (i, [vp, i]),
+        "igc_ctx_set_predicate": (i, [vp, i]),
         "igc_ctx_set_csr_form": (i, [vp, i]),
         "igc_assign_fp32": (i, [vp, vp, u32, u32, vp, u32, vp]),
         "igc_valu_probe": (i, [vp, vp]),
@@ -482,6 +484,12 @@ class Context:
         IGC_PREFILTER_MIN_N strings) or IGC_PREFILTER_ALWAYS."""
         m = int(mode) if not isinstance(mode, bool) else (IGC_PREFILTER_AUTO if mode else IGC_PREFILTER_OFF)
         _check(lib().igc_ctx_set_prefilter(self._h, m))
+
+    def set_predicate(self, predicate: int) -> None:
+        """Pair predicate of the kernels: IGC_PRED_AUTO (default, the cheaper for
+        (n, editDist)), IGC_PRED_LANDAU_VISHKIN (where editDist - 1 <= 13) or
+        IGC_PRED_MYERS; the same graph either way."""
+        _check(lib().igc_ctx_set_predicate(self._h, int(predicate)))
 
     def set_csr_form(self, form: int) -> None:
         """CSR assembly: IGC_CSR_AUTO (split pass + 32-bit radix passes where they

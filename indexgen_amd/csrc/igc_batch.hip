@@ -182,7 +182,7 @@ static int plan_batch(igc_ctx *ctx, const igc_set *sets, uint32_t count, BatchPl
     P.by_node.resize(count);
     // group key: predicate kind, K for Landau-Vishkin, word class
     auto key = [&](uint32_t s) -> int {
-        const PredKind pk = pred_kind(sets[s].n, K[s]);
+        const PredKind pk = ctx_pred(ctx, sets[s].n, K[s]);
         const int k = pk == PredKind::LV ? K[s] : 0;
         return ((int)pk * 64 + k) * 3 + word_class(sets[s].n);
     };
@@ -210,7 +210,7 @@ static int plan_batch(igc_ctx *ctx, const igc_set *sets, uint32_t count, BatchPl
         const igc_set &x = sets[s];
         SetDesc d = P.by_node[s];
         if (r == 0 || key(s) != key(run[r - 1])) {
-            const PredKind pk = pred_kind(x.n, K[s]);
+            const PredKind pk = ctx_pred(ctx, x.n, K[s]);
             P.groups.push_back(BatchGroup{pk, pk == PredKind::LV ? K[s] : 0, word_class(x.n), (uint32_t)P.desc.size(), 0, 0});
         }
         BatchGroup &g = P.groups.back();

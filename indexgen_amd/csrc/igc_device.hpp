@@ -99,7 +99,8 @@ uint32_t grid_for(uint64_t work, uint32_t per_block, uint32_t cap_blocks);
 inline bool wide_words(uint32_t n) { return n > 16; }
 inline bool binary_words(uint32_t n) { return n > IGC_MAX_QUATERNARY_LEN; }
 // 0: 32-bit words, 1: 64-bit two-plane words, 2: 64-bit binary words
-inline int word_class(uint32_t n) { return n <= 16 ? 0 : (binary_words(n) ? 2 : 1); }
+inline int word_class(uint32_t n) { return word_class_of(n); }
+static_assert(IGC_MAX_QUATERNARY_LEN == 32, "word_class_of splits at n = 32");
 inline size_t word_bytes(uint32_t n) { return wide_words(n) ? 8 : 4; }
 // Columns a pair test covers per register: two 16-bit lanes for n <= 16.
 inline int lane_cols(uint32_t n) { return wide_words(n) ? 1 : 2; }
@@ -107,11 +108,11 @@ inline int lane_cols(uint32_t n) { return wide_words(n) ? 1 : 2; }
 // Launches the pair kernel of the predicate pred_kind(n, K) picks, with cpl
 // columns per lane, over `tiles` tiles of geometry G; `packed` holds words of
 // the width of n.
-int launch_pairs(uint32_t n, int K, int cpl, bool emit, hipStream_t s, uint64_t tiles, const void *packed,
+int launch_pairs(PredKind pk, uint32_t n, int K, int cpl, bool emit, hipStream_t s, uint64_t tiles, const void *packed,
                  const TileGeom &G, uint32_t node_off, uint64_t *edges, uint64_t cap, unsigned long long *counter,
                  const RowFilter &f = RowFilter{});
 int launch_pack(hipStream_t s, const uint8_t *d_syms, uint32_t N, uint32_t n, void *packed, uint32_t *bad);
-int pred_variant(uint32_t n, int K);
+int pred_variant(PredKind pk, uint32_t n, int K);
 constexpr int kPredVariants = 3 * (kLvLaunchMaxK + 1) + 6;
 
 // Whether a set of N strings gets the row filter under prefilter `mode`: it
@@ -153,6 +154,7 @@ struct igc_ctx {
     std::vector<igc::FilterSet> fs_host; // the last filter set-up's table (read by its copy)
     igc::DevBuf corder[IGC_MAX_LEN + 1]; // class order along the curve, per string length
     int prefilter = IGC_PREFILTER_AUTO; // exact row skipping (IGC_PREFILTER_*)
+    int predicate = IGC_PRED_AUTO;      // forced pair predicate (IGC_PRED_*)
     igc::HostBuf host_rp, host_col, stage; // pinned: batch CSR copies, staged copies
     hipEvent_t ev[5] = {};
     igc_timing last{};
@@ -181,4 +183,17 @@ int32_t *kmeans_assign_ptr(igc_ctx *ctx);
 
 // igc_graph_build_batch without taking the context lock (caller holds it)
 int batch_build_host_locked(igc_ctx *ctx, const igc_set *sets, uint32_t count, igc_csr *outs);
+} // namespace igc
+
+namespace igc {
+// The predicate the context's kernels run for (n, K): pred_kind unless the
+// context forces one (every pair stays every pair; Landau-Vishkin only where
+// it is instantiated).
+inline PredKind ctx_pred(const igc_ctx *ctx, uint32_t n, int K)
+{
+    const PredKind pk = pred_kind(n, K);
+    if (pk == PredKind::ALL || ctx->predicate == IGC_PRED_AUTO) return pk;
+    if (ctx->predicate == IGC_PRED_LANDAU_VISHKIN) return K <= kLvLaunchMaxK ? PredKind::LV : pk;
+    return PredKind::MYERS;
+}
 } // namespace igc

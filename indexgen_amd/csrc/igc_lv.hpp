@@ -17,7 +17,7 @@
 //   lvt_test     Landau-Vishkin furthest-reaching diagonals on thermometer
 //                codes, O(K^2) instructions independent of n;
 //   myers_lanes  the bit-parallel Myers recurrence, O(n) instructions.
-// pred_kind() picks the cheaper one for (n, K).
+// pred_kind() picks the faster one for (n, K) by measured times.
 #pragma once
 #include <stdint.h>
 
@@ -35,8 +35,8 @@ namespace igc {
 constexpr int kAllPairs = 1000;
 // largest K lvt_residue is defined for
 constexpr int kLvMaxK = 15;
-// largest K for which pred_kind() picks Landau-Vishkin at any n <= 32 (the
-// kernels are instantiated for 0..kLvLaunchMaxK)
+// largest K the Landau-Vishkin kernels are instantiated for; pred_kind() picks
+// Myers above it
 constexpr int kLvLaunchMaxK = 13;
 
 template <class W> struct WordTraits;
@@ -351,33 +351,38 @@ IGC_HD inline uint32_t myers_bin(uint64_t t, uint64_t p, uint32_t n, int K)
 }
 
 // ---------------------------------------------------------------------------
-// Which predicate a (length, threshold) pair uses: Landau-Vishkin while its
-// cost is below Myers', Myers otherwise, and no test at all when K >= n.
-// n <= 16 runs two pairs per register (u16x2), 17 <= n <= 32 one (uint32_t).
+// Which predicate a (length, threshold) pair uses: the one with the lower
+// measured time per pair, and no test at all when K >= n.  Word classes:
+// 0 = two pairs per register (u16x2, n <= 16), 1 = one pair per 32-bit lane
+// (17 <= n <= 32), 2 = binary words on 64-bit lanes (33 <= n <= 64).
 enum class PredKind : int { LV = 0, MYERS = 1, ALL = 2 };
 
-// VALU instructions per two pairs on gfx950, counted in the hipcc -O3
-// assembly of kernels that run just one test (v_bitop3_b32 merges most of the
-// logic): Landau-Vishkin per K on u16x2 and on uint32_t (twice its one-pair
-// count), Myers per symbol plus a fixed part.
-IGC_HD inline uint32_t lv_cost2(int K, bool wide)
+IGC_HD inline int word_class_of(uint32_t n) { return n <= 16 ? 0 : (n <= 32 ? 1 : 2); }
+
+// Femtoseconds per pair of the count-only pair kernel at its fastest tile
+// shape on MI355X, N = 65536 random strings (indexgen_amd/predicate_sweep.py):
+// Landau-Vishkin per K for each word class (its cost does not depend on n),
+// Myers a least-squares line in n per word class (one step per symbol).
+IGC_HD inline uint32_t lv_fs(int wc, int K)
 {
-    const uint32_t c16[kLvMaxK + 1] = {11, 15, 29, 42, 67, 90, 125, 158, 203, 246, 301, 354, 419, 482, 557, 630};
-    const uint32_t c32[kLvMaxK + 1] = {7, 11, 24, 38, 62, 86, 120, 154, 198, 242, 296, 349, 414, 477, 552, 625};
-    return wide ? 2u * c32[K] : c16[K];
+    const uint32_t t[3][kLvLaunchMaxK + 1] = {
+        {116, 134, 291, 425, 726, 1005, 1484, 1971, 2556, 3107, 3771, 4369, 5283, 6028},
+        {150, 192, 493, 774, 1243, 1844, 2625, 3592, 4502, 5708, 7062, 8361, 9990, 11455},
+        {152, 218, 672, 1150, 1986, 2997, 4239, 5559, 7189, 8921, 11123, 13123, 15748, 18382}};
+    return t[wc][K];
 }
 
-IGC_HD inline uint32_t myers_cost2(uint32_t n, bool wide) { return wide ? 2u * (15u * n + 10u) : 16u * n + 12u; }
+IGC_HD inline uint32_t myers_fs(int wc, uint32_t n)
+{
+    const uint32_t a[3] = {386, 250, 259}, b[3] = {145, 241, 515};
+    return a[wc] + b[wc] * n;
+}
 
-// Binary words run both tests on 64-bit lanes, where every operation costs
-// about twice its 32-bit form on both sides, so the comparison is the 32-bit
-// one at the binary length; Landau-Vishkin is capped at the instantiated K.
 IGC_HD inline PredKind pred_kind(uint32_t n, int K)
 {
     if (K >= kAllPairs || K >= (int)n) return PredKind::ALL;
-    if (n > 32) return (K <= kLvLaunchMaxK && lv_cost2(K, true) <= myers_cost2(n, true)) ? PredKind::LV : PredKind::MYERS;
-    const bool wide = n > 16;
-    if (K <= kLvMaxK && lv_cost2(K, wide) <= myers_cost2(n, wide)) return PredKind::LV;
+    const int wc = word_class_of(n);
+    if (K <= kLvLaunchMaxK && lv_fs(wc, K) <= myers_fs(wc, n)) return PredKind::LV;
     return PredKind::MYERS;
 }
 

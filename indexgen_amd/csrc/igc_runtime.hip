@@ -109,10 +109,10 @@ static int size_bucket(uint32_t N)
 // Index of the kernel variant a (length, threshold) pair runs, for the table
 // of autotuned shapes: Landau-Vishkin on word class c (word_class) at
 // c*(kLvLaunchMaxK+1)+K, then Myers and every-pair on each word class.
-int pred_variant(uint32_t n, int K)
+int pred_variant(PredKind pk, uint32_t n, int K)
 {
     const int wc = word_class(n);
-    switch (pred_kind(n, K)) {
+    switch (pk) {
     case PredKind::LV: return wc * (kLvLaunchMaxK + 1) + K;
     case PredKind::MYERS: return 3 * (kLvLaunchMaxK + 1) + wc;
     default: return 3 * (kLvLaunchMaxK + 1) + 3 + wc;
@@ -121,7 +121,7 @@ int pred_variant(uint32_t n, int K)
 
 Shape pick_shape(igc_ctx *ctx, uint32_t n, int K, uint32_t N)
 {
-    const Shape t = ctx->tuned[pred_variant(n, K)][size_bucket(N)];
+    const Shape t = ctx->tuned[pred_variant(ctx_pred(ctx, n, K), n, K)][size_bucket(N)];
     return t.cpl ? t : default_shape(N);
 }
 
@@ -170,12 +170,12 @@ static int launch_pairs_lv(hipStream_t s, uint64_t tiles, const void *packed, co
 static_assert(kLvLaunchMaxK == 13, "launch_pairs_lv instantiates K = 0..13");
 
 template <int CPL, bool EMIT>
-static int launch_pairs_c(hipStream_t s, uint64_t tiles, const void *packed, const TileGeom &G, uint32_t node_off,
+static int launch_pairs_c(PredKind pk, hipStream_t s, uint64_t tiles, const void *packed, const TileGeom &G, uint32_t node_off,
                           uint32_t n, int K, const RowFilter &f, uint64_t *edges, uint64_t cap,
                           unsigned long long *counter)
 {
     const int wc = word_class(n);
-    switch (pred_kind(n, K)) {
+    switch (pk) {
     case PredKind::LV:
         if (wc == 2)
             return launch_pairs_lv<uint64_t, CPL, EMIT, true>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
@@ -194,19 +194,19 @@ static int launch_pairs_c(hipStream_t s, uint64_t tiles, const void *packed, con
     }
 }
 
-int launch_pairs(uint32_t n, int K, int cpl, bool emit, hipStream_t s, uint64_t tiles, const void *packed,
+int launch_pairs(PredKind pk, uint32_t n, int K, int cpl, bool emit, hipStream_t s, uint64_t tiles, const void *packed,
                  const TileGeom &G, uint32_t node_off, uint64_t *edges, uint64_t cap, unsigned long long *counter,
                  const RowFilter &f)
 {
     if (tiles == 0) return IGC_OK;
     if (tiles > 0x7FFFFFFFull) return set_error(IGC_ERR_INVALID, "too many tiles (%llu)", (unsigned long long)tiles);
     int rc;
-    if (cpl == 1) rc = emit ? launch_pairs_c<1, true>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter)
-                            : launch_pairs_c<1, false>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
-    else if (cpl == 2) rc = emit ? launch_pairs_c<2, true>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter)
-                                 : launch_pairs_c<2, false>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
-    else if (cpl == 4) rc = emit ? launch_pairs_c<4, true>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter)
-                                 : launch_pairs_c<4, false>(s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
+    if (cpl == 1) rc = emit ? launch_pairs_c<1, true>(pk, s, tiles, packed, G, node_off, n, K, f, edges, cap, counter)
+                            : launch_pairs_c<1, false>(pk, s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
+    else if (cpl == 2) rc = emit ? launch_pairs_c<2, true>(pk, s, tiles, packed, G, node_off, n, K, f, edges, cap, counter)
+                                 : launch_pairs_c<2, false>(pk, s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
+    else if (cpl == 4) rc = emit ? launch_pairs_c<4, true>(pk, s, tiles, packed, G, node_off, n, K, f, edges, cap, counter)
+                                 : launch_pairs_c<4, false>(pk, s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);
     else return set_error(IGC_ERR_INVALID, "unsupported columns-per-lane %d", cpl);
     if (rc) return rc;
     HIP_TRY(hipGetLastError());
@@ -560,7 +560,8 @@ static int build_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, 
         RowFilter flt{};
         if (use_filter && (rc = setup_filter(ctx, d_syms, N, n, K, 64u * cpl * lane_cols(n), flt))) return rc;
         HIP_TRY(hipEventRecord(ctx->ev[1], s));
-        rc = launch_pairs(n, K, cpl, !count_only, s, tiles, use_filter ? ctx->packed_s.p : ctx->packed.p, G, 0u,
+        rc = launch_pairs(ctx_pred(ctx, n, K), n, K, cpl, !count_only, s, tiles,
+                          use_filter ? ctx->packed_s.p : ctx->packed.p, G, 0u,
                           count_only ? nullptr : ctx->edges.as<uint64_t>(), cap, counter, flt);
         if (rc) return rc;
         HIP_TRY(hipEventRecord(ctx->ev[2], s));
@@ -688,6 +689,17 @@ IGC_API int igc_ctx_set_prefilter(igc_ctx *ctx, int mode)
         return set_error(IGC_ERR_INVALID, "prefilter mode %d is not 0, 1 or 2", mode);
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->prefilter = mode;
+    return IGC_OK;
+}
+
+IGC_API int igc_ctx_set_predicate(igc_ctx *ctx, int predicate)
+{
+    if (!ctx) return set_error(IGC_ERR_INVALID, "null context");
+    if (predicate != IGC_PRED_AUTO && predicate != IGC_PRED_LANDAU_VISHKIN && predicate != IGC_PRED_MYERS)
+        return set_error(IGC_ERR_INVALID, "predicate %d is not IGC_PRED_AUTO, IGC_PRED_LANDAU_VISHKIN or IGC_PRED_MYERS",
+                         predicate);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->predicate = predicate;
     return IGC_OK;
 }
 
@@ -871,7 +883,8 @@ IGC_API int igc_autotune(igc_ctx *ctx, uint32_t n, uint32_t edit_dist, uint32_t 
             float t_min = 0;
             for (uint32_t r = 0; r <= reps; ++r) { // first run warms up
                 HIP_TRY(hipEventRecord(ctx->ev[0], s));
-                if ((rc = launch_pairs(n, K, cpl, false, s, tiles, ctx->packed.p, G, 0u, nullptr, 0, counter)))
+                if ((rc = launch_pairs(ctx_pred(ctx, n, K), n, K, cpl, false, s, tiles, ctx->packed.p, G, 0u,
+                                       nullptr, 0, counter)))
                     return rc;
                 HIP_TRY(hipEventRecord(ctx->ev[1], s));
                 HIP_TRY(hipEventSynchronize(ctx->ev[1]));
@@ -896,7 +909,7 @@ IGC_API int igc_autotune(igc_ctx *ctx, uint32_t n, uint32_t edit_dist, uint32_t 
             break;
         }
     }
-    ctx->tuned[pred_variant(n, K)][size_bucket(N)] = best_shape;
+    ctx->tuned[pred_variant(ctx_pred(ctx, n, K), n, K)][size_bucket(N)] = best_shape;
     *count = k;
     return IGC_OK;
 }
@@ -922,7 +935,7 @@ IGC_API int igc_pair_plan_of(igc_ctx *ctx, uint32_t n, uint32_t edit_dist, uint3
     const bool filtered = filter_pays(ctx->prefilter, N, n, K);
     const Shape sh = build_shape(ctx, n, K, N, filtered);
     const TileGeom G = choose_geometry(N, sh.cpl * lane_cols(n), sh.q);
-    const PredKind pk = pred_kind(n, K);
+    const PredKind pk = ctx_pred(ctx, n, K);
     out->predicate = pk == PredKind::LV ? IGC_PRED_LANDAU_VISHKIN : (pk == PredKind::MYERS ? IGC_PRED_MYERS : IGC_PRED_ALL_PAIRS);
     out->k = (int)edit_dist - 1;
     out->lane_cols = lane_cols(n);

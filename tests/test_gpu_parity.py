@@ -264,6 +264,32 @@ def test_binary_sets_in_a_batch(gpu_ctx):
     assert "binary" in str(e.value)
 
 
+@pytest.mark.parametrize("n", [6, 10, 16, 20, 32, 40, 64])
+def test_forced_predicates_agree(gpu_ctx, n):
+    """Landau-Vishkin and Myers forced at the same (n, K), on every word class:
+    both give the oracle's CSR, single sets and a batch."""
+    rng = np.random.default_rng(6000 + n)
+    N = 900
+    syms = _binary_set(rng, N, n, N // 3) if n > 32 else rng.integers(0, 4, size=(N, n), dtype=np.uint8)
+    if n <= 32:
+        _near_copies(rng, syms, N // 3)
+    eds = sorted({e for e in (1, 2, 4, 7, 10, 12, 14) if e <= n})
+    want = {ed: O.graph_csr(syms, ed) for ed in eds}
+    try:
+        for pred in (igc.IGC_PRED_LANDAU_VISHKIN, igc.IGC_PRED_MYERS):
+            gpu_ctx.set_predicate(pred)
+            for ed in eds:
+                banner(f"forced predicate {pred} n={n} min_ed={ed}")
+                assert gpu_ctx.pair_plan(n, ed, N)["predicate"] == igc.PREDICATES[pred]
+                adj = gpu_ctx.graph_build(syms, ed)
+                assert np.array_equal(adj.row_ptr, want[ed][0]) and np.array_equal(adj.col, want[ed][1]), (pred, ed)
+            outs = gpu_ctx.graph_build_batch([(syms, ed) for ed in eds])
+            for ed, adj in zip(eds, outs):
+                assert np.array_equal(adj.row_ptr, want[ed][0]) and np.array_equal(adj.col, want[ed][1]), (pred, ed)
+    finally:
+        gpu_ctx.set_predicate(igc.IGC_PRED_AUTO)
+
+
 def test_dense_rows_use_every_sort_path(gpu_ctx):
     # random n=12 strings at editDist 5 give rows of ~90 neighbours (wave sort
     # <= 64 and LDS bitonic sort <= 4096); a cluster of 5000 strings sharing 10

@@ -145,11 +145,14 @@ def test_oracle_pins_wide_distances():
 def test_dispatch_rule(harness):
     """pred_kind: every pair when K >= n; Landau-Vishkin only up to the largest
     K the kernels are instantiated for, and for every K up to it at some n;
-    at the benchmark shape (n=13, K=3) Landau-Vishkin."""
+    Landau-Vishkin's reach in K grows with n within a word class (Myers' cost
+    grows with n, Landau-Vishkin's does not); at the benchmark shape (n=13,
+    K=3) Landau-Vishkin."""
     kmax = harness.lv_launch_max_k()
     seen, lv_k = set(), set()
-    for n in range(1, 33):
-        for K in range(0, 40):
+    reach = {}
+    for n in range(1, 65):
+        for K in range(0, 70):
             k = harness.pred_kind_of(n, K)
             seen.add(k)
             if K >= n:
@@ -159,12 +162,17 @@ def test_dispatch_rule(harness):
             if k == 0:
                 assert K <= kmax, (n, K)
                 lv_k.add(K)
+                # Landau-Vishkin at K implies it at every smaller K
+                assert all(harness.pred_kind_of(n, j) == 0 for j in range(K)), (n, K)
+                reach[n] = K
+    for lo, hi in ((1, 16), (17, 32), (33, 64)):
+        r = [reach.get(n, -1) for n in range(lo, hi + 1)]
+        assert r == sorted(r), (lo, hi, r)
     assert seen == {0, 1, 2}
     assert lv_k == set(range(kmax + 1))
     assert harness.pred_kind_of(13, 3) == 0
     assert harness.pred_kind_of(32, 3) == 0
     assert harness.pred_kind_of(16, 14) == 1
-
 
 def _bin_call(fn, A, B, K):
     A = np.ascontiguousarray(A, np.uint8)
@@ -249,15 +257,16 @@ def test_oracle_pins_binary_distances():
 
 
 def test_binary_dispatch_rule(harness):
-    """Binary lengths: Landau-Vishkin up to the instantiated K, Myers above,
-    every pair once K >= n."""
+    """Binary lengths: every pair once K >= n; Landau-Vishkin at every
+    instantiated K from n = 40 on (Myers' 64-bit steps cost n each), and at
+    every K up to 12 from n = 33."""
     kmax = harness.lv_launch_max_k()
     for n in range(33, 65):
         for K in range(0, 70):
             k = harness.pred_kind_of(n, K)
             if K >= n:
                 assert k == 2
-            elif K <= kmax:
-                assert k == 0, (n, K)
-            else:
+            elif K > kmax:
                 assert k == 1, (n, K)
+            elif K <= 12 or n >= 40:
+                assert k == 0, (n, K)
