@@ -250,6 +250,59 @@ IGC_API int igc_graph_build_batch(igc_ctx *ctx, const igc_set *sets, uint32_t co
 IGC_API int igc_graph_build_batch_device(igc_ctx *ctx, const igc_set *sets, uint32_t count, igc_csr *union_out,
                                          uint32_t *node_off, uint64_t *set_edges);
 
+/* ---- candidate generators on the device ---- */
+
+/* applyFilters (src/CandidateGenerator.cpp:24-50): keep strings whose longest
+ * run of one symbol is at most max_run (0: no run filter) and whose fraction of
+ * symbols 1 and 2 (C, G) lies in [min_gc, max_gc], compared in double as the
+ * reference does (both 0: no GC filter). */
+typedef struct igc_filter {
+    uint32_t max_run;
+    uint32_t pad;
+    double min_gc;
+    double max_gc;
+} igc_filter;
+
+/* Generated candidates: count x n symbols in device memory owned by the
+ * context, in the reference generator's order; valid until the next generator
+ * call on the context.  Feed them to igc_graph_build_device (count < 2^31). */
+typedef struct igc_candidates {
+    const uint8_t *d_syms;
+    uint64_t count;      /* strings kept */
+    uint64_t enumerated; /* words tried */
+    uint64_t serial;     /* the context's generation number (checked by igc_candidates_to_host) */
+    uint32_t n;
+    float ms;            /* device time of the generation (HIP events) */
+} igc_candidates;
+
+/* longest word the enumerating generators (VT, differential VT, all strings)
+ * accept: 4^21 words */
+#define IGC_GEN_MAX_WORD_LEN 21
+
+/* LinearCode candidates: every message m of k symbols in NextBase4 order
+ * (src/Utils.cpp:525-545), codeword m*G + bias over GF(4) (labels 0, 1, x,
+ * x+1 of GF(2)[x]/(x^2+x+1), src/Candidates/GF4.cpp:15-30), then the filter
+ * (NULL: none).  `gen` is the k x n generator matrix after the caller's row
+ * and column permutations, `bias` n symbols (NULL: zero).  1 <= k <= 15,
+ * n <= 32.  Replaces CodedVecs + applyFilters (src/Candidates/LinearCodes.cpp:
+ * 204-328, src/Candidates.cpp:178-188) for the LinearCode method. */
+IGC_API int igc_gen_linear(igc_ctx *ctx, const uint8_t *gen, uint32_t k, uint32_t n, const uint8_t *bias,
+                           const igc_filter *filter, igc_candidates *out);
+/* VTCode candidates (GenerateVTCodes, src/Candidates/VTCodes.cpp): every word
+ * of length n in base-4 index order with sum_{p>=1} p*[w_p >= w_{p-1}] == a
+ * (mod n) and sum_p w_p == b (mod 4), then the filter. */
+IGC_API int igc_gen_vt(igc_ctx *ctx, uint32_t n, int32_t a, int32_t b, const igc_filter *filter, igc_candidates *out);
+/* DifferentialVTCode candidates (GenerateDifferentialVTCodes,
+ * src/Candidates/DifferentialVTCodes.cpp): y_p = (w_p - w_{p+1}) mod 4,
+ * y_{n-1} = w_{n-1}, kept when sum_p (p+1)*y_p == syndrome (mod 4n). */
+IGC_API int igc_gen_diff_vt(igc_ctx *ctx, uint32_t n, int32_t syndrome, const igc_filter *filter,
+                            igc_candidates *out);
+/* AllStrings candidates: every word of length n in index order, filtered. */
+IGC_API int igc_gen_all(igc_ctx *ctx, uint32_t n, const igc_filter *filter, igc_candidates *out);
+/* Copies the first min(cap, count) strings of the context's latest generation
+ * to host memory (cap x n bytes). */
+IGC_API int igc_candidates_to_host(igc_ctx *ctx, const igc_candidates *c, uint8_t *out, uint64_t cap);
+
 /* ---- K-means on one-hot symbol features (cluster refinement) ---- */
 
 /* run the Lloyd iterations eagerly instead of replaying a captured hipGraph */

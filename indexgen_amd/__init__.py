@@ -146,6 +146,18 @@ class _PairPlan(ctypes.Structure):
                 ("tiles", ctypes.c_uint64), ("row_filter", ctypes.c_int), ("pad", ctypes.c_int)]
 
 
+class _Filter(ctypes.Structure):
+    _fields_ = [("max_run", ctypes.c_uint32), ("pad", ctypes.c_uint32), ("min_gc", ctypes.c_double),
+                ("max_gc", ctypes.c_double)]
+
+
+class _Cands(ctypes.Structure):
+    _fields_ = [("d_syms", ctypes.c_void_p), ("count", ctypes.c_uint64), ("enumerated", ctypes.c_uint64),
+                ("serial", ctypes.c_uint64), ("n", ctypes.c_uint32), ("ms", ctypes.c_float)]
+
+
+IGC_GEN_MAX_WORD_LEN = 21
+
 PREDICATES = {0: "landau_vishkin", 1: "myers", 2: "all_pairs"}
 IGC_PRED_AUTO, IGC_PRED_LANDAU_VISHKIN, IGC_PRED_MYERS = -1, 0, 1
 
@@ -219,6 +231,11 @@ def lib() -> ctypes.CDLL:
         "igc_ctx_set_csr_form": (i, [vp, i]),
         "igc_assign_fp32": (i, [vp, vp, u32, u32, vp, u32, vp]),
         "igc_valu_probe": (i, [vp, vp]),
+        "igc_gen_linear": (i, [vp, vp, u32, u32, vp, ctypes.POINTER(_Filter), ctypes.POINTER(_Cands)]),
+        "igc_gen_vt": (i, [vp, u32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(_Filter), ctypes.POINTER(_Cands)]),
+        "igc_gen_diff_vt": (i, [vp, u32, ctypes.c_int32, ctypes.POINTER(_Filter), ctypes.POINTER(_Cands)]),
+        "igc_gen_all": (i, [vp, u32, ctypes.POINTER(_Filter), ctypes.POINTER(_Cands)]),
+        "igc_candidates_to_host": (i, [vp, ctypes.POINTER(_Cands), vp, u64]),
         "igc_last_error": (ctypes.c_char_p, []),
         "igc_backend": (ctypes.c_char_p, []),
         "igc_version": (ctypes.c_char_p, []),
@@ -347,6 +364,19 @@ class Timing:
 
 
 @dataclass
+class DeviceCandidates:
+    """Candidates a generator left in device memory (owned by the context;
+    valid until its next generator call), in the reference generator's order."""
+
+    d_syms: int  # device address of count x n symbols
+    count: int
+    enumerated: int
+    n: int
+    ms: float
+    _raw: _Cands = None
+
+
+@dataclass
 class DeviceCSR:
     """CSR left in device memory by Context.graph_build_device (owned by the
     context; valid until its next build)."""
@@ -394,6 +424,60 @@ class Context:
                                             ctypes.byref(c)))
         return DeviceCSR(int(c.n_nodes), int(c.n_edges), ctypes.cast(c.row_ptr, ctypes.c_void_p).value or 0,
                          ctypes.cast(c.col, ctypes.c_void_p).value or 0, c)
+
+    # -- candidate generators (the reference's CandidateGenerator methods + applyFilters)
+
+    @staticmethod
+    def _filter(max_run, min_gc, max_gc):
+        return _Filter(int(max_run), 0, float(min_gc), float(max_gc))
+
+    def _cands(self, c: _Cands) -> DeviceCandidates:
+        return DeviceCandidates(int(c.d_syms or 0), int(c.count), int(c.enumerated), int(c.n), float(c.ms), c)
+
+    def gen_linear(self, G, bias=None, max_run: int = 0, min_gc: float = 0.0, max_gc: float = 0.0) -> DeviceCandidates:
+        """LinearCode candidates for generator matrix G (k x n over GF(4), after
+        permutations) and bias, filtered, left on the device."""
+        G = np.ascontiguousarray(G, dtype=np.uint8)
+        if G.ndim != 2:
+            raise ValueError("generator matrix must be 2-D (k, n)")
+        k, n = G.shape
+        b = None if bias is None else np.ascontiguousarray(bias, dtype=np.uint8)
+        if b is not None and b.shape != (n,):
+            raise ValueError(f"bias must have {n} entries")
+        c = _Cands()
+        f = self._filter(max_run, min_gc, max_gc)
+        _check(lib().igc_gen_linear(self._h, G.ctypes.data, k, n, None if b is None else b.ctypes.data,
+                                    ctypes.byref(f), ctypes.byref(c)))
+        return self._cands(c)
+
+    def gen_vt(self, n: int, a: int, b: int, max_run: int = 0, min_gc: float = 0.0, max_gc: float = 0.0):
+        c = _Cands()
+        f = self._filter(max_run, min_gc, max_gc)
+        _check(lib().igc_gen_vt(self._h, int(n), int(a), int(b), ctypes.byref(f), ctypes.byref(c)))
+        return self._cands(c)
+
+    def gen_diff_vt(self, n: int, syndrome: int, max_run: int = 0, min_gc: float = 0.0, max_gc: float = 0.0):
+        c = _Cands()
+        f = self._filter(max_run, min_gc, max_gc)
+        _check(lib().igc_gen_diff_vt(self._h, int(n), int(syndrome), ctypes.byref(f), ctypes.byref(c)))
+        return self._cands(c)
+
+    def gen_all(self, n: int, max_run: int = 0, min_gc: float = 0.0, max_gc: float = 0.0):
+        c = _Cands()
+        f = self._filter(max_run, min_gc, max_gc)
+        _check(lib().igc_gen_all(self._h, int(n), ctypes.byref(f), ctypes.byref(c)))
+        return self._cands(c)
+
+    def candidates_to_host(self, c: DeviceCandidates) -> np.ndarray:
+        out = np.zeros((max(c.count, 1), c.n), dtype=np.uint8)
+        _check(lib().igc_candidates_to_host(self._h, ctypes.byref(c._raw), out.ctypes.data, c.count))
+        return out[: c.count]
+
+    def graph_build_candidates(self, c: DeviceCandidates, min_ed: int, flags: int = 0) -> AdjList:
+        """Conflict graph of generated candidates without a host round trip of
+        the strings: the build reads them where the generator left them."""
+        d = self.graph_build_device(c.d_syms, c.count, c.n, min_ed)
+        return self.device_csr_to_host(d, flags)
 
     def device_csr_to_host(self, dcsr: DeviceCSR, flags: int = 0) -> AdjList:
         c = _Csr()

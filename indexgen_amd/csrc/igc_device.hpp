@@ -112,6 +112,9 @@ int launch_pairs(PredKind pk, uint32_t n, int K, int cpl, bool emit, hipStream_t
                  const TileGeom &G, uint32_t node_off, uint64_t *edges, uint64_t cap, unsigned long long *counter,
                  const RowFilter &f = RowFilter{});
 int launch_pack(hipStream_t s, const uint8_t *d_syms, uint32_t N, uint32_t n, void *packed, uint32_t *bad);
+// offsets[0..L] = exclusive scan of counts[0..L-1] (offsets[L] = total); bsum
+// holds ceil(L / kScanTile) partial sums
+void scan_counts(hipStream_t s, const uint32_t *counts, uint32_t L, uint64_t *bsum, uint64_t *offsets);
 int pred_variant(PredKind pk, uint32_t n, int K);
 constexpr int kPredVariants = 3 * (kLvLaunchMaxK + 1) + 6;
 
@@ -156,6 +159,8 @@ struct igc_ctx {
     int prefilter = IGC_PREFILTER_AUTO; // exact row skipping (IGC_PREFILTER_*)
     int predicate = IGC_PRED_AUTO;      // forced pair predicate (IGC_PRED_*)
     igc::HostBuf host_rp, host_col, stage; // pinned: batch CSR copies, staged copies
+    igc::DevBuf gen_out, gen_cnt, gen_off, gen_rows; // candidate generators: output, per-block counts/offsets, rows
+    uint64_t gen_serial = 0;                         // generations so far
     hipEvent_t ev[5] = {};
     igc_timing last{};
     int cu_count = 256;

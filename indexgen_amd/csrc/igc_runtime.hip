@@ -265,7 +265,7 @@ static uint32_t bits_for(uint64_t n) // smallest b with n <= 2^b (at least 1)
     return b;
 }
 
-static void scan_counts(hipStream_t s, const uint32_t *counts, uint32_t L, uint64_t *bsum, uint64_t *offsets)
+void scan_counts(hipStream_t s, const uint32_t *counts, uint32_t L, uint64_t *bsum, uint64_t *offsets)
 {
     const uint32_t nsb = (uint32_t)(((uint64_t)L + kScanTile - 1) / kScanTile);
     hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(kScanThreads), 0, s, counts, L, bsum);
@@ -659,7 +659,8 @@ IGC_API int igc_ctx_destroy(igc_ctx *ctx)
         for (DevBuf *b : {&ctx->syms, &ctx->packed, &ctx->edges, &ctx->misc, &ctx->row_ptr, &ctx->col, &ctx->keys[0],
                           &ctx->keys[1], &ctx->rcount, &ctx->roffset, &ctx->segmeta, &ctx->nbase, &ctx->bsum,
                           &ctx->sets, &ctx->bounds,
-                          &ctx->feat, &ctx->feat_s, &ctx->nkey, &ctx->perm, &ctx->packed_s, &ctx->fitems, &ctx->fbias, &ctx->fcount, &ctx->fsets})
+                          &ctx->feat, &ctx->feat_s, &ctx->nkey, &ctx->perm, &ctx->packed_s, &ctx->fitems, &ctx->fbias,
+                          &ctx->fcount, &ctx->fsets, &ctx->gen_out, &ctx->gen_cnt, &ctx->gen_off, &ctx->gen_rows})
             b->release();
         for (DevBuf &b : ctx->corder) b.release();
         for (HostBuf *b : {&ctx->host_rp, &ctx->host_col, &ctx->stage}) b->release();

@@ -600,3 +600,125 @@ ORC_EXPORT void orc_kmeans(const uint8_t *syms, int N, int n, int k, int iters, 
     if (C_out) memcpy(C_out, C, sizeof(float) * (size_t)k * D);
     free(C); free(cn2); free(cnt); free(size); free(seeds);
 }
+
+/* ------------------------------------------------------------------ */
+/* Candidate generators with applyFilters (src/CandidateGenerator.cpp:24-50).
+ * Each enumerates its words in the reference's order, keeps those that pass
+ * the generator's test and the filters (orc_filter's rules), writes the first
+ * `cap` kept strings to out (N x n symbols) and returns how many were kept.
+ *
+ *   orc_gen_linear  CodedVecs (src/Candidates/LinearCodes.cpp:204-328): every
+ *                   message m of length k in NextBase4 order (src/Utils.cpp:
+ *                   525-545; last digit fastest), codeword m*G + bias over
+ *                   GF(4) (src/Candidates/GF4.cpp:15-60: addition is XOR of
+ *                   the labels, multiplication the table below); G is the
+ *                   generator matrix after the caller's permutations.
+ *   orc_gen_vt      GenerateVTCodes (src/Candidates/VTCodes.cpp): words in
+ *                   base-4 index order (last symbol least significant), kept
+ *                   when sum over p >= 1 of p*[w[p] >= w[p-1]] == a (mod n) and
+ *                   sum of symbols == b (mod 4).
+ *   orc_gen_dvt     GenerateDifferentialVTCodes (src/Candidates/
+ *                   DifferentialVTCodes.cpp): y[p] = (w[p] - w[p+1]) mod 4 for
+ *                   p < n-1, y[n-1] = w[n-1]; kept when sum (p+1)*y[p] == s
+ *                   (mod 4n).
+ *   orc_gen_all     AllStringsGenerator: every word in index order.
+ * Pinned by tests/golden/gen_golden.npz, written by the compiled reference's
+ * own generators (tests/golden/make_golden.py). */
+static const uint8_t orc_gf4_mul[4][4] = {{0, 0, 0, 0}, {0, 1, 2, 3}, {0, 2, 3, 1}, {0, 3, 1, 2}};
+
+static int orc_keep(const uint8_t *w, int n, int max_run, double min_gc, double max_gc)
+{
+    const int use_run = max_run > 0;
+    const int use_gc = (min_gc > 0.0 || max_gc > 0.0);
+    if (use_run && orc_max_run(w, n) > max_run) return 0;
+    if (use_gc && !orc_gc_ok(w, n, min_gc, max_gc)) return 0;
+    return 1;
+}
+
+static void orc_index_word(uint64_t t, int n, uint8_t *w)
+{
+    for (int p = n - 1; p >= 0; --p) {
+        w[p] = (uint8_t)(t & 3u);
+        t >>= 2;
+    }
+}
+
+ORC_EXPORT int64_t orc_gen_linear(const uint8_t *G, int k, int n, const uint8_t *bias, int max_run, double min_gc,
+                                  double max_gc, uint8_t *out, int64_t cap)
+{
+    if (k < 1 || k > 15 || n < 1 || n > 64) return -1;
+    uint8_t m[16], w[64];
+    int64_t kept = 0;
+    const uint64_t total = 1ull << (2 * k);
+    for (uint64_t t = 0; t < total; ++t) {
+        orc_index_word(t, k, m);
+        for (int j = 0; j < n; ++j) {
+            uint8_t c = bias ? bias[j] : 0;
+            for (int i = 0; i < k; ++i) c ^= orc_gf4_mul[m[i]][G[(size_t)i * n + j]];
+            w[j] = c;
+        }
+        if (!orc_keep(w, n, max_run, min_gc, max_gc)) continue;
+        if (kept < cap) memcpy(out + (size_t)kept * n, w, (size_t)n);
+        ++kept;
+    }
+    return kept;
+}
+
+ORC_EXPORT int64_t orc_gen_vt(int n, int a, int b, int max_run, double min_gc, double max_gc, uint8_t *out,
+                              int64_t cap)
+{
+    if (n < 1 || n > 15) return -1;
+    uint8_t w[16];
+    int64_t kept = 0;
+    const long long am = ((a % n) + n) % n, bm = ((b % 4) + 4) % 4;
+    const uint64_t total = 1ull << (2 * n);
+    for (uint64_t t = 0; t < total; ++t) {
+        orc_index_word(t, n, w);
+        long long alpha = 0, sum = 0;
+        for (int p = 1; p < n; ++p)
+            if (w[p] >= w[p - 1]) alpha += p;
+        for (int p = 0; p < n; ++p) sum += w[p];
+        if (alpha % n != am || sum % 4 != bm) continue;
+        if (!orc_keep(w, n, max_run, min_gc, max_gc)) continue;
+        if (kept < cap) memcpy(out + (size_t)kept * n, w, (size_t)n);
+        ++kept;
+    }
+    return kept;
+}
+
+ORC_EXPORT int64_t orc_gen_dvt(int n, int s, int max_run, double min_gc, double max_gc, uint8_t *out, int64_t cap)
+{
+    if (n < 1 || n > 15) return -1;
+    uint8_t w[16];
+    int64_t kept = 0;
+    const long long mod = 4ll * n, sm = ((s % mod) + mod) % mod;
+    const uint64_t total = 1ull << (2 * n);
+    for (uint64_t t = 0; t < total; ++t) {
+        orc_index_word(t, n, w);
+        long long syn = 0;
+        for (int p = 0; p < n; ++p) {
+            const int y = p < n - 1 ? (((int)w[p] - (int)w[p + 1]) % 4 + 4) % 4 : w[n - 1];
+            syn += (long long)(p + 1) * y;
+        }
+        if (syn % mod != sm) continue;
+        if (!orc_keep(w, n, max_run, min_gc, max_gc)) continue;
+        if (kept < cap) memcpy(out + (size_t)kept * n, w, (size_t)n);
+        ++kept;
+    }
+    return kept;
+}
+
+ORC_EXPORT int64_t orc_gen_all(int n, int max_run, double min_gc, double max_gc, uint8_t *out, int64_t cap)
+{
+    if (n < 1 || n > 15) return -1;
+    uint8_t w[16];
+    int64_t kept = 0;
+    const uint64_t total = 1ull << (2 * n);
+    for (uint64_t t = 0; t < total; ++t) {
+        orc_index_word(t, n, w);
+        if (!orc_keep(w, n, max_run, min_gc, max_gc)) continue;
+        if (kept < cap) memcpy(out + (size_t)kept * n, w, (size_t)n);
+        ++kept;
+    }
+    return kept;
+}

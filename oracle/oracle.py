@@ -71,6 +71,13 @@ def lib() -> ctypes.CDLL:
         L.orc_kmeans.restype = None
         L.orc_assign_fp32.argtypes = [u8p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
         L.orc_assign_fp32.restype = None
+        d, vp, i, i64 = ctypes.c_double, ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
+        L.orc_gen_linear.argtypes = [vp, i, i, vp, i, d, d, vp, i64]
+        L.orc_gen_vt.argtypes = [i, i, i, i, d, d, vp, i64]
+        L.orc_gen_dvt.argtypes = [i, i, i, d, d, vp, i64]
+        L.orc_gen_all.argtypes = [i, i, d, d, vp, i64]
+        for f in (L.orc_gen_linear, L.orc_gen_vt, L.orc_gen_dvt, L.orc_gen_all):
+            f.restype = ctypes.c_int64
         _lib = L
     return _lib
 
@@ -205,3 +212,35 @@ def assign_fp32(syms: np.ndarray, C: np.ndarray) -> np.ndarray:
     out = np.zeros(N, dtype=np.int32)
     lib().orc_assign_fp32(ps, N, n, C.ctypes.data, C.shape[0], out.ctypes.data)
     return out
+
+
+def _gen(call, n):
+    """Runs an orc_gen_* restatement twice: once to count, once to fill."""
+    cnt = call(None, 0)
+    if cnt < 0:
+        raise ValueError("generator arguments out of range")
+    out = np.zeros((max(cnt, 1), n), dtype=np.uint8)
+    assert call(out.ctypes.data, cnt) == cnt
+    return out[:cnt]
+
+
+def gen_linear(G: np.ndarray, bias=None, max_run: int = 0, min_gc: float = 0.0, max_gc: float = 0.0) -> np.ndarray:
+    """Candidates of the LinearCode generator for generator matrix G (k x n over
+    GF(4)) and bias, after the filters, in the reference's order."""
+    G = np.ascontiguousarray(G, dtype=np.uint8)
+    k, n = G.shape
+    b = None if bias is None else np.ascontiguousarray(bias, dtype=np.uint8)
+    bp = None if b is None else b.ctypes.data
+    return _gen(lambda o, c: lib().orc_gen_linear(G.ctypes.data, k, n, bp, max_run, min_gc, max_gc, o, c), n)
+
+
+def gen_vt(n: int, a: int, b: int, max_run: int = 0, min_gc: float = 0.0, max_gc: float = 0.0) -> np.ndarray:
+    return _gen(lambda o, c: lib().orc_gen_vt(n, a, b, max_run, min_gc, max_gc, o, c), n)
+
+
+def gen_dvt(n: int, s: int, max_run: int = 0, min_gc: float = 0.0, max_gc: float = 0.0) -> np.ndarray:
+    return _gen(lambda o, c: lib().orc_gen_dvt(n, s, max_run, min_gc, max_gc, o, c), n)
+
+
+def gen_all(n: int, max_run: int = 0, min_gc: float = 0.0, max_gc: float = 0.0) -> np.ndarray:
+    return _gen(lambda o, c: lib().orc_gen_all(n, max_run, min_gc, max_gc, o, c), n)

@@ -22,6 +22,10 @@
 //                              Candidates(params) for the LinearCode generator
 //                              with MANUAL vectors (src/Candidates.cpp:178-188),
 //                              one candidate per line
+//   vt n a b T maxRun minGC maxGC
+//   dvt n s T maxRun minGC maxGC
+//   all n maxRun minGC maxGC   the same for the VTCode, DifferentialVTCode and
+//                              AllStrings generators (T threads)
 #include "CandidateGenerator.hpp"
 #include "Candidates.hpp"
 #include "EditDistance.hpp"
@@ -144,6 +148,34 @@ int main(int argc, char **argv)
         std::vector<std::string> cands = Candidates(params);
         std::cout.rdbuf(old);
         for (const auto &s : cands) std::cout << s << '\n';
+        return 0;
+    }
+    if ((mode == "vt" && argc == 9) || (mode == "dvt" && argc == 8) || (mode == "all" && argc == 6)) {
+        const int n = std::atoi(argv[2]);
+        std::unique_ptr<GenerationConstraints> c;
+        GenerationMethod m;
+        int T = 1, f = 3;
+        if (mode == "vt") {
+            c = std::make_unique<VTCodeConstraints>(std::atoi(argv[3]), std::atoi(argv[4]));
+            m = GenerationMethod::VT_CODE;
+            T = std::atoi(argv[5]);
+            f = 6;
+        } else if (mode == "dvt") {
+            c = std::make_unique<DifferentialVTCodeConstraints>(std::atoi(argv[3]));
+            m = GenerationMethod::DIFFERENTIAL_VT_CODE;
+            T = std::atoi(argv[4]);
+            f = 5;
+        } else {
+            c = std::make_unique<AllStringsConstraints>();
+            m = GenerationMethod::ALL_STRINGS;
+        }
+        Params params(n, 3, std::atoi(argv[f]), std::atof(argv[f + 1]), std::atof(argv[f + 2]), T, 0, m, std::move(c));
+        std::streambuf *old = std::cout.rdbuf();
+        std::ostringstream sink;
+        std::cout.rdbuf(sink.rdbuf());
+        std::vector<std::string> cands = Candidates(params);
+        std::cout.rdbuf(old);
+        for (const auto &x : cands) std::cout << x << '\n';
         return 0;
     }
     std::cerr << "bad arguments" << std::endl;

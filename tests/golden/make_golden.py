@@ -16,6 +16,16 @@ Outputs (data only -- inputs and the reference's outputs):
                         (Codebooks/N11/...), regenerated with the reference's
                         LinearCode generator, with the published edge count
                         ("Number of Ones in Matrix") and codebook digest
+  gen_golden.npz        candidate lists of the reference's LinearCode, VTCode,
+                        DifferentialVTCode and AllStrings generators after its
+                        filters (lists above 20,000 strings as count + SHA-256
+                        of the newline-joined strings); for LinearCode also the
+                        generator matrix and
+                        bias the reference used (read off its unfiltered list:
+                        message 0 is the bias, unit message e_i gives row i)
+                        and a digest of that unfiltered list
+
+    python tests/golden/make_golden.py gen     # only gen_golden.npz
 """
 from __future__ import annotations
 
@@ -181,13 +191,94 @@ def make_published():
         print("published", rel, syms.shape, "ones", info["ones"], "codewords", info["size"])
 
 
+def _driver_words(args) -> list:
+    return subprocess.run([DRIVER] + [str(a) for a in args], capture_output=True, text=True, check=True).stdout.split()
+
+
+def _syms(words, n) -> np.ndarray:
+    if not words:
+        return np.zeros((0, n), np.uint8)
+    return (np.frombuffer("".join(words).encode(), np.uint8).reshape(len(words), n) - 48).astype(np.uint8)
+
+
+GEN_K = {2: 1, 3: 3, 4: 5, 5: 7}  # n - k of the reference's codes by minimum Hamming distance
+GEN_FULL_ROWS = 20000  # longer kept lists are stored as their count and digest only
+
+
+def _store_kept(out, key, words, n):
+    out[f"{key}_count"] = np.int64(len(words))
+    out[f"{key}_digest"] = np.bytes_(hashlib.sha256("\n".join(words).encode()).hexdigest().encode())
+    if len(words) <= GEN_FULL_ROWS:
+        out[f"{key}_syms"] = _syms(words, n)
+
+
+def make_gen():
+    rng = np.random.default_rng(777)
+    out = {}
+    linear = [(8, 2, False, (0, 0.0, 0.0)), (9, 2, True, (3, 0.3, 0.7)), (10, 3, True, (3, 0.3, 0.7)),
+              (12, 4, True, (2, 0.25, 0.75)), (13, 5, False, (0, 0.0, 0.0)), (14, 5, True, (3, 0.0, 0.6))]
+    for rel in PUBLISHED:
+        info, _ = parse_published(os.path.join(REF, rel))
+        linear.append((info["n"], info["min_hd"], (info["bias"], info["row_perm"], info["col_perm"]),
+                       (info["max_run"], info["min_gc"], info["max_gc"])))
+    for c, (n, hd, vec, (mr, lo, hi)) in enumerate(linear):
+        k = n - GEN_K[hd]
+        if isinstance(vec, tuple):
+            bias, rowp, colp = (v or "-" for v in vec)
+        elif vec:
+            bias = ",".join(str(int(x)) for x in rng.integers(0, 4, n))
+            rowp = ",".join(str(int(x)) for x in rng.permutation(k))
+            colp = ",".join(str(int(x)) for x in rng.permutation(n))
+        else:
+            bias = rowp = colp = "-"
+        full = _driver_words(["linear", n, hd, bias, rowp, colp, 0, 0, 0])
+        assert len(full) == 4 ** k, (n, hd, len(full))
+        kept = _driver_words(["linear", n, hd, bias, rowp, colp, mr, lo, hi])
+        F = _syms(full, n)
+        b = F[0].copy()
+        G = np.stack([F[4 ** (k - 1 - i)] ^ b for i in range(k)]).astype(np.uint8)
+        out[f"linear{c}_G"] = G
+        out[f"linear{c}_bias"] = b
+        out[f"linear{c}_filter"] = np.array([mr, lo, hi], np.float64)
+        _store_kept(out, f"linear{c}", kept, n)
+        out[f"linear{c}_full_digest"] = np.bytes_(hashlib.sha256("\n".join(full).encode()).hexdigest().encode())
+        print("linear", n, hd, "k", k, "full", len(full), "kept", len(kept))
+    vt = [(8, 0, 0, 1, (0, 0.0, 0.0)), (9, 4, 3, 3, (3, 0.3, 0.7)), (10, 7, 1, 2, (2, 0.0, 0.0)),
+          (11, -3, 6, 4, (3, 0.3, 0.7))]
+    for c, (n, a, b, T, (mr, lo, hi)) in enumerate(vt):
+        kept = _driver_words(["vt", n, a, b, T, mr, lo, hi])
+        out[f"vt{c}_params"] = np.array([n, a, b], np.int64)
+        out[f"vt{c}_filter"] = np.array([mr, lo, hi], np.float64)
+        _store_kept(out, f"vt{c}", kept, n)
+        print("vt", n, a, b, "kept", len(kept))
+    dvt = [(8, 0, 1, (0, 0.0, 0.0)), (9, 13, 3, (3, 0.3, 0.7)), (10, -21, 2, (0, 0.2, 0.8)), (11, 40, 4, (3, 0.3, 0.7))]
+    for c, (n, sy, T, (mr, lo, hi)) in enumerate(dvt):
+        kept = _driver_words(["dvt", n, sy, T, mr, lo, hi])
+        out[f"dvt{c}_params"] = np.array([n, sy], np.int64)
+        out[f"dvt{c}_filter"] = np.array([mr, lo, hi], np.float64)
+        _store_kept(out, f"dvt{c}", kept, n)
+        print("dvt", n, sy, "kept", len(kept))
+    alls = [(6, (0, 0.0, 0.0)), (7, (2, 0.4, 0.6)), (9, (3, 0.3, 0.7))]
+    for c, (n, (mr, lo, hi)) in enumerate(alls):
+        kept = _driver_words(["all", n, mr, lo, hi])
+        out[f"all{c}_params"] = np.array([n], np.int64)
+        out[f"all{c}_filter"] = np.array([mr, lo, hi], np.float64)
+        _store_kept(out, f"all{c}", kept, n)
+        print("all", n, "kept", len(kept))
+    np.savez_compressed(os.path.join(HERE, "gen_golden.npz"), **out)
+
+
 def main():
     if not os.path.exists(DRIVER):
         sys.exit("build the reference driver first: make -C oracle ref")
+    if sys.argv[1:] == ["gen"]:
+        make_gen()
+        return
     rng = np.random.default_rng(20251203)
     make_pairs(rng)
     make_graphs(rng)
     make_published()
+    make_gen()
 
 
 if __name__ == "__main__":
