@@ -102,6 +102,51 @@ def cpu_baseline(syms: np.ndarray, min_ed: int, target_s: float = 12.0):
     }
 
 
+def generators_block(reps: int = 3) -> dict:
+    """On-device candidate generators (words enumerated per second, best of
+    reps, HIP events) against the reference's own generators (oracle/_ref
+    ref_driver, the same method and filters, host threads) on a bounded case."""
+    import indexgen_amd as igc
+    from oracle import oracle as O
+    ctx = igc.Context(0)
+    filt = (3, 0.3, 0.7)
+    rng = np.random.default_rng(5)
+    G = rng.integers(0, 4, size=(15, 20), dtype=np.uint8)
+    cases = {
+        "vt_n16": lambda: ctx.gen_vt(16, 0, 0, *filt),
+        "diff_vt_n16": lambda: ctx.gen_diff_vt(16, 0, *filt),
+        "linear_k15_n20": lambda: ctx.gen_linear(G, None, *filt),
+        "all_n16": lambda: ctx.gen_all(16, *filt),
+    }
+    out = {}
+    for name, fn in cases.items():
+        fn()
+        best = None
+        for _ in range(reps):
+            c = fn()
+            best = c.ms if best is None else min(best, c.ms)
+        out[name] = {"words": c.enumerated, "kept": c.count, "ms": round(best, 3),
+                     "gwords_s": round(c.enumerated / (best / 1e3) / 1e9, 1)}
+        print(f"[bench] generator {name}: {out[name]}", file=sys.stderr, flush=True)
+    ref = {}
+    T = host_threads()
+    if os.path.exists(O.REF_DRIVER):
+        for name, args, n_words in (("vt_n15", ["vt", 15, 0, 0, T, *filt], 4 ** 15),
+                                    ("diff_vt_n15", ["dvt", 15, 0, T, *filt], 4 ** 15)):
+            t0 = time.perf_counter()
+            words = subprocess.run([O.REF_DRIVER] + [str(x) for x in args], capture_output=True, text=True,
+                                   check=True).stdout.split()
+            secs = time.perf_counter() - t0
+            dev = ctx.gen_vt(15, 0, 0, *filt) if name == "vt_n15" else ctx.gen_diff_vt(15, 0, *filt)
+            assert dev.count == len(words), (name, dev.count, len(words))
+            ref[name] = {"words": n_words, "kept": len(words), "s": round(secs, 2), "threads": T,
+                         "gwords_s": round(n_words / secs / 1e9, 4),
+                         "device_ms": round(dev.ms, 3), "device_gwords_s": round(n_words / (dev.ms / 1e3) / 1e9, 1)}
+            print(f"[bench] reference generator {name}: {ref[name]}", file=sys.stderr, flush=True)
+    return {"metric": "candidate words enumerated per second", "unit": "Gwords/s", "device": out,
+            "reference_cpu": ref or None, "filter": {"max_run": filt[0], "gc": list(filt[1:])}}
+
+
 def batched_block(ctx, torch, steps: int = 3, seed: int = 3, sets_n: int = 64, scale: float = 1.0,
                   label: str = "batched"):
     """Uneven sets: one batched build for all of them against the plain form
@@ -224,9 +269,15 @@ def main():
     ap.add_argument("--serving-jobs", type=int, default=200, help="jobs of the serving stream to replay (0: skip)")
     ap.add_argument("--no-refine", action="store_true", help="skip the cluster-refinement block")
     ap.add_argument("--valu-probe", action="store_true", help="only measure the int32 VALU issue rates")
+    ap.add_argument("--generators", action="store_true",
+                    help="only measure the on-device candidate generators against the reference's")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (scales rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.generators:
+        import torch  # noqa: F401
+        print(json.dumps(generators_block()))
+        return
     if args.valu_probe:
         import torch  # noqa: F401  (initialises the runtime the same way as a bench run)
         import indexgen_amd as igc

@@ -135,9 +135,15 @@ __device__ __forceinline__ uint32_t gen_thread(const GenParams &P, const uint2 *
             if (t0 + j < P.total && gen_keep<KIND>(P, wl, wh)) keep |= 1u << j;
         }
     } else {
+        // t0 is a multiple of 16: word t0 + j is word t0 with its last two
+        // symbols (digits 0 and 1 of the index) set from j
+        uint32_t l0, h0;
+        index_planes(t0, P.n, l0, h0);
+        const uint32_t p1 = P.n - 1, p2 = P.n >= 2 ? P.n - 2 : 0;
 #pragma unroll
         for (int j = 0; j < kGenItems; ++j) {
-            index_planes(t0 + j, P.n, lo[j], hi[j]);
+            lo[j] = l0 | ((uint32_t)(j & 1) << p1) | (P.n >= 2 ? (uint32_t)((j >> 2) & 1) << p2 : 0u);
+            hi[j] = h0 | ((uint32_t)((j >> 1) & 1) << p1) | (P.n >= 2 ? (uint32_t)((j >> 3) & 1) << p2 : 0u);
             if (t0 + j < P.total && gen_keep<KIND>(P, lo[j], hi[j])) keep |= 1u << j;
         }
     }
@@ -178,20 +184,48 @@ __global__ __launch_bounds__(kGenThreads) void k_gen_write(GenParams P, const ui
 {
     __shared__ uint2 rows_s[4 * kGenMaxK];
     __shared__ uint64_t sh[kScanThreads / 64 + 1];
+    __shared__ uint2 staged[kGenThreads / 64][64 * kGenItems]; // a wave's kept words, in output order
     if constexpr (KIND == GEN_LINEAR) load_rows(P, rows_s);
     uint32_t lo[kGenItems], hi[kGenItems];
     const uint64_t t0 = (uint64_t)blockIdx.x * kGenBlock + (uint64_t)threadIdx.x * kGenItems;
-    uint32_t keep = gen_thread<KIND>(P, rows_s, t0, lo, hi);
+    const uint32_t keep = gen_thread<KIND>(P, rows_s, t0, lo, hi);
     uint64_t excl;
     block_exclusive_scan((uint64_t)__popc(keep), sh, excl);
-    uint64_t o = blk_off[blockIdx.x] + excl;
-    const uint32_t n = P.n;
+    const uint64_t o = blk_off[blockIdx.x] + excl;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the wave's kept words are output words wo .. wo + wc - 1
+    const uint64_t wo = uniform64(o);
+    const uint64_t end = o + __popc(keep);
+    const uint64_t wend = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(end >> 32), 63, 64) << 32) |
+                          (uint32_t)__shfl((int)(uint32_t)end, 63, 64);
+    const uint32_t wc = (uint32_t)(wend - wo);
+    uint32_t r = (uint32_t)(o - wo);
 #pragma unroll
-    for (int j = 0; j < kGenItems; ++j) {
-        if (!((keep >> j) & 1u)) continue;
-        uint8_t *w = out + o * n;
-        for (uint32_t p = 0; p < n; ++p) w[p] = (uint8_t)(((lo[j] >> p) & 1u) | (((hi[j] >> p) & 1u) << 1));
-        ++o;
+    for (int j = 0; j < kGenItems; ++j)
+        if ((keep >> j) & 1u) staged[wave][r++] = make_uint2(lo[j], hi[j]);
+    __syncthreads();
+    // bytes [b0, b1) of the output, written as aligned 32-bit words by
+    // consecutive lanes; the (at most 3 + 3) bytes outside the aligned part
+    // are byte stores (the neighbouring wave writes the rest of those words).
+    const uint32_t n = P.n;
+    const uint32_t inv = ((1u << 20) + n - 1) / n; // r / n = (r * inv) >> 20 for r < 2^15, n <= 32
+    const uint64_t b0 = wo * n, b1 = b0 + (uint64_t)wc * n;
+    const uint64_t a0 = (b0 + 3) & ~3ull, a1 = b1 & ~3ull;
+    const uint2 *st = staged[wave];
+    auto sym_at = [&](uint64_t pos) -> uint32_t {
+        const uint32_t rr = (uint32_t)(pos - b0);
+        const uint32_t w = (rr * inv) >> 20, p = rr - w * n;
+        const uint2 pl = st[w];
+        return ((pl.x >> p) & 1u) | (((pl.y >> p) & 1u) << 1);
+    };
+    if (a0 < a1) {
+        for (uint64_t q = a0 + 4ull * lane; q < a1; q += 256)
+            *reinterpret_cast<uint32_t *>(out + q) =
+                sym_at(q) | (sym_at(q + 1) << 8) | (sym_at(q + 2) << 16) | (sym_at(q + 3) << 24);
+        if (lane < a0 - b0) out[b0 + lane] = (uint8_t)sym_at(b0 + lane);
+        if (lane >= 32 && lane - 32 < b1 - a1) out[a1 + lane - 32] = (uint8_t)sym_at(a1 + lane - 32);
+    } else if (lane < b1 - b0) {
+        out[b0 + lane] = (uint8_t)sym_at(b0 + lane);
     }
 }
 
