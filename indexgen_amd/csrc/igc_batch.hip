@@ -63,6 +63,46 @@ static __device__ __forceinline__ uint32_t find_set_by(const SetDesc *__restrict
     return lo;
 }
 
+// One non-empty set's symbols in the caller's device memory and their byte
+// offset in the concatenated buffer (offsets ascending).
+struct SymSrc {
+    const uint8_t *p;
+    uint64_t off;
+};
+
+// Concatenates the sets' symbol arrays in one launch (instead of one copy per
+// set): thread t assembles output bytes [16t, 16t + 16) and writes them with
+// one aligned 16-byte store; the tail is written byte by byte.
+static __global__ void k_gather_syms(const SymSrc *__restrict__ src, uint32_t nsrc, uint64_t total,
+                                     uint8_t *__restrict__ dst)
+{
+    const uint64_t b0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 16;
+    if (b0 >= total) return;
+    uint32_t lo = 0, hi = nsrc - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (src[mid].off <= b0) lo = mid;
+        else hi = mid - 1;
+    }
+    uint32_t s = lo;
+    uint64_t next = s + 1 < nsrc ? src[s + 1].off : total;
+    uint32_t w[4] = {0u, 0u, 0u, 0u};
+    const uint32_t nb = (uint32_t)min<uint64_t>(16, total - b0);
+    for (uint32_t q = 0; q < nb; ++q) {
+        const uint64_t b = b0 + q;
+        while (b >= next) {
+            ++s;
+            next = s + 1 < nsrc ? src[s + 1].off : total;
+        }
+        w[q >> 2] |= (uint32_t)src[s].p[b - src[s].off] << (8 * (q & 3));
+    }
+    if (nb == 16) {
+        *reinterpret_cast<uint4 *>(dst + b0) = make_uint4(w[0], w[1], w[2], w[3]);
+    } else {
+        for (uint32_t q = 0; q < nb; ++q) dst[b0 + q] = (uint8_t)(w[q >> 2] >> (8 * (q & 3)));
+    }
+}
+
 static __global__ void k_pack_batch(const uint8_t *__restrict__ syms, const SetDesc *__restrict__ sets,
                                     uint32_t nsets, uint32_t total, uint8_t *__restrict__ packed,
                                     uint32_t *__restrict__ nbase, uint32_t *__restrict__ bad)
@@ -416,35 +456,78 @@ static int batch_device_locked(igc_ctx *ctx, const uint8_t *d_syms, const BatchP
     return set_error(IGC_ERR_DEVICE, "edge buffer kept overflowing");
 }
 
-// Host CSR of set s from the union CSR already copied to pinned host memory;
-// with IGC_UPPER_TRIANGLE each row keeps its neighbours above the row.
-static int split_set(const uint64_t *rp, const uint32_t *col, uint32_t off, uint32_t N, uint32_t flags, igc_csr *out)
+// Copies count entries of the union col (device) to dst, less shift (the set's
+// first node): a small slice by one direct copy, a large one through the two
+// pinned 8 MiB staging chunks with the subtraction fused into the copy out of
+// each chunk (the next chunk's transfer runs meanwhile).  No pinned buffer of
+// the union's size is needed.
+static int d2h_col_shift(igc_ctx *ctx, uint32_t *dst, const uint32_t *src, uint64_t count, uint32_t shift)
+{
+    constexpr uint64_t kChunk = (8u << 20) / sizeof(uint32_t); // entries per staging chunk
+    hipStream_t st = ctx->stream;
+    if (count == 0) return IGC_OK;
+    if (count <= kChunk) {
+        HIP_TRY(hipMemcpyAsync(dst, src, sizeof(uint32_t) * count, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (uint64_t q = 0; q < count; ++q) dst[q] -= shift;
+        return IGC_OK;
+    }
+    int rc = ctx->stage.ensure(2 * kChunk * sizeof(uint32_t));
+    if (rc) return rc;
+    const uint64_t chunks = (count + kChunk - 1) / kChunk;
+    auto part = [&](uint64_t c) { return std::min(kChunk, count - c * kChunk); };
+    for (uint64_t c = 0; c <= chunks; ++c) {
+        if (c < chunks) { // chunk c - 2 used this buffer and was consumed in the previous iteration
+            HIP_TRY(hipMemcpyAsync(ctx->stage.as<uint32_t>() + (c & 1) * kChunk, src + c * kChunk,
+                                   sizeof(uint32_t) * part(c), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipEventRecord(ctx->ev[c & 1], st));
+        }
+        if (c >= 1) {
+            const uint64_t p = c - 1;
+            HIP_TRY(hipEventSynchronize(ctx->ev[p & 1]));
+            const uint32_t *in = ctx->stage.as<uint32_t>() + (p & 1) * kChunk;
+            uint32_t *out = dst + p * kChunk;
+            const uint64_t m = part(p);
+            for (uint64_t q = 0; q < m; ++q) out[q] = in[q] - shift;
+        }
+    }
+    return IGC_OK;
+}
+
+// Host CSR of the set whose nodes start at off: its rows of the union CSR (the
+// union row_ptr already on the host in rp), ids shifted back to the set; with
+// IGC_UPPER_TRIANGLE each row keeps its neighbours above the row; a thread
+// order in the flags is applied last, as in the single-set call.
+static int copy_set(igc_ctx *ctx, const uint64_t *rp, uint32_t off, uint32_t N, uint32_t flags, igc_csr *out)
 {
     const uint64_t base = rp[off];
     const uint64_t all = rp[off + N] - base;
-    const bool upper = (flags & IGC_UPPER_TRIANGLE) != 0;
-    const uint64_t ents = upper ? all / 2 : all;
     out->n_nodes = N;
     out->location = IGC_ON_HOST;
     out->n_edges = all / 2;
     out->row_ptr = static_cast<uint64_t *>(host_alloc(sizeof(uint64_t) * ((size_t)N + 1)));
-    out->col = static_cast<uint32_t *>(host_alloc(sizeof(uint32_t) * std::max<uint64_t>(ents, 1)));
+    out->col = static_cast<uint32_t *>(host_alloc(sizeof(uint32_t) * std::max<uint64_t>(all, 1)));
     if (!out->row_ptr || !out->col) {
         igc_csr_free(out);
         return set_error(IGC_ERR_OOM, "host CSR allocation failed");
     }
-    if (upper) {
+    for (uint32_t x = 0; x <= N; ++x) out->row_ptr[x] = rp[off + x] - base;
+    const int rc = d2h_col_shift(ctx, out->col, ctx->col.as<uint32_t>() + base, all, off);
+    if (rc) {
+        igc_csr_free(out);
+        return rc;
+    }
+    if (flags & IGC_UPPER_TRIANGLE) { // in place: the write index never passes the read index
         uint64_t w = 0;
         for (uint32_t x = 0; x < N; ++x) {
+            const uint64_t q0 = out->row_ptr[x], q1 = out->row_ptr[x + 1];
             out->row_ptr[x] = w;
-            for (uint64_t q = rp[off + x]; q < rp[off + x + 1]; ++q)
-                if (col[q] - off > x) out->col[w++] = col[q] - off;
+            for (uint64_t q = q0; q < q1; ++q)
+                if (out->col[q] > x) out->col[w++] = out->col[q];
         }
         out->row_ptr[N] = w;
         return IGC_OK;
     }
-    for (uint32_t x = 0; x <= N; ++x) out->row_ptr[x] = rp[off + x] - base;
-    for (uint64_t q = 0; q < ents; ++q) out->col[q] = col[base + q] - off;
     const uint32_t T = (flags >> 16) & 0xFFFFu;
     if (T > 1) reorder_threads(*out, T);
     return IGC_OK;
@@ -470,6 +553,43 @@ IGC_API int igc_graph_build_batch(igc_ctx *ctx, const igc_set *sets, uint32_t co
 
 namespace igc {
 
+// Copies the sets' host symbol arrays, concatenated, to dst through two pinned
+// 8 MiB chunks: one DMA per chunk instead of one pageable copy per set (a
+// batch of a server may hold hundreds of small jobs).  A chunk buffer is
+// refilled only after its previous copy has completed.
+static int h2d_concat_staged(igc_ctx *ctx, uint8_t *dst, const igc_set *sets, uint32_t count, uint64_t total)
+{
+    constexpr size_t kChunk = 8u << 20;
+    if (total == 0) return IGC_OK;
+    int rc = ctx->stage.ensure(2 * kChunk);
+    if (rc) return rc;
+    hipStream_t st = ctx->stream;
+    uint32_t set = 0;
+    uint64_t in_set = 0, done = 0;
+    for (uint32_t c = 0; done < total; ++c) {
+        uint8_t *buf = ctx->stage.as<uint8_t>() + (c & 1) * kChunk;
+        if (c >= 2) HIP_TRY(hipEventSynchronize(ctx->ev[c & 1]));
+        const uint64_t want = std::min<uint64_t>(kChunk, total - done);
+        uint64_t fill = 0;
+        while (fill < want && set < count) {
+            const uint64_t bytes = (uint64_t)sets[set].N * sets[set].n;
+            const uint64_t take = std::min<uint64_t>(bytes - in_set, want - fill);
+            if (take) std::memcpy(buf + fill, sets[set].syms + in_set, take);
+            fill += take;
+            in_set += take;
+            if (in_set == bytes) {
+                ++set;
+                in_set = 0;
+            }
+        }
+        if (fill != want) return set_error(IGC_ERR_INVALID, "set sizes changed during the copy");
+        HIP_TRY(hipMemcpyAsync(dst + done, buf, fill, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipEventRecord(ctx->ev[c & 1], st));
+        done += fill;
+    }
+    return IGC_OK;
+}
+
 int batch_build_host_locked(igc_ctx *ctx, const igc_set *sets, uint32_t count, igc_csr *outs)
 {
     for (uint32_t s = 0; s < count; ++s) std::memset(&outs[s], 0, sizeof(igc_csr));
@@ -478,32 +598,17 @@ int batch_build_host_locked(igc_ctx *ctx, const igc_set *sets, uint32_t count, i
     int rc = plan_batch(ctx, sets, count, P);
     if (rc) return rc;
     if ((rc = ctx->syms.ensure(std::max<size_t>(P.sym_bytes, 1)))) return rc;
-    {
-        uint64_t off = 0;
-        for (uint32_t s = 0; s < count; ++s) {
-            const size_t bytes = (size_t)sets[s].N * sets[s].n;
-            if (bytes)
-                HIP_TRY(hipMemcpyAsync(ctx->syms.as<uint8_t>() + off, sets[s].syms, bytes, hipMemcpyHostToDevice,
-                                       ctx->stream));
-            off += bytes;
-        }
-    }
+    if ((rc = h2d_concat_staged(ctx, ctx->syms.as<uint8_t>(), sets, count, P.sym_bytes))) return rc;
     std::vector<uint64_t> bounds;
     uint64_t E = 0;
     if ((rc = batch_device_locked(ctx, ctx->syms.as<uint8_t>(), P, bounds, &E))) return rc;
-    // union CSR -> pinned staging -> per-set host CSRs
-    const uint64_t ents = 2 * E;
+    // union row_ptr -> pinned host memory; each set's col slice straight into its own host CSR
     if ((rc = ctx->host_rp.ensure(sizeof(uint64_t) * ((size_t)P.total + 1)))) return rc;
-    if ((rc = ctx->host_col.ensure(sizeof(uint32_t) * std::max<uint64_t>(ents, 1)))) return rc;
     HIP_TRY(hipMemcpyAsync(ctx->host_rp.p, ctx->row_ptr.p, sizeof(uint64_t) * ((size_t)P.total + 1),
                            hipMemcpyDeviceToHost, ctx->stream));
-    if (ents)
-        HIP_TRY(hipMemcpyAsync(ctx->host_col.p, ctx->col.p, sizeof(uint32_t) * ents, hipMemcpyDeviceToHost,
-                               ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     for (uint32_t s = 0; s < count; ++s) {
-        rc = split_set(ctx->host_rp.as<uint64_t>(), ctx->host_col.as<uint32_t>(), P.node_off_by_set[s], sets[s].N,
-                       sets[s].flags, &outs[s]);
+        rc = copy_set(ctx, ctx->host_rp.as<uint64_t>(), P.node_off_by_set[s], sets[s].N, sets[s].flags, &outs[s]);
         if (rc) {
             for (uint32_t t = 0; t < s; ++t) igc_csr_free(&outs[t]);
             return rc;
@@ -526,16 +631,26 @@ IGC_API int igc_graph_build_batch_device(igc_ctx *ctx, const igc_set *sets, uint
     BatchPlan P;
     int rc = plan_batch(ctx, sets, count, P);
     if (rc) return rc;
-    if ((rc = ctx->syms.ensure(std::max<size_t>(P.sym_bytes, 1)))) return rc;
+    if ((rc = ctx->syms.ensure(std::max<size_t>(P.sym_bytes, 16)))) return rc;
+    // the sets' symbols concatenated by one gather launch; the table stays
+    // alive until batch_device_locked has synchronised the stream
+    std::vector<SymSrc> src;
     {
         uint64_t off = 0;
         for (uint32_t s = 0; s < count; ++s) {
-            const size_t bytes = (size_t)sets[s].N * sets[s].n;
-            if (bytes)
-                HIP_TRY(hipMemcpyAsync(ctx->syms.as<uint8_t>() + off, sets[s].syms, bytes, hipMemcpyDeviceToDevice,
-                                       ctx->stream));
+            const uint64_t bytes = (uint64_t)sets[s].N * sets[s].n;
+            if (bytes) src.push_back(SymSrc{sets[s].syms, off});
             off += bytes;
         }
+    }
+    if (!src.empty()) {
+        if ((rc = ctx->srcs.ensure(sizeof(SymSrc) * src.size()))) return rc;
+        HIP_TRY(hipMemcpyAsync(ctx->srcs.p, src.data(), sizeof(SymSrc) * src.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+        const uint64_t threads = (P.sym_bytes + 15) / 16;
+        hipLaunchKernelGGL(k_gather_syms, dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, ctx->stream,
+                           ctx->srcs.as<SymSrc>(), (uint32_t)src.size(), P.sym_bytes, ctx->syms.as<uint8_t>());
+        HIP_TRY(hipGetLastError());
     }
     std::vector<uint64_t> bounds;
     uint64_t E = 0;

@@ -146,7 +146,7 @@ struct igc_ctx {
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     std::mutex mu;
-    igc::DevBuf syms, packed, edges, misc, row_ptr, col, bsum, sets, bounds;
+    igc::DevBuf syms, packed, edges, misc, row_ptr, col, bsum, sets, bounds, srcs;
     igc::DevBuf keys[2];         // radix-sort ping-pong buffers (2 entries per edge)
     igc::DevBuf rcount, roffset; // per-tile digit counts and their scan
     igc::DevBuf segmeta, nbase;  // segment table of the 32-bit CSR form; batch node -> first node of its set
@@ -158,7 +158,7 @@ struct igc_ctx {
     igc::DevBuf corder[IGC_MAX_LEN + 1]; // class order along the curve, per string length
     int prefilter = IGC_PREFILTER_AUTO; // exact row skipping (IGC_PREFILTER_*)
     int predicate = IGC_PRED_AUTO;      // forced pair predicate (IGC_PRED_*)
-    igc::HostBuf host_rp, host_col, stage; // pinned: batch CSR copies, staged copies
+    igc::HostBuf host_rp, stage; // pinned: batch row pointers, staged copies
     igc::DevBuf gen_out, gen_cnt, gen_off, gen_rows; // candidate generators: output, per-block counts/offsets, rows
     uint64_t gen_serial = 0;                         // generations so far
     hipEvent_t ev[5] = {};

@@ -658,12 +658,12 @@ IGC_API int igc_ctx_destroy(igc_ctx *ctx)
         (void)hipStreamSynchronize(ctx->stream);
         for (DevBuf *b : {&ctx->syms, &ctx->packed, &ctx->edges, &ctx->misc, &ctx->row_ptr, &ctx->col, &ctx->keys[0],
                           &ctx->keys[1], &ctx->rcount, &ctx->roffset, &ctx->segmeta, &ctx->nbase, &ctx->bsum,
-                          &ctx->sets, &ctx->bounds,
+                          &ctx->sets, &ctx->bounds, &ctx->srcs,
                           &ctx->feat, &ctx->feat_s, &ctx->nkey, &ctx->perm, &ctx->packed_s, &ctx->fitems, &ctx->fbias,
                           &ctx->fcount, &ctx->fsets, &ctx->gen_out, &ctx->gen_cnt, &ctx->gen_off, &ctx->gen_rows})
             b->release();
         for (DevBuf &b : ctx->corder) b.release();
-        for (HostBuf *b : {&ctx->host_rp, &ctx->host_col, &ctx->stage}) b->release();
+        for (HostBuf *b : {&ctx->host_rp, &ctx->stage}) b->release();
         kmeans_release(ctx);
         for (auto &e : ctx->ev)
             if (e) (void)hipEventDestroy(e);

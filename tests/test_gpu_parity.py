@@ -415,6 +415,61 @@ def test_batch_prefilter_sets_match_oracle(gpu_ctx):
         assert np.array_equal(a.row_ptr, b.row_ptr) and np.array_equal(a.col, b.col)
 
 
+def test_batch_device_sets_match_host_batch(gpu_ctx):
+    """Sets in caller device buffers (gathered into one symbol array by a single
+    launch): the union CSR's row slices equal the host-input batch's per-set
+    CSRs; empty sets and byte counts that are not multiples of 16 included."""
+    import ctypes
+    igc.lib()
+    hip = ctypes.CDLL("libamdhip64.so.7")  # the HIP runtime libigc.so is linked against
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    rng = np.random.default_rng(77)
+    sets = []
+    for N, n, ed in [(0, 11, 3), (1, 7, 2), (2, 13, 4), (333, 11, 3), (0, 5, 1), (1000, 13, 4), (17, 3, 1),
+                     (2500, 16, 5), (4097, 10, 3), (5, 1, 1), (700, 24, 6), (300, 40, 6)]:
+        syms = rng.integers(0, 2 if n > 32 else 4, size=(N, n), dtype=np.uint8)
+        if n > 32:  # binary words: near copies by bit flips
+            for r in range(N // 3):
+                syms[r] = syms[int(rng.integers(0, N))]
+                syms[r, int(rng.integers(0, n))] ^= 1
+        elif N:
+            _near_copies(rng, syms, N // 3)
+        sets.append((syms, ed))
+    banner(f"device batch of {len(sets)} sets")
+    bufs = []
+    for syms, _ in sets:
+        p = ctypes.c_void_p()
+        if syms.size:
+            a = np.ascontiguousarray(syms)
+            assert hip.hipMalloc(ctypes.byref(p), a.nbytes) == 0
+            assert hip.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0  # host to device
+        bufs.append(p.value or 0)
+    spec = [(p, s.shape[0], s.shape[1], ed) for p, (s, ed) in zip(bufs, sets)]
+    try:
+        _check_device_batch(gpu_ctx, sets, spec)
+    finally:
+        for p in bufs:
+            if p:
+                hip.hipFree(ctypes.c_void_p(p))
+
+
+def _check_device_batch(gpu_ctx, sets, spec):
+    for _ in range(2):  # the second call reuses the context's buffers
+        d, off, edges = gpu_ctx.graph_build_batch_device(spec)
+        union = gpu_ctx.device_csr_to_host(d)
+        outs = gpu_ctx.graph_build_batch(sets)
+        for k, ((syms, ed), adj) in enumerate(zip(sets, outs)):
+            a, b = int(off[k]), int(off[k + 1])
+            assert b - a == syms.shape[0]
+            rp = union.row_ptr[a:b + 1] - union.row_ptr[a]
+            col = union.col[int(union.row_ptr[a]):int(union.row_ptr[b])] - a
+            assert np.array_equal(rp, adj.row_ptr), (k, syms.shape, ed)
+            assert np.array_equal(col, adj.col), (k, syms.shape, ed)
+            assert int(edges[k]) == adj.n_edges
+
+
 def test_batched_config_sets_alone_and_in_the_mix(gpu_ctx):
     # a scaled-down batched workload: each set's graph is the same alone and in the batch
     sets = W.batched(seed=3, sets=24, scale=0.05)
