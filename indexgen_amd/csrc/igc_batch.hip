@@ -456,49 +456,10 @@ static int batch_device_locked(igc_ctx *ctx, const uint8_t *d_syms, const BatchP
     return set_error(IGC_ERR_DEVICE, "edge buffer kept overflowing");
 }
 
-// Copies count entries of the union col (device) to dst, less shift (the set's
-// first node): a small slice by one direct copy, a large one through the two
-// pinned 8 MiB staging chunks with the subtraction fused into the copy out of
-// each chunk (the next chunk's transfer runs meanwhile).  No pinned buffer of
-// the union's size is needed.
-static int d2h_col_shift(igc_ctx *ctx, uint32_t *dst, const uint32_t *src, uint64_t count, uint32_t shift)
-{
-    constexpr uint64_t kChunk = (8u << 20) / sizeof(uint32_t); // entries per staging chunk
-    hipStream_t st = ctx->stream;
-    if (count == 0) return IGC_OK;
-    if (count <= kChunk) {
-        HIP_TRY(hipMemcpyAsync(dst, src, sizeof(uint32_t) * count, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        for (uint64_t q = 0; q < count; ++q) dst[q] -= shift;
-        return IGC_OK;
-    }
-    int rc = ctx->stage.ensure(2 * kChunk * sizeof(uint32_t));
-    if (rc) return rc;
-    const uint64_t chunks = (count + kChunk - 1) / kChunk;
-    auto part = [&](uint64_t c) { return std::min(kChunk, count - c * kChunk); };
-    for (uint64_t c = 0; c <= chunks; ++c) {
-        if (c < chunks) { // chunk c - 2 used this buffer and was consumed in the previous iteration
-            HIP_TRY(hipMemcpyAsync(ctx->stage.as<uint32_t>() + (c & 1) * kChunk, src + c * kChunk,
-                                   sizeof(uint32_t) * part(c), hipMemcpyDeviceToHost, st));
-            HIP_TRY(hipEventRecord(ctx->ev[c & 1], st));
-        }
-        if (c >= 1) {
-            const uint64_t p = c - 1;
-            HIP_TRY(hipEventSynchronize(ctx->ev[p & 1]));
-            const uint32_t *in = ctx->stage.as<uint32_t>() + (p & 1) * kChunk;
-            uint32_t *out = dst + p * kChunk;
-            const uint64_t m = part(p);
-            for (uint64_t q = 0; q < m; ++q) out[q] = in[q] - shift;
-        }
-    }
-    return IGC_OK;
-}
-
-// Host CSR of the set whose nodes start at off: its rows of the union CSR (the
-// union row_ptr already on the host in rp), ids shifted back to the set; with
-// IGC_UPPER_TRIANGLE each row keeps its neighbours above the row; a thread
-// order in the flags is applied last, as in the single-set call.
-static int copy_set(igc_ctx *ctx, const uint64_t *rp, uint32_t off, uint32_t N, uint32_t flags, igc_csr *out)
+// Host arrays of the set whose nodes start at off (row pointers from the union
+// row_ptr already on the host in rp; the col entries are filled by
+// d2h_split_cols).
+static int alloc_set(const uint64_t *rp, uint32_t off, uint32_t N, igc_csr *out)
 {
     const uint64_t base = rp[off];
     const uint64_t all = rp[off + N] - base;
@@ -512,12 +473,58 @@ static int copy_set(igc_ctx *ctx, const uint64_t *rp, uint32_t off, uint32_t N, 
         return set_error(IGC_ERR_OOM, "host CSR allocation failed");
     }
     for (uint32_t x = 0; x <= N; ++x) out->row_ptr[x] = rp[off + x] - base;
-    const int rc = d2h_col_shift(ctx, out->col, ctx->col.as<uint32_t>() + base, all, off);
-    if (rc) {
-        igc_csr_free(out);
-        return rc;
+    return IGC_OK;
+}
+
+// Streams the union col (total entries) to the sets' host CSRs in 8 MiB chunks
+// through the two pinned staging buffers: one transfer per chunk whatever the
+// number of sets (the next chunk's transfer runs while the host distributes
+// the current one), no pinned buffer of the union's size.  Set slices are
+// contiguous and ascending in caller order; each entry goes to its set less
+// the set's first node.
+static int d2h_split_cols(igc_ctx *ctx, const uint64_t *rp, const uint32_t *node_off, const igc_set *sets,
+                          uint32_t count, uint64_t total, igc_csr *outs)
+{
+    constexpr uint64_t kChunk = (8u << 20) / sizeof(uint32_t); // entries per staging chunk
+    if (total == 0) return IGC_OK;
+    int rc = ctx->stage.ensure(2 * kChunk * sizeof(uint32_t));
+    if (rc) return rc;
+    hipStream_t st = ctx->stream;
+    const uint32_t *src = ctx->col.as<uint32_t>();
+    const uint64_t chunks = (total + kChunk - 1) / kChunk;
+    auto part = [&](uint64_t c) { return std::min(kChunk, total - c * kChunk); };
+    uint32_t first = 0; // first set whose slice may still be incomplete
+    for (uint64_t c = 0; c <= chunks; ++c) {
+        if (c < chunks) { // chunk c - 2 used this buffer and was consumed in the previous iteration
+            HIP_TRY(hipMemcpyAsync(ctx->stage.as<uint32_t>() + (c & 1) * kChunk, src + c * kChunk,
+                                   sizeof(uint32_t) * part(c), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipEventRecord(ctx->ev[c & 1], st));
+        }
+        if (c == 0) continue;
+        const uint64_t p = c - 1, c0 = p * kChunk, c1 = c0 + part(p);
+        HIP_TRY(hipEventSynchronize(ctx->ev[p & 1]));
+        const uint32_t *in = ctx->stage.as<uint32_t>() + (p & 1) * kChunk;
+        while (first < count && rp[node_off[first] + sets[first].N] <= c0) ++first;
+        for (uint32_t t = first; t < count; ++t) {
+            const uint64_t b = rp[node_off[t]], e = rp[node_off[t] + sets[t].N];
+            if (b >= c1) break;
+            const uint64_t lo = std::max(b, c0), hi = std::min(e, c1);
+            uint32_t *out = outs[t].col + (lo - b);
+            const uint32_t *from = in + (lo - c0);
+            const uint32_t shift = node_off[t];
+            for (uint64_t q = 0; q < hi - lo; ++q) out[q] = from[q] - shift;
+        }
     }
-    if (flags & IGC_UPPER_TRIANGLE) { // in place: the write index never passes the read index
+    return IGC_OK;
+}
+
+// Applies the set's flags to its host CSR: IGC_UPPER_TRIANGLE keeps each row's
+// neighbours above the row (in place: the write index never passes the read
+// index); a thread order is applied as in the single-set call.
+static void finish_set(uint32_t flags, igc_csr *out)
+{
+    const uint32_t N = out->n_nodes;
+    if (flags & IGC_UPPER_TRIANGLE) {
         uint64_t w = 0;
         for (uint32_t x = 0; x < N; ++x) {
             const uint64_t q0 = out->row_ptr[x], q1 = out->row_ptr[x + 1];
@@ -526,11 +533,10 @@ static int copy_set(igc_ctx *ctx, const uint64_t *rp, uint32_t off, uint32_t N, 
                 if (out->col[q] > x) out->col[w++] = out->col[q];
         }
         out->row_ptr[N] = w;
-        return IGC_OK;
+        return;
     }
     const uint32_t T = (flags >> 16) & 0xFFFFu;
     if (T > 1) reorder_threads(*out, T);
-    return IGC_OK;
 }
 
 } // namespace igc
@@ -607,13 +613,18 @@ int batch_build_host_locked(igc_ctx *ctx, const igc_set *sets, uint32_t count, i
     HIP_TRY(hipMemcpyAsync(ctx->host_rp.p, ctx->row_ptr.p, sizeof(uint64_t) * ((size_t)P.total + 1),
                            hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const uint64_t *rp = ctx->host_rp.as<uint64_t>();
     for (uint32_t s = 0; s < count; ++s) {
-        rc = copy_set(ctx, ctx->host_rp.as<uint64_t>(), P.node_off_by_set[s], sets[s].N, sets[s].flags, &outs[s]);
-        if (rc) {
+        if ((rc = alloc_set(rp, P.node_off_by_set[s], sets[s].N, &outs[s]))) {
             for (uint32_t t = 0; t < s; ++t) igc_csr_free(&outs[t]);
             return rc;
         }
     }
+    if ((rc = d2h_split_cols(ctx, rp, P.node_off_by_set.data(), sets, count, rp[P.total], outs))) {
+        for (uint32_t t = 0; t < count; ++t) igc_csr_free(&outs[t]);
+        return rc;
+    }
+    for (uint32_t s = 0; s < count; ++s) finish_set(sets[s].flags, &outs[s]);
     return IGC_OK;
 }
 
