@@ -334,6 +334,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the uneven-sets comparison block")
     ap.add_argument("--no-autotune", action="store_true", help="use the default tile shape")
     ap.add_argument("--serving-jobs", type=int, default=200, help="jobs of the serving stream to replay (0: skip)")
+    ap.add_argument("--saturated-jobs", type=int, default=200,
+                    help="jobs of the stream submitted at once for the server-capacity block (0: skip)")
     ap.add_argument("--no-refine", action="store_true", help="skip the cluster-refinement block")
     ap.add_argument("--valu-probe", action="store_true", help="only measure the int32 VALU issue rates")
     ap.add_argument("--generators", action="store_true",
@@ -494,7 +496,8 @@ def main():
         result["step_latency_ms"] = result["latency_ms"]
         result["latency_ms"] = {k: result["serving"]["latency_ms"][k] for k in ("p50", "p99")}
         result["latency_source"] = "serving stream job latency (arrival to codebook); step_latency_ms is one build"
-        result["serving_saturated"] = saturated_block(args.serving_jobs, host_threads=host_threads())
+    if args.saturated_jobs > 0:
+        result["serving_saturated"] = saturated_block(args.saturated_jobs, host_threads=host_threads())
     if not args.no_batched:
         result["batched"] = batched_block(ctx, torch)
         result["many_small_sets"] = batched_block(ctx, torch, steps=5, seed=4, sets_n=256, scale=0.08,

@@ -476,46 +476,32 @@ static int alloc_set(const uint64_t *rp, uint32_t off, uint32_t N, igc_csr *out)
     return IGC_OK;
 }
 
-// Streams the union col (total entries) to the sets' host CSRs in 8 MiB chunks
-// through the two pinned staging buffers: one transfer per chunk whatever the
-// number of sets (the next chunk's transfer runs while the host distributes
-// the current one), no pinned buffer of the union's size.  Set slices are
-// contiguous and ascending in caller order; each entry goes to its set less
-// the set's first node.
+// Streams the union col (total entries) to the sets' host CSRs through the
+// pinned staging ring (d2h_stream: one transfer per chunk whatever the number
+// of sets, chunks distributed on worker threads), no pinned buffer of the
+// union's size.  Set slices are contiguous and ascending in caller order; each
+// entry goes to its set less the set's first node.
 static int d2h_split_cols(igc_ctx *ctx, const uint64_t *rp, const uint32_t *node_off, const igc_set *sets,
                           uint32_t count, uint64_t total, igc_csr *outs)
 {
-    constexpr uint64_t kChunk = (8u << 20) / sizeof(uint32_t); // entries per staging chunk
     if (total == 0) return IGC_OK;
-    int rc = ctx->stage.ensure(2 * kChunk * sizeof(uint32_t));
-    if (rc) return rc;
-    hipStream_t st = ctx->stream;
-    const uint32_t *src = ctx->col.as<uint32_t>();
-    const uint64_t chunks = (total + kChunk - 1) / kChunk;
-    auto part = [&](uint64_t c) { return std::min(kChunk, total - c * kChunk); };
-    uint32_t first = 0; // first set whose slice may still be incomplete
-    for (uint64_t c = 0; c <= chunks; ++c) {
-        if (c < chunks) { // chunk c - 2 used this buffer and was consumed in the previous iteration
-            HIP_TRY(hipMemcpyAsync(ctx->stage.as<uint32_t>() + (c & 1) * kChunk, src + c * kChunk,
-                                   sizeof(uint32_t) * part(c), hipMemcpyDeviceToHost, st));
-            HIP_TRY(hipEventRecord(ctx->ev[c & 1], st));
-        }
-        if (c == 0) continue;
-        const uint64_t p = c - 1, c0 = p * kChunk, c1 = c0 + part(p);
-        HIP_TRY(hipEventSynchronize(ctx->ev[p & 1]));
-        const uint32_t *in = ctx->stage.as<uint32_t>() + (p & 1) * kChunk;
-        while (first < count && rp[node_off[first] + sets[first].N] <= c0) ++first;
-        for (uint32_t t = first; t < count; ++t) {
-            const uint64_t b = rp[node_off[t]], e = rp[node_off[t] + sets[t].N];
-            if (b >= c1) break;
-            const uint64_t lo = std::max(b, c0), hi = std::min(e, c1);
-            uint32_t *out = outs[t].col + (lo - b);
-            const uint32_t *from = in + (lo - c0);
-            const uint32_t shift = node_off[t];
-            for (uint64_t q = 0; q < hi - lo; ++q) out[q] = from[q] - shift;
-        }
-    }
-    return IGC_OK;
+    std::vector<uint64_t> ends(count); // end of each set's slice (ascending)
+    for (uint32_t t = 0; t < count; ++t) ends[t] = rp[node_off[t] + sets[t].N];
+    return d2h_stream(ctx, ctx->col.p, sizeof(uint32_t) * total,
+                      [&](const uint8_t *chunk, size_t off, size_t bytes) {
+                          const uint64_t c0 = off / sizeof(uint32_t), c1 = c0 + bytes / sizeof(uint32_t);
+                          const uint32_t *in = reinterpret_cast<const uint32_t *>(chunk);
+                          uint32_t t = (uint32_t)(std::upper_bound(ends.begin(), ends.end(), c0) - ends.begin());
+                          for (; t < count; ++t) {
+                              const uint64_t b = rp[node_off[t]], e = ends[t];
+                              if (b >= c1) break;
+                              const uint64_t lo = std::max(b, c0), hi = std::min(e, c1);
+                              uint32_t *out = outs[t].col + (lo - b);
+                              const uint32_t *from = in + (lo - c0);
+                              const uint32_t shift = node_off[t];
+                              for (uint64_t q = 0; q < hi - lo; ++q) out[q] = from[q] - shift;
+                          }
+                      });
 }
 
 // Applies the set's flags to its host CSR: IGC_UPPER_TRIANGLE keeps each row's

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <mutex>
 
 #define HIP_TRY(expr)                                                                                         \
@@ -162,6 +163,7 @@ struct igc_ctx {
     igc::DevBuf gen_out, gen_cnt, gen_off, gen_rows; // candidate generators: output, per-block counts/offsets, rows
     uint64_t gen_serial = 0;                         // generations so far
     hipEvent_t ev[5] = {};
+    std::vector<hipEvent_t> cev; // staging-ring events of d2h_stream
     igc_timing last{};
     int cu_count = 256;
     // tile shapes chosen by the autotuner, keyed by (kernel variant, N bucket);
@@ -185,6 +187,14 @@ void kmeans_release(igc_ctx *ctx);
 int kmeans_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n, uint32_t k, uint32_t iters,
                          uint64_t seed, bool use_graph, float *ms_out);
 int32_t *kmeans_assign_ptr(igc_ctx *ctx);
+
+// Device-to-host streaming through a ring of pinned staging chunks consumed on
+// worker threads (igc_runtime.hip); the caller holds the context lock and the
+// source is complete (the context's stream has reached it).
+constexpr size_t kD2HChunk = 4u << 20;
+constexpr uint32_t kD2HThreads = 8;
+using D2HConsumer = std::function<void(const uint8_t *chunk, size_t offset, size_t bytes)>;
+int d2h_stream(igc_ctx *ctx, const void *src, size_t bytes, const D2HConsumer &consume);
 
 // igc_graph_build_batch without taking the context lock (caller holds it)
 int batch_build_host_locked(igc_ctx *ctx, const igc_set *sets, uint32_t count, igc_csr *outs);

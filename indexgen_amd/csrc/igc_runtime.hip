@@ -15,7 +15,10 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
+#include <atomic>
+#include <memory>
 
 namespace igc {
 
@@ -596,6 +599,81 @@ static int build_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, 
     return set_error(IGC_ERR_DEVICE, "edge buffer kept overflowing");
 }
 
+// Streams bytes of device memory at src to the host through a ring of pinned
+// staging chunks.  The calling thread enqueues the chunk transfers on the
+// context's stream (a ring slot is refilled once its previous chunk has been
+// consumed); T worker threads take chunks round-robin, wait for their
+// transfer and hand each to consume(chunk, byte offset, size) -- the host-side
+// work (moving into pageable memory, which also faults its pages in, or
+// splitting into per-set arrays) runs on T cores while the copy engine fills
+// the next slots.  Every chunk is consumed exactly once; chunk boundaries are
+// multiples of kD2HChunk.
+int d2h_stream(igc_ctx *ctx, const void *src, size_t bytes, const D2HConsumer &consume)
+{
+    if (bytes == 0) return IGC_OK;
+    const size_t chunks = (bytes + kD2HChunk - 1) / kD2HChunk;
+    const uint32_t T = (uint32_t)std::min<size_t>(chunks, kD2HThreads);
+    const uint32_t R = 2 * T;
+    int rc = ctx->stage.ensure((size_t)R * kD2HChunk);
+    if (rc) return rc;
+    while (ctx->cev.size() < R) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        ctx->cev.push_back(e);
+    }
+    const int dev = ctx->device;
+    hipStream_t st = ctx->stream;
+    uint8_t *ring = ctx->stage.as<uint8_t>();
+    auto size_of = [&](size_t c) { return std::min(kD2HChunk, bytes - c * kD2HChunk); };
+    std::unique_ptr<std::atomic<uint8_t>[]> done(new std::atomic<uint8_t>[chunks]);
+    for (size_t c = 0; c < chunks; ++c) done[c].store(0, std::memory_order_relaxed);
+    std::atomic<size_t> enqueued{0};
+    std::atomic<int> failed{IGC_OK};
+    if (T == 1) { // one chunk: no worker thread
+        HIP_TRY(hipMemcpyAsync(ring, src, bytes, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        consume(ring, 0, bytes);
+        return IGC_OK;
+    }
+    std::vector<std::thread> workers;
+    workers.reserve(T);
+    for (uint32_t w = 0; w < T; ++w)
+        workers.emplace_back([&, w] {
+            (void)hipSetDevice(dev);
+            for (size_t c = w; c < chunks; c += T) {
+                while (enqueued.load(std::memory_order_acquire) <= c) {
+                    if (failed.load(std::memory_order_relaxed) != IGC_OK) return;
+                    std::this_thread::yield();
+                }
+                if (hipEventSynchronize(ctx->cev[c % R]) != hipSuccess) {
+                    failed.store(IGC_ERR_DEVICE);
+                    return;
+                }
+                consume(ring + (c % R) * kD2HChunk, c * kD2HChunk, size_of(c));
+                done[c].store(1, std::memory_order_release);
+            }
+        });
+    for (size_t c = 0; c < chunks && failed.load() == IGC_OK; ++c) {
+        if (c >= R) // slot c % R held chunk c - R, a chunk of the same worker
+            while (!done[c - R].load(std::memory_order_acquire) && failed.load() == IGC_OK) std::this_thread::yield();
+        if (failed.load() != IGC_OK) break;
+        const hipError_t e1 = hipMemcpyAsync(ring + (c % R) * kD2HChunk, static_cast<const uint8_t *>(src) + c * kD2HChunk,
+                                             size_of(c), hipMemcpyDeviceToHost, st);
+        const hipError_t e2 = e1 == hipSuccess ? hipEventRecord(ctx->cev[c % R], st) : e1;
+        if (e2 != hipSuccess) {
+            failed.store(set_error(IGC_ERR_DEVICE, "staged device-to-host copy failed: %s", hipGetErrorString(e2)));
+            break;
+        }
+        enqueued.store(c + 1, std::memory_order_release);
+    }
+    for (auto &t : workers) t.join();
+    if (failed.load() != IGC_OK) {
+        (void)hipStreamSynchronize(st);
+        return set_error(IGC_ERR_DEVICE, "staged device-to-host copy failed");
+    }
+    return IGC_OK;
+}
+
 } // namespace igc
 
 // ---------------------------------------------------------------------------
@@ -667,6 +745,7 @@ IGC_API int igc_ctx_destroy(igc_ctx *ctx)
         kmeans_release(ctx);
         for (auto &e : ctx->ev)
             if (e) (void)hipEventDestroy(e);
+        for (auto &e : ctx->cev) (void)hipEventDestroy(e);
         if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     }
     delete ctx;
@@ -754,35 +833,12 @@ IGC_API int igc_csr_free(igc_csr *csr)
     return IGC_OK;
 }
 
-// Device-to-host copy into pageable memory through two pinned staging chunks:
-// the copy engine fills one chunk while the host moves the other out, so the
-// transfer runs at PCIe speed rather than at the pageable-copy rate.
+// Device-to-host copy into pageable memory (d2h_stream with a plain move).
 static int d2h_staged(igc_ctx *ctx, void *dst, const void *src, size_t bytes)
 {
-    constexpr size_t kChunk = 8u << 20;
-    hipStream_t s = ctx->stream;
-    if (bytes <= kChunk) {
-        HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        return IGC_OK;
-    }
-    int rc = ctx->stage.ensure(2 * kChunk);
-    if (rc) return rc;
-    const size_t chunks = (bytes + kChunk - 1) / kChunk;
-    auto part = [&](size_t c) { return std::min(kChunk, bytes - c * kChunk); };
-    for (size_t c = 0; c <= chunks; ++c) {
-        if (c < chunks) {
-            HIP_TRY(hipMemcpyAsync(ctx->stage.as<uint8_t>() + (c & 1) * kChunk, static_cast<const uint8_t *>(src) + c * kChunk,
-                                   part(c), hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipEventRecord(ctx->ev[c & 1], s));
-        }
-        if (c >= 1) {
-            const size_t p = c - 1;
-            HIP_TRY(hipEventSynchronize(ctx->ev[p & 1]));
-            std::memcpy(static_cast<uint8_t *>(dst) + p * kChunk, ctx->stage.as<uint8_t>() + (p & 1) * kChunk, part(p));
-        }
-    }
-    return IGC_OK;
+    uint8_t *out = static_cast<uint8_t *>(dst);
+    return igc::d2h_stream(ctx, src, bytes,
+                      [out](const uint8_t *chunk, size_t off, size_t n) { std::memcpy(out + off, chunk, n); });
 }
 
 // Copies a device CSR of E edges (2E entries, or E for an upper-triangle CSR)
