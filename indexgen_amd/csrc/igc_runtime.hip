@@ -15,6 +15,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 #include <atomic>
@@ -637,22 +638,27 @@ int d2h_stream(igc_ctx *ctx, const void *src, size_t bytes, const D2HConsumer &c
     }
     std::vector<std::thread> workers;
     workers.reserve(T);
-    for (uint32_t w = 0; w < T; ++w)
-        workers.emplace_back([&, w] {
-            (void)hipSetDevice(dev);
-            for (size_t c = w; c < chunks; c += T) {
-                while (enqueued.load(std::memory_order_acquire) <= c) {
-                    if (failed.load(std::memory_order_relaxed) != IGC_OK) return;
-                    std::this_thread::yield();
+    for (uint32_t w = 0; w < T && failed.load() == IGC_OK; ++w) {
+        try {
+            workers.emplace_back([&, w] {
+                (void)hipSetDevice(dev);
+                for (size_t c = w; c < chunks; c += T) {
+                    while (enqueued.load(std::memory_order_acquire) <= c) {
+                        if (failed.load(std::memory_order_relaxed) != IGC_OK) return;
+                        std::this_thread::yield();
+                    }
+                    if (hipEventSynchronize(ctx->cev[c % R]) != hipSuccess) {
+                        failed.store(IGC_ERR_DEVICE);
+                        return;
+                    }
+                    consume(ring + (c % R) * kD2HChunk, c * kD2HChunk, size_of(c));
+                    done[c].store(1, std::memory_order_release);
                 }
-                if (hipEventSynchronize(ctx->cev[c % R]) != hipSuccess) {
-                    failed.store(IGC_ERR_DEVICE);
-                    return;
-                }
-                consume(ring + (c % R) * kD2HChunk, c * kD2HChunk, size_of(c));
-                done[c].store(1, std::memory_order_release);
-            }
-        });
+            });
+        } catch (const std::system_error &) { // no thread: the started workers see the flag and stop
+            failed.store(set_error(IGC_ERR_OOM, "staged device-to-host copy: no worker thread"));
+        }
+    }
     for (size_t c = 0; c < chunks && failed.load() == IGC_OK; ++c) {
         if (c >= R) // slot c % R held chunk c - R, a chunk of the same worker
             while (!done[c - R].load(std::memory_order_acquire) && failed.load() == IGC_OK) std::this_thread::yield();
@@ -669,7 +675,8 @@ int d2h_stream(igc_ctx *ctx, const void *src, size_t bytes, const D2HConsumer &c
     for (auto &t : workers) t.join();
     if (failed.load() != IGC_OK) {
         (void)hipStreamSynchronize(st);
-        return set_error(IGC_ERR_DEVICE, "staged device-to-host copy failed");
+        return failed.load() == IGC_ERR_OOM ? IGC_ERR_OOM
+                                            : set_error(IGC_ERR_DEVICE, "staged device-to-host copy failed");
     }
     return IGC_OK;
 }
