@@ -470,6 +470,47 @@ def _check_device_batch(gpu_ctx, sets, spec):
             assert int(edges[k]) == adj.n_edges
 
 
+def _upper_of(rp, col):
+    """Upper-triangle CSR (each row keeps its neighbours above the row) of a
+    symmetric one."""
+    N = rp.size - 1
+    rows = np.repeat(np.arange(N, dtype=np.int64), np.diff(rp.astype(np.int64)))
+    keep = col.astype(np.int64) > rows
+    urp = np.zeros(N + 1, dtype=np.uint64)
+    urp[1:] = np.cumsum(np.bincount(rows[keep], minlength=N))
+    return urp, col[keep]
+
+
+def test_multi_chunk_host_copies(gpu_ctx):
+    """Host CSRs spanning several 4 MiB staging chunks (moved by worker threads,
+    and in a batch split among the sets a chunk holds): single-set and batched
+    results equal the oracle, with the thread order and the upper triangle
+    applied after the streamed copy."""
+    rng = np.random.default_rng(5)
+    big = rng.integers(0, 4, size=(16000, 11), dtype=np.uint8)  # ~4.4e6 entries, ~17 MiB of col
+    banner("multi-chunk host copies: N=16000 n=11 ed=5")
+    rp, col = O.graph_csr(big, 5)
+    assert col.size * 4 > 3 * (4 << 20)
+    adj = gpu_ctx.graph_build(big, 5)
+    assert np.array_equal(adj.row_ptr, rp) and np.array_equal(adj.col, col)
+    small = [rng.integers(0, 4, size=(N, n), dtype=np.uint8) for N, n in ((900, 10), (1, 12), (2500, 13))]
+    sets = [(small[0], 3), (big, 5), (small[1], 2), (small[2], 4), (big[:9000].copy(), 5)]
+    singles = [gpu_ctx.graph_build(s, ed) for s, ed in sets]
+    outs = gpu_ctx.graph_build_batch(sets)
+    for k, (a, b) in enumerate(zip(outs, singles)):
+        assert np.array_equal(a.row_ptr, b.row_ptr) and np.array_equal(a.col, b.col), k
+    assert np.array_equal(outs[1].col, col)
+    rp4, col4 = O.graph_csr(big, 5, T=4)
+    outs4 = gpu_ctx.graph_build_batch(sets, flags=igc.order_threads(4))
+    assert np.array_equal(outs4[1].row_ptr, rp4) and np.array_equal(outs4[1].col, col4)
+    urp, ucol = _upper_of(rp, col)
+    up = gpu_ctx.graph_build_batch(sets, flags=igc.IGC_UPPER_TRIANGLE)
+    assert np.array_equal(up[1].row_ptr, urp) and np.array_equal(up[1].col, ucol)
+    for (s, _), o, full in zip(sets, up, singles):
+        u_rp, u_col = _upper_of(full.row_ptr, full.col)
+        assert np.array_equal(o.row_ptr, u_rp) and np.array_equal(o.col, u_col)
+
+
 def test_batched_config_sets_alone_and_in_the_mix(gpu_ctx):
     # a scaled-down batched workload: each set's graph is the same alone and in the batch
     sets = W.batched(seed=3, sets=24, scale=0.05)
