@@ -219,7 +219,10 @@ IGC_API int igc_csr_to_host(igc_ctx *ctx, const igc_csr *dev, uint32_t flags, ig
  * src/SparseMat.cpp:599-609).  IGC_ORDER_ASCENDING leaves it unchanged. */
 IGC_API int igc_csr_reorder(igc_csr *csr, uint32_t flags);
 
-/* Releases a host CSR's arrays (no-op for device CSRs owned by a context). */
+/* Releases a host CSR's arrays (no-op for device CSRs owned by a context).
+ * Arrays of 4 MiB and more go back to a process-wide cache of host blocks whose
+ * pages are already faulted in (at most min(16 GiB, physical memory / 8)), from
+ * which later host CSRs are allocated. */
 IGC_API int igc_csr_free(igc_csr *csr);
 
 /* ---- many candidate sets in one launch ---- */

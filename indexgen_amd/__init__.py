@@ -341,13 +341,34 @@ class AdjList:
         return c
 
 
+class _CsrOwner:
+    """Holds a library-allocated host CSR; releases it (igc_csr_free) when the
+    last numpy view of its arrays is gone."""
+
+    def __init__(self, c: _Csr):
+        self.c = c
+
+    def __del__(self):
+        if _lib is not None:
+            _lib.igc_csr_free(ctypes.byref(self.c))
+
+
+def _view(ptr, ctype, count: int, owner: _CsrOwner, dtype) -> np.ndarray:
+    addr = ctypes.cast(ptr, ctypes.c_void_p).value
+    if not addr:
+        return np.zeros(count, dtype=dtype)
+    buf = (ctype * max(count, 1)).from_address(addr)
+    buf._owner = owner  # the arrays keep the CSR alive; no copy of the library's arrays
+    return np.frombuffer(buf, dtype=dtype)[:count]
+
+
 def _adj_from_c(c: _Csr) -> AdjList:
     N = int(c.n_nodes)
-    rp = np.ctypeslib.as_array(c.row_ptr, shape=(N + 1,)).copy()
+    owner = _CsrOwner(c)
+    rp = _view(c.row_ptr, ctypes.c_uint64, N + 1, owner, np.uint64)
     ents = int(rp[N])
-    col = np.ctypeslib.as_array(c.col, shape=(max(ents, 1),))[:ents].copy()
+    col = _view(c.col, ctypes.c_uint32, ents, owner, np.uint32)
     upper = ents != 2 * int(c.n_edges)
-    lib().igc_csr_free(ctypes.byref(c))
     return AdjList(rp, col, upper)
 
 

@@ -16,10 +16,12 @@ const char *last_error();
 // within row x, neighbour y is keyed by (min(x,y) mod T, min(x,y), max(x,y)).
 void reorder_threads(igc_csr &csr, uint32_t T);
 
-// Host memory for arrays handed to the caller (released with std::free by
-// igc_csr_free).  Blocks of 4 MiB and more are 2 MiB aligned and advised for
-// transparent huge pages, so filling them takes a few page faults instead of
-// one per 4 KiB page.
+// Host memory for arrays handed to the caller (released with host_free by
+// igc_csr_free).  Blocks of 4 MiB and more are 2 MiB aligned, advised for
+// transparent huge pages, and recycled through a bounded cache of blocks whose
+// pages are already faulted in; host_cache_trim() frees the cached blocks.
 void *host_alloc(size_t bytes);
+void host_free(void *p);
+void host_cache_trim();
 
 } // namespace igc

@@ -831,8 +831,8 @@ IGC_API int igc_csr_free(igc_csr *csr)
 {
     if (!csr) return IGC_OK;
     if (csr->location == IGC_ON_HOST) {
-        std::free(csr->row_ptr);
-        std::free(csr->col);
+        host_free(csr->row_ptr);
+        host_free(csr->col);
     }
     csr->row_ptr = nullptr;
     csr->col = nullptr;
