@@ -349,8 +349,11 @@ class _CsrOwner:
         self.c = c
 
     def __del__(self):
-        if _lib is not None:
-            _lib.igc_csr_free(ctypes.byref(self.c))
+        try:
+            if _lib is not None:
+                _lib.igc_csr_free(ctypes.byref(self.c))
+        except Exception:  # interpreter shutdown: module globals may already be gone
+            pass
 
 
 def _view(ptr, ctype, count: int, owner: _CsrOwner, dtype) -> np.ndarray:
