@@ -150,6 +150,7 @@ class Greedy {
         uint32_t maxd = 0;
         try {
             st_.resize(N_);
+            dead_.assign(((size_t)N_ + 63) / 64, 0ull);
             for (uint32_t v = 0; v < N_; ++v) {
                 st_[v].deg = (uint32_t)(rp_[v + 1] - rp_[v]);
                 maxd = std::max(maxd, st_[v].deg);
@@ -194,7 +195,7 @@ class Greedy {
             for (uint64_t q = v0; q < v1; ++q) {
                 const uint32_t u = col_[q];
                 ball_[nb] = u;
-                nb += st_[u].deg != kDead ? 1u : 0u;
+                nb += 1u - dead(u);
             }
             ball_[nb++] = v;
             for (uint32_t bi = 0; bi < nb; ++bi) {
@@ -202,12 +203,16 @@ class Greedy {
                 const uint64_t x0 = rp_[x], x1 = rp_[x + 1];
                 // x's neighbours that move (active, degree > 0), compacted the same
                 // way; a move changes only the moved node's degree, so every test
-                // is already fixed before the first move
+                // is already fixed before the first move.  An active neighbour of
+                // x has degree >= 1 while x is active (a node's degree is its
+                // number of active neighbours), so "active" alone decides, read
+                // from the bitmap of deleted nodes (N/8 bytes, cache-resident)
+                // instead of the node records
                 uint32_t m = 0;
                 for (uint64_t q = x0; q < x1; ++q) {
                     const uint32_t u = col_[q];
                     live_[m] = u;
-                    m += st_[u].deg - 1u < kDead - 1u ? 1u : 0u;
+                    m += 1u - dead(u);
                 }
                 for (uint32_t e = 0; e < m; ++e) {
                     // node records kAhead moves ahead, their bucket slots half as far
@@ -225,6 +230,7 @@ class Greedy {
                 }
                 remove(st_[x].deg, x);
                 st_[x].deg = kDead;
+                dead_[x >> 6] |= 1ull << (x & 63);
                 --active;
             }
         }
@@ -267,6 +273,8 @@ class Greedy {
     const uint32_t *col_;
     uint32_t nb_ = 0, tracker_ = 0;
     std::vector<Node> st_;
+    std::vector<uint64_t> dead_; // one bit per deleted node: the neighbour scans test only this
+    uint32_t dead(uint32_t u) const { return (uint32_t)(dead_[u >> 6] >> (u & 63)) & 1u; }
     std::vector<Bucket> bk_;
     std::unique_ptr<uint32_t[]> store_;
 };
