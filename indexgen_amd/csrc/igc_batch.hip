@@ -325,21 +325,55 @@ static int launch_sets_w(const BatchGroup &g, hipStream_t s, const uint8_t *pack
     }
 }
 
-// The group's tiles in launches of at most 2^31 - 1 workgroups.
-static int launch_group(igc_ctx *ctx, const BatchGroup &g, const FilterBase &fb, const SetDesc *d_group,
-                        uint64_t *edges, uint64_t cap, unsigned long long *counter, uint32_t *launches)
+// The group's tiles in launches of at most 2^31 - 1 workgroups, on stream st.
+static int launch_group(igc_ctx *ctx, hipStream_t st, const BatchGroup &g, const FilterBase &fb,
+                        const SetDesc *d_group, uint64_t *edges, uint64_t cap, unsigned long long *counter,
+                        uint32_t *launches)
 {
     constexpr uint64_t kMaxGrid = 0x7FFFFFFFull;
     for (uint64_t t0 = 0; t0 < g.tiles; t0 += kMaxGrid) {
         const uint32_t nt = (uint32_t)std::min<uint64_t>(g.tiles - t0, kMaxGrid);
         const uint8_t *pk = ctx->packed.as<uint8_t>(), *pks = ctx->packed_s.as<uint8_t>();
         const int rc =
-            g.wc == 2   ? launch_sets_w<uint64_t, true>(g, ctx->stream, pk, pks, fb, d_group, t0, nt, edges, cap, counter)
-            : g.wc == 1 ? launch_sets_w<uint64_t>(g, ctx->stream, pk, pks, fb, d_group, t0, nt, edges, cap, counter)
-                        : launch_sets_w<uint32_t>(g, ctx->stream, pk, pks, fb, d_group, t0, nt, edges, cap, counter);
+            g.wc == 2   ? launch_sets_w<uint64_t, true>(g, st, pk, pks, fb, d_group, t0, nt, edges, cap, counter)
+            : g.wc == 1 ? launch_sets_w<uint64_t>(g, st, pk, pks, fb, d_group, t0, nt, edges, cap, counter)
+                        : launch_sets_w<uint32_t>(g, st, pk, pks, fb, d_group, t0, nt, edges, cap, counter);
         if (rc) return rc;
         HIP_TRY(hipGetLastError());
         ++*launches;
+    }
+    return IGC_OK;
+}
+
+// All groups' launches.  With several groups, group g > 0 runs on auxiliary
+// stream (g - 1) mod kAuxStreams (default priority), forked from and joined
+// back to the context's stream by events, so one group's tail of workgroups
+// overlaps the next group instead of leaving CUs idle between launches.  All
+// launches append to the same edge list through its atomic counter.
+static int launch_groups(igc_ctx *ctx, const BatchPlan &P, const FilterBase &fb, const SetDesc *d_groups,
+                         uint64_t cap, unsigned long long *counter, uint32_t *launches)
+{
+    hipStream_t s = ctx->stream;
+    uint64_t *edges = ctx->edges.as<uint64_t>();
+    const size_t ng = P.groups.size();
+    const uint32_t naux = (uint32_t)std::min<size_t>(ng > 0 ? ng - 1 : 0, igc_ctx::kAuxStreams);
+    int rc;
+    for (uint32_t a = 0; a < naux; ++a) {
+        if (!ctx->aux[a]) HIP_TRY(hipStreamCreateWithFlags(&ctx->aux[a], hipStreamNonBlocking));
+        if (!ctx->aux_ev[a]) HIP_TRY(hipEventCreateWithFlags(&ctx->aux_ev[a], hipEventDisableTiming));
+    }
+    if (naux) {
+        HIP_TRY(hipEventRecord(ctx->aux_ev[0], s)); // fork point: pack and filter set-up done
+        for (uint32_t a = 0; a < naux; ++a) HIP_TRY(hipStreamWaitEvent(ctx->aux[a], ctx->aux_ev[0], 0));
+    }
+    for (size_t gi = 0; gi < ng; ++gi) {
+        const BatchGroup &g = P.groups[gi];
+        hipStream_t st = gi == 0 ? s : ctx->aux[(gi - 1) % igc_ctx::kAuxStreams];
+        if ((rc = launch_group(ctx, st, g, fb, d_groups + g.first, edges, cap, counter, launches))) return rc;
+    }
+    for (uint32_t a = 0; a < naux; ++a) { // join
+        HIP_TRY(hipEventRecord(ctx->aux_ev[a], ctx->aux[a]));
+        HIP_TRY(hipStreamWaitEvent(s, ctx->aux_ev[a], 0));
     }
     return IGC_OK;
 }
@@ -419,9 +453,7 @@ static int batch_device_locked(igc_ctx *ctx, const uint8_t *d_syms, const BatchP
             launches += 8;
         }
         HIP_TRY(hipEventRecord(ctx->ev[1], s));
-        for (const BatchGroup &g : P.groups)
-            if ((rc = launch_group(ctx, g, fb, d_groups + g.first, ctx->edges.as<uint64_t>(), cap, counter, &launches)))
-                return rc;
+        if ((rc = launch_groups(ctx, P, fb, d_groups, cap, counter, &launches))) return rc;
         HIP_TRY(hipEventRecord(ctx->ev[2], s));
         uint32_t host_misc[kMiscWords];
         HIP_TRY(hipMemcpyAsync(host_misc, misc, sizeof host_misc, hipMemcpyDeviceToHost, s));

@@ -164,6 +164,11 @@ struct igc_ctx {
     uint64_t gen_serial = 0;                         // generations so far
     hipEvent_t ev[5] = {};
     std::vector<hipEvent_t> cev; // staging-ring events of d2h_stream
+    // auxiliary streams (default priority) and their fork/join events: a batch's
+    // predicate groups run concurrently (igc_batch.hip)
+    static constexpr uint32_t kAuxStreams = 3;
+    hipStream_t aux[kAuxStreams] = {};
+    hipEvent_t aux_ev[kAuxStreams] = {};
     igc_timing last{};
     int cu_count = 256;
     // tile shapes chosen by the autotuner, keyed by (kernel variant, N bucket);

@@ -753,6 +753,11 @@ IGC_API int igc_ctx_destroy(igc_ctx *ctx)
         for (auto &e : ctx->ev)
             if (e) (void)hipEventDestroy(e);
         for (auto &e : ctx->cev) (void)hipEventDestroy(e);
+        for (uint32_t a = 0; a < igc_ctx::kAuxStreams; ++a) {
+            if (ctx->aux[a]) (void)hipStreamSynchronize(ctx->aux[a]);
+            if (ctx->aux_ev[a]) (void)hipEventDestroy(ctx->aux_ev[a]);
+            if (ctx->aux[a]) (void)hipStreamDestroy(ctx->aux[a]);
+        }
         if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     }
     delete ctx;
