@@ -670,6 +670,43 @@ def test_refine_matches_oracle(gpu_ctx):
     assert 0 < got.size <= syms.shape[0]
 
 
+def test_concurrent_calls_from_host_threads(gpu_ctx):
+    """The C ABI is callable from several host threads at once, on one context
+    (calls serialise on its lock) and on separate contexts (their work overlaps):
+    every result equals the sequential one, with host CSRs large enough to use the
+    threaded staging copy and the process-wide host block cache."""
+    import threading
+    rng = np.random.default_rng(91)
+    jobs = [(rng.integers(0, 4, size=(N, n), dtype=np.uint8), ed)
+            for N, n, ed in ((12000, 11, 5), (3000, 13, 4), (9000, 10, 4), (500, 16, 5), (7000, 12, 5))]
+    ref = [gpu_ctx.graph_build(s, ed) for s, ed in jobs]
+    banner(f"{len(jobs)} jobs from 4 threads, shared and own contexts")
+    own = [igc.Context(0) for _ in range(2)]
+    errors, results = [], {}
+
+    def worker(w):
+        try:
+            ctx = gpu_ctx if w < 2 else own[w - 2]
+            for rep in range(3):
+                for k, (s, ed) in enumerate(jobs):
+                    a = ctx.graph_build(s, ed)
+                    results[(w, rep, k)] = (a.row_ptr.copy(), a.col.copy())
+        except Exception as e:  # reported by the main thread
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=worker, args=(w,)) for w in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=240)
+    for c in own:
+        c.close()
+    assert not errors, errors
+    assert len(results) == 4 * 3 * len(jobs)
+    for (w, rep, k), (rp, col) in results.items():
+        assert np.array_equal(rp, ref[k].row_ptr) and np.array_equal(col, ref[k].col), (w, rep, k)
+
+
 def test_server_results_same_alone_and_in_the_mix(gpu_ctx):
     specs = W.serving(seed=5, jobs=40, mu=np.log(3000), sigma=1.0, lo=100, hi=20000)
     data = [(sp.draw(), sp.edit_dist) for sp in specs]
