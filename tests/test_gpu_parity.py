@@ -670,6 +670,28 @@ def test_refine_matches_oracle(gpu_ctx):
     assert 0 < got.size <= syms.shape[0]
 
 
+def test_batch_symbols_over_several_staging_chunks(gpu_ctx):
+    """A host batch whose concatenated symbols (~30 MB) fill several 8 MiB
+    pinned staging chunks, set boundaries falling inside chunks: every set's CSR
+    equals its single build (which copies its symbols by itself)."""
+    rng = np.random.default_rng(17)
+    sets = []
+    for N, n, ed in ((700_000, 11, 2), (3, 11, 2), (900_000, 11, 2), (1, 9, 1), (1_100_000, 12, 2), (999, 13, 3)):
+        syms = rng.integers(0, 4, size=(N, n), dtype=np.uint8)
+        if N > 10:
+            syms[: N // 50] = syms[N // 50: 2 * (N // 50)]  # duplicates: edges at editDist 2
+        sets.append((syms, ed))
+    total = sum(s.nbytes for s, _ in sets)
+    banner(f"host batch with {total / 2**20:.1f} MiB of symbols")
+    assert total > 3 * (8 << 20)
+    outs = gpu_ctx.graph_build_batch(sets)
+    for k, ((s, ed), o) in enumerate(zip(sets, outs)):
+        single = gpu_ctx.graph_build(s, ed)
+        assert np.array_equal(o.row_ptr, single.row_ptr) and np.array_equal(o.col, single.col), k
+        if s.shape[0] > 10:
+            assert o.n_edges >= s.shape[0] // 50
+
+
 def test_concurrent_calls_from_host_threads(gpu_ctx):
     """The C ABI is callable from several host threads at once, on one context
     (calls serialise on its lock) and on separate contexts (their work overlaps):
