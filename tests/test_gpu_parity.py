@@ -670,6 +670,49 @@ def test_refine_matches_oracle(gpu_ctx):
     assert 0 < got.size <= syms.shape[0]
 
 
+def test_batched_full_config_properties(gpu_ctx):
+    """The full uneven-sets workload (64 sets, n 10..16, editDist 3..5, up to
+    150,000 strings, 8.2e10 pairs): every set's edge count in the batch equals its
+    single build, sampled sets are identical alone and in the mix, sampled rows
+    equal the oracle's row scan, and the device-input batch gives the same union."""
+    sets = W.batched(seed=3, sets=64, scale=1.0)
+    banner(f"full batched workload: {len(sets)} sets, {sum(s.pairs for s in sets):.3e} pairs")
+    outs = gpu_ctx.graph_build_batch([(s.syms, s.edit_dist) for s in sets])
+    rng = np.random.default_rng(12)
+    sampled = set(int(k) for k in rng.choice(len(sets), 6, replace=False))
+    for k, (s, o) in enumerate(zip(sets, outs)):
+        assert o.n_nodes == s.syms.shape[0] and int(o.row_ptr[-1]) == 2 * o.n_edges
+        if k in sampled:
+            alone = gpu_ctx.graph_build(s.syms, s.edit_dist)
+            assert np.array_equal(o.row_ptr, alone.row_ptr) and np.array_equal(o.col, alone.col), k
+            for i in rng.integers(0, s.syms.shape[0], 3):
+                assert np.array_equal(o.neighbors(int(i)), O.row_neighbors(s.syms, s.edit_dist, int(i))), (k, i)
+        else:
+            assert gpu_ctx.graph_count(s.syms, s.edit_dist) == o.n_edges, k
+    total = sum(o.n_edges for o in outs)
+    del outs
+    counts_only = sum(gpu_ctx.graph_count(s.syms, s.edit_dist) for s in sets)
+    assert counts_only == total
+
+
+def test_refine_full_config_deterministic(gpu_ctx):
+    """The full cluster-refinement workload (500,000 strings n=16 with the
+    syndrome mod 17, editDist 5, k=256, 10 Lloyd iterations, 5 rounds) twice:
+    the same codebook, and each round's candidates are the previous round's
+    codewords."""
+    syms = W.vt_refine(seed=4)
+    banner(f"full refinement N={syms.shape[0]}")
+    a, sa = gpu_ctx.refine(syms, 5, 256, kmeans_iters=10, max_rounds=5, converge_n=3, seed=1, host_threads=8)
+    b, sb = gpu_ctx.refine(syms, 5, 256, kmeans_iters=10, max_rounds=5, converge_n=3, seed=1, host_threads=8)
+    assert np.array_equal(a, b)
+    rounds = sa["rounds"]
+    assert rounds[0]["candidates"] == syms.shape[0]
+    for r0, r1 in zip(rounds, rounds[1:]):
+        assert r1["candidates"] == r0["codewords"] and r0["codewords"] <= r0["candidates"]
+    assert a.size == rounds[-1]["codewords"] and len(np.unique(a)) == a.size
+    assert [r["codewords"] for r in rounds] == [r["codewords"] for r in sb["rounds"]]
+
+
 def test_batch_symbols_over_several_staging_chunks(gpu_ctx):
     """A host batch whose concatenated symbols (~30 MB) fill several 8 MiB
     pinned staging chunks, set boundaries falling inside chunks: every set's CSR
