@@ -247,7 +247,10 @@ typedef struct igc_set {
 IGC_API int igc_graph_build_batch(igc_ctx *ctx, const igc_set *sets, uint32_t count, igc_csr *outs);
 
 /* Same with set symbols in device memory; the union CSR (global node ids) stays
- * in device memory owned by the context.  node_off[count+1] receives each set's
+ * in device memory owned by the context.  Every non-empty set's `syms` must be
+ * device memory of the context's device: the sets are gathered by a kernel that
+ * reads them directly (pointers are not validated, to keep the call free of a
+ * per-set runtime query).  node_off[count+1] receives each set's
  * first global id (set s owns rows node_off[s]..node_off[s+1]-1, ids shifted
  * by node_off[s]); set_edges[count] its edge count. */
 IGC_API int igc_graph_build_batch_device(igc_ctx *ctx, const igc_set *sets, uint32_t count, igc_csr *union_out,
