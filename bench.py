@@ -210,6 +210,40 @@ def batched_block(ctx, torch, steps: int = 3, seed: int = 3, sets_n: int = 64, s
     }
 
 
+def process_per_set_block(batched_ms: float, sample: int = 8, seed: int = 4, sets_n: int = 256, scale: float = 0.08):
+    """The reference's own per-cluster GPU form, measured on a sample of the
+    per-cluster sets: one process per set with candidates and edges passing
+    through files, as FillAdjListGPU runs its script (src/SparseMat.cpp:482-533) --
+    here the script is this library's CLI (igc_cli --graph-only: same files,
+    this library's kernels), and the pair file is read back as
+    AdjList::FromBinaryFile does.  The per-set mean is extrapolated to all sets."""
+    from indexgen_amd import workloads as W
+    cli = os.path.join(ROOT, "indexgen_amd", "igc_cli")
+    if not os.path.exists(cli):
+        return None
+    sets = W.batched(seed=seed, sets=sets_n, scale=scale)
+    tmp = tempfile.mkdtemp(prefix="igc_proc_")
+    pick = sets[:sample]
+    t0 = time.perf_counter()
+    edges = 0
+    for k, x in enumerate(pick):
+        cf, pf = os.path.join(tmp, f"c{k}.txt"), os.path.join(tmp, f"e{k}.bin")
+        with open(cf, "w") as f:
+            f.write("\n".join("".join(chr(48 + c) for c in r) for r in x.syms.tolist()) + "\n")
+        subprocess.run([cli, "--cands", cf, "--min-ed", str(x.edit_dist), "--pair-file", pf, "--graph-only"],
+                       check=True, capture_output=True)
+        edges += np.fromfile(pf, dtype="<i4").size // 2
+    per_set = (time.perf_counter() - t0) / len(pick)
+    return {
+        "form": "one process per set, candidates and edges through files (FillAdjListGPU's script call), "
+                "this library's CLI as the script",
+        "sample_sets": len(pick), "sample_edges": edges,
+        "per_set_s": round(per_set, 4),
+        "extrapolated_s": round(per_set * len(sets), 2),
+        "speedup_of_batched_device_build": round(per_set * len(sets) / (batched_ms / 1e3), 0),
+    }
+
+
 def refine_block(ctx, host_threads: int = 16, seed: int = 4):
     """The cluster-refinement workload: 500,000 strings of length 16 with a
     Varshamov-Tenengolts-style syndrome (sum i*x_i = 0 mod 17) passing the run
@@ -502,6 +536,8 @@ def main():
         result["batched"] = batched_block(ctx, torch)
         result["many_small_sets"] = batched_block(ctx, torch, steps=5, seed=4, sets_n=256, scale=0.08,
                                                   label="per-cluster sets")
+        result["many_small_sets"]["reference_form"] = process_per_set_block(
+            result["many_small_sets"]["device_resident"]["batched_ms"])
     if not args.no_refine:
         result["refine"] = refine_block(ctx, host_threads=host_threads())
     if not args.no_cpu_baseline:

@@ -5,6 +5,11 @@
 //
 //   igc_cli --cands FILE --min-ed D [--order-threads T] [--out DIR] [--verify]
 //           [--pair-file PATH] [--refine K --kmeans-iters I --rounds R --converge C]
+//   igc_cli --cands FILE --min-ed D --pair-file PATH --graph-only
+//
+// --graph-only does what the reference's per-call GPU script does
+// (src/gpu_graph_generator.py, run by FillAdjListGPU, src/SparseMat.cpp:482-533):
+// candidates file in, LE int32 pair file out, no greedy and no codebook.
 //
 // FILE holds one candidate per line ('0'..'3' or A,C,G,T), all of one length.
 #include "igc.h"
@@ -40,7 +45,7 @@ int main(int argc, char **argv)
 {
     std::string cands, out = ".", pair_file;
     int min_ed = -1, order_t = 1;
-    bool verify = false;
+    bool verify = false, graph_only = false;
     uint32_t refine_k = 0, km_iters = 10, rounds = 5, converge = 3;
     for (int a = 1; a < argc; ++a) {
         std::string s = argv[a];
@@ -50,6 +55,7 @@ int main(int argc, char **argv)
         else if (s == "--order-threads") order_t = std::atoi(next());
         else if (s == "--out") out = next();
         else if (s == "--verify") verify = true;
+        else if (s == "--graph-only") graph_only = true;
         else if (s == "--pair-file") pair_file = next();
         else if (s == "--refine") refine_k = (uint32_t)std::atoi(next());
         else if (s == "--kmeans-iters") km_iters = (uint32_t)std::atoi(next());
@@ -60,9 +66,9 @@ int main(int argc, char **argv)
             return 2;
         }
     }
-    if (cands.empty() || min_ed < 0) {
+    if (cands.empty() || min_ed < 0 || (graph_only && (pair_file.empty() || refine_k))) {
         std::cerr << "usage: igc_cli --cands FILE --min-ed D [--order-threads T] [--out DIR] [--verify] "
-                     "[--pair-file PATH] [--refine K ...]" << std::endl;
+                     "[--pair-file PATH] [--refine K ...] | --pair-file PATH --graph-only" << std::endl;
         return 2;
     }
     std::ifstream in(cands);
@@ -122,6 +128,12 @@ int main(int argc, char **argv)
         ones = 2 * (long long)g.n_edges;
         std::cout << "Fill AdjList Time:\t" << fill_s << "\tseconds" << std::endl;
         if (!pair_file.empty() && igc_write_pair_file(&g, pair_file.c_str()) != IGC_OK) return die("pair file");
+        if (graph_only) {
+            std::cout << "Edges: " << g.n_edges << std::endl;
+            igc_csr_free(&g);
+            igc_ctx_destroy(ctx);
+            return 0;
+        }
         if (igc_greedy(&g, order.data(), &count) != IGC_OK) return die("igc_greedy");
         proc_s = std::chrono::duration<double>(clk::now() - t1).count();
         std::cout << "Process Matrix Time:\t" << proc_s << "\tseconds" << std::endl;

@@ -865,3 +865,16 @@ def test_cli_writes_the_reference_codebook_file(gpu_ctx, tmp_path):
     assert files == [f"CodeSize-{len(want):07d}_CodeLen-11_MinED-3.txt"]
     body = open(tmp_path / files[0]).read().split("=========================================== \n")[1].split()
     assert body == want
+    # --graph-only: what the reference's per-call GPU script does (pair file, no codebook)
+    pf = tmp_path / "edges.bin"
+    out = subprocess.run([os.path.join(root, "indexgen_amd", "igc_cli"), "--cands", str(cf), "--min-ed", "3",
+                          "--pair-file", str(pf), "--graph-only"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    pairs = np.fromfile(pf, dtype="<i4").reshape(-1, 2)
+    assert (pairs[:, 0] < pairs[:, 1]).all()
+    rp1, col1 = O.graph_csr(syms, 3)
+    rows = np.repeat(np.arange(syms.shape[0]), np.diff(rp1.astype(np.int64)))
+    keep = col1.astype(np.int64) > rows
+    want_pairs = np.stack([rows[keep], col1[keep].astype(np.int64)], 1)
+    got = pairs[np.lexsort((pairs[:, 1], pairs[:, 0]))].astype(np.int64)
+    assert np.array_equal(got, want_pairs)
