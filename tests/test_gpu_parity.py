@@ -878,3 +878,23 @@ def test_cli_writes_the_reference_codebook_file(gpu_ctx, tmp_path):
     want_pairs = np.stack([rows[keep], col1[keep].astype(np.int64)], 1)
     got = pairs[np.lexsort((pairs[:, 1], pairs[:, 0]))].astype(np.int64)
     assert np.array_equal(got, want_pairs)
+
+
+def test_cli_refine_writes_the_refined_codebook(gpu_ctx, tmp_path):
+    """igc_cli --refine runs the clustered loop (igc_refine, seed 1) and writes its
+    codebook file: the words equal Context.refine's for the same parameters."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    syms = W.vt_refine(seed=4, total=5000)
+    cf = tmp_path / "cands.txt"
+    cf.write_text("\n".join(igc.to_strings(syms)) + "\n")
+    out = subprocess.run([os.path.join(root, "indexgen_amd", "igc_cli"), "--cands", str(cf), "--min-ed", "5",
+                          "--out", str(tmp_path), "--refine", "8", "--kmeans-iters", "5", "--rounds", "3",
+                          "--converge", "2"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    order, _ = gpu_ctx.refine(syms, 5, 8, kmeans_iters=5, max_rounds=3, converge_n=2, seed=1, host_threads=4)
+    want = igc.to_strings(syms[order])
+    files = [f for f in os.listdir(tmp_path) if f.startswith("CodeSize-")]
+    assert files == [f"CodeSize-{len(want):07d}_CodeLen-16_MinED-5.txt"]
+    body = open(tmp_path / files[0]).read().split("=========================================== \n")[1].split()
+    assert body == want
