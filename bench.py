@@ -2,17 +2,22 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload linear_large]
 
-A step is one full build of the conflict graph of one candidate set whose
-symbols are already resident in HBM: pack -> all-pairs edit-distance kernel ->
-CSR (degree, scan, scatter, row sort), ending with the CSR in HBM.  The default
-workload is the single large set: n=13, editDist=4, 200,000 candidates (GF(4)
-[13,8] code words + uniform filtered padding), 2.0e10 pairs per step.
+A step is one full build of the conflict graph of one candidate set: the
+candidates' symbols (N x n bytes in pinned host memory) -> host-to-device copy
+-> pack -> exact row filter set-up -> all-pairs edit-distance kernel -> exact
+edge count -> CSR (split pass + radix passes), ending with the CSR in HBM.  The
+default workload is the single large set: n=13, editDist=4, 200,000 candidates
+(GF(4) [13,8] code words + uniform filtered padding), 2.0e10 pairs per step.
+After the timed steps the last build's edge set is hashed and compared with the
+digest the compiled reference (FillAdjList, src/SparseMat.cpp:567-615) gave for
+the same set (tests/golden/full_size_golden.npz); a mismatch exits non-zero.
 
-Prints one JSON line: Gpairs/s (value), per-step latency p50/p99, the hot
-kernel's roofline (VALU-bound integer work, HIP-event timing on the stream the
-kernel runs on) and a CPU baseline: the IndexGen reference's own FillAdjList
-(compiled from its sources into oracle/_ref) timed on a bounded sample of the
-same workload with the host's threads.
+Prints one JSON line: Gpairs/s (value); latency_ms p50/p99 of the serving
+stream (all 2,000 jobs, real-time arrivals); the pair kernel's roofline (issued
+VALU instructions from the committed rocprof counters over the live HIP-event
+kernel time, against the int32 VALU peak); a CPU baseline: the reference's own
+FillAdjList (compiled from its sources into oracle/_ref) on the full set with
+the lease's host threads, its edge count checked against the GPU's.
 """
 from __future__ import annotations
 
@@ -32,74 +37,120 @@ sys.path.insert(0, ROOT)
 # MI355X: 256 CUs x 4 SIMD x 16 lanes per clock x 2.4 GHz = int32 VALU lane-ops/s
 PEAK_VALU_TOPS = 256 * 64 * 2.4e9 / 1e12
 PEAK_HBM_GBS = 8000.0
+GOLDEN = os.path.join(ROOT, "tests", "golden", "full_size_golden.npz")
 
-# Measured issue rate of single int32 VALU instruction types (xor, shifts,
-# ffbl, max3, or3, add3, add) in independent chains on this MI355X:
-# 604 G wave-instructions/s = 38.7 T lane-ops/s (profiles/valu_issue_probe.md).
-PROBE_VALU_TOPS = 38.7
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def workload(name: str):
     from indexgen_amd import workloads as W
     if name == "linear_large":
-        return W.linear_large(seed=2), 13, 4, "n=13 editDist=4 N=200000 GF(4)[13,8] codewords + uniform filtered padding"
+        return W.linear_large(seed=2), 13, 4, "linear", \
+            "n=13 editDist=4 N=200000 GF(4)[13,8] codewords + uniform filtered padding (seed 2)"
     if name == "plain_small":
-        return W.plain_small(seed=1), 11, 3, "n=11 editDist=3 20000 uniform strings, dedup, maxRun<=3, GC in [0.3,0.7]"
+        return W.plain_small(seed=1), 11, 3, "plain", \
+            "n=11 editDist=3 20000 uniform strings (seed 1), dedup, maxRun<=3, GC in [0.3,0.7]"
     raise SystemExit(f"unknown workload {name}")
 
 
 def host_threads() -> int:
+    """Host threads of this lease: the affinity set, capped by OMP_NUM_THREADS
+    when the environment sets it (the box's CPU share)."""
     try:
         avail = len(os.sched_getaffinity(0))
     except AttributeError:
         avail = os.cpu_count() or 1
-    return max(1, min(16, avail))
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return max(1, min(avail, cap) if cap > 0 else avail)
 
 
-def cpu_baseline(syms: np.ndarray, min_ed: int, target_s: float = 12.0):
-    """Reference FillAdjList (oracle/_ref/ref_driver fill) on the first M
-    candidates, M chosen so the run takes about target_s seconds."""
+def write_cands(path: str, syms: np.ndarray) -> None:
+    s = np.ascontiguousarray(syms, dtype=np.uint8) + ord("0")
+    lines = np.full((s.shape[0], s.shape[1] + 1), ord("\n"), dtype=np.uint8)
+    lines[:, :-1] = s
+    with open(path, "wb") as f:
+        f.write(lines.tobytes())
+
+
+def ref_run(mode: str, T: int, min_ed: int, syms: np.ndarray) -> dict:
+    """oracle/_ref/ref_driver fill|solve on one set: the reference's own code."""
+    from oracle import oracle as O
+    d = tempfile.mkdtemp(prefix="igc_cpu_")
+    path = os.path.join(d, "c.txt")
+    write_cands(path, syms)
+    out = subprocess.run([O.REF_DRIVER, mode, str(T), str(min_ed), path], capture_output=True, text=True,
+                         check=True).stdout.split()
+    os.remove(path)
+    os.rmdir(d)
+    return {out[k]: float(out[k + 1]) for k in range(0, len(out), 2)}
+
+
+def cpu_baseline(syms: np.ndarray, min_ed: int, gpu_ones: int) -> dict:
+    """The reference's FillAdjList (src/SparseMat.cpp:567-615, built from its
+    sources) on the whole set with the lease's threads; its 2|E| must equal the
+    GPU's.  Without the reference binary: the oracle's C restatement."""
     from oracle import oracle as O
     T = host_threads()
-    driver = O.REF_DRIVER
-    tmp = tempfile.mkdtemp(prefix="igc_cpu_")
-
-    def run_ref(M):
-        path = os.path.join(tmp, "c.txt")
-        with open(path, "w") as f:
-            f.write("\n".join("".join(chr(48 + c) for c in r) for r in syms[:M].tolist()) + "\n")
-        t0 = time.perf_counter()
-        out = subprocess.run([driver, "fill", str(T), str(min_ed), path], capture_output=True, text=True,
-                             check=True).stdout.split()
-        wall = time.perf_counter() - t0
-        return float(out[3]), int(out[1]), wall
-
-    def run_port(M):
-        t0 = time.perf_counter()
-        e = O.edge_count(syms[:M], min_ed, workers=T)
-        return time.perf_counter() - t0, 2 * e, None
-
-    if os.path.exists(driver):
-        kind, fn = "reference", run_ref
+    N = syms.shape[0]
+    pairs = N * (N - 1) // 2
+    if os.path.exists(O.REF_DRIVER):
+        r = ref_run("fill", T, min_ed, syms)
+        kind, secs, ones = "reference", r["fill_s"], int(r["ones"])
+        what = "reference FillAdjList (src/SparseMat.cpp:567-615) built from source"
     else:
-        kind, fn = "port", run_port
-    M = min(syms.shape[0], 6000)
-    secs, ones, _ = fn(M)
-    rate = (M * (M - 1) / 2) / max(secs, 1e-6)
-    M2 = int(min(syms.shape[0], max(M, np.sqrt(2 * rate * target_s))))
-    if M2 > M:
-        M = M2
-        secs, ones, _ = fn(M)
-    pairs = M * (M - 1) / 2
-    return {
-        "value": pairs / secs / 1e9,
-        "unit": "Gpairs/s",
-        "cores": T,
-        "kind": kind,
-        "sample": f"first {M} candidates of the same set ({pairs:.3e} pairs, {ones} ones), "
-                  f"{'reference FillAdjList (src/SparseMat.cpp:567-615) built from source' if kind == 'reference' else 'oracle restatement'}"
-                  f" with {T} threads, {secs:.2f} s",
-    }
+        t0 = time.perf_counter()
+        ones = 2 * O.edge_count(syms, min_ed, workers=T)
+        kind, secs, what = "port", time.perf_counter() - t0, "oracle restatement"
+    if ones != gpu_ones:
+        raise SystemExit(f"CPU baseline edge count {ones} differs from the GPU's {gpu_ones}")
+    return {"value": pairs / secs / 1e9, "unit": "Gpairs/s", "cores": T, "kind": kind,
+            "sample": f"the whole set: {N} candidates, {pairs:.3e} pairs, {ones} ones (equal to the GPU's); "
+                      f"{what} with {T} threads (nproc {os.cpu_count()}), {secs:.2f} s"}
+
+
+def cpu_configs_block(T: int) -> dict:
+    """The reference's CPU codebook path (ref_driver solve: FillAdjList +
+    Codebook's selection loop, no progress dumps) on bounded samples of the other
+    workloads: the whole n=11 set, the first jobs of the serving stream and the
+    first sets of the batched workload up to ~4e9 pairs each."""
+    from indexgen_amd import workloads as W
+    from oracle import oracle as O
+    if not os.path.exists(O.REF_DRIVER):
+        return None
+    out = {"threads": T, "form": "ref_driver solve: reference FillAdjList + Codebook selection loop"}
+
+    def run(items, label):
+        pairs = fill = greedy = 0.0
+        ones = []
+        for syms, ed in items:
+            r = ref_run("solve", T, ed, syms)
+            pairs += syms.shape[0] * (syms.shape[0] - 1) / 2
+            fill += r["fill_s"]
+            greedy += r["greedy_s"]
+            ones.append(int(r["ones"]))
+        log(f"cpu {label}: {len(items)} sets, {pairs:.3e} pairs, fill {fill:.2f} s, greedy {greedy:.2f} s")
+        return {"sets": len(items), "pairs": pairs, "fill_s": round(fill, 3), "greedy_s": round(greedy, 3),
+                "fill_gpairs_s": round(pairs / max(fill, 1e-9) / 1e9, 4),
+                "codebook_gpairs_s": round(pairs / max(fill + greedy, 1e-9) / 1e9, 4)}, ones
+
+    out["plain_small"], _ = run([(W.plain_small(seed=1), 3)], "plain_small")
+    specs, acc, items = W.serving(seed=5), 0, []
+    for sp in specs:
+        if acc + sp.pairs > 4e9 and items:
+            break
+        acc += sp.pairs
+        items.append((sp.draw(), sp.edit_dist))
+    out["serving_first_jobs"], _ = run(items, "serving")
+    sets, acc, items = W.batched(seed=3), 0, []
+    for cs in sets:
+        if acc + cs.pairs > 4e9 and items:
+            continue
+        acc += cs.pairs
+        items.append((cs.syms, cs.edit_dist))
+    out["batched_sets"], _ = run(items, "batched")
+    return out
 
 
 def generators_block(reps: int = 3) -> dict:
@@ -127,7 +178,7 @@ def generators_block(reps: int = 3) -> dict:
             best = c.ms if best is None else min(best, c.ms)
         out[name] = {"words": c.enumerated, "kept": c.count, "ms": round(best, 3),
                      "gwords_s": round(c.enumerated / (best / 1e3) / 1e9, 1)}
-        print(f"[bench] generator {name}: {out[name]}", file=sys.stderr, flush=True)
+        log(f"generator {name}: {out[name]}")
     ref = {}
     T = host_threads()
     if os.path.exists(O.REF_DRIVER):
@@ -142,7 +193,7 @@ def generators_block(reps: int = 3) -> dict:
             ref[name] = {"words": n_words, "kept": len(words), "s": round(secs, 2), "threads": T,
                          "gwords_s": round(n_words / secs / 1e9, 4),
                          "device_ms": round(dev.ms, 3), "device_gwords_s": round(n_words / (dev.ms / 1e3) / 1e9, 1)}
-            print(f"[bench] reference generator {name}: {ref[name]}", file=sys.stderr, flush=True)
+            log(f"reference generator {name}: {ref[name]}")
     return {"metric": "candidate words enumerated per second", "unit": "Gwords/s", "device": out,
             "reference_cpu": ref or None, "filter": {"max_run": filt[0], "gc": list(filt[1:])}}
 
@@ -169,14 +220,13 @@ def batched_block(ctx, torch, steps: int = 3, seed: int = 3, sets_n: int = 64, s
         lat.append((time.perf_counter() - s0) * 1e3)
     batched_s = (time.perf_counter() - t0) / steps
     tb = ctx.last_timing()
-    # plain, device-resident: one build per set from its device symbols
     t0 = time.perf_counter()
     plain_dev_edges = 0
     for p, N, n, ed in spec:
         plain_dev_edges += ctx.graph_build_device(p, N, n, ed).n_edges
     plain_dev_s = time.perf_counter() - t0
-    # host API both ways: symbols in, per-set host CSRs out
     hsets = [(x.syms, x.edit_dist) for x in sets]
+    ctx.graph_build_batch(hsets)  # warm the host block cache
     t0 = time.perf_counter()
     host_edges = sum(a.n_edges for a in ctx.graph_build_batch(hsets))
     batched_host_s = time.perf_counter() - t0
@@ -186,6 +236,7 @@ def batched_block(ctx, torch, steps: int = 3, seed: int = 3, sets_n: int = 64, s
         plain_edges += ctx.graph_build(x.syms, x.edit_dist).n_edges
     plain_s = time.perf_counter() - t0
     assert plain_edges == plain_dev_edges == host_edges == int(edges.sum()), "batched and per-set builds disagree"
+    log(f"{label}: batched {batched_s * 1e3:.2f} ms in HBM, {batched_host_s * 1e3:.1f} ms through the host API")
     return {
         "workload": f"{label}: {len(sets)} sets, n in 10..16, editDist in 3..5, sizes lognormal (seed {seed}, scale {scale})",
         "pairs": pairs,
@@ -194,6 +245,7 @@ def batched_block(ctx, torch, steps: int = 3, seed: int = 3, sets_n: int = 64, s
             "batched_gpairs_s": round(pairs / batched_s / 1e9, 3),
             "batched_ms": round(batched_s * 1e3, 3),
             "batched_pairs_kernel_ms": round(tb.pairs_ms, 3),
+            "batched_csr_ms": round(tb.csr_ms, 3),
             "batched_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                                    "p99": round(float(np.percentile(lat, 99)), 3)},
             "plain_per_set_gpairs_s": round(pairs / plain_dev_s / 1e9, 3),
@@ -210,37 +262,38 @@ def batched_block(ctx, torch, steps: int = 3, seed: int = 3, sets_n: int = 64, s
     }
 
 
-def process_per_set_block(batched_ms: float, sample: int = 8, seed: int = 4, sets_n: int = 256, scale: float = 0.08):
-    """The reference's own per-cluster GPU form, measured on a sample of the
-    per-cluster sets: one process per set with candidates and edges passing
-    through files, as FillAdjListGPU runs its script (src/SparseMat.cpp:482-533) --
-    here the script is this library's CLI (igc_cli --graph-only: same files,
-    this library's kernels), and the pair file is read back as
-    AdjList::FromBinaryFile does.  The per-set mean is extrapolated to all sets."""
+def process_per_set_block(batched_host_ms: float, seed: int, sets_n: int, scale: float) -> dict | None:
+    """The reference's own per-cluster GPU form on every set of the batch: one
+    process per set with candidates and edges passing through files, as
+    FillAdjListGPU runs its script (src/SparseMat.cpp:482-533) -- here the script
+    is this library's CLI (igc_cli --graph-only: the same files, this library's
+    kernels, so a lower bound on the reference's cost) -- and the pair file read
+    back as AdjList::FromBinaryFile does.  Compared with the batched build
+    through the host API (symbols in, host CSRs out), the like-for-like form."""
     from indexgen_amd import workloads as W
     cli = os.path.join(ROOT, "indexgen_amd", "igc_cli")
     if not os.path.exists(cli):
         return None
     sets = W.batched(seed=seed, sets=sets_n, scale=scale)
     tmp = tempfile.mkdtemp(prefix="igc_proc_")
-    pick = sets[:sample]
     t0 = time.perf_counter()
     edges = 0
-    for k, x in enumerate(pick):
-        cf, pf = os.path.join(tmp, f"c{k}.txt"), os.path.join(tmp, f"e{k}.bin")
-        with open(cf, "w") as f:
-            f.write("\n".join("".join(chr(48 + c) for c in r) for r in x.syms.tolist()) + "\n")
+    for k, x in enumerate(sets):
+        cf, pf = os.path.join(tmp, "c.txt"), os.path.join(tmp, "e.bin")
+        write_cands(cf, x.syms)
         subprocess.run([cli, "--cands", cf, "--min-ed", str(x.edit_dist), "--pair-file", pf, "--graph-only"],
                        check=True, capture_output=True)
         edges += np.fromfile(pf, dtype="<i4").size // 2
-    per_set = (time.perf_counter() - t0) / len(pick)
+        os.remove(pf)
+    total = time.perf_counter() - t0
+    log(f"per-process form: {len(sets)} sets in {total:.2f} s")
     return {
         "form": "one process per set, candidates and edges through files (FillAdjListGPU's script call), "
-                "this library's CLI as the script",
-        "sample_sets": len(pick), "sample_edges": edges,
-        "per_set_s": round(per_set, 4),
-        "extrapolated_s": round(per_set * len(sets), 2),
-        "speedup_of_batched_device_build": round(per_set * len(sets) / (batched_ms / 1e3), 0),
+                "this library's CLI as the script; every set of the batch",
+        "sets": len(sets), "edges": edges, "total_s": round(total, 3),
+        "per_set_s": round(total / len(sets), 4),
+        "batched_host_api_ms": round(batched_host_ms, 3),
+        "speedup_of_batched_host_api": round(total / (batched_host_ms / 1e3), 1),
     }
 
 
@@ -252,6 +305,7 @@ def refine_block(ctx, host_threads: int = 16, seed: int = 4):
     the greedy per cluster on host threads; at most 5 rounds."""
     from indexgen_amd import workloads as W
     syms = W.vt_refine(seed=seed)
+    ctx.refine(syms[:20000], 5, k=16, kmeans_iters=2, max_rounds=1, converge_n=3, seed=1, host_threads=host_threads)
     t0 = time.perf_counter()
     order, st = ctx.refine(syms, 5, k=256, kmeans_iters=10, max_rounds=5, converge_n=3, seed=1,
                            host_threads=host_threads)
@@ -259,6 +313,7 @@ def refine_block(ctx, host_threads: int = 16, seed: int = 4):
     rounds = st["rounds"]
     pairs = sum(r["pairs"] for r in rounds)
     dev_pairs_s = sum(r["pairs_ms"] for r in rounds) / 1e3
+    log(f"refine: {len(rounds)} rounds, {order.size} codewords, {wall:.3f} s")
     return {
         "workload": f"{syms.shape[0]} strings n=16 (VT syndrome mod 17, run/GC filtered, seed {seed}), editDist 5, "
                     f"k=256, 10 Lloyd iterations, up to 5 rounds",
@@ -269,7 +324,27 @@ def refine_block(ctx, host_threads: int = 16, seed: int = 4):
     }
 
 
-def serving_block(jobs: int = 200, pipelines: int = 2, host_threads: int = 16, seed: int = 5):
+def draw_jobs(specs, threads: int):
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max(1, min(threads, 16))) as ex:
+        return list(ex.map(lambda sp: sp.draw(), specs))
+
+
+def greedy_profile(specs, res) -> dict:
+    """Per-job host greedy time against edge count (the serving tail)."""
+    g = np.array([r["done_s"] - r["greedy_start_s"] for r in res])
+    e = np.array([r["n_edges"] for r in res], dtype=np.float64)
+    big = e >= 1e6
+    top = np.argsort(e)[-8:][::-1]
+    return {
+        "ns_per_edge_jobs_over_1M_edges": round(float(g[big].sum() / e[big].sum() * 1e9), 2) if big.any() else None,
+        "densest_jobs": [{"job": int(k), "N": int(specs[k].size), "n": specs[k].n, "edit_dist": specs[k].edit_dist,
+                          "edges": int(e[k]), "greedy_s": round(float(g[k]), 3),
+                          "latency_s": round(res[k]["done_s"] - res[k]["submit_s"], 3)} for k in top],
+    }
+
+
+def serving_block(jobs: int, pipelines: int = 2, host_threads: int = 16, seed: int = 5):
     """The codebook-request stream (Poisson 20 jobs/s, n in 11..16, editDist in
     3..5, sizes lognormal(ln 30000, 1.2) clipped to [500, 400000]) replayed in
     real time: each job is submitted at its arrival time; latency = completion
@@ -277,28 +352,36 @@ def serving_block(jobs: int = 200, pipelines: int = 2, host_threads: int = 16, s
     import indexgen_amd as igc
     from indexgen_amd import workloads as W
     specs = W.serving(seed=seed)[:jobs]
-    data = [(sp.draw(), sp.edit_dist) for sp in specs]
+    t0d = time.perf_counter()
+    data = draw_jobs(specs, host_threads)
+    log(f"serving: drew {sum(d.shape[0] for d in data)} strings for {jobs} jobs in {time.perf_counter() - t0d:.1f} s")
     srv = igc.Server(0, pipelines=pipelines, host_threads=host_threads)
-    srv.wait(srv.submit(data[0][0][:64], data[0][1]))  # warm the pipelines
+    srv.wait(srv.submit(data[0][:64], specs[0].edit_dist))  # warm the pipelines
     t0 = srv.clock()
     ids = []
-    for sp, (syms, ed) in zip(specs, data):
+    next_log = 10.0
+    for sp, syms in zip(specs, data):
         delay = t0 + sp.arrival_s - srv.clock()
         if delay > 0:
             time.sleep(delay)
-        ids.append(srv.submit(syms, ed))
+        ids.append(srv.submit(syms, sp.edit_dist))
+        if sp.arrival_s >= next_log:
+            log(f"serving: {len(ids)}/{jobs} jobs submitted at t={sp.arrival_s:.0f} s")
+            next_log += 10.0
     res = [srv.wait(j) for j in ids]
     info = srv.stats()
     srv.close()
-    print(f"[bench] serving stream of {jobs} jobs done", file=sys.stderr, flush=True)
+    del data
+    log(f"serving stream of {jobs} jobs done")
     bad = [r for r in res if r["status"] != 0]
+    if bad:
+        raise SystemExit(f"{len(bad)} serving jobs failed: {bad[0]['error']}")
     lat = np.array([r["done_s"] - r["submit_s"] for r in res]) * 1e3
     gpu_lat = np.array([r["gpu_done_s"] - r["submit_s"] for r in res]) * 1e3
     makespan = max(r["done_s"] for r in res) - min(r["submit_s"] for r in res)
     pairs = sum(sp.pairs for sp in specs)
     return {
-        "workload": f"first {jobs} jobs of the seeded stream (seed {seed}), real-time arrivals over "
-                    f"{specs[-1].arrival_s:.1f} s",
+        "workload": f"the seeded stream (seed {seed}): {jobs} jobs, real-time arrivals over {specs[-1].arrival_s:.1f} s",
         "jobs": jobs, "failed": len(bad), "pairs": pairs, "edges": int(sum(r["n_edges"] for r in res)),
         "throughput_gpairs_s": round(pairs / makespan / 1e9, 3),
         "offered_gpairs_s": round(pairs / specs[-1].arrival_s / 1e9, 3),
@@ -308,11 +391,13 @@ def serving_block(jobs: int = 200, pipelines: int = 2, host_threads: int = 16, s
                              "p99": round(float(np.percentile(gpu_lat, 99)), 2)},
         "makespan_s": round(makespan, 3),
         "launches": int(info["batches"]), "max_jobs_per_launch": int(info["max_batch_jobs"]),
+        "max_hbm_reserved_gb": round(info["max_in_use_bytes"] / 2 ** 30, 2),
         "gpu_busy_s": round(info["gpu_busy_s"], 3), "pipelines": pipelines, "greedy_threads": host_threads,
+        "greedy": greedy_profile(specs, res),
     }
 
 
-def saturated_block(jobs: int = 200, pipelines: int = 2, host_threads: int = 16, seed: int = 5):
+def saturated_block(jobs: int, pipelines: int = 2, host_threads: int = 16, seed: int = 5):
     """Server capacity on the same mixed jobs: every job submitted at once (the
     queue is never empty, so each pipeline takes up to 512 queued jobs that fit
     the HBM budget per batched build), against the plain form -- one host-API
@@ -321,24 +406,23 @@ def saturated_block(jobs: int = 200, pipelines: int = 2, host_threads: int = 16,
     import indexgen_amd as igc
     from indexgen_amd import workloads as W
     specs = W.serving(seed=seed)[:jobs]
-    data = [(sp.draw(), sp.edit_dist) for sp in specs]
+    data = draw_jobs(specs, host_threads)
     pairs = sum(sp.pairs for sp in specs)
     srv = igc.Server(0, pipelines=pipelines, host_threads=host_threads)
-    srv.wait(srv.submit(data[0][0][:64], data[0][1]))
+    srv.wait(srv.submit(data[0][:64], specs[0].edit_dist))
     t0 = srv.clock()
-    ids = [srv.submit(syms, ed) for syms, ed in data]
+    ids = [srv.submit(syms, sp.edit_dist) for syms, sp in zip(data, specs)]
     res = [srv.wait(j) for j in ids]
     info = srv.stats()
     srv.close()
     assert all(r["status"] == 0 for r in res), "a saturated job failed"
     graph_s = max(r["gpu_done_s"] for r in res) - t0
     done_s = max(r["done_s"] for r in res) - t0
-    print(f"[bench] saturated server: {jobs} jobs, graphs {graph_s:.3f} s, codebooks {done_s:.3f} s",
-          file=sys.stderr, flush=True)
+    log(f"saturated server: {jobs} jobs, graphs {graph_s:.3f} s, codebooks {done_s:.3f} s")
     ctx = igc.Context(0)
-    ctx.graph_build(data[0][0][:64], data[0][1])
+    ctx.graph_build(data[0][:64], specs[0].edit_dist)
     t1 = time.perf_counter()
-    plain_edges = sum(ctx.graph_build(syms, ed).n_edges for syms, ed in data)
+    plain_edges = sum(ctx.graph_build(syms, sp.edit_dist).n_edges for syms, sp in zip(data, specs))
     plain_s = time.perf_counter() - t1
     ctx.close()
     served_edges = int(sum(r["n_edges"] for r in res))
@@ -357,6 +441,43 @@ def saturated_block(jobs: int = 200, pipelines: int = 2, host_threads: int = 16,
     }
 
 
+def counter_record(name: str, plan: dict):
+    """The committed rocprof counter record of the pair kernel for this
+    workload, when it was taken on the launch the run makes."""
+    path = os.path.join(ROOT, "profiles", "k_pairs_traffic.json")
+    if not os.path.exists(path):
+        return None
+    rec = json.load(open(path)).get(name)
+    if rec and (rec.get("slots"), rec.get("q"), rec.get("lane_cols"), rec.get("row_filter")) == (
+            plan["slots"], plan["q"], plan["lane_cols"], plan["row_filter"]):
+        return rec
+    return None
+
+
+def verify_build(ctx, dcsr, syms, key: str) -> dict:
+    """The last timed build against the compiled reference's digests: 2|E|, the
+    sorted i<j edge list, and the codebook under the reference's 8-thread
+    neighbour order."""
+    import indexgen_amd as igc
+    if not os.path.exists(GOLDEN):
+        return {"checked": False, "reason": "no fixture"}
+    z = np.load(GOLDEN)
+    want_ones = int(z[f"{key}_ones"])
+    want_edges = z[f"{key}_edges_sha256"].item().decode()
+    adj = ctx.device_csr_to_host(dcsr)
+    got_edges = igc.edges_sha256(adj)
+    T = int(z["threads"])
+    ordered = ctx.device_csr_to_host(dcsr, igc.order_threads(T))
+    cb = igc.codebook_sha256(syms, igc.greedy_order(ordered))
+    want_cb = z[f"{key}_codebook_sha256"].item().decode()
+    out = {"checked": True, "matrix_ones": 2 * adj.n_edges, "reference_ones": want_ones,
+           "edges_sha256": got_edges, "edges_equal": got_edges == want_edges and 2 * adj.n_edges == want_ones,
+           "codebook_equal_T%d" % T: cb == want_cb,
+           "fixture": "tests/golden/full_size_golden.npz (reference FillAdjList + Codebook, 8 threads)"}
+    out["ok"] = out["edges_equal"] and out["codebook_equal_T%d" % T]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -364,68 +485,52 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="linear_large")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--no-batched", action="store_true", help="skip the uneven-sets comparison block")
+    ap.add_argument("--no-cpu-configs", action="store_true", help="skip the CPU samples of the other workloads")
+    ap.add_argument("--no-batched", action="store_true", help="skip the uneven-sets comparison blocks")
+    ap.add_argument("--no-reference-forms", action="store_true",
+                    help="skip the one-process-per-set form (every set of both batches)")
     ap.add_argument("--no-autotune", action="store_true", help="use the default tile shape")
-    ap.add_argument("--serving-jobs", type=int, default=200, help="jobs of the serving stream to replay (0: skip)")
-    ap.add_argument("--saturated-jobs", type=int, default=200,
+    ap.add_argument("--serving-jobs", type=int, default=2000, help="jobs of the serving stream to replay (0: skip)")
+    ap.add_argument("--saturated-jobs", type=int, default=400,
                     help="jobs of the stream submitted at once for the server-capacity block (0: skip)")
     ap.add_argument("--no-refine", action="store_true", help="skip the cluster-refinement block")
-    ap.add_argument("--valu-probe", action="store_true", help="only measure the int32 VALU issue rates")
     ap.add_argument("--generators", action="store_true",
                     help="only measure the on-device candidate generators against the reference's")
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="only run 3 device-to-device copies of MIB MiB (scales rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this run has one MI355X per box, and the conflict-graph build here "
+              "runs on a single GPU (sharding the pair triangle over GPUs is designed, not built). Use --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
     if args.generators:
         import torch  # noqa: F401
         print(json.dumps(generators_block()))
         return
-    if args.valu_probe:
-        import torch  # noqa: F401  (initialises the runtime the same way as a bench run)
-        import indexgen_amd as igc
-        rates = igc.Context(0).valu_probe()
-        print(json.dumps({"wave_instr_per_s": rates,
-                          "lane_ops_T": {k: round(v * 64 / 1e12, 3) for k, v in rates.items()}}))
-        return
-    if args.copy_calibration:
-        import torch
-        a = torch.empty(args.copy_calibration << 20, dtype=torch.uint8, device="cuda:0").fill_(1)
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes": a.numel(), "copies": 3}))
-        return
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the conflict-graph build here runs on a single MI355X "
-              "(sharding the pair triangle over GPUs is designed but not built). Use --gpus 1.", file=sys.stderr)
-        sys.exit(2)
 
     import torch
     import indexgen_amd as igc
 
-    syms, n, min_ed, desc = workload(args.workload)
+    syms, n, min_ed, key, desc = workload(args.workload)
     N = syms.shape[0]
     pairs = N * (N - 1) // 2
-    print(f"[bench] workload {args.workload}: {desc}; pairs/step={pairs:.3e}", file=sys.stderr, flush=True)
+    log(f"workload {args.workload}: {desc}; pairs/step={pairs:.3e}; host threads {host_threads()}")
 
-    d_syms = torch.from_numpy(np.ascontiguousarray(syms)).to("cuda:0")
-    torch.cuda.synchronize()
+    # candidates in pinned host memory: each step copies them to the device
+    h_syms = torch.from_numpy(np.ascontiguousarray(syms)).pin_memory()
+    hptr = h_syms.data_ptr()
     ctx = igc.Context(0)
-    ptr = d_syms.data_ptr()
     tune = ctx.autotune(n, min_ed, N, reps=2) if not args.no_autotune else []
     plan = ctx.pair_plan(n, min_ed, N)
 
     for _ in range(args.warmup):
-        ctx.graph_build_device(ptr, N, n, min_ed)
+        ctx.graph_build_to_device(hptr, min_ed, n=n, N=N)
     torch.cuda.synchronize()
 
     step_ms, pairs_ms, csr_ms = [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         s0 = time.perf_counter()
-        dcsr = ctx.graph_build_device(ptr, N, n, min_ed)
+        dcsr = ctx.graph_build_to_device(hptr, min_ed, n=n, N=N)
         step_ms.append((time.perf_counter() - s0) * 1e3)
         t = ctx.last_timing()
         pairs_ms.append(t.pairs_ms)
@@ -434,16 +539,28 @@ def main():
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed * 1e3 / args.steps
     value = pairs / (elapsed / args.steps) / 1e9
+    log(f"headline: {value:.1f} Gpairs/s, {ms_per_step:.3f} ms per step (host-to-device copy included)")
 
-    # the same builds without the exact row filter, for comparison
+    verify = verify_build(ctx, dcsr, syms, key)
+    if verify.get("checked"):
+        log(f"verify: edges equal {verify['edges_equal']}, codebook equal {verify['ok']}")
+
+    # the same builds with the symbols already in HBM, and without the row filter
+    d_syms = torch.from_numpy(np.ascontiguousarray(syms)).to("cuda:0")
+    torch.cuda.synchronize()
+    dev_ms = []
+    for _ in range(max(3, min(args.steps, 10))):
+        s0 = time.perf_counter()
+        ctx.graph_build_device(d_syms.data_ptr(), N, n, min_ed)
+        dev_ms.append((time.perf_counter() - s0) * 1e3)
     ctx.set_prefilter(False)
-    ctx.graph_build_device(ptr, N, n, min_ed)
+    ctx.graph_build_device(d_syms.data_ptr(), N, n, min_ed)
     unf_ms = []
-    for _ in range(max(2, min(args.steps, 5))):
-        ctx.graph_build_device(ptr, N, n, min_ed)
+    for _ in range(3):
+        ctx.graph_build_device(d_syms.data_ptr(), N, n, min_ed)
         unf_ms.append(ctx.last_timing().pairs_ms)
     ctx.set_prefilter(True)
-    dcsr = ctx.graph_build_device(ptr, N, n, min_ed)
+    dcsr = ctx.graph_build_to_device(hptr, min_ed, n=n, N=N)
 
     # outputs: greedy codebook of the last build (host), reported separately
     g0 = time.perf_counter()
@@ -455,47 +572,38 @@ def main():
 
     kern_ms = float(np.mean(pairs_ms))
     edge_bytes = dcsr.n_edges * 8  # the kernel's compacted (i,j) writes
-    # Algorithmic ops per pair as the measurement plan defines them for this
-    # stage: one Myers column step per symbol, 17 int ops, so 17*n per pair.
-    # This kernel computes the same predicate with fewer instructions (the
-    # Landau-Vishkin recurrence, two pairs per register), so the fraction below
-    # exceeds 1 by the algorithmic gain; the physical issue rate comes from the
-    # rocprof counters.
-    myers_ops = 17 * n
-    achieved = pairs * myers_ops / (kern_ms / 1e3) / 1e12
-    traffic, issued = None, None
-    tpath = os.path.join(ROOT, "profiles", "k_pairs_traffic.json")
-    if os.path.exists(tpath):
-        rec = json.load(open(tpath)).get(args.workload)
-        if rec and (rec.get("slots"), rec.get("q"), rec.get("lane_cols"), rec.get("row_filter")) == (
-                plan["slots"], plan["q"], plan["lane_cols"], plan["row_filter"]):
-            # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE passes
-            traffic = int(rec["fetch_size_kib"] * 1024 * rec["fetch_scale"]
-                          + rec["write_size_kib"] * 1024 * rec["write_scale"])
-            issued = {
-                "valu_ops_per_pair": rec["valu_ops_per_pair"],
-                "achieved": round(pairs * rec["valu_ops_per_pair"] / (kern_ms / 1e3) / 1e12, 3),
-                "peak": round(PEAK_VALU_TOPS, 3),
-                "probe_peak": PROBE_VALU_TOPS,
-                "frac": round(pairs * rec["valu_ops_per_pair"] / (kern_ms / 1e3) / 1e12 / PEAK_VALU_TOPS, 4),
-                "source": "SQ_INSTS_VALU x 64 / pairs from " + rec["source"],
-            }
+    rec = counter_record(args.workload, plan)
+    myers_x = pairs * 17 * n / (kern_ms / 1e3) / 1e12 / PEAK_VALU_TOPS
+    if rec:
+        # issued VALU lane-ops of one launch (SQ_INSTS_VALU x 64, exact for this
+        # launch shape and input) over the live kernel time
+        ops = rec["sq_insts_valu"] * 64
+        achieved = ops / (kern_ms / 1e3) / 1e12
+        traffic = int(rec["fetch_size_kib"] * 1024 * rec["fetch_scale"] + rec["write_size_kib"] * 1024 *
+                      rec["write_scale"])
+    else:
+        achieved, traffic = None, None
     roofline = {
         "bound": "valu",
-        "achieved": round(achieved, 3),
+        "achieved": round(achieved, 3) if achieved is not None else None,
         "peak": round(PEAK_VALU_TOPS, 3),
-        "unit": "Tops/s int32 VALU (algorithmic: 17*n ops per pair)",
-        "frac": round(achieved / PEAK_VALU_TOPS, 4),
+        "unit": "Tops/s int32 VALU lane-ops issued (SQ_INSTS_VALU x 64 per launch / kernel time)",
+        "frac": round(achieved / PEAK_VALU_TOPS, 4) if achieved is not None else None,
         "traffic": traffic,
         "algorithmic_bytes": int(N * 4 + edge_bytes),
         "kernel": f"k_pairs<{plan['predicate']} K={plan['k']}, {plan['lane_cols']} pairs/register, "
-                  f"{plan['slots']} slots>",
+                  f"{plan['slots']} slots{', row filter' if plan['row_filter'] else ''}>",
         "tile": {"cols_per_lane": plan["slots"] * plan["lane_cols"], "rows_per_tile": plan["tile_rows"],
                  "cols_per_tile": plan["tile_cols"], "workgroups": plan["tiles"]},
         "kernel_ms": round(kern_ms, 4),
         "pairs_per_launch": pairs,
-        "algorithmic_ops_per_pair": myers_ops,
-        "issued": issued,
+        "valu_instr_per_pair": round(rec["sq_insts_valu"] * 64 / pairs, 3) if rec else None,
+        "row_window_fraction_processed": rec.get("row_window_fraction") if rec else None,
+        "counters": rec["source"] if rec else "no committed counters for this launch shape",
+        "myers_equivalent_x": round(myers_x, 2),
+        "myers_equivalent_note": "pairs x 17n ops (one Myers column step per symbol) / kernel time / VALU peak: "
+                                 "the algorithmic gain of Landau-Vishkin on thermometer codes, two pairs per "
+                                 "register and the row filter, not a fraction of peak",
         "edge_write_GBs": round(edge_bytes / (kern_ms / 1e3) / 1e9, 3),
         "autotune": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in e.items()} for e in tune],
     }
@@ -512,11 +620,15 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic",
-        "config": {"workload": args.workload, "n": n, "edit_dist": min_ed, "candidates": N, "pairs": pairs,
-                   "edges": dcsr.n_edges, "matrix_ones": 2 * dcsr.n_edges},
-        "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 4),
-                       "p99": round(float(np.percentile(step_ms, 99)), 4)},
+        "config": {"workload": args.workload, "description": desc, "n": n, "edit_dist": min_ed, "candidates": N,
+                   "pairs": pairs, "edges": dcsr.n_edges, "matrix_ones": 2 * dcsr.n_edges,
+                   "step": "pinned host symbols -> device copy -> pack -> row filter -> pairs -> CSR in HBM"},
+        "verify": verify,
+        "latency_ms": None,
+        "step_latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 4),
+                            "p99": round(float(np.percentile(step_ms, 99)), 4)},
         "breakdown_ms": {"pairs_kernel": round(kern_ms, 4), "csr": round(float(np.mean(csr_ms)), 4),
+                         "step_symbols_in_hbm": round(float(np.median(dev_ms)), 4),
                          "d2h_csr": round(d2h_ms, 2), "greedy_host": round(greedy_ms, 2),
                          "codewords": int(order.size)},
         "roofline": roofline,
@@ -525,24 +637,36 @@ def main():
                       "speedup": round(float(np.mean(unf_ms)) / kern_ms, 3),
                       "note": "exact row skipping by half-string symbol counts; same graph"},
     }
+    del d_syms
+    torch.cuda.synchronize()
+    T = host_threads()
     if args.serving_jobs > 0:
-        result["serving"] = serving_block(args.serving_jobs, host_threads=host_threads())
-        result["step_latency_ms"] = result["latency_ms"]
+        result["serving"] = serving_block(args.serving_jobs, host_threads=min(T, 16))
         result["latency_ms"] = {k: result["serving"]["latency_ms"][k] for k in ("p50", "p99")}
-        result["latency_source"] = "serving stream job latency (arrival to codebook); step_latency_ms is one build"
+        result["latency_source"] = (f"serving stream job latency, arrival to codebook, all "
+                                    f"{args.serving_jobs} jobs; step_latency_ms is one headline build")
     if args.saturated_jobs > 0:
-        result["serving_saturated"] = saturated_block(args.saturated_jobs, host_threads=host_threads())
+        result["serving_saturated"] = saturated_block(args.saturated_jobs, host_threads=min(T, 16))
     if not args.no_batched:
         result["batched"] = batched_block(ctx, torch)
         result["many_small_sets"] = batched_block(ctx, torch, steps=5, seed=4, sets_n=256, scale=0.08,
                                                   label="per-cluster sets")
-        result["many_small_sets"]["reference_form"] = process_per_set_block(
-            result["many_small_sets"]["device_resident"]["batched_ms"])
+        if not args.no_reference_forms:
+            result["many_small_sets"]["reference_form"] = process_per_set_block(
+                result["many_small_sets"]["host_api"]["batched_ms"], seed=4, sets_n=256, scale=0.08)
+            result["batched"]["reference_form"] = process_per_set_block(
+                result["batched"]["host_api"]["batched_ms"], seed=3, sets_n=64, scale=1.0)
     if not args.no_refine:
-        result["refine"] = refine_block(ctx, host_threads=host_threads())
+        result["refine"] = refine_block(ctx, host_threads=min(T, 16))
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(syms, min_ed, args.cpu_seconds)
+        log(f"cpu baseline: reference FillAdjList on all {N} candidates with {T} threads")
+        result["cpu_baseline"] = cpu_baseline(syms, min_ed, 2 * dcsr.n_edges)
+        if not args.no_cpu_configs:
+            result["cpu_configs"] = cpu_configs_block(T)
     print(json.dumps(result), flush=True)
+    if verify.get("checked") and not verify["ok"]:
+        log("the built graph or codebook differs from the reference's")
+        sys.exit(3)
 
 
 if __name__ == "__main__":

@@ -72,6 +72,11 @@ extern "C" {
 /* CSR of the upper triangle only: row i lists its neighbours j > i, ascending
  * (|E| entries instead of 2|E|; the pair-file order of the reference) */
 #define IGC_UPPER_TRIANGLE 1u
+/* igc_graph_build: leave the CSR in device memory owned by the context (as
+ * igc_graph_build_device does) instead of copying it to the host; the
+ * symbols still come from host memory (pinned memory makes the copy an
+ * asynchronous DMA).  Ascending order only. */
+#define IGC_RESULT_ON_DEVICE 2u
 
 /* where a CSR's arrays live */
 #define IGC_ON_HOST 0u
@@ -141,6 +146,15 @@ IGC_API int igc_ctx_set_predicate(igc_ctx *ctx, int predicate);
 #define IGC_CSR_AUTO 0
 #define IGC_CSR_WIDE_KEYS 1
 IGC_API int igc_ctx_set_csr_form(igc_ctx *ctx, int form);
+/* Capacity of a build's first edge buffer, in edges (0, the default: twice
+ * the expected edge count of random strings of that length and threshold, or
+ * 2^26 edges where no estimate exists).  A build whose graph has more edges
+ * regrows the buffer to the exact count and runs its pair kernels again, so
+ * the result never depends on it (igc_timing.retries counts the regrowths).
+ * For tests of that path and for callers that know their graph's density.
+ * No counterpart in the reference (its per-tile edge buffer is fixed,
+ * src/cuda_edit_distance.py:320). */
+IGC_API int igc_ctx_set_edge_cap(igc_ctx *ctx, uint64_t edges);
 IGC_API int igc_last_timing(igc_ctx *ctx, igc_timing *out);
 
 /* ---- single candidate set ---- */
@@ -392,8 +406,12 @@ typedef struct igc_job_result {
     uint32_t *order;
     uint64_t n_edges;
     double submit_s, admit_s, gpu_done_s, done_s;
+    double greedy_start_s; /* a greedy worker took the job's CSR */
 } igc_job_result;
 
+/* in_use_bytes: HBM the pipelines have reserved now (their buffers plus the
+ * running batches' estimates); max_in_use_bytes: the highest it has been.
+ * Both stay within the budget, and the buffers within them. */
 typedef struct igc_server_stats {
     uint64_t batches, jobs_done, pairs_done, queued;
     uint64_t in_use_bytes, max_in_use_bytes, max_batch_jobs;
@@ -407,11 +425,15 @@ typedef struct igc_server_stats {
  * all jobs admitted together run as one batched build. */
 IGC_API int igc_server_create(int device, uint64_t hbm_budget_bytes, uint32_t pipelines, uint32_t host_threads,
                               igc_server **out);
-/* Queues a codebook job (the candidates are copied); returns at once. */
+/* Queues a codebook job (the candidates are copied); returns at once.  The
+ * symbols are checked here: codes above 3 (above 1 for n > 32) give
+ * IGC_ERR_INVALID and queue nothing. */
 IGC_API int igc_server_submit(igc_server *srv, const uint8_t *syms, uint32_t N, uint32_t n, uint32_t edit_dist,
                               uint64_t *job_id);
 /* Waits up to timeout_ms (< 0: forever) for a job; IGC_ERR_BUSY on timeout.
- * On success the job is retired and its result handed over. */
+ * On success the job is retired and its result handed over; when the job
+ * itself failed (out->status != IGC_OK), igc_last_error() in the waiting
+ * thread gives its message. */
 IGC_API int igc_server_wait(igc_server *srv, uint64_t job_id, int64_t timeout_ms, igc_job_result *out);
 IGC_API int igc_server_poll(igc_server *srv, uint64_t job_id, igc_job_result *out);
 IGC_API int igc_job_free(igc_job_result *res);
@@ -433,7 +455,13 @@ IGC_API int igc_greedy(const igc_csr *csr, uint32_t *order, uint32_t *count);
  * read by AdjList::FromBinaryFile, src/SparseMat.cpp:338-390). */
 IGC_API int igc_write_pair_file(const igc_csr *csr, const char *path);
 
-/* Header values of the reference's codebook file (src/Utils.cpp:311-379). */
+/* Header values of the reference's codebook file (src/Utils.cpp:311-379).
+ * A clustered run (cluster_k > 0, the reference's clustering.enabled) writes
+ * the clustering lines instead of the ones count, and when it converged
+ * (cluster_iterations > 0) the clustering and solving times in place of the
+ * fill and process times, fill_s then holding the total clustering time and
+ * process_s the total solving time (src/Utils.cpp:345-368,
+ * src/SparseMat.cpp:1100-1122). */
 typedef struct igc_codebook_info {
     int code_len;
     int min_ed;
@@ -448,6 +476,10 @@ typedef struct igc_codebook_info {
     double fill_s;
     double process_s;
     double overall_s;
+    int cluster_k;              /* clustering.k; <= 0: not a clustered run */
+    int cluster_iterations;     /* rounds until convergence (0: did not converge) */
+    int convergence_iterations; /* clustering.convergenceIterations */
+    const char *cluster_method; /* clustering.method; NULL: "" */
 } igc_codebook_info;
 
 /* Writes dir/CodeSize-%07d_CodeLen-%02d_MinED-%d.txt (src/Utils.cpp:381-418):
@@ -459,11 +491,6 @@ IGC_API int igc_write_codebook(const char *dir, const uint8_t *syms, uint32_t n,
 
 /* ---- diagnostics ---- */
 
-/* Issue rate of the pair kernel's int32 instruction types (xor, lshrrev, ffbl,
- * max3, or3, add3, add), each in independent chains over the whole GPU:
- * wave-level VALU instructions per second, 7 values.  No reference
- * counterpart: it measures the roofline the pair kernel is compared with. */
-IGC_API int igc_valu_probe(igc_ctx *ctx, double *wave_instr_per_s);
 IGC_API const char *igc_last_error(void);
 /* "hip:gfx950" when a device context can be created, else "none". */
 IGC_API const char *igc_backend(void);

@@ -37,6 +37,7 @@ IGC_ERR_BUSY = 6
 IGC_MAX_LEN = 64  # strings longer than IGC_MAX_QUATERNARY_LEN must be binary (symbols 0/1)
 IGC_MAX_QUATERNARY_LEN = 32
 IGC_UPPER_TRIANGLE = 1  # flag: CSR rows list only neighbours above the row
+IGC_RESULT_ON_DEVICE = 2  # flag: igc_graph_build leaves the CSR in HBM
 IGC_PREFILTER_OFF, IGC_PREFILTER_AUTO, IGC_PREFILTER_ALWAYS = 0, 1, 2
 IGC_PREFILTER_MIN_N = 16384  # smallest set the automatic mode sorts
 IGC_CSR_AUTO, IGC_CSR_WIDE_KEYS = 0, 1
@@ -127,6 +128,7 @@ class _JobResult(ctypes.Structure):
         ("admit_s", ctypes.c_double),
         ("gpu_done_s", ctypes.c_double),
         ("done_s", ctypes.c_double),
+        ("greedy_start_s", ctypes.c_double),
     ]
 
 
@@ -177,6 +179,10 @@ class _CodebookInfo(ctypes.Structure):
         ("fill_s", ctypes.c_double),
         ("process_s", ctypes.c_double),
         ("overall_s", ctypes.c_double),
+        ("cluster_k", ctypes.c_int),
+        ("cluster_iterations", ctypes.c_int),
+        ("convergence_iterations", ctypes.c_int),
+        ("cluster_method", ctypes.c_char_p),
     ]
 
 
@@ -229,8 +235,8 @@ def lib() -> ctypes.CDLL:
         "igc_ctx_set_prefilter": (i, [vp, i]),
         "igc_ctx_set_predicate": (i, [vp, i]),
         "igc_ctx_set_csr_form": (i, [vp, i]),
+        "igc_ctx_set_edge_cap": (i, [vp, u64]),
         "igc_assign_fp32": (i, [vp, vp, u32, u32, vp, u32, vp]),
-        "igc_valu_probe": (i, [vp, vp]),
         "igc_gen_linear": (i, [vp, vp, u32, u32, vp, ctypes.POINTER(_Filter), ctypes.POINTER(_Cands)]),
         "igc_gen_vt": (i, [vp, u32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(_Filter), ctypes.POINTER(_Cands)]),
         "igc_gen_diff_vt": (i, [vp, u32, ctypes.c_int32, ctypes.POINTER(_Filter), ctypes.POINTER(_Cands)]),
@@ -442,6 +448,23 @@ class Context:
         _check(lib().igc_graph_build(self._h, s.ctypes.data, N, n, int(min_ed), int(flags), ctypes.byref(c)))
         return _adj_from_c(c)
 
+    def graph_build_to_device(self, syms, min_ed: int, n: int | None = None, N: int | None = None,
+                              flags: int = 0) -> DeviceCSR:
+        """Symbols from host memory (a numpy array, or the address of N x n
+        bytes, e.g. a pinned torch tensor's data_ptr()), CSR left in HBM
+        (igc_graph_build with IGC_RESULT_ON_DEVICE)."""
+        if isinstance(syms, int):
+            ptr, N, n = syms, int(N), int(n)
+        else:
+            s = to_symbols(syms)
+            N, n = s.shape
+            ptr = s.ctypes.data
+        c = _Csr()
+        _check(lib().igc_graph_build(self._h, ctypes.c_void_p(ptr), N, n, int(min_ed),
+                                     int(flags) | IGC_RESULT_ON_DEVICE, ctypes.byref(c)))
+        return DeviceCSR(int(c.n_nodes), int(c.n_edges), ctypes.cast(c.row_ptr, ctypes.c_void_p).value or 0,
+                         ctypes.cast(c.col, ctypes.c_void_p).value or 0, c)
+
     def graph_build_device(self, d_syms_ptr: int, N: int, n: int, min_ed: int) -> DeviceCSR:
         c = _Csr()
         _check(lib().igc_graph_build_device(self._h, ctypes.c_void_p(d_syms_ptr), int(N), int(n), int(min_ed),
@@ -604,6 +627,12 @@ class Context:
         fit) or IGC_CSR_WIDE_KEYS (64-bit keys); the same CSR either way."""
         _check(lib().igc_ctx_set_csr_form(self._h, int(form)))
 
+    def set_edge_cap(self, edges: int) -> None:
+        """First edge buffer of a build, in edges (0: the density estimate); a
+        graph with more edges regrows it to the exact count and replays the pair
+        kernels (last_timing().retries), with the same result."""
+        _check(lib().igc_ctx_set_edge_cap(self._h, int(edges)))
+
     def pair_plan(self, n: int, min_ed: int, N: int) -> dict:
         """The pair launch this context makes for N strings at (n, min_ed)."""
         p = _PairPlan()
@@ -611,12 +640,6 @@ class Context:
         return {"predicate": PREDICATES[p.predicate], "k": p.k, "lane_cols": p.lane_cols, "slots": p.slots,
                 "q": p.q, "tile_cols": p.tile_cols, "tile_rows": p.tile_rows, "tiles": int(p.tiles),
                 "row_filter": bool(p.row_filter)}
-
-    def valu_probe(self) -> dict:
-        """Wave-level VALU instructions per second for each int32 op type of the pair kernel."""
-        out = np.zeros(7, dtype=np.float64)
-        _check(lib().igc_valu_probe(self._h, out.ctypes.data))
-        return dict(zip(["xor", "lshrrev", "ffbl", "max3", "or3", "add3", "add"], out.tolist()))
 
     def graph_count(self, syms, min_ed: int) -> int:
         s = to_symbols(syms)
@@ -666,10 +689,11 @@ class Server:
         order = (np.ctypeslib.as_array(r.order, shape=(r.codewords,)).copy() if r.codewords
                  else np.zeros(0, dtype=np.uint32))
         lib().igc_job_free(ctypes.byref(r))
-        return {"id": int(r.id), "status": int(r.status), "order": order, "n_nodes": int(r.n_nodes),
+        err = lib().igc_last_error().decode() if r.status else ""
+        return {"id": int(r.id), "status": int(r.status), "error": err, "order": order, "n_nodes": int(r.n_nodes),
                 "n_edges": int(r.n_edges), "submit_s": r.submit_s, "admit_s": r.admit_s,
-                "gpu_done_s": r.gpu_done_s, "done_s": r.done_s, "batch": int(r.batch),
-                "batch_jobs": int(r.batch_jobs)}
+                "gpu_done_s": r.gpu_done_s, "greedy_start_s": r.greedy_start_s, "done_s": r.done_s,
+                "batch": int(r.batch), "batch_jobs": int(r.batch_jobs)}
 
     def stats(self) -> dict:
         x = _ServerInfo()
@@ -721,6 +745,33 @@ def greedy_order(adj: AdjList) -> np.ndarray:
     return order[: cnt.value].copy()
 
 
+def edges_sha256(adj: AdjList) -> str:
+    """SHA-256 of the graph's edge set as LE uint32 pairs (i, j), i < j, sorted
+    by (i, j) -- the reference's pair list in a canonical order.  Needs
+    ascending neighbour lists (the default order, or an upper-triangle CSR)."""
+    import hashlib
+    rp = adj.row_ptr.astype(np.int64)
+    N = rp.size - 1
+    rows = np.repeat(np.arange(N, dtype=np.uint32), np.diff(rp))
+    keep = adj.col > rows
+    pairs = np.empty((int(keep.sum()), 2), dtype="<u4")
+    pairs[:, 0] = rows[keep]
+    pairs[:, 1] = adj.col[keep]
+    return hashlib.sha256(pairs.tobytes()).hexdigest()
+
+
+def codebook_sha256(candidates, order) -> str:
+    """SHA-256 of the codebook as its words ('0'..'3') joined by newlines, in
+    selection order."""
+    import hashlib
+    syms = to_symbols(candidates)
+    words = np.ascontiguousarray(syms[np.asarray(order, dtype=np.int64)], dtype=np.uint8) + ord("0")
+    n = syms.shape[1]
+    lines = np.full((words.shape[0], n + 1), ord("\n"), dtype=np.uint8)
+    lines[:, :n] = words
+    return hashlib.sha256(lines.tobytes()[:-1] if words.shape[0] else b"").hexdigest()
+
+
 def codebook(adj: AdjList, candidates) -> list:
     """Codebook (src/SparseMat.cpp:679-731): codewords in selection order."""
     syms = to_symbols(candidates)
@@ -756,14 +807,18 @@ def write_pair_file(adj: AdjList, path: str) -> None:
 def write_codebook(directory: str, candidates, order: Sequence[int], *, code_len: int, min_ed: int,
                    max_run: int = 0, min_gc: float = 0.0, max_gc: float = 0.0, method_line: str | None = None,
                    candidate_num: int = 0, matrix_ones: int = 0, threads: int = 1, candidates_s: float = 0.0,
-                   fill_s: float = 0.0, process_s: float = 0.0, overall_s: float = 0.0) -> str:
+                   fill_s: float = 0.0, process_s: float = 0.0, overall_s: float = 0.0, cluster_k: int = -1,
+                   cluster_iterations: int = -1, convergence_iterations: int = 3,
+                   cluster_method: str | None = None) -> str:
     """ToFile (src/Utils.cpp:381-418): writes CodeSize-..._CodeLen-.._MinED-..txt and
-    returns its path."""
+    returns its path.  cluster_k > 0 writes the clustered run's header
+    (fill_s = total clustering time, process_s = total solving time)."""
     syms = to_symbols(candidates)
     order = np.ascontiguousarray(order, dtype=np.uint32)
     info = _CodebookInfo(code_len, min_ed, max_run, min_gc, max_gc,
                          method_line.encode() if method_line else None, candidate_num, matrix_ones, threads,
-                         candidates_s, fill_s, process_s, overall_s)
+                         candidates_s, fill_s, process_s, overall_s, cluster_k, cluster_iterations,
+                         convergence_iterations, cluster_method.encode() if cluster_method else None)
     name = ctypes.create_string_buffer(128)
     _check(lib().igc_write_codebook(directory.encode(), syms.ctypes.data, syms.shape[1], order.ctypes.data,
                                     order.size, ctypes.byref(info), name, 128))

@@ -19,8 +19,7 @@ LIB = os.path.join(HERE, "libigc.so")
 CLI = os.path.join(HERE, "igc_cli")
 
 # device code and its host orchestration (hipcc)
-SOURCES = ["igc_runtime.hip", "igc_batch.hip", "igc_kmeans.hip", "igc_refine.hip", "igc_server.hip", "igc_probe.hip",
-           "igc_gen.hip"]
+SOURCES = ["igc_runtime.hip", "igc_batch.hip", "igc_kmeans.hip", "igc_refine.hip", "igc_server.hip", "igc_gen.hip"]
 # plain host C++ (greedy, output writers): g++, no HIP
 HOST_SOURCES = ["igc_host.cpp"]
 
@@ -43,31 +42,36 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: libigc.so cannot be built")
 
 
-def build(verbose: bool = False) -> str:
-    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    objs = []
-    for h in HOST_SOURCES:
-        obj = os.path.join(CSRC, os.path.splitext(h)[0] + ".host.o")
-        cmd = ["g++", "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-I", os.path.join(ROOT, "include"),
-               "-c", "-o", obj, os.path.join(CSRC, h)]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
-        objs.append(obj)
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-shared", "-I", os.path.join(ROOT, "include"), "-o", tmp, *srcs, "-x", "none", *objs,
-           "-lpthread"]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
+
+
+def build(verbose: bool = False) -> str:
+    """Compiles every translation unit (in parallel, one process each) and
+    links libigc.so, then the CLI; always rebuilds."""
+    from concurrent.futures import ThreadPoolExecutor
+    inc = ["-I", os.path.join(ROOT, "include")]
+    jobs = []
+    for h in HOST_SOURCES:
+        obj = os.path.join(CSRC, os.path.splitext(h)[0] + ".host.o")
+        jobs.append((obj, ["g++", "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden", *inc, "-c", "-o", obj,
+                           os.path.join(CSRC, h)]))
+    for src in SOURCES:
+        obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
+        jobs.append((obj, [hipcc(), *HIPCC_FLAGS, *inc, "-c", "-o", obj, os.path.join(CSRC, src)]))
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    with ThreadPoolExecutor(workers) as ex:
+        for f in [ex.submit(_run, cmd, verbose) for _, cmd in jobs]:
+            f.result()
+    tmp = LIB + ".tmp"
+    _run([hipcc(), *HIPCC_FLAGS, "-shared", "-o", tmp, *[o for o, _ in jobs], "-lpthread"], verbose)
     os.replace(tmp, LIB)
     cli_src = os.path.join(CSRC, "igc_cli.cpp")
     if os.path.exists(cli_src):
-        cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-o", CLI + ".tmp", cli_src,
-               "-L", HERE, "-ligc", "-Wl,-rpath,$ORIGIN", "-lpthread"]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        _run(["g++", "-O2", "-std=c++17", *inc, "-o", CLI + ".tmp", cli_src, "-L", HERE, "-ligc",
+              "-Wl,-rpath,$ORIGIN", "-lpthread"], verbose)
         os.replace(CLI + ".tmp", CLI)
     return LIB
 

@@ -269,7 +269,7 @@ static int plan_batch(igc_ctx *ctx, const igc_set *sets, uint32_t count, BatchPl
         P.order.push_back(s);
         P.desc.push_back(d);
     }
-    P.cap = std::max<uint64_t>((uint64_t)est_edges, 1);
+    P.cap = ctx->first_edge_cap ? ctx->first_edge_cap : std::max<uint64_t>((uint64_t)est_edges, 1);
     if (P.windows > 0xFFFFFFFFull) return set_error(IGC_ERR_INVALID, "batch has too many filter windows");
     return IGC_OK;
 }

@@ -4,7 +4,8 @@
 // src/Utils.cpp:394-418), optionally verify (VerifyDist GPU mode, :203-288).
 //
 //   igc_cli --cands FILE --min-ed D [--order-threads T] [--out DIR] [--verify]
-//           [--pair-file PATH] [--refine K --kmeans-iters I --rounds R --converge C]
+//           [--pair-file PATH] [--refine K --kmeans-iters I --rounds R --converge C
+//            --cluster-method NAME]
 //   igc_cli --cands FILE --min-ed D --pair-file PATH --graph-only
 //
 // --graph-only does what the reference's per-call GPU script does
@@ -47,6 +48,7 @@ int main(int argc, char **argv)
     int min_ed = -1, order_t = 1;
     bool verify = false, graph_only = false;
     uint32_t refine_k = 0, km_iters = 10, rounds = 5, converge = 3;
+    std::string cluster_method = "kmeans_fp32";
     for (int a = 1; a < argc; ++a) {
         std::string s = argv[a];
         auto next = [&]() -> const char * { return a + 1 < argc ? argv[++a] : ""; };
@@ -61,6 +63,7 @@ int main(int argc, char **argv)
         else if (s == "--kmeans-iters") km_iters = (uint32_t)std::atoi(next());
         else if (s == "--rounds") rounds = (uint32_t)std::atoi(next());
         else if (s == "--converge") converge = (uint32_t)std::atoi(next());
+        else if (s == "--cluster-method") cluster_method = next();
         else {
             std::cerr << "unknown argument " << s << std::endl;
             return 2;
@@ -108,16 +111,23 @@ int main(int argc, char **argv)
     uint32_t count = 0;
     long long ones = 0;
     double fill_s = 0, proc_s = 0;
+    int cluster_iters = -1;
     if (refine_k) {
         igc_refine_stats st;
         const unsigned hw = std::thread::hardware_concurrency();
         if (igc_refine(ctx, syms.data(), N, n, (uint32_t)min_ed, refine_k, km_iters, rounds, converge, 1,
                        hw ? hw : 4, order.data(), &count, &st) != IGC_OK)
             return die("igc_refine");
-        for (uint32_t r = 0; r < st.rounds; ++r)
+        for (uint32_t r = 0; r < st.rounds; ++r) {
             std::cout << "round " << r + 1 << ": " << st.round[r].candidates << " candidates, "
                       << st.round[r].clusters << " clusters -> " << st.round[r].codewords << " codewords" << std::endl;
-        fill_s = std::chrono::duration<double>(clk::now() - t0).count();
+            // the reference's clustered header: clustering time, then solving
+            // (graphs + greedy) time (src/SparseMat.cpp:1113-1118)
+            fill_s += st.round[r].cluster_s;
+            proc_s += st.round[r].graph_s + st.round[r].greedy_s;
+        }
+        // rounds to convergence; 0 when the round limit ended the loop first
+        cluster_iters = st.converged ? (int)st.rounds : 0;
     } else {
         igc_csr g;
         const uint32_t flags = order_t > 1 ? IGC_ORDER_THREADS(order_t) : IGC_ORDER_ASCENDING;
@@ -151,6 +161,13 @@ int main(int argc, char **argv)
     info.fill_s = fill_s;
     info.process_s = proc_s;
     info.overall_s = std::chrono::duration<double>(clk::now() - t0).count();
+    if (refine_k) {
+        info.cluster_k = (int)refine_k;
+        info.cluster_iterations = cluster_iters;
+        info.convergence_iterations = (int)converge;
+        info.cluster_method = cluster_method.c_str();
+        info.matrix_ones = 0;
+    }
     char name[128];
     if (igc_write_codebook(out.c_str(), syms.data(), n, order.data(), count, &info, name, sizeof name) != IGC_OK)
         return die("igc_write_codebook");

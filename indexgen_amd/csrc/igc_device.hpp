@@ -23,18 +23,81 @@
 
 namespace igc {
 
-// grow-only device buffer
+// Device-memory budget shared by several contexts (the server's pipelines).
+// Each context draws from it through its own MemSlot: `used` is what the
+// slot's buffers hold, `charged` what the slot has reserved (>= used).  The
+// sum of all charges never exceeds the limit, so neither do the buffers.
+struct MemPool {
+    std::mutex mu;
+    uint64_t limit = ~0ull;
+    uint64_t charged = 0; // sum over slots
+    uint64_t peak = 0;    // highest sum of charges
+};
+
+struct MemSlot {
+    MemPool *pool = nullptr;
+    uint64_t used = 0, charged = 0;
+
+    // Reserves at least `bytes` (and at least what the slot holds); false when
+    // the pool's limit would be exceeded.
+    bool reserve(uint64_t bytes)
+    {
+        std::lock_guard<std::mutex> lk(pool->mu);
+        const uint64_t want = std::max(bytes, used);
+        if (want <= charged) return true;
+        if (pool->charged + (want - charged) > pool->limit) return false;
+        pool->charged += want - charged;
+        charged = want;
+        pool->peak = std::max(pool->peak, pool->charged);
+        return true;
+    }
+    // A buffer of the slot grows by delta bytes: within the reservation, or
+    // by extending it when the pool allows.
+    bool grow(uint64_t delta)
+    {
+        std::lock_guard<std::mutex> lk(pool->mu);
+        if (used + delta > charged) {
+            const uint64_t need = used + delta - charged;
+            if (pool->charged + need > pool->limit) return false;
+            pool->charged += need;
+            charged += need;
+            pool->peak = std::max(pool->peak, pool->charged);
+        }
+        used += delta;
+        return true;
+    }
+    void shrink(uint64_t delta)
+    {
+        std::lock_guard<std::mutex> lk(pool->mu);
+        used -= std::min(used, delta);
+    }
+    // Gives back the reservation above what the buffers hold.
+    void settle()
+    {
+        std::lock_guard<std::mutex> lk(pool->mu);
+        pool->charged -= charged - used;
+        charged = used;
+    }
+};
+
+// grow-only device buffer; with a slot, every allocation is charged to it
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    MemSlot *slot = nullptr;
     int ensure(size_t want)
     {
         if (want <= bytes && p) return IGC_OK;
         release();
         const size_t sz = std::max<size_t>(want, 256);
+        if (slot && !slot->grow(sz))
+            return set_error(IGC_ERR_OOM, "device buffer of %zu bytes exceeds the HBM budget (%llu of %llu bytes "
+                                          "reserved)", sz, (unsigned long long)slot->pool->charged,
+                             (unsigned long long)slot->pool->limit);
         const hipError_t e = hipMalloc(&p, sz);
         if (e != hipSuccess) {
             p = nullptr;
+            if (slot) slot->shrink(sz);
             return set_error(IGC_ERR_OOM, "hipMalloc(%zu) failed: %s", sz, hipGetErrorString(e));
         }
         bytes = sz;
@@ -42,7 +105,10 @@ struct DevBuf {
     }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p) {
+            (void)hipFree(p);
+            if (slot) slot->shrink(bytes);
+        }
         p = nullptr;
         bytes = 0;
     }
@@ -91,6 +157,10 @@ struct Shape {
 TileGeom choose_geometry(uint32_t N, int cols_per_lane, uint32_t q);
 uint64_t tile_count(const TileGeom &G);
 Shape default_shape(uint32_t N);
+// First edge buffer (entries) for N strings of length n at threshold K: twice
+// the density estimate, or kUnknownDensityCap entries where there is none (an
+// overflow is regrown to the exact count and the pairs replayed).
+constexpr uint64_t kUnknownDensityCap = 1ull << 26;
 uint64_t initial_edge_cap(uint32_t N, uint32_t n, int K);
 int effective_k(uint32_t n, uint32_t edit_dist);
 uint32_t grid_for(uint64_t work, uint32_t per_block, uint32_t cap_blocks);
@@ -175,9 +245,48 @@ struct igc_ctx {
     // cpl 0 = not tuned
     igc::Shape tuned[igc::kPredVariants][8] = {};
     void *km = nullptr; // K-means buffers (igc_kmeans.hip)
+    // first edge buffer of a build, in entries (0: sized by the density
+    // estimate); a smaller one is regrown to the exact count and the pairs replayed
+    uint64_t first_edge_cap = 0;
+    // charge account of the context's device buffers when it draws from a
+    // shared budget (the server's pipelines); no pool = not charged
+    igc::MemSlot mem;
+
+    // every device buffer the context owns directly (not the K-means buffers)
+    template <class F> void for_each_devbuf(F f)
+    {
+        for (igc::DevBuf *b : {&syms, &packed, &edges, &misc, &row_ptr, &col, &bsum, &sets, &bounds, &srcs, &keys[0],
+                               &keys[1], &rcount, &roffset, &segmeta, &nbase, &feat, &feat_s, &nkey, &perm, &packed_s,
+                               &fitems, &fbias, &fcount, &fsets, &gen_out, &gen_cnt, &gen_off, &gen_rows})
+            f(*b);
+        for (igc::DevBuf &b : corder) f(b);
+    }
 };
 
 namespace igc {
+// Sizes of the segmented CSR form (a split pass on the row's high hb bits,
+// then 32-bit passes over rowbits + dbits low bits) for an edge capacity cap.
+struct SegPlan {
+    uint32_t N = 0, dbits = 0, rowbits = 0, hb = 0, nseg = 0, lb = 0, passes = 0, bits = 0, Lmax = 0;
+    bool upper = false;
+    uint64_t cap = 0, items_cap = 0, tiles64 = 0, ub = 0;
+    size_t slots = 0;
+    size_t bytes_keys0 = 0, bytes_keys1 = 0, bytes_rcount = 0, bytes_roffset = 0, bytes_bsum = 0, bytes_segmeta = 0;
+};
+// Device buffers of the segmented form: the CSR (row_ptr N+1, col items_cap)
+// and its temporaries, each at least the plan's size.
+struct SegBufs {
+    uint64_t *row_ptr;
+    uint32_t *col;
+    uint32_t *keys0, *keys1, *rcount;
+    uint64_t *roffset, *bsum;
+    SegMeta *segmeta;
+};
+int seg_plan(uint32_t N, uint64_t cap, bool upper, uint32_t dbits, uint32_t rowbits, uint32_t hb, SegPlan &P);
+// Asynchronous on s; reads the edge count from counter (misc words 0..1) and
+// uses misc word 4.  Every buffer is written before it is read (no clearing).
+void launch_csr_seg_into(hipStream_t s, const SegPlan &P, const SegBufs &B, const uint64_t *edges,
+                         unsigned long long *counter, uint32_t *misc, uint32_t *launches, const uint32_t *base);
 // Builds the CSR of the edge list in ctx->edges (count in counter, capacity cap)
 // over N nodes into ctx->row_ptr / ctx->col; asynchronous on ctx->stream.
 int launch_csr(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *counter, uint32_t *misc,
@@ -187,11 +296,21 @@ Shape pick_shape(igc_ctx *ctx, uint32_t n, int K, uint32_t N);
 int filter_sets(igc_ctx *ctx, const uint8_t *d_syms, const void *packed, void *packed_s, uint2 *feat_s,
                 uint32_t *perm, uint4 *items, uint32_t *bias, uint32_t *count, std::vector<igc::FilterSet> &fs);
 
+// initial_edge_cap, or the context's forced first capacity (igc_ctx_set_edge_cap)
+uint64_t first_edge_cap(const igc_ctx *ctx, uint32_t N, uint32_t n, int K);
+
 int kmeans_attach(igc_ctx *ctx);
 void kmeans_release(igc_ctx *ctx);
 int kmeans_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n, uint32_t k, uint32_t iters,
                          uint64_t seed, bool use_graph, float *ms_out);
 int32_t *kmeans_assign_ptr(igc_ctx *ctx);
+
+// Charges every device buffer of ctx (present and future) to a slot of a
+// shared pool; buffers already held are charged at once.
+void ctx_attach_pool(igc_ctx *ctx, MemPool *pool);
+// Frees every device buffer of ctx (they are regrown on the next build) and
+// gives back the slot's reservation; the caller holds ctx->mu.
+void ctx_trim_locked(igc_ctx *ctx);
 
 // Device-to-host streaming through a ring of pinned staging chunks consumed on
 // worker threads (igc_runtime.hip); the caller holds the context lock and the

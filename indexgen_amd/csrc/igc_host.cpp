@@ -355,13 +355,29 @@ IGC_API int igc_write_codebook(const char *dir, const uint8_t *syms, uint32_t n,
     std::fprintf(f, "%s\n\n", info->method_line ? info->method_line : "Using Generation Method: External candidates");
     std::fprintf(f, "--- Results Summary ---\n");
     std::fprintf(f, "Number of Candidates:\t\t%lld\n", (long long)info->candidate_num);
-    std::fprintf(f, "Number of Ones in Matrix:\t%lld\n", (long long)info->matrix_ones);
+    const bool clustered = info->cluster_k > 0;
+    if (clustered) {
+        std::fprintf(f, "Clustering Method:\t\t%s\n", info->cluster_method ? info->cluster_method : "");
+        std::fprintf(f, "Number of Clusters (K):\t\t%d\n", info->cluster_k);
+        std::fprintf(f, "Required Identical Iterations:\t%d\n", info->convergence_iterations);
+        std::fprintf(f, "Iterations to Converge:\t\t%d\n", info->cluster_iterations);
+    } else {
+        std::fprintf(f, "Number of Ones in Matrix:\t%lld\n", (long long)info->matrix_ones);
+    }
     std::fprintf(f, "Number of Code Words:\t\t%u\n\n", count);
     std::fprintf(f, "--- Performance Metrics ---\n");
     std::fprintf(f, "Number of Threads:\t\t%d\n", info->threads);
     std::fprintf(f, "Candidate Generation Time:\t%.2f\tseconds\n", info->candidates_s);
-    std::fprintf(f, "Fill Adjacency List Time:\t%.2f\tseconds\n", info->fill_s);
-    std::fprintf(f, "Process Matrix Time:\t\t%.2f\tseconds\n", info->process_s);
+    if (clustered && info->cluster_iterations > 0) {
+        const double it = info->cluster_iterations;
+        std::fprintf(f, "Total Clustering Time:\t\t%.2f\tseconds\n", info->fill_s);
+        std::fprintf(f, "Avg Clustering Time/Iter:\t%.2f\tseconds\n", info->fill_s / it);
+        std::fprintf(f, "Total Solving Time:\t\t%.2f\tseconds\n", info->process_s);
+        std::fprintf(f, "Avg Solving Time/Iter:\t\t%.2f\tseconds\n", info->process_s / it);
+    } else {
+        std::fprintf(f, "Fill Adjacency List Time:\t%.2f\tseconds\n", info->fill_s);
+        std::fprintf(f, "Process Matrix Time:\t\t%.2f\tseconds\n", info->process_s);
+    }
     std::fprintf(f, "Overall Execution Time:\t\t%.2f\tseconds\n", info->overall_s);
     std::fprintf(f, "=========================================== \n");
     std::vector<char> line(n + 1);

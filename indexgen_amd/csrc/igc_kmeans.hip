@@ -302,8 +302,14 @@ void kmeans_release(igc_ctx *ctx)
 
 int kmeans_attach(igc_ctx *ctx)
 {
-    if (!ctx->km) ctx->km = new (std::nothrow) KmBufs();
-    return ctx->km ? IGC_OK : set_error(IGC_ERR_OOM, "host allocation failed");
+    if (!ctx->km) {
+        KmBufs *b = new (std::nothrow) KmBufs();
+        if (!b) return set_error(IGC_ERR_OOM, "host allocation failed");
+        if (ctx->mem.pool)
+            for (DevBuf *d : {&b->C, &b->cn2, &b->cnt, &b->size, &b->seeds, &b->assign}) d->slot = &ctx->mem;
+        ctx->km = b;
+    }
+    return IGC_OK;
 }
 
 int32_t *kmeans_assign_ptr(igc_ctx *ctx) { return reinterpret_cast<KmBufs *>(ctx->km)->assign.as<int32_t>(); }

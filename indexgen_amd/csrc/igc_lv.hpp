@@ -360,7 +360,7 @@ enum class PredKind : int { LV = 0, MYERS = 1, ALL = 2 };
 IGC_HD inline int word_class_of(uint32_t n) { return n <= 16 ? 0 : (n <= 32 ? 1 : 2); }
 
 // Femtoseconds per pair of the count-only pair kernel at its fastest tile
-// shape on MI355X, N = 65536 random strings (indexgen_amd/predicate_sweep.py):
+// shape on MI355X, N = 65536 random strings, each predicate forced in turn:
 // Landau-Vishkin per K for each word class (its cost does not depend on n),
 // Myers a least-squares line in n per word class (one step per symbol).
 IGC_HD inline uint32_t lv_fs(int wc, int K)

@@ -45,7 +45,10 @@ using namespace igc;
 namespace igc {
 
 // P(Levenshtein <= K) for uniformly random filtered 4-ary strings, measured
-// offline; used only to size the first edge buffer (the build is exact either way).
+// offline; used only to size the first edge buffer (the build is exact either
+// way).  Longer strings are sparser at the same K, so n > 16 takes the n = 16
+// row as an upper bound; elsewhere (n < 10, K > 5, binary strings) there is no
+// estimate (negative).
 static double density_prior(uint32_t n, int K)
 {
     static const double tab[7][6] = {
@@ -58,17 +61,24 @@ static double density_prior(uint32_t n, int K)
         {1e-7, 1e-7, 1.5e-6, 2.3e-5, 3.6e-4, 3.3e-3},   // n=15
         {1e-7, 1e-7, 1.5e-6, 7.5e-6, 1.2e-4, 1.3e-3},   // n=16
     };
-    if (n >= 10 && n <= 16 && K >= 0 && K <= 5) return tab[n - 10][K];
-    return 0.2;
+    if (K < 0 || K > 5 || n < 10 || n > IGC_MAX_QUATERNARY_LEN) return -1.0;
+    return tab[std::min<uint32_t>(n, 16) - 10][K];
 }
 
 uint64_t initial_edge_cap(uint32_t N, uint32_t n, int K)
 {
     const uint64_t pairs = (uint64_t)N * (N - 1) / 2;
-    if (pairs <= (1ull << 22)) return std::max<uint64_t>(pairs, 1);
-    double est = (double)pairs * density_prior(n, K) * 2.0 + (double)(1u << 20);
+    if (pairs <= (1ull << 22) || K >= (int)n) return std::max<uint64_t>(pairs, 1); // K >= n: every pair
+    const double prior = density_prior(n, K);
+    double est = prior >= 0 ? (double)pairs * prior * 2.0 + (double)(1u << 20)
+                            : (double)std::min<uint64_t>(pairs, kUnknownDensityCap);
     if (est > (double)pairs) est = (double)pairs;
     return (uint64_t)est;
+}
+
+uint64_t first_edge_cap(const igc_ctx *ctx, uint32_t N, uint32_t n, int K)
+{
+    return ctx->first_edge_cap ? ctx->first_edge_cap : initial_edge_cap(N, n, K);
 }
 
 TileGeom choose_geometry(uint32_t N, int cols_per_lane, uint32_t q)
@@ -279,77 +289,104 @@ void scan_counts(hipStream_t s, const uint32_t *counts, uint32_t L, uint64_t *bs
 
 // Segmented CSR (igc_csr.hpp): the split pass over the edge list, then 32-bit
 // passes over the low key bits.  rowbits + dbits <= 32 and the split is at
-// most kRadixMaxBits bits (the caller checks).
+// most kRadixMaxBits bits (the caller checks).  seg_plan sizes the buffers;
+// launch_csr_seg_into launches into buffers of at least those sizes.
+int seg_plan(uint32_t N, uint64_t cap, bool upper, uint32_t dbits, uint32_t rowbits, uint32_t hb, SegPlan &P)
+{
+    P = SegPlan{};
+    P.N = N;
+    P.cap = cap;
+    P.upper = upper;
+    P.dbits = dbits;
+    P.rowbits = rowbits;
+    P.hb = hb;
+    P.items_cap = (upper ? 1 : 2) * std::max<uint64_t>(cap, 1);
+    P.nseg = 1u << hb;
+    // split pass: 64-bit keys from the edge list, digit = the row's high bits
+    P.tiles64 = (P.items_cap + kRadixTile - 1) / kRadixTile;
+    P.ub = (P.items_cap + kSegTile - 1) / kSegTile + P.nseg; // tiles of all segments, at most
+    P.lb = rowbits + dbits;                                    // low key bits
+    P.passes = std::max<uint32_t>(1, (P.lb + kSegMaxBits - 1) / kSegMaxBits);
+    P.bits = (P.lb + P.passes - 1) / P.passes; // at most kSegMaxBits
+    if ((P.tiles64 << hb) > 0xFFFFFFFFull || (P.ub << P.bits) > 0xFFFFFFFFull)
+        return set_error(IGC_ERR_INVALID, "edge buffer of %llu entries is too large", (unsigned long long)cap);
+    const uint32_t L64 = (uint32_t)(P.tiles64 << hb);
+    const uint32_t L32 = (uint32_t)(P.ub << P.bits);
+    P.Lmax = std::max(L64, L32);
+    P.slots = (size_t)P.ub * kSegTile; // padded 32-bit key slots
+    P.bytes_keys0 = sizeof(uint32_t) * P.slots;
+    P.bytes_keys1 = sizeof(uint32_t) * std::max<size_t>(P.slots, P.items_cap);
+    P.bytes_rcount = sizeof(uint32_t) * (size_t)P.Lmax;
+    P.bytes_roffset = sizeof(uint64_t) * ((size_t)P.Lmax + 1);
+    P.bytes_bsum = sizeof(uint64_t) * (((size_t)P.Lmax + kScanTile - 1) / kScanTile);
+    P.bytes_segmeta = sizeof(SegMeta) * P.nseg;
+    return IGC_OK;
+}
+
+void launch_csr_seg_into(hipStream_t s, const SegPlan &P, const SegBufs &B, const uint64_t *edges,
+                         unsigned long long *counter, uint32_t *misc, uint32_t *launches, const uint32_t *base)
+{
+    uint32_t *total = misc + 4; // tiles of all segments, on the device
+    RadixPass rp{};
+    rp.edges = edges;
+    rp.cnt = counter;
+    rp.cap = P.cap;
+    rp.bsh = P.rowbits + P.hb;
+    rp.dbits = P.dbits;
+    rp.base = base;
+    rp.shift = P.lb;
+    rp.bits = P.hb;
+    rp.from_edges = 1;
+    rp.upper = P.upper ? 1u : 0u;
+    const uint32_t t64 = (uint32_t)P.tiles64;
+    const uint32_t L64 = (uint32_t)(P.tiles64 << P.hb);
+    hipLaunchKernelGGL(k_radix_count, dim3(t64), dim3(kRadixThreads), 0, s, rp, t64, B.rcount);
+    scan_counts(s, B.rcount, L64, B.bsum, B.roffset);
+    hipLaunchKernelGGL(k_seg_meta, dim3(1), dim3(512), 0, s, B.roffset, t64, P.nseg, B.segmeta, total);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(t64), dim3(kRadixThreads), 0, s, rp, t64, B.roffset, 0u, nullptr,
+                       nullptr, nullptr, B.segmeta, B.keys0);
+    *launches += 6;
+    // 32-bit passes, least significant digit first; the last writes col and src
+    for (uint32_t pass = 0; pass < P.passes; ++pass) {
+        const uint32_t shift = pass * P.bits;
+        const uint32_t pb = std::min(P.bits, P.lb - shift);
+        const uint32_t L = (uint32_t)(P.ub << pb);
+        const bool last = pass + 1 == P.passes;
+        uint32_t *in = (pass & 1) ? B.keys1 : B.keys0;
+        uint32_t *out = (pass & 1) ? B.keys0 : B.keys1; // keys, or src on the last pass
+        hipLaunchKernelGGL(k_seg_count, dim3((uint32_t)P.ub), dim3(kSegThreads), 0, s, in, B.segmeta, P.nseg, total,
+                           shift, pb, B.rcount);
+        scan_counts(s, B.rcount, L, B.bsum, B.roffset);
+        hipLaunchKernelGGL(k_seg_scatter, dim3((uint32_t)P.ub), dim3(kSegThreads), 0, s, in, B.segmeta, P.nseg,
+                           total, shift, pb, B.roffset, (uint32_t)last, out, P.dbits, P.rowbits, base, B.col, out);
+        *launches += 5;
+        if (last)
+            hipLaunchKernelGGL(k_csr_row_ptr, dim3(grid_for((uint64_t)P.N + 1, 256, 8192)), dim3(256), 0, s, out,
+                               counter, P.cap, P.upper ? 1u : 2u, P.N, B.row_ptr);
+    }
+    *launches += 1;
+}
+
 static int launch_csr_seg(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *counter, uint32_t *misc,
                           uint32_t *launches, bool upper, uint32_t dbits, uint32_t rowbits, uint32_t hb,
                           const uint32_t *base)
 {
-    hipStream_t s = ctx->stream;
+    SegPlan P;
     int rc;
-    const uint64_t items_cap = (upper ? 1 : 2) * std::max<uint64_t>(cap, 1);
-    const uint32_t nseg = 1u << hb;
-    // split pass: 64-bit keys from the edge list, digit = the row's high bits
-    const uint64_t tiles64 = (items_cap + kRadixTile - 1) / kRadixTile;
-    const uint64_t ub = (items_cap + kSegTile - 1) / kSegTile + nseg; // tiles of all segments, at most
-    const uint32_t lb = rowbits + dbits;                                // low key bits
-    const uint32_t passes = std::max<uint32_t>(1, (lb + kSegMaxBits - 1) / kSegMaxBits);
-    const uint32_t bits = (lb + passes - 1) / passes; // at most kSegMaxBits
-    if ((tiles64 << hb) > 0xFFFFFFFFull || (ub << bits) > 0xFFFFFFFFull)
-        return set_error(IGC_ERR_INVALID, "edge buffer of %llu entries is too large", (unsigned long long)cap);
-    const uint32_t L64 = (uint32_t)(tiles64 << hb);
-    const uint32_t L32 = (uint32_t)(ub << bits);
-    const uint32_t Lmax = std::max(L64, L32);
-    const size_t slots = (size_t)ub * kSegTile; // padded 32-bit key slots
+    if ((rc = seg_plan(N, cap, upper, dbits, rowbits, hb, P))) return rc;
     if ((rc = ctx->row_ptr.ensure(sizeof(uint64_t) * ((size_t)N + 1)))) return rc;
-    if ((rc = ctx->col.ensure(sizeof(uint32_t) * items_cap))) return rc;
-    if ((rc = ctx->keys[0].ensure(sizeof(uint32_t) * slots))) return rc;
-    if ((rc = ctx->keys[1].ensure(sizeof(uint32_t) * std::max<size_t>(slots, items_cap)))) return rc;
-    if ((rc = ctx->rcount.ensure(sizeof(uint32_t) * (size_t)Lmax))) return rc;
-    if ((rc = ctx->roffset.ensure(sizeof(uint64_t) * ((size_t)Lmax + 1)))) return rc;
-    if ((rc = ctx->bsum.ensure(sizeof(uint64_t) * (((size_t)Lmax + kScanTile - 1) / kScanTile)))) return rc;
-    if ((rc = ctx->segmeta.ensure(sizeof(SegMeta) * nseg))) return rc;
-    uint32_t *counts = ctx->rcount.as<uint32_t>();
-    uint64_t *offsets = ctx->roffset.as<uint64_t>();
-    SegMeta *meta = ctx->segmeta.as<SegMeta>();
-    uint32_t *total = misc + 4; // tiles of all segments, on the device
-    RadixPass rp{};
-    rp.edges = ctx->edges.as<uint64_t>();
-    rp.cnt = counter;
-    rp.cap = cap;
-    rp.bsh = rowbits + hb;
-    rp.dbits = dbits;
-    rp.base = base;
-    rp.shift = lb;
-    rp.bits = hb;
-    rp.from_edges = 1;
-    rp.upper = upper ? 1u : 0u;
-    const uint32_t t64 = (uint32_t)tiles64;
-    hipLaunchKernelGGL(k_radix_count, dim3(t64), dim3(kRadixThreads), 0, s, rp, t64, counts);
-    scan_counts(s, counts, L64, ctx->bsum.as<uint64_t>(), offsets);
-    hipLaunchKernelGGL(k_seg_meta, dim3(1), dim3(512), 0, s, offsets, t64, nseg, meta, total);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(t64), dim3(kRadixThreads), 0, s, rp, t64, offsets, 0u, nullptr, nullptr,
-                       nullptr, meta, ctx->keys[0].as<uint32_t>());
-    *launches += 6;
-    // 32-bit passes, least significant digit first; the last writes col and src
-    for (uint32_t pass = 0; pass < passes; ++pass) {
-        const uint32_t shift = pass * bits;
-        const uint32_t pb = std::min(bits, lb - shift);
-        const uint32_t L = (uint32_t)(ub << pb);
-        const bool last = pass + 1 == passes;
-        uint32_t *in = ctx->keys[pass & 1].as<uint32_t>();
-        uint32_t *out = ctx->keys[(pass + 1) & 1].as<uint32_t>(); // keys, or src on the last pass
-        hipLaunchKernelGGL(k_seg_count, dim3((uint32_t)ub), dim3(kSegThreads), 0, s, in, meta, nseg, total, shift, pb,
-                           counts);
-        scan_counts(s, counts, L, ctx->bsum.as<uint64_t>(), offsets);
-        hipLaunchKernelGGL(k_seg_scatter, dim3((uint32_t)ub), dim3(kSegThreads), 0, s, in, meta, nseg, total, shift,
-                           pb, offsets, (uint32_t)last, out, dbits, rowbits, base, ctx->col.as<uint32_t>(), out);
-        *launches += 5;
-        if (last)
-            hipLaunchKernelGGL(k_csr_row_ptr, dim3(grid_for((uint64_t)N + 1, 256, 8192)), dim3(256), 0, s, out,
-                               counter, cap, upper ? 1u : 2u, N, ctx->row_ptr.as<uint64_t>());
-    }
+    if ((rc = ctx->col.ensure(sizeof(uint32_t) * P.items_cap))) return rc;
+    if ((rc = ctx->keys[0].ensure(P.bytes_keys0))) return rc;
+    if ((rc = ctx->keys[1].ensure(P.bytes_keys1))) return rc;
+    if ((rc = ctx->rcount.ensure(P.bytes_rcount))) return rc;
+    if ((rc = ctx->roffset.ensure(P.bytes_roffset))) return rc;
+    if ((rc = ctx->bsum.ensure(P.bytes_bsum))) return rc;
+    if ((rc = ctx->segmeta.ensure(P.bytes_segmeta))) return rc;
+    SegBufs B{ctx->row_ptr.as<uint64_t>(), ctx->col.as<uint32_t>(), ctx->keys[0].as<uint32_t>(),
+              ctx->keys[1].as<uint32_t>(), ctx->rcount.as<uint32_t>(), ctx->roffset.as<uint64_t>(),
+              ctx->bsum.as<uint64_t>(), ctx->segmeta.as<SegMeta>()};
+    launch_csr_seg_into(ctx->stream, P, B, ctx->edges.as<uint64_t>(), counter, misc, launches, base);
     HIP_TRY(hipGetLastError());
-    *launches += 1;
     return IGC_OK;
 }
 
@@ -553,7 +590,7 @@ static int build_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, 
     uint32_t *misc = ctx->misc.as<uint32_t>();
     unsigned long long *counter = reinterpret_cast<unsigned long long *>(misc);
 
-    uint64_t cap = count_only ? 0 : initial_edge_cap(N, n, K);
+    uint64_t cap = count_only ? 0 : first_edge_cap(ctx, N, n, K);
     for (int attempt = 0; attempt < 3; ++attempt) {
         if (!count_only) {
             if ((rc = ctx->edges.ensure(sizeof(uint64_t) * (size_t)cap))) return rc;
@@ -681,6 +718,28 @@ int d2h_stream(igc_ctx *ctx, const void *src, size_t bytes, const D2HConsumer &c
     return IGC_OK;
 }
 
+void ctx_attach_pool(igc_ctx *ctx, MemPool *pool)
+{
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->mem.pool = pool;
+    uint64_t held = 0;
+    ctx->for_each_devbuf([&](DevBuf &b) {
+        b.slot = &ctx->mem;
+        held += b.p ? b.bytes : 0;
+    });
+    ctx->mem.grow(held); // buffers made before the attachment (none on a fresh context)
+    kmeans_release(ctx);
+}
+
+void ctx_trim_locked(igc_ctx *ctx)
+{
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->for_each_devbuf([](DevBuf &b) { b.release(); });
+    kmeans_release(ctx);
+    if (ctx->mem.pool) ctx->mem.settle();
+}
+
 } // namespace igc
 
 // ---------------------------------------------------------------------------
@@ -741,13 +800,7 @@ IGC_API int igc_ctx_destroy(igc_ctx *ctx)
         std::lock_guard<std::mutex> lk(ctx->mu);
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
-        for (DevBuf *b : {&ctx->syms, &ctx->packed, &ctx->edges, &ctx->misc, &ctx->row_ptr, &ctx->col, &ctx->keys[0],
-                          &ctx->keys[1], &ctx->rcount, &ctx->roffset, &ctx->segmeta, &ctx->nbase, &ctx->bsum,
-                          &ctx->sets, &ctx->bounds, &ctx->srcs,
-                          &ctx->feat, &ctx->feat_s, &ctx->nkey, &ctx->perm, &ctx->packed_s, &ctx->fitems, &ctx->fbias,
-                          &ctx->fcount, &ctx->fsets, &ctx->gen_out, &ctx->gen_cnt, &ctx->gen_off, &ctx->gen_rows})
-            b->release();
-        for (DevBuf &b : ctx->corder) b.release();
+        ctx->for_each_devbuf([](DevBuf &b) { b.release(); });
         for (HostBuf *b : {&ctx->host_rp, &ctx->stage}) b->release();
         kmeans_release(ctx);
         for (auto &e : ctx->ev)
@@ -759,6 +812,7 @@ IGC_API int igc_ctx_destroy(igc_ctx *ctx)
             if (ctx->aux[a]) (void)hipStreamDestroy(ctx->aux[a]);
         }
         if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+        if (ctx->mem.pool) ctx->mem.settle();
     }
     delete ctx;
     return IGC_OK;
@@ -802,6 +856,14 @@ IGC_API int igc_ctx_set_csr_form(igc_ctx *ctx, int form)
         return set_error(IGC_ERR_INVALID, "CSR form %d is not 0 or 1", form);
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->csr_form = form;
+    return IGC_OK;
+}
+
+IGC_API int igc_ctx_set_edge_cap(igc_ctx *ctx, uint64_t edges)
+{
+    if (!ctx) return set_error(IGC_ERR_INVALID, "null context");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->first_edge_cap = edges;
     return IGC_OK;
 }
 
@@ -901,6 +963,9 @@ IGC_API int igc_graph_build(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint3
     if (rc) return rc;
     if (!out || (!syms && N)) return set_error(IGC_ERR_INVALID, "null argument");
     std::memset(out, 0, sizeof *out);
+    const bool on_device = (flags & IGC_RESULT_ON_DEVICE) != 0;
+    if (on_device && ((flags >> 16) & 0xFFFFu) > 1)
+        return set_error(IGC_ERR_INVALID, "a thread order needs the host CSR (drop IGC_RESULT_ON_DEVICE)");
     std::lock_guard<std::mutex> lk(ctx->mu);
     HIP_TRY(hipSetDevice(ctx->device));
     const size_t bytes = (size_t)N * n;
@@ -910,6 +975,14 @@ IGC_API int igc_graph_build(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint3
     rc = build_device_locked(ctx, ctx->syms.as<uint8_t>(), N, n, edit_dist, &E, false,
                              (flags & IGC_UPPER_TRIANGLE) != 0);
     if (rc) return rc;
+    if (on_device) {
+        out->n_nodes = N;
+        out->location = IGC_ON_DEVICE;
+        out->n_edges = E;
+        out->row_ptr = ctx->row_ptr.as<uint64_t>();
+        out->col = ctx->col.as<uint32_t>();
+        return IGC_OK;
+    }
     return csr_to_host_locked(ctx, N, E, ctx->row_ptr.as<uint64_t>(), ctx->col.as<uint32_t>(), flags, out);
 }
 

@@ -1,22 +1,35 @@
 // igc_server.hip -- continuous-batching codebook server on one GPU.
 //
 // Jobs (candidate set + editDist) are submitted from any host thread and
-// queued FIFO.  Each GPU pipeline (its own context and stream; up to two, so
-// one batch's pair kernel overlaps the previous batch's CSR assembly and
-// copies) takes, as soon as it is free, every queued job that fits the HBM
-// budget -- the budget is shared by the batches in flight -- and builds their
-// conflict graphs in one batched build (igc_batch.hip).  The per-job CSRs
-// go to a pool of host threads running the greedy independent set, while the
-// pipeline already serves the next batch.  A job's latency is its completion
-// time minus its submission time.
+// queued FIFO; their symbols are checked at submission.  Each GPU pipeline
+// (its own context and stream; up to two, so one batch's pair kernel overlaps
+// the previous batch's CSR assembly and copies) takes, as soon as it is free,
+// every queued job that fits the HBM budget and builds their conflict graphs
+// in one batched build (igc_batch.hip).  The per-job CSRs go to a pool of host
+// threads running the greedy independent set, while the pipeline already
+// serves the next batch.  A job's latency is its completion time minus its
+// submission time.
+//
+// HBM budget.  Every device buffer of a pipeline context is charged to the
+// server's pool (MemPool): a pipeline reserves max(its buffers, the batch's
+// estimate) before a batch, a buffer that grows beyond the reservation extends
+// it only while the sum of all reservations stays within the budget, and
+// otherwise the allocation fails with IGC_ERR_OOM.  So the device memory the
+// pipelines hold never exceeds the budget, also when a job has far more edges
+// than estimated.  A batch that runs out is re-run job by job; a job that runs
+// out alone is re-run once with every pipeline's buffers freed and the whole
+// budget reserved, and fails with IGC_ERR_OOM only then.  After a batch an
+// idle pipeline keeps at most its share of the budget in buffers.
 //
 // Admission estimate per job: symbols + packed words + row pointers + the
 // per-string arrays of the row filter and the batch (counts 8 + 8, permutation
 // 4, bucket keys 4, set base 4, window items ~3, sorted words) + 48 B per edge
-// of the density prior's estimate (edge list 8, col 8, the radix sort's two
-// key buffers 32 in the 64-bit form; less in the 32-bit form).  A job whose estimate alone exceeds the budget is refused with
-// IGC_ERR_ADMISSION; otherwise FIFO order is kept (a job that does not fit
-// waits for memory; nothing behind it overtakes it).
+// of the density estimate (edge list 8, col 8, the radix sort's two key
+// buffers 32 in the 64-bit form; less in the 32-bit form).  A job whose
+// estimate alone exceeds the budget is refused with IGC_ERR_ADMISSION;
+// otherwise FIFO order is kept (a job that does not fit waits for memory;
+// nothing behind it overtakes it).  Host CSRs waiting for the greedy are
+// bounded too: no batch is admitted while they hold more than the host budget.
 //
 // The reference has no server: its clustered loop runs one
 // FillAdjListGPU process per cluster on worker threads
@@ -25,8 +38,11 @@
 
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
 #include <cstring>
 #include <deque>
+#include <string>
+#include <unistd.h>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -38,12 +54,15 @@ struct SJob {
     std::vector<uint8_t> syms;
     uint32_t N = 0, n = 0, ed = 0;
     uint64_t est = 0;
-    double t_submit = 0, t_admit = 0, t_gpu = 0, t_done = 0;
+    double t_submit = 0, t_admit = 0, t_gpu = 0, t_greedy = 0, t_done = 0;
     igc_csr csr{};
-    uint64_t n_edges = 0;
+    uint64_t n_edges = 0, host_bytes = 0;
     std::vector<uint32_t> order;
     int status = IGC_OK;
+    std::string err; // message of a failed job, handed to the waiting thread
     bool done = false;
+    bool solo = false;    // admitted alone (after its batch ran out of budget)
+    bool retried = false; // already re-run with the whole budget
     uint32_t batch = 0, batch_jobs = 0;
 };
 
@@ -54,13 +73,26 @@ static uint64_t job_bytes(uint32_t N, uint32_t n, uint32_t ed)
     return (uint64_t)N * n + 2ull * N * word_bytes(n) + ((uint64_t)N + 1) * 8 + (uint64_t)N * 32 + edges * 48;
 }
 
+// Host memory the CSRs waiting for the greedy may hold before the pipelines
+// stop admitting: a quarter of physical memory, at most 64 GiB.
+static uint64_t default_host_budget()
+{
+    const long pages = sysconf(_SC_PHYS_PAGES), psz = sysconf(_SC_PAGE_SIZE);
+    const uint64_t phys = pages > 0 && psz > 0 ? (uint64_t)pages * (uint64_t)psz : (64ull << 30);
+    return std::max<uint64_t>(1ull << 30, std::min<uint64_t>(64ull << 30, phys / 4));
+}
+
 } // namespace igc
 
 using namespace igc;
 
 struct igc_server {
     std::vector<igc_ctx *> ctx;
-    uint64_t budget = 0, in_use = 0;
+    std::vector<uint8_t> busy; // per pipeline: a batch is running
+    MemPool pool;              // the HBM budget, charged by every pipeline context's buffers
+    uint64_t budget = 0;
+    uint64_t host_budget = 0, host_pending = 0; // host CSR bytes waiting for the greedy
+    int exclusive = -1;        // pipeline re-running a job with the whole budget
     uint32_t max_batch = 512;
     std::mutex mu;
     std::condition_variable cv_queue, cv_greedy, cv_done;
@@ -71,95 +103,180 @@ struct igc_server {
     uint32_t gpu_alive = 0;
     uint64_t next_id = 1;
     uint32_t batches = 0;
-    uint64_t pairs_done = 0, jobs_done = 0, max_in_use = 0, max_batch_seen = 0;
+    uint64_t pairs_done = 0, jobs_done = 0, max_batch_seen = 0;
     double gpu_busy_s = 0;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 };
 
-static void gpu_loop(igc_server *S, igc_ctx *ctx)
+static void finish_job(igc_server *S, SJob *j, int status, const char *msg)
 {
+    j->status = status;
+    if (status) j->err = msg ? msg : "";
+    j->done = true;
+    j->t_done = S->now();
+}
+
+// Frees the buffers of every idle pipeline other than p (S->mu held; an idle
+// pipeline holds no context lock).  Returns whether any memory was released.
+static bool trim_idle_others(igc_server *S, uint32_t p)
+{
+    bool freed = false;
+    for (uint32_t q = 0; q < S->ctx.size(); ++q) {
+        if (q == p || S->busy[q] || S->ctx[q]->mem.used == 0) continue;
+        std::lock_guard<std::mutex> clk(S->ctx[q]->mu);
+        ctx_trim_locked(S->ctx[q]);
+        freed = true;
+    }
+    return freed;
+}
+
+// One batched build of the jobs on pipeline p's context (S->mu not held).
+static int run_batch(igc_server *S, igc_ctx *ctx, const std::vector<SJob *> &batch, std::vector<igc_csr> &outs,
+                     std::string &msg)
+{
+    std::vector<igc_set> sets(batch.size());
+    for (size_t t = 0; t < batch.size(); ++t) sets[t] = igc_set{batch[t]->syms.data(), batch[t]->N, batch[t]->n,
+                                                                batch[t]->ed, 0u};
+    outs.assign(batch.size(), igc_csr{});
+    std::lock_guard<std::mutex> clk(ctx->mu);
+    int rc = hipSetDevice(ctx->device) == hipSuccess
+                 ? batch_build_host_locked(ctx, sets.data(), (uint32_t)sets.size(), outs.data())
+                 : set_error(IGC_ERR_DEVICE, "hipSetDevice(%d) failed", ctx->device);
+    if (rc) msg = last_error();
+    return rc;
+}
+
+static void gpu_loop(igc_server *S, uint32_t p)
+{
+    igc_ctx *ctx = S->ctx[p];
+    MemSlot &slot = ctx->mem;
     std::unique_lock<std::mutex> lk(S->mu);
     for (;;) {
-        S->cv_queue.wait(lk, [&] { return S->stop || !S->queue.empty(); });
-        if (S->stop && S->queue.empty()) {
+        // admit when jobs wait, no pipeline runs alone, and the host CSRs
+        // waiting for the greedy leave room (a destroy drains the queue first)
+        S->cv_queue.wait(lk, [&] {
+            return (S->stop && S->queue.empty()) ||
+                   (!S->queue.empty() && S->exclusive < 0 &&
+                    (S->host_pending <= S->host_budget || S->host_pending == 0));
+        });
+        if (S->queue.empty()) {
             S->gpu_alive--;
             S->cv_greedy.notify_all();
             return;
         }
-        // admit FIFO while the batch fits the shared budget
+        // admit FIFO while the batch's estimate fits next to what the other
+        // pipelines have reserved
         std::vector<SJob *> batch;
         uint64_t bytes = 0;
         while (!S->queue.empty() && batch.size() < S->max_batch) {
             SJob *j = S->queue.front();
             if (j->est > S->budget) { // can never fit: refuse
                 S->queue.pop_front();
-                j->status = set_error(IGC_ERR_ADMISSION, "job %llu needs ~%llu bytes, budget %llu",
-                                      (unsigned long long)j->id, (unsigned long long)j->est,
-                                      (unsigned long long)S->budget);
-                j->done = true;
-                j->t_done = S->now();
+                char m[160];
+                std::snprintf(m, sizeof m, "job %llu needs ~%llu bytes of HBM, the budget is %llu",
+                              (unsigned long long)j->id, (unsigned long long)j->est, (unsigned long long)S->budget);
+                finish_job(S, j, IGC_ERR_ADMISSION, m);
                 S->cv_done.notify_all();
                 continue;
             }
-            if (S->in_use + bytes + j->est > S->budget) break;
+            if (j->solo && !batch.empty()) break;
+            if (!slot.reserve(bytes + j->est)) break;
             S->queue.pop_front();
             bytes += j->est;
             batch.push_back(j);
+            if (j->solo) break;
         }
         if (batch.empty()) {
-            // memory is held by the other pipeline; wait for it to release
-            S->cv_queue.wait(lk);
+            if (S->queue.empty()) continue;
+            // the head job does not fit next to the other pipelines' buffers:
+            // free idle ones, else wait for a running batch to end
+            if (trim_idle_others(S, p)) continue;
+            bool busy_other = false;
+            for (uint32_t q = 0; q < S->ctx.size(); ++q) busy_other |= q != p && S->busy[q];
+            if (!busy_other && slot.used) {
+                std::lock_guard<std::mutex> clk(ctx->mu);
+                ctx_trim_locked(ctx);
+                continue;
+            }
+            S->cv_queue.wait_for(lk, std::chrono::milliseconds(busy_other ? 1000 : 10));
             continue;
         }
-        S->in_use += bytes;
-        S->max_in_use = std::max(S->max_in_use, S->in_use);
+        S->busy[p] = 1;
         S->max_batch_seen = std::max<uint64_t>(S->max_batch_seen, batch.size());
         const uint32_t bno = ++S->batches;
         const double ta = S->now();
+        uint64_t pairs = 0;
         for (SJob *j : batch) {
             j->t_admit = ta;
             j->batch = bno;
             j->batch_jobs = (uint32_t)batch.size();
-        }
-        lk.unlock();
-
-        std::vector<igc_set> sets(batch.size());
-        uint64_t pairs = 0;
-        for (size_t t = 0; t < batch.size(); ++t) {
-            SJob *j = batch[t];
-            sets[t] = igc_set{j->syms.data(), j->N, j->n, j->ed, 0u};
             pairs += (uint64_t)j->N * (j->N > 0 ? j->N - 1 : 0) / 2;
         }
-        std::vector<igc_csr> outs(batch.size());
-        int rc;
-        {
-            std::lock_guard<std::mutex> clk(ctx->mu);
-            rc = hipSetDevice(ctx->device) == hipSuccess ? batch_build_host_locked(ctx, sets.data(),
-                                                                                  (uint32_t)sets.size(), outs.data())
-                                                         : IGC_ERR_DEVICE;
-        }
-        const double tg = S->now();
-
+        lk.unlock();
+        std::vector<igc_csr> outs;
+        std::string msg;
+        int rc = run_batch(S, ctx, batch, outs, msg);
         lk.lock();
-        S->in_use -= bytes;
-        S->gpu_busy_s += tg - ta;
-        S->pairs_done += pairs;
-        for (size_t t = 0; t < batch.size(); ++t) {
-            SJob *j = batch[t];
-            j->t_gpu = tg;
-            if (rc) {
-                j->status = rc;
-                j->done = true;
-                j->t_done = tg;
-            } else {
-                j->csr = outs[t];
-                j->n_edges = outs[t].n_edges;
-                j->syms.clear();
-                j->syms.shrink_to_fit();
-                S->greedy_q.push_back(j);
+        S->busy[p] = 0;
+        if (rc == IGC_ERR_OOM && batch.size() > 1) {
+            // over the budget: run these jobs one by one, in their order, ahead of the queue
+            for (auto it = batch.rbegin(); it != batch.rend(); ++it) {
+                (*it)->solo = true;
+                S->queue.push_front(*it);
+            }
+        } else if (rc == IGC_ERR_OOM && !batch[0]->retried) {
+            // one job over the budget next to the other pipelines: run it again
+            // with every pipeline's buffers freed and the whole budget reserved
+            SJob *j = batch[0];
+            j->retried = true;
+            S->cv_queue.wait(lk, [&] { return S->exclusive < 0; });
+            S->exclusive = (int)p;
+            S->cv_queue.wait(lk, [&] {
+                for (uint32_t q = 0; q < S->ctx.size(); ++q)
+                    if (q != p && S->busy[q]) return false;
+                return true;
+            });
+            trim_idle_others(S, p);
+            {
+                std::lock_guard<std::mutex> clk(ctx->mu);
+                ctx_trim_locked(ctx);
+            }
+            (void)slot.reserve(S->budget);
+            S->busy[p] = 1;
+            lk.unlock();
+            rc = run_batch(S, ctx, batch, outs, msg);
+            lk.lock();
+            S->busy[p] = 0;
+            S->exclusive = -1;
+        }
+        const bool requeued = rc == IGC_ERR_OOM && batch.size() > 1;
+        const double tg = S->now();
+        if (!requeued) {
+            S->gpu_busy_s += tg - ta;
+            if (!rc) S->pairs_done += pairs;
+            for (size_t t = 0; t < batch.size(); ++t) {
+                SJob *j = batch[t];
+                j->t_gpu = tg;
+                if (rc) {
+                    finish_job(S, j, rc, msg.c_str());
+                } else {
+                    j->csr = outs[t];
+                    j->n_edges = outs[t].n_edges;
+                    j->host_bytes = 8ull * ((uint64_t)j->N + 1) + 8ull * outs[t].n_edges;
+                    S->host_pending += j->host_bytes;
+                    j->syms.clear();
+                    j->syms.shrink_to_fit();
+                    S->greedy_q.push_back(j);
+                }
             }
         }
+        // an idle pipeline keeps at most its share of the budget in buffers
+        if (slot.used > S->budget / S->ctx.size()) {
+            std::lock_guard<std::mutex> clk(ctx->mu);
+            ctx_trim_locked(ctx);
+        }
+        slot.settle();
         S->cv_greedy.notify_all();
         S->cv_done.notify_all();
         S->cv_queue.notify_all(); // memory released: a waiting pipeline may admit now
@@ -175,18 +292,20 @@ static void greedy_loop(igc_server *S)
         if (S->greedy_q.empty()) return;
         SJob *j = S->greedy_q.front();
         S->greedy_q.pop_front();
+        j->t_greedy = S->now();
         lk.unlock();
         j->order.resize(j->csr.n_nodes);
         uint32_t cnt = 0;
         const int rc = igc_greedy(&j->csr, j->order.data(), &cnt);
+        const std::string msg = rc ? std::string(last_error()) : std::string();
         j->order.resize(cnt);
         igc_csr_free(&j->csr);
         lk.lock();
-        j->status = rc;
-        j->done = true;
-        j->t_done = S->now();
+        S->host_pending -= std::min(S->host_pending, j->host_bytes);
+        finish_job(S, j, rc, msg.c_str());
         S->jobs_done++;
         S->cv_done.notify_all();
+        S->cv_queue.notify_all(); // host memory released
     }
 }
 
@@ -202,6 +321,7 @@ static void fill_result(const SJob *j, igc_job_result *out)
     out->admit_s = j->t_admit;
     out->gpu_done_s = j->t_gpu;
     out->done_s = j->t_done;
+    out->greedy_start_s = j->t_greedy;
     out->batch = j->batch;
     out->batch_jobs = j->batch_jobs;
     if (!j->order.empty()) {
@@ -223,6 +343,8 @@ IGC_API int igc_server_create(int device, uint64_t hbm_budget_bytes, uint32_t pi
     igc_server *S = new (std::nothrow) igc_server();
     if (!S) return set_error(IGC_ERR_OOM, "host allocation failed");
     S->budget = hbm_budget_bytes ? hbm_budget_bytes : (200ull << 30);
+    S->pool.limit = S->budget;
+    S->host_budget = default_host_budget();
     for (uint32_t p = 0; p < pipelines; ++p) {
         igc_ctx *c = nullptr;
         const int rc = igc_ctx_create(device, &c);
@@ -231,10 +353,12 @@ IGC_API int igc_server_create(int device, uint64_t hbm_budget_bytes, uint32_t pi
             delete S;
             return rc;
         }
+        ctx_attach_pool(c, &S->pool);
         S->ctx.push_back(c);
     }
+    S->busy.assign(S->ctx.size(), 0);
     S->gpu_alive = (uint32_t)S->ctx.size();
-    for (igc_ctx *c : S->ctx) S->gpu_threads.emplace_back(gpu_loop, S, c);
+    for (uint32_t p = 0; p < S->ctx.size(); ++p) S->gpu_threads.emplace_back(gpu_loop, S, p);
     for (uint32_t t = 0; t < host_threads; ++t) S->workers.emplace_back(greedy_loop, S);
     *out = S;
     return IGC_OK;
@@ -245,10 +369,21 @@ IGC_API int igc_server_submit(igc_server *S, const uint8_t *syms, uint32_t N, ui
 {
     if (!S || (!syms && N) || !job_id) return set_error(IGC_ERR_INVALID, "null argument");
     if (n < 1 || n > IGC_MAX_LEN) return set_error(IGC_ERR_INVALID, "string length n=%u outside 1..%d", n, IGC_MAX_LEN);
+    // symbols are checked here, so one bad job cannot fail the batch it would share
+    const size_t bytes = (size_t)N * n;
+    const uint8_t top = n > IGC_MAX_QUATERNARY_LEN ? 1 : 3;
+    uint8_t any = 0;
+    for (size_t b = 0; b < bytes; ++b) any |= (uint8_t)(syms[b] > top);
+    if (any)
+        return set_error(IGC_ERR_INVALID, n > IGC_MAX_QUATERNARY_LEN
+                                              ? "symbol code above 1 in a binary string (strings longer than %d "
+                                                "take symbols 0/1 only)"
+                                              : "symbol code above 3 in input (strings up to %d)",
+                         IGC_MAX_QUATERNARY_LEN);
     SJob *j = new (std::nothrow) SJob();
     if (!j) return set_error(IGC_ERR_OOM, "host allocation failed");
     try {
-        j->syms.assign(syms, syms + (size_t)N * n);
+        j->syms.assign(syms, syms + bytes);
     } catch (...) {
         delete j;
         return set_error(IGC_ERR_OOM, "host allocation failed");
@@ -278,6 +413,7 @@ IGC_API int igc_server_wait(igc_server *S, uint64_t job_id, int64_t timeout_ms, 
     if (timeout_ms < 0) S->cv_done.wait(lk, ready);
     else if (!S->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) return IGC_ERR_BUSY;
     fill_result(j, out);
+    if (j->status) set_error(j->status, "%s", j->err.c_str()); // the job's error, for igc_last_error()
     S->jobs.erase(it);
     delete j;
     return IGC_OK;
@@ -307,8 +443,11 @@ IGC_API int igc_server_stats(igc_server *S, igc_server_info *out)
     out->jobs_done = S->jobs_done;
     out->pairs_done = S->pairs_done;
     out->queued = S->queue.size();
-    out->in_use_bytes = S->in_use;
-    out->max_in_use_bytes = S->max_in_use;
+    {
+        std::lock_guard<std::mutex> plk(S->pool.mu);
+        out->in_use_bytes = S->pool.charged;
+        out->max_in_use_bytes = S->pool.peak;
+    }
     out->max_batch_jobs = S->max_batch_seen;
     out->gpu_busy_s = S->gpu_busy_s;
     return IGC_OK;

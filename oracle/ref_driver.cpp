@@ -18,6 +18,38 @@
 //                              prints "ones <2|E|> fill_s <t> greedy_s <t>"
 //   fill T minED cands         FillAdjList only, prints "ones <n> fill_s <t>"
 //                              (used as the timed CPU baseline)
+//   digest T minED cands dir   FillAdjList with T threads, then the edge set as
+//                              dir/upper.bin (LE uint32 pairs (i, j), i < j,
+//                              sorted by (i, j); read back from AdjList::ToFile)
+//                              and Codebook run inside dir -> dir/codebook.txt;
+//                              prints "ones <2|E|> fill_s <t> greedy_s <t>".
+//                              For full-size sets, where adj.txt would be large
+//                              to keep as a fixture.
+//   solve T minED cands        the reference's CPU codebook path without files:
+//                              FillAdjList (T threads), then Codebook's
+//                              selection loop (AdjList::FindMinDel until the
+//                              list is empty, then the remaining nodes,
+//                              src/SparseMat.cpp:698-729) without its progress
+//                              dumps; prints "ones <2|E|> fill_s <t> greedy_s <t>
+//                              codewords <c>" (the CPU baseline of the bench)
+//   gpu minED cands dir        the reference's own CodebookAdjList (src/SparseMat.cpp:734-777)
+//                              with useGPU = true, run inside dir: its
+//                              FillAdjListGPU writes the candidates, runs
+//                              $INDEXGEN_PYTHON $INDEXGEN_ROOT/src/gpu_graph_generator.py
+//                              and reads the pair file back (:454-534); with
+//                              INDEXGEN_ROOT pointing at a tree whose script is
+//                              indexgen_amd/gpu_graph_generator.py this drives
+//                              libigc.so through the reference's unmodified
+//                              host.  Writes dir/codebook.txt; prints "ones <n>
+//                              fill_s <t> greedy_s <t>"
+//   header words n minED maxRun minGC maxGC T candfile candNum ones K iters conv method cand_s fill_s proc_s all_s dir
+//                              the reference's own codebook file: ToFile
+//                              (src/Utils.cpp:394-418) of the words in `words`
+//                              (one per line) with Params for the FileRead
+//                              method (file=candfile), clustering.method /
+//                              convergenceIterations = method / conv, clusterK
+//                              = K and clusterIterations = iters (K <= 0: an
+//                              unclustered run), written inside dir
 //   linear n minHD bias rowp colp maxRun minGC maxGC
 //                              Candidates(params) for the LinearCode generator
 //                              with MANUAL vectors (src/Candidates.cpp:178-188),
@@ -33,7 +65,9 @@
 #include "SparseMat.hpp"
 #include "Utils.hpp"
 
+#include <algorithm>
 #include <chrono>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <filesystem>
@@ -50,6 +84,11 @@ void FillAdjList(AdjList &adjList, const std::vector<std::string> &candidates, c
                  const int saveInterval, const bool resume, long long int &matrixOnesNum);
 void Codebook(AdjList &adjList, std::vector<std::string> &codebook, const std::vector<std::string> &candidates,
               const int saveInterval, const bool resume);
+std::string FileName(const int codeLen, const int codeSize, const int minED); // src/Utils.cpp:381-392
+void CodebookAdjList(const std::vector<std::string> &candidates, std::vector<std::string> &codebook, const int minED,
+                     const int threadNum, const int saveInterval, long long int &matrixOnesNum,
+                     std::chrono::duration<double> &fillAdjListTime, std::chrono::duration<double> &processMatrixTime,
+                     const bool useGPU, double maxGPUMemoryGB, const std::string &candFilename, bool isBinary);
 
 static std::vector<std::string> read_lines(const std::string &path)
 {
@@ -90,6 +129,141 @@ int main(int argc, char **argv)
             int wf = FastEditDistance(text, pat);
             std::cout << banded << ' ' << wf << '\n';
         }
+        return 0;
+    }
+    if (mode == "digest" && argc == 6) {
+        int T = std::atoi(argv[2]);
+        int minED = std::atoi(argv[3]);
+        std::vector<std::string> cands = read_lines(argv[4]);
+        std::string dir = argv[5];
+        AdjList adj;
+        long long ones = 0;
+        auto t0 = std::chrono::steady_clock::now();
+        FillAdjList(adj, cands, minED, T, 0, false, ones);
+        auto t1 = std::chrono::steady_clock::now();
+        const double fill_s = std::chrono::duration<double>(t1 - t0).count();
+        {
+            // the adjacency as the reference itself writes it ("i\tj" per list entry)
+            const std::string adj_path = dir + "/adj.txt";
+            adj.ToFile(adj_path);
+            FILE *f = std::fopen(adj_path.c_str(), "r");
+            if (!f) {
+                std::cerr << "cannot read " << adj_path << std::endl;
+                return 1;
+            }
+            std::vector<uint64_t> keys;
+            keys.reserve((size_t)(ones / 2));
+            long long a, b;
+            long long entries = 0;
+            while (std::fscanf(f, "%lld %lld", &a, &b) == 2) {
+                ++entries;
+                if (a < b) keys.push_back(((uint64_t)a << 32) | (uint64_t)b);
+            }
+            std::fclose(f);
+            std::remove(adj_path.c_str());
+            if (entries != ones || (long long)keys.size() * 2 != ones) {
+                std::cerr << "adjacency file holds " << entries << " entries, " << keys.size()
+                          << " with i < j; FillAdjList reported " << ones << std::endl;
+                return 1;
+            }
+            std::sort(keys.begin(), keys.end());
+            FILE *o = std::fopen((dir + "/upper.bin").c_str(), "wb");
+            if (!o) {
+                std::cerr << "cannot write upper.bin" << std::endl;
+                return 1;
+            }
+            std::vector<uint32_t> buf;
+            buf.reserve(1 << 20);
+            for (uint64_t k : keys) {
+                buf.push_back((uint32_t)(k >> 32));
+                buf.push_back((uint32_t)k);
+                if (buf.size() >= (1u << 20)) {
+                    std::fwrite(buf.data(), 4, buf.size(), o);
+                    buf.clear();
+                }
+            }
+            std::fwrite(buf.data(), 4, buf.size(), o);
+            std::fclose(o);
+        }
+        if (chdir(dir.c_str()) != 0) {
+            std::cerr << "chdir failed" << std::endl;
+            return 1;
+        }
+        std::vector<std::string> codebook;
+        std::streambuf *old = std::cout.rdbuf();
+        std::ostringstream sink;
+        std::cout.rdbuf(sink.rdbuf());
+        auto t2 = std::chrono::steady_clock::now();
+        Codebook(adj, codebook, cands, 0, false);
+        auto t3 = std::chrono::steady_clock::now();
+        std::cout.rdbuf(old);
+        std::ofstream cb("codebook.txt");
+        for (const auto &w : codebook) cb << w << '\n';
+        std::cout << "ones " << ones << " fill_s " << fill_s << " greedy_s "
+                  << std::chrono::duration<double>(t3 - t2).count() << std::endl;
+        return 0;
+    }
+    if (mode == "gpu" && argc == 5) {
+        const int minED = std::atoi(argv[2]);
+        std::vector<std::string> cands = read_lines(argv[3]);
+        if (chdir(argv[4]) != 0) {
+            std::cerr << "chdir failed" << std::endl;
+            return 1;
+        }
+        std::vector<std::string> codebook;
+        long long ones = 0;
+        std::chrono::duration<double> fill{}, proc{};
+        std::streambuf *old = std::cout.rdbuf();
+        std::ostringstream sink;
+        std::cout.rdbuf(sink.rdbuf());
+        CodebookAdjList(cands, codebook, minED, 1, 0, ones, fill, proc, true, 10.0, "", false);
+        std::cout.rdbuf(old);
+        std::ofstream cb("codebook.txt");
+        for (const auto &w : codebook) cb << w << '\n';
+        std::cout << "ones " << ones << " fill_s " << fill.count() << " greedy_s " << proc.count() << std::endl;
+        return 0;
+    }
+    if (mode == "solve" && argc == 5) {
+        const int T = std::atoi(argv[2]);
+        const int minED = std::atoi(argv[3]);
+        std::vector<std::string> cands = read_lines(argv[4]);
+        AdjList adj;
+        long long ones = 0;
+        auto t0 = std::chrono::steady_clock::now();
+        FillAdjList(adj, cands, minED, T, 0, false, ones); // ends with RowsBySum
+        auto t1 = std::chrono::steady_clock::now();
+        std::vector<bool> remaining(cands.size(), true);
+        double t_min = 0, t_del = 0;
+        size_t words = 0;
+        while (!adj.empty()) {
+            (void)adj.FindMinDel(remaining, t_min, t_del);
+            ++words;
+        }
+        for (bool r : remaining) words += r ? 1 : 0;
+        auto t2 = std::chrono::steady_clock::now();
+        std::cout << "ones " << ones << " fill_s " << std::chrono::duration<double>(t1 - t0).count() << " greedy_s "
+                  << std::chrono::duration<double>(t2 - t1).count() << " codewords " << words << std::endl;
+        return 0;
+    }
+    if (mode == "header" && argc == 21) {
+        std::vector<std::string> words = read_lines(argv[2]);
+        const int n = std::atoi(argv[3]);
+        Params params(n, std::atoi(argv[4]), std::atoi(argv[5]), std::atof(argv[6]), std::atof(argv[7]),
+                      std::atoi(argv[8]), 0, GenerationMethod::FILE_READ,
+                      std::make_unique<FileReadConstraints>(std::string(argv[9])));
+        const int K = std::atoi(argv[12]);
+        params.clustering.enabled = K > 0;
+        params.clustering.k = K;
+        params.clustering.convergenceIterations = std::atoi(argv[14]);
+        params.clustering.method = argv[15];
+        using dur = std::chrono::duration<double>;
+        if (chdir(argv[20]) != 0) {
+            std::cerr << "chdir failed" << std::endl;
+            return 1;
+        }
+        ToFile(words, params, std::atoi(argv[10]), std::atoll(argv[11]), dur(std::atof(argv[16])),
+               dur(std::atof(argv[17])), dur(std::atof(argv[18])), dur(std::atof(argv[19])), K, std::atoi(argv[13]));
+        std::cout << FileName(n, (int)words.size(), std::atoi(argv[4])) << std::endl;
         return 0;
     }
     if ((mode == "graph" && argc == 6) || (mode == "fill" && argc == 5)) {

@@ -25,7 +25,17 @@ Outputs (data only -- inputs and the reference's outputs):
                         message 0 is the bias, unit message e_i gives row i)
                         and a digest of that unfiltered list
 
+  full_size_golden.npz  the benchmark workloads at full size (n=11 plain set,
+                        n=13 linear-code set, the 64 uneven sets) through the
+                        reference's FillAdjList and Codebook with 8 threads:
+                        per set the candidates' digest, 2|E| ("ones"), the
+                        SHA-256 of the sorted i<j edge list (LE uint32 pairs)
+                        and of the codebook (newline-joined words, selection
+                        order), and the reference's own fill / greedy seconds;
+                        the n=11 set also with one thread ("plain_T1_*")
+
     python tests/golden/make_golden.py gen     # only gen_golden.npz
+    python tests/golden/make_golden.py full    # only full_size_golden.npz (~30 min on 8 cores)
 """
 from __future__ import annotations
 
@@ -268,11 +278,73 @@ def make_gen():
     np.savez_compressed(os.path.join(HERE, "gen_golden.npz"), **out)
 
 
+FULL_THREADS = 8
+
+
+def ref_digest(syms: np.ndarray, min_ed: int, T: int = FULL_THREADS) -> dict:
+    """The reference's FillAdjList + Codebook on one set, reduced to digests."""
+    d = tempfile.mkdtemp(prefix="igc_full_")
+    cf = os.path.join(d, "c.txt")
+    with open(cf, "w") as f:
+        for r in syms:
+            f.write(sym_str(r) + "\n")
+    out = subprocess.run([DRIVER, "digest", str(T), str(min_ed), cf, d], capture_output=True, text=True,
+                         check=True).stdout.split()
+    h = hashlib.sha256()
+    with open(os.path.join(d, "upper.bin"), "rb") as f:
+        for chunk in iter(lambda: f.read(1 << 24), b""):
+            h.update(chunk)
+    upper_bytes = os.path.getsize(os.path.join(d, "upper.bin"))
+    words = [w.strip() for w in open(os.path.join(d, "codebook.txt")) if w.strip()]
+    for fn in os.listdir(d):
+        os.remove(os.path.join(d, fn))
+    os.rmdir(d)
+    ones = int(out[1])
+    assert upper_bytes == 4 * ones, (upper_bytes, ones)
+    return {"ones": np.int64(ones), "edges_sha256": np.bytes_(h.hexdigest().encode()),
+            "codebook_sha256": np.bytes_(hashlib.sha256("\n".join(words).encode()).hexdigest().encode()),
+            "codewords": np.int64(len(words)), "fill_s": np.float64(out[3]), "greedy_s": np.float64(out[5])}
+
+
+def make_full():
+    """Full-size fixtures of the benchmark workloads (default seeds of
+    indexgen_amd.workloads)."""
+    sys.path.insert(0, ROOT)
+    from indexgen_amd import workloads as W
+    sets = [("plain", W.plain_small(seed=1), 3), ("linear", W.linear_large(seed=2), 4)]
+    for s, cs in enumerate(W.batched(seed=3)):
+        sets.append((f"batched{s}", cs.syms, cs.edit_dist))
+    out = {"threads": np.int32(FULL_THREADS), "names": np.array([nm for nm, _, _ in sets])}
+    path = os.path.join(HERE, "full_size_golden.npz")
+    if os.path.exists(path):  # resume: keep the sets already done
+        with np.load(path) as old:
+            out.update({k: old[k] for k in old.files if k not in ("threads", "names")})
+    # the plain set once more with one thread: the neighbour order a caller of
+    # the library gets by default (ascending), for the drop-in checks
+    sets.append(("plain_T1", sets[0][1], 3))
+    for nm, syms, ed in sets:
+        sha = np.bytes_(hashlib.sha256(np.ascontiguousarray(syms).tobytes()).hexdigest().encode())
+        if f"{nm}_ones" in out and out.get(f"{nm}_syms_sha256") == sha:
+            continue
+        r = ref_digest(syms, ed, 1 if nm.endswith("_T1") else FULL_THREADS)
+        out[f"{nm}_syms_sha256"] = sha
+        out[f"{nm}_shape"] = np.array(syms.shape, np.int64)
+        out[f"{nm}_min_ed"] = np.int32(ed)
+        for k, v in r.items():
+            out[f"{nm}_{k}"] = v
+        print(nm, syms.shape, "ed", ed, "ones", int(r["ones"]), "codewords", int(r["codewords"]),
+              "fill_s %.1f greedy_s %.1f" % (r["fill_s"], r["greedy_s"]), flush=True)
+        np.savez_compressed(path, **out)
+
+
 def main():
     if not os.path.exists(DRIVER):
         sys.exit("build the reference driver first: make -C oracle ref")
     if sys.argv[1:] == ["gen"]:
         make_gen()
+        return
+    if sys.argv[1:] == ["full"]:
+        make_full()
         return
     rng = np.random.default_rng(20251203)
     make_pairs(rng)
