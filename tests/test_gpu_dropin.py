@@ -43,7 +43,8 @@ def plain(tmp_path_factory):
 
 def _check(out_dir, stdout, plain):
     words = [w for w in open(os.path.join(out_dir, "codebook.txt")).read().split("\n") if w]
-    ones = int(stdout.split()[1])
+    line = [l for l in stdout.split("\n") if l.startswith("ones ")][-1]  # the script prints its own line first
+    ones = int(line.split()[1])
     assert ones == plain["ones"]
     assert len(words) == plain["codewords"]
     assert hashlib.sha256("\n".join(words).encode()).hexdigest() == plain["codebook"]
