@@ -178,6 +178,23 @@ IGC_API int igc_graph_build(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint3
 IGC_API int igc_graph_build_device(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n,
                                    uint32_t edit_dist, igc_csr *out);
 
+/* Same as igc_graph_build_device, for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): no counterpart in the reference.  The CSR goes to caller device
+ * buffers -- d_row_ptr (N + 1 entries) and d_col (col_cap entries, >= 2|E|) --
+ * and every temporary comes from a fixed device array of 128 MiB that each GPU
+ * the library runs on holds until the process ends; the call allocates nothing,
+ * host or device.  Limit: packed words (4 or 8 B per string), edge list (8 B
+ * per edge) and sort temporaries (about 16 B per edge) within the 128 MiB,
+ * about 5 million edges; beyond it, or when col_cap < 2|E|, IGC_ERR_INVALID
+ * with the size needed (*n_edges is set once the edges are counted).
+ * Ascending order, no row filter (same graph).  Calls on one GPU share the
+ * array and are serialised by the library; the array keeps the last call's
+ * contents (what must start at zero is cleared by a kernel each call). */
+IGC_API int igc_graph_build_device_noalloc(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n,
+                                           uint32_t edit_dist, uint64_t *d_row_ptr, uint32_t *d_col,
+                                           uint64_t col_cap, uint64_t *n_edges);
+
 /* One timed tile shape of the pair kernel: `cpl` register slots per lane
  * (1, 2, 4; each slot tests lane_cols columns: 2 for n <= 16, 1 above), tiles
  * of 256*cpl*lane_cols columns and 1/q as many rows, best time over the

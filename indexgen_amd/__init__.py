@@ -207,6 +207,7 @@ def lib() -> ctypes.CDLL:
         "igc_last_timing": (i, [vp, ctypes.POINTER(_Timing)]),
         "igc_graph_build": (i, [vp, u8p, u32, u32, u32, u32, ctypes.POINTER(_Csr)]),
         "igc_graph_build_device": (i, [vp, u8p, u32, u32, u32, ctypes.POINTER(_Csr)]),
+        "igc_graph_build_device_noalloc": (i, [vp, vp, u32, u32, u32, vp, vp, u64, ctypes.POINTER(u64)]),
         "igc_graph_count": (i, [vp, u8p, u32, u32, u32, ctypes.POINTER(u64)]),
         "igc_graph_build_batch": (i, [vp, ctypes.POINTER(_Set), u32, ctypes.POINTER(_Csr)]),
         "igc_kmeans": (i, [vp, u8p, u32, u32, u32, u32, u64, u32, vp, vp]),
@@ -471,6 +472,16 @@ class Context:
                                             ctypes.byref(c)))
         return DeviceCSR(int(c.n_nodes), int(c.n_edges), ctypes.cast(c.row_ptr, ctypes.c_void_p).value or 0,
                          ctypes.cast(c.col, ctypes.c_void_p).value or 0, c)
+
+    def graph_build_device_noalloc(self, d_syms_ptr: int, N: int, n: int, min_ed: int, d_row_ptr: int,
+                                   d_col: int, col_cap: int) -> int:
+        """igc_graph_build_device into caller device buffers (row_ptr N+1 u64,
+        col col_cap u32) with no allocation; returns the edge count."""
+        e = ctypes.c_uint64()
+        _check(lib().igc_graph_build_device_noalloc(self._h, ctypes.c_void_p(d_syms_ptr), int(N), int(n),
+                                                    int(min_ed), ctypes.c_void_p(d_row_ptr), ctypes.c_void_p(d_col),
+                                                    int(col_cap), ctypes.byref(e)))
+        return int(e.value)
 
     # -- candidate generators (the reference's CandidateGenerator methods + applyFilters)
 

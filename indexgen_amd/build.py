@@ -19,7 +19,8 @@ LIB = os.path.join(HERE, "libigc.so")
 CLI = os.path.join(HERE, "igc_cli")
 
 # device code and its host orchestration (hipcc)
-SOURCES = ["igc_runtime.hip", "igc_batch.hip", "igc_kmeans.hip", "igc_refine.hip", "igc_server.hip", "igc_gen.hip"]
+SOURCES = ["igc_runtime.hip", "igc_batch.hip", "igc_kmeans.hip", "igc_refine.hip", "igc_server.hip", "igc_gen.hip",
+           "igc_noalloc.hip"]
 # plain host C++ (greedy, output writers): g++, no HIP
 HOST_SOURCES = ["igc_host.cpp"]
 

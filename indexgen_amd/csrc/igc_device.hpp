@@ -305,6 +305,10 @@ int kmeans_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32
                          uint64_t seed, bool use_graph, float *ms_out);
 int32_t *kmeans_assign_ptr(igc_ctx *ctx);
 
+// Looks up the device address of the no-allocation build's scratch array on
+// `device` (the current device) once; igc_ctx_create calls it (igc_noalloc.hip).
+int scratch_attach(int device);
+
 // Charges every device buffer of ctx (present and future) to a slot of a
 // shared pool; buffers already held are charged at once.
 void ctx_attach_pool(igc_ctx *ctx, MemPool *pool);

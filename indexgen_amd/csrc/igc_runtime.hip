@@ -783,6 +783,9 @@ IGC_API int igc_ctx_create(int device, igc_ctx **out)
         return set_error(IGC_ERR_DEVICE, "hipStreamCreate failed");
     }
     c->stream = c->own_stream;
+    // the no-allocation build's scratch address on this device (a failure
+    // here is reported by that entry point, which retries the lookup)
+    (void)scratch_attach(device);
     for (auto &e : c->ev) {
         if (hipEventCreate(&e) != hipSuccess) {
             delete c;
