@@ -155,6 +155,11 @@ IGC_API int igc_ctx_set_csr_form(igc_ctx *ctx, int form);
  * No counterpart in the reference (its per-tile edge buffer is fixed,
  * src/cuda_edit_distance.py:320). */
 IGC_API int igc_ctx_set_edge_cap(igc_ctx *ctx, uint64_t edges);
+/* igc_refine replays one captured hipGraph per round for its device work up
+ * to the cluster assignment (on, the default), or launches the same kernels
+ * directly (off); the results are the same.  For tests and measurement; no
+ * counterpart in the reference. */
+IGC_API int igc_ctx_set_graph_replay(igc_ctx *ctx, int on);
 IGC_API int igc_last_timing(igc_ctx *ctx, igc_timing *out);
 
 /* ---- single candidate set ---- */
@@ -447,6 +452,17 @@ IGC_API int igc_server_create(int device, uint64_t hbm_budget_bytes, uint32_t pi
  * IGC_ERR_INVALID and queue nothing. */
 IGC_API int igc_server_submit(igc_server *srv, const uint8_t *syms, uint32_t N, uint32_t n, uint32_t edit_dist,
                               uint64_t *job_id);
+/* Queues a refinement job: the clustered codebook loop of igc_refine (same
+ * parameters, host threads = the server's greedy threads) on N quaternary
+ * strings.  It runs alone on one pipeline, which keeps the job's strings,
+ * its round state and its captured round graph on the device across the
+ * rounds, while the other pipeline goes on serving; the result's order holds
+ * the final codebook as indices into syms, equal to igc_refine's.  No
+ * counterpart in the reference (its clustered loop is a single blocking run,
+ * src/SparseMat.cpp:918-1095). */
+IGC_API int igc_server_submit_refine(igc_server *srv, const uint8_t *syms, uint32_t N, uint32_t n,
+                                     uint32_t edit_dist, uint32_t k, uint32_t kmeans_iters, uint32_t max_rounds,
+                                     uint32_t converge_n, uint64_t seed, uint64_t *job_id);
 /* Waits up to timeout_ms (< 0: forever) for a job; IGC_ERR_BUSY on timeout.
  * On success the job is retired and its result handed over; when the job
  * itself failed (out->status != IGC_OK), igc_last_error() in the waiting

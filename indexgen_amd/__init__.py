@@ -224,6 +224,7 @@ def lib() -> ctypes.CDLL:
                                    ctypes.c_char_p, ctypes.c_size_t]),
         "igc_server_create": (i, [i, u64, u32, u32, ctypes.POINTER(vp)]),
         "igc_server_submit": (i, [vp, u8p, u32, u32, u32, ctypes.POINTER(u64)]),
+        "igc_server_submit_refine": (i, [vp, u8p, u32, u32, u32, u32, u32, u32, u32, u64, ctypes.POINTER(u64)]),
         "igc_server_wait": (i, [vp, u64, ctypes.c_int64, ctypes.POINTER(_JobResult)]),
         "igc_server_poll": (i, [vp, u64, ctypes.POINTER(_JobResult)]),
         "igc_job_free": (i, [ctypes.POINTER(_JobResult)]),
@@ -237,6 +238,7 @@ def lib() -> ctypes.CDLL:
         "igc_ctx_set_predicate": (i, [vp, i]),
         "igc_ctx_set_csr_form": (i, [vp, i]),
         "igc_ctx_set_edge_cap": (i, [vp, u64]),
+        "igc_ctx_set_graph_replay": (i, [vp, i]),
         "igc_assign_fp32": (i, [vp, vp, u32, u32, vp, u32, vp]),
         "igc_gen_linear": (i, [vp, vp, u32, u32, vp, ctypes.POINTER(_Filter), ctypes.POINTER(_Cands)]),
         "igc_gen_vt": (i, [vp, u32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(_Filter), ctypes.POINTER(_Cands)]),
@@ -638,6 +640,11 @@ class Context:
         fit) or IGC_CSR_WIDE_KEYS (64-bit keys); the same CSR either way."""
         _check(lib().igc_ctx_set_csr_form(self._h, int(form)))
 
+    def set_graph_replay(self, on: bool) -> None:
+        """refine replays its captured round graph (True, default) or launches
+        the same kernels directly (False); the same results."""
+        _check(lib().igc_ctx_set_graph_replay(self._h, 1 if on else 0))
+
     def set_edge_cap(self, edges: int) -> None:
         """First edge buffer of a build, in edges (0: the density estimate); a
         graph with more edges regrows it to the exact count and replays the pair
@@ -688,6 +695,17 @@ class Server:
         jid = ctypes.c_uint64()
         _check(lib().igc_server_submit(self._h, s.ctypes.data if s.size else None, s.shape[0], s.shape[1],
                                        int(min_ed), ctypes.byref(jid)))
+        return int(jid.value)
+
+    def submit_refine(self, candidates, min_ed: int, k: int, kmeans_iters: int = 10, max_rounds: int = 5,
+                      converge_n: int = 3, seed: int = 1) -> int:
+        """A refinement job (the clustered loop of Context.refine); its result's
+        order is the final codebook."""
+        s = to_symbols(candidates)
+        jid = ctypes.c_uint64()
+        _check(lib().igc_server_submit_refine(self._h, s.ctypes.data if s.size else None, s.shape[0], s.shape[1],
+                                              int(min_ed), int(k), int(kmeans_iters), int(max_rounds),
+                                              int(converge_n), int(seed), ctypes.byref(jid)))
         return int(jid.value)
 
     def wait(self, job_id: int, timeout_ms: int = -1):

@@ -614,18 +614,15 @@ static int h2d_concat_staged(igc_ctx *ctx, uint8_t *dst, const igc_set *sets, ui
     return IGC_OK;
 }
 
-int batch_build_host_locked(igc_ctx *ctx, const igc_set *sets, uint32_t count, igc_csr *outs)
+// The batch built from its sets' symbols concatenated at d_syms (device
+// memory, set order), every set's CSR copied to host arrays.
+static int batch_to_host_locked(igc_ctx *ctx, const uint8_t *d_syms, const BatchPlan &P, const igc_set *sets,
+                                uint32_t count, igc_csr *outs)
 {
-    for (uint32_t s = 0; s < count; ++s) std::memset(&outs[s], 0, sizeof(igc_csr));
-    if (count == 0) return IGC_OK;
-    BatchPlan P;
-    int rc = plan_batch(ctx, sets, count, P);
-    if (rc) return rc;
-    if ((rc = ctx->syms.ensure(std::max<size_t>(P.sym_bytes, 1)))) return rc;
-    if ((rc = h2d_concat_staged(ctx, ctx->syms.as<uint8_t>(), sets, count, P.sym_bytes))) return rc;
     std::vector<uint64_t> bounds;
     uint64_t E = 0;
-    if ((rc = batch_device_locked(ctx, ctx->syms.as<uint8_t>(), P, bounds, &E))) return rc;
+    int rc;
+    if ((rc = batch_device_locked(ctx, d_syms, P, bounds, &E))) return rc;
     // union row_ptr -> pinned host memory; each set's col slice straight into its own host CSR
     if ((rc = ctx->host_rp.ensure(sizeof(uint64_t) * ((size_t)P.total + 1)))) return rc;
     HIP_TRY(hipMemcpyAsync(ctx->host_rp.p, ctx->row_ptr.p, sizeof(uint64_t) * ((size_t)P.total + 1),
@@ -644,6 +641,29 @@ int batch_build_host_locked(igc_ctx *ctx, const igc_set *sets, uint32_t count, i
     }
     for (uint32_t s = 0; s < count; ++s) finish_set(sets[s].flags, &outs[s]);
     return IGC_OK;
+}
+
+int batch_build_host_locked(igc_ctx *ctx, const igc_set *sets, uint32_t count, igc_csr *outs)
+{
+    for (uint32_t s = 0; s < count; ++s) std::memset(&outs[s], 0, sizeof(igc_csr));
+    if (count == 0) return IGC_OK;
+    BatchPlan P;
+    int rc = plan_batch(ctx, sets, count, P);
+    if (rc) return rc;
+    if ((rc = ctx->syms.ensure(std::max<size_t>(P.sym_bytes, 1)))) return rc;
+    if ((rc = h2d_concat_staged(ctx, ctx->syms.as<uint8_t>(), sets, count, P.sym_bytes))) return rc;
+    return batch_to_host_locked(ctx, ctx->syms.as<uint8_t>(), P, sets, count, outs);
+}
+
+int batch_build_dev_syms_locked(igc_ctx *ctx, const uint8_t *d_syms, const igc_set *sets, uint32_t count,
+                                igc_csr *outs)
+{
+    for (uint32_t s = 0; s < count; ++s) std::memset(&outs[s], 0, sizeof(igc_csr));
+    if (count == 0) return IGC_OK;
+    BatchPlan P;
+    int rc = plan_batch(ctx, sets, count, P);
+    if (rc) return rc;
+    return batch_to_host_locked(ctx, d_syms, P, sets, count, outs);
 }
 
 } // namespace igc

@@ -12,6 +12,7 @@
 #include <cstdint>
 #include <functional>
 #include <mutex>
+#include <vector>
 
 #define HIP_TRY(expr)                                                                                         \
     do {                                                                                                      \
@@ -248,6 +249,8 @@ struct igc_ctx {
     // first edge buffer of a build, in entries (0: sized by the density
     // estimate); a smaller one is regrown to the exact count and the pairs replayed
     uint64_t first_edge_cap = 0;
+    // the refinement loop replays its captured round (igc_ctx_set_graph_replay)
+    bool graph_replay = true;
     // charge account of the context's device buffers when it draws from a
     // shared budget (the server's pipelines); no pool = not charged
     igc::MemSlot mem;
@@ -304,6 +307,16 @@ void kmeans_release(igc_ctx *ctx);
 int kmeans_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n, uint32_t k, uint32_t iters,
                          uint64_t seed, bool use_graph, float *ms_out);
 int32_t *kmeans_assign_ptr(igc_ctx *ctx);
+uint32_t *kmeans_flags_ptr(igc_ctx *ctx); // kMiscWords words; [2] bad symbol flag
+// The refinement loop's K-means: buffers for up to Nmax points (kept in
+// place, so captured launches stay valid), the seeds of a round (host_seeds
+// must live until the stream has copied them), and the whole K-means --
+// pack, seeds, `iters` Lloyd iterations, final assignment -- as kernels only,
+// with grids for Nmax and the round's count read from dN (capturable).
+int kmeans_prepare_locked(igc_ctx *ctx, uint32_t Nmax, uint32_t n, uint32_t k);
+int kmeans_seed_locked(igc_ctx *ctx, uint32_t N, uint32_t k, uint64_t seed, std::vector<uint32_t> &host_seeds);
+void kmeans_launch_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t Nmax, const uint32_t *dN, uint32_t n,
+                          uint32_t k, uint32_t iters);
 
 // Looks up the device address of the no-allocation build's scratch array on
 // `device` (the current device) once; igc_ctx_create calls it (igc_noalloc.hip).
@@ -326,6 +339,15 @@ int d2h_stream(igc_ctx *ctx, const void *src, size_t bytes, const D2HConsumer &c
 
 // igc_graph_build_batch without taking the context lock (caller holds it)
 int batch_build_host_locked(igc_ctx *ctx, const igc_set *sets, uint32_t count, igc_csr *outs);
+// The same with the sets' symbols already concatenated in device memory at
+// d_syms, in set order (sets[s].syms is not read; it must be non-null when N > 0)
+// The refinement loop (igc_refine.hip) on a locked context.
+int refine_locked(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint32_t n, uint32_t edit_dist, uint32_t k,
+                  uint32_t kmeans_iters, uint32_t max_rounds, uint32_t converge_n, uint64_t seed,
+                  uint32_t host_threads, bool use_graph, uint32_t *out_idx, uint32_t *out_count,
+                  igc_refine_stats *stats);
+int batch_build_dev_syms_locked(igc_ctx *ctx, const uint8_t *d_syms, const igc_set *sets, uint32_t count,
+                                igc_csr *outs);
 } // namespace igc
 
 namespace igc {

@@ -35,13 +35,17 @@ __device__ __forceinline__ uint32_t sym_at(W w, uint32_t p)
 
 // assign[i] = argmin_c (cn2[c] - 2 * sum_p C[c][4p + s_p]), first index on ties.
 // The centroid table goes through LDS in chunks of kc centroids (<= 32 KiB).
+// dN (may be null) holds the point count in device memory instead of N, so a
+// launch captured for the largest round serves every round (grid-stride loops).
 template <class W>
 __global__ __launch_bounds__(kKmThreads) void k_km_assign(const W *__restrict__ packed, uint32_t N,
-                                                                  uint32_t n, const float *__restrict__ C,
+                                                                  const uint32_t *__restrict__ dN, uint32_t n,
+                                                                  const float *__restrict__ C,
                                                                   const float *__restrict__ cn2, uint32_t k,
                                                                   uint32_t kc, int32_t *__restrict__ assign)
 {
     extern __shared__ float sh[];
+    if (dN) N = *dN;
     const uint32_t D = 4 * n;
     float *sC = sh;
     float *sN = sh + (size_t)kc * D;
@@ -82,11 +86,13 @@ __global__ __launch_bounds__(kKmThreads) void k_km_assign(const W *__restrict__ 
 // global atomic each.
 template <class W>
 __global__ __launch_bounds__(kKmThreads) void k_km_accum(const W *__restrict__ packed, uint32_t N,
-                                                                 uint32_t n, const int32_t *__restrict__ assign,
+                                                                 const uint32_t *__restrict__ dN, uint32_t n,
+                                                                 const int32_t *__restrict__ assign,
                                                                  uint32_t k, uint32_t *__restrict__ cnt,
                                                                  uint32_t *__restrict__ size)
 {
     extern __shared__ uint32_t hs[];
+    if (dN) N = *dN;
     const uint32_t D = 4 * n;
     const uint32_t words = (k * D + 1) / 2;
     uint32_t *hc = hs;
@@ -178,8 +184,10 @@ void kmeans_seeds(uint32_t N, uint32_t k, uint64_t seed, std::vector<uint32_t> &
     }
 }
 
+// K-means buffers: their own packed words and flags (not the context's, which
+// the graph builds regrow), so launches captured with them stay valid.
 struct KmBufs {
-    DevBuf C, cn2, cnt, size, seeds, assign;
+    DevBuf C, cn2, cnt, size, seeds, assign, packed, flags;
 };
 
 static KmBufs &km_bufs(igc_ctx *ctx) { return *reinterpret_cast<KmBufs *>(ctx->km); }
@@ -192,9 +200,26 @@ static uint32_t km_grid(igc_ctx *ctx, uint32_t N)
     return std::max<uint32_t>(grid_for(N, kKmThreads, (uint32_t)ctx->cu_count * 4), (N + 32767) / 32768);
 }
 
-// Launches one Lloyd iteration on ctx->stream (capturable: kernels only).
+// Packs the strings for K-means (the count in dN when non-null, else N).
 template <class W>
-static void lloyd_iteration(igc_ctx *ctx, KmBufs &b, uint32_t N, uint32_t n, uint32_t k)
+__global__ void k_km_pack(const uint8_t *__restrict__ syms, uint32_t N, const uint32_t *__restrict__ dN, uint32_t n,
+                          W *__restrict__ packed, uint32_t *__restrict__ bad)
+{
+    if (dN) N = *dN;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const uint8_t *p = syms + (size_t)i * n;
+    uint32_t or_all = 0;
+    for (uint32_t q = 0; q < n; ++q) or_all |= p[q];
+    if constexpr (sizeof(W) == 8) packed[i] = pack_word64(p, n);
+    else packed[i] = pack_word(p, n);
+    if (or_all > 3u) atomicOr(bad, 1u);
+}
+
+// Launches one Lloyd iteration on ctx->stream (capturable: kernels only).
+// Grids are sized for N points; dN (may be null) holds the actual count.
+template <class W>
+static void lloyd_iteration(igc_ctx *ctx, KmBufs &b, uint32_t N, const uint32_t *dN, uint32_t n, uint32_t k)
 {
     hipStream_t s = ctx->stream;
     const uint32_t D = 4 * n;
@@ -202,55 +227,95 @@ static void lloyd_iteration(igc_ctx *ctx, KmBufs &b, uint32_t N, uint32_t n, uin
     const size_t lds_assign = sizeof(float) * ((size_t)kc * D + kc);
     const size_t lds_accum = sizeof(uint32_t) * (((size_t)k * D + 1) / 2 + k);
     const uint32_t grid = km_grid(ctx, N);
-    hipLaunchKernelGGL(k_km_assign<W>, dim3(grid), dim3(kKmThreads), lds_assign, s, ctx->packed.as<W>(), N, n,
+    hipLaunchKernelGGL(k_km_assign<W>, dim3(grid), dim3(kKmThreads), lds_assign, s, b.packed.as<W>(), N, dN, n,
                        b.C.as<float>(), b.cn2.as<float>(), k, kc, b.assign.as<int32_t>());
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for((uint64_t)k * D, 256, 1024)), dim3(256), 0, s, b.cnt.as<uint32_t>(),
                        (uint64_t)k * D, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, b.size.as<uint32_t>(), (uint64_t)k, 0u);
-    hipLaunchKernelGGL(k_km_accum<W>, dim3(grid), dim3(kKmThreads), lds_accum, s, ctx->packed.as<W>(), N, n,
+    hipLaunchKernelGGL(k_km_accum<W>, dim3(grid), dim3(kKmThreads), lds_accum, s, b.packed.as<W>(), N, dN, n,
                        b.assign.as<int32_t>(), k, b.cnt.as<uint32_t>(), b.size.as<uint32_t>());
     hipLaunchKernelGGL(k_km_update, dim3(k), dim3(64), 0, s, b.cnt.as<uint32_t>(), b.size.as<uint32_t>(), D,
                        b.C.as<float>(), b.cn2.as<float>());
 }
 
-// K-means on symbols already in device memory: seeds -> `iters` Lloyd
-// iterations (assign, histogram, update) -> a final assign against the last
-// centroids, left in the context's assign buffer.
-template <class W>
-static int kmeans_device_w(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n, uint32_t k, uint32_t iters,
-                           uint64_t seed, bool use_graph, float *ms_out)
+static int km_check(uint32_t N, uint32_t n, uint32_t k)
 {
     if (k < 1 || k > N) return set_error(IGC_ERR_INVALID, "k=%u must be in 1..N (N=%u)", k, N);
-    const uint32_t D = 4 * n;
-    if (((size_t)k * D + 1) / 2 * 4 + (size_t)k * 4 > 64 * 1024)
+    if (((size_t)k * 4 * n + 1) / 2 * 4 + (size_t)k * 4 > 64 * 1024)
         return set_error(IGC_ERR_INVALID, "k=%u too large for the LDS histogram (k*4n/2 + k words <= 16384)", k);
-    KmBufs &b = km_bufs(ctx);
+    return IGC_OK;
+}
+
+// Device buffers of a K-means of up to N points (they stay put afterwards, so
+// launches captured with them can be replayed).
+template <class W>
+static int km_buffers(igc_ctx *ctx, KmBufs &b, uint32_t N, uint32_t n, uint32_t k)
+{
+    const uint32_t D = 4 * n;
     int rc;
-    if ((rc = ctx->packed.ensure(sizeof(W) * (size_t)N))) return rc;
-    if ((rc = ctx->misc.ensure(sizeof(uint32_t) * kMiscWords))) return rc;
+    if ((rc = b.packed.ensure(sizeof(W) * (size_t)N))) return rc;
+    if ((rc = b.flags.ensure(sizeof(uint32_t) * kMiscWords))) return rc;
     if ((rc = b.C.ensure(sizeof(float) * (size_t)k * D))) return rc;
     if ((rc = b.cn2.ensure(sizeof(float) * k))) return rc;
     if ((rc = b.cnt.ensure(sizeof(uint32_t) * (size_t)k * D))) return rc;
     if ((rc = b.size.ensure(sizeof(uint32_t) * k))) return rc;
     if ((rc = b.seeds.ensure(sizeof(uint32_t) * k))) return rc;
     if ((rc = b.assign.ensure(sizeof(int32_t) * (size_t)N))) return rc;
+    return IGC_OK;
+}
+
+// The whole K-means as kernels only (capturable): pack, seeds -> centroids
+// (b.seeds already filled), `iters` Lloyd iterations, then (with final) the
+// assignment against the last centroids.  Grids for N points; dN (may be
+// null) holds the actual count.
+template <class W>
+static void km_launch(igc_ctx *ctx, KmBufs &b, const uint8_t *d_syms, uint32_t N, const uint32_t *dN, uint32_t n,
+                      uint32_t k, uint32_t iters, bool final = true)
+{
+    hipStream_t s = ctx->stream;
+    const uint32_t D = 4 * n;
+    uint32_t *misc = b.flags.as<uint32_t>();
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, misc, (uint64_t)kMiscWords, 0u);
+    hipLaunchKernelGGL(k_km_pack<W>, dim3((N + 255) / 256), dim3(256), 0, s, d_syms, N, dN, n, b.packed.as<W>(),
+                       misc + 2);
+    hipLaunchKernelGGL(k_km_init<W>, dim3(k), dim3(64), 0, s, b.packed.as<W>(), b.seeds.as<uint32_t>(), n, k,
+                       b.C.as<float>(), b.cn2.as<float>());
+    for (uint32_t t = 0; t < iters; ++t) lloyd_iteration<W>(ctx, b, N, dN, n, k);
+    if (!final) return;
+    const uint32_t kc = km_chunk(D);
+    hipLaunchKernelGGL(k_km_assign<W>, dim3(km_grid(ctx, N)), dim3(kKmThreads), sizeof(float) * ((size_t)kc * D + kc),
+                       s, b.packed.as<W>(), N, dN, n, b.C.as<float>(), b.cn2.as<float>(), k, kc,
+                       b.assign.as<int32_t>());
+}
+
+// K-means on symbols already in device memory: seeds -> `iters` Lloyd
+// iterations (assign, histogram, update) -> a final assign against the last
+// centroids, left in the context's assign buffer.  With use_graph one Lloyd
+// iteration is captured and replayed `iters` times.
+template <class W>
+static int kmeans_device_w(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32_t n, uint32_t k, uint32_t iters,
+                           uint64_t seed, bool use_graph, float *ms_out)
+{
+    int rc;
+    if ((rc = km_check(N, n, k))) return rc;
+    KmBufs &b = km_bufs(ctx);
+    if ((rc = km_buffers<W>(ctx, b, N, n, k))) return rc;
     std::vector<uint32_t> seeds;
     kmeans_seeds(N, k, seed, seeds);
     hipStream_t s = ctx->stream;
-    uint32_t *misc = ctx->misc.as<uint32_t>();
+    uint32_t *misc = b.flags.as<uint32_t>();
     HIP_TRY(hipEventRecord(ctx->ev[0], s));
     HIP_TRY(hipMemcpyAsync(b.seeds.p, seeds.data(), sizeof(uint32_t) * k, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, misc, (uint64_t)kMiscWords, 0u);
-    hipLaunchKernelGGL(k_pack<W>, dim3((N + 255) / 256), dim3(256), 0, s, d_syms, N, n, ctx->packed.as<W>(),
-                       misc + 2);
-    hipLaunchKernelGGL(k_km_init<W>, dim3(k), dim3(64), 0, s, ctx->packed.as<W>(), b.seeds.as<uint32_t>(), n, k,
-                       b.C.as<float>(), b.cn2.as<float>());
-    HIP_TRY(hipGetLastError());
-    if (use_graph && iters > 0) {
+    if (!use_graph || iters == 0) {
+        km_launch<W>(ctx, b, d_syms, N, nullptr, n, k, iters);
+        HIP_TRY(hipGetLastError());
+    } else {
+        km_launch<W>(ctx, b, d_syms, N, nullptr, n, k, 0, false); // pack, seeds
+        HIP_TRY(hipGetLastError());
         hipGraph_t g = nullptr;
         hipGraphExec_t ge = nullptr;
         HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        lloyd_iteration<W>(ctx, b, N, n, k);
+        lloyd_iteration<W>(ctx, b, N, nullptr, n, k);
         hipError_t e = hipStreamEndCapture(s, &g);
         if (e != hipSuccess) return set_error(IGC_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
         e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
@@ -262,16 +327,13 @@ static int kmeans_device_w(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint
         (void)hipGraphExecDestroy(ge);
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) return set_error(IGC_ERR_DEVICE, "graph launch failed: %s", hipGetErrorString(e));
-    } else {
-        for (uint32_t t = 0; t < iters; ++t) lloyd_iteration<W>(ctx, b, N, n, k);
+        // final assignment against the last centroids
+        const uint32_t D = 4 * n, kc = km_chunk(D);
+        hipLaunchKernelGGL(k_km_assign<W>, dim3(km_grid(ctx, N)), dim3(kKmThreads),
+                           sizeof(float) * ((size_t)kc * D + kc), s, b.packed.as<W>(), N, nullptr, n,
+                           b.C.as<float>(), b.cn2.as<float>(), k, kc, b.assign.as<int32_t>());
         HIP_TRY(hipGetLastError());
     }
-    // final assignment against the last centroids
-    const uint32_t kc = km_chunk(D);
-    hipLaunchKernelGGL(k_km_assign<W>, dim3(km_grid(ctx, N)), dim3(kKmThreads), sizeof(float) * ((size_t)kc * D + kc),
-                       s, ctx->packed.as<W>(), N, n, b.C.as<float>(), b.cn2.as<float>(), k, kc,
-                       b.assign.as<int32_t>());
-    HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[1], s));
     uint32_t host_misc[kMiscWords];
     HIP_TRY(hipMemcpyAsync(host_misc, misc, sizeof host_misc, hipMemcpyDeviceToHost, s));
@@ -291,11 +353,37 @@ int kmeans_device_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint32
     return kmeans_device_w<uint32_t>(ctx, d_syms, N, n, k, iters, seed, use_graph, ms_out);
 }
 
+int kmeans_prepare_locked(igc_ctx *ctx, uint32_t Nmax, uint32_t n, uint32_t k)
+{
+    int rc;
+    if ((rc = kmeans_attach(ctx))) return rc;
+    if ((rc = km_check(Nmax, n, k))) return rc;
+    KmBufs &b = km_bufs(ctx);
+    return wide_words(n) ? km_buffers<uint64_t>(ctx, b, Nmax, n, k) : km_buffers<uint32_t>(ctx, b, Nmax, n, k);
+}
+
+int kmeans_seed_locked(igc_ctx *ctx, uint32_t N, uint32_t k, uint64_t seed, std::vector<uint32_t> &host_seeds)
+{
+    kmeans_seeds(N, k, seed, host_seeds);
+    HIP_TRY(hipMemcpyAsync(km_bufs(ctx).seeds.p, host_seeds.data(), sizeof(uint32_t) * k, hipMemcpyHostToDevice,
+                           ctx->stream));
+    return IGC_OK;
+}
+
+void kmeans_launch_locked(igc_ctx *ctx, const uint8_t *d_syms, uint32_t Nmax, const uint32_t *dN, uint32_t n,
+                          uint32_t k, uint32_t iters)
+{
+    KmBufs &b = km_bufs(ctx);
+    if (wide_words(n)) km_launch<uint64_t>(ctx, b, d_syms, Nmax, dN, n, k, iters);
+    else km_launch<uint32_t>(ctx, b, d_syms, Nmax, dN, n, k, iters);
+}
+
 void kmeans_release(igc_ctx *ctx)
 {
     if (!ctx->km) return;
     KmBufs *b = reinterpret_cast<KmBufs *>(ctx->km);
-    for (DevBuf *d : {&b->C, &b->cn2, &b->cnt, &b->size, &b->seeds, &b->assign}) d->release();
+    for (DevBuf *d : {&b->C, &b->cn2, &b->cnt, &b->size, &b->seeds, &b->assign, &b->packed, &b->flags})
+        d->release();
     delete b;
     ctx->km = nullptr;
 }
@@ -306,13 +394,15 @@ int kmeans_attach(igc_ctx *ctx)
         KmBufs *b = new (std::nothrow) KmBufs();
         if (!b) return set_error(IGC_ERR_OOM, "host allocation failed");
         if (ctx->mem.pool)
-            for (DevBuf *d : {&b->C, &b->cn2, &b->cnt, &b->size, &b->seeds, &b->assign}) d->slot = &ctx->mem;
+            for (DevBuf *d : {&b->C, &b->cn2, &b->cnt, &b->size, &b->seeds, &b->assign, &b->packed, &b->flags})
+                d->slot = &ctx->mem;
         ctx->km = b;
     }
     return IGC_OK;
 }
 
 int32_t *kmeans_assign_ptr(igc_ctx *ctx) { return reinterpret_cast<KmBufs *>(ctx->km)->assign.as<int32_t>(); }
+uint32_t *kmeans_flags_ptr(igc_ctx *ctx) { return reinterpret_cast<KmBufs *>(ctx->km)->flags.as<uint32_t>(); }
 
 } // namespace igc
 
@@ -380,11 +470,11 @@ IGC_API int igc_assign_fp32(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint3
     const size_t lds = sizeof(float) * ((size_t)kc * D + kc);
     if (wide)
         hipLaunchKernelGGL(k_km_assign<uint64_t>, dim3(km_grid(ctx, N)), dim3(kKmThreads), lds, s,
-                           ctx->packed.as<uint64_t>(), N, n, b.C.as<float>(), b.cn2.as<float>(), k, kc,
+                           ctx->packed.as<uint64_t>(), N, nullptr, n, b.C.as<float>(), b.cn2.as<float>(), k, kc,
                            b.assign.as<int32_t>());
     else
         hipLaunchKernelGGL(k_km_assign<uint32_t>, dim3(km_grid(ctx, N)), dim3(kKmThreads), lds, s,
-                           ctx->packed.as<uint32_t>(), N, n, b.C.as<float>(), b.cn2.as<float>(), k, kc,
+                           ctx->packed.as<uint32_t>(), N, nullptr, n, b.C.as<float>(), b.cn2.as<float>(), k, kc,
                            b.assign.as<int32_t>());
     HIP_TRY(hipGetLastError());
     uint32_t host_misc[kMiscWords];

@@ -870,6 +870,14 @@ IGC_API int igc_ctx_set_edge_cap(igc_ctx *ctx, uint64_t edges)
     return IGC_OK;
 }
 
+IGC_API int igc_ctx_set_graph_replay(igc_ctx *ctx, int on)
+{
+    if (!ctx) return set_error(IGC_ERR_INVALID, "null context");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->graph_replay = on != 0;
+    return IGC_OK;
+}
+
 IGC_API int igc_last_timing(igc_ctx *ctx, igc_timing *out)
 {
     if (!ctx || !out) return set_error(IGC_ERR_INVALID, "null argument");

@@ -53,6 +53,11 @@ struct SJob {
     uint64_t id = 0;
     std::vector<uint8_t> syms;
     uint32_t N = 0, n = 0, ed = 0;
+    // a refinement job (igc_server_submit_refine): the clustered loop's
+    // parameters; it runs alone on a pipeline, its state on the device
+    bool refine = false;
+    uint32_t k = 0, km_iters = 0, max_rounds = 0, converge_n = 0;
+    uint64_t seed = 0;
     uint64_t est = 0;
     double t_submit = 0, t_admit = 0, t_gpu = 0, t_greedy = 0, t_done = 0;
     igc_csr csr{};
@@ -71,6 +76,17 @@ static uint64_t job_bytes(uint32_t N, uint32_t n, uint32_t ed)
     const int K = ed == 0 ? -1 : effective_k(n, ed);
     const uint64_t edges = (K < 0 || N < 2) ? 0 : initial_edge_cap(N, n, K);
     return (uint64_t)N * n + 2ull * N * word_bytes(n) + ((uint64_t)N + 1) * 8 + (uint64_t)N * 32 + edges * 48;
+}
+
+// A refinement job's reservation: the resident strings (set, round, cluster
+// order: 3 B per symbol), index lists, the K-means' packed words and
+// assignment, and the first round's batched build (its clusters' pairs,
+// about N^2 / 2k, at the density estimate).
+static uint64_t refine_bytes(uint32_t N, uint32_t n, uint32_t ed, uint32_t k)
+{
+    const uint32_t kk = std::max<uint32_t>(1, std::min<uint32_t>(k, std::max<uint32_t>(N, 1)));
+    const uint32_t per = std::max<uint32_t>(2, N / kk);
+    return 3ull * N * n + 24ull * N + (uint64_t)kk * job_bytes(per, n, ed);
 }
 
 // Host memory the CSRs waiting for the greedy may hold before the pipelines
@@ -94,6 +110,7 @@ struct igc_server {
     uint64_t host_budget = 0, host_pending = 0; // host CSR bytes waiting for the greedy
     int exclusive = -1;        // pipeline re-running a job with the whole budget
     uint32_t max_batch = 512;
+    uint32_t refine_threads = 8; // host greedy threads of a refinement job
     std::mutex mu;
     std::condition_variable cv_queue, cv_greedy, cv_done;
     std::deque<SJob *> queue, greedy_q;
@@ -129,6 +146,24 @@ static bool trim_idle_others(igc_server *S, uint32_t p)
         freed = true;
     }
     return freed;
+}
+
+// A refinement job on pipeline p's context (S->mu not held): the whole loop,
+// its strings and round state staying on the device; the per-cluster greedy
+// runs on `threads` host threads.
+static int run_refine(igc_ctx *ctx, SJob *j, uint32_t threads, std::string &msg)
+{
+    std::lock_guard<std::mutex> clk(ctx->mu);
+    int rc = hipSetDevice(ctx->device) == hipSuccess ? IGC_OK : set_error(IGC_ERR_DEVICE, "hipSetDevice failed");
+    if (!rc) {
+        j->order.resize(std::max<uint32_t>(j->N, 1));
+        uint32_t cnt = 0;
+        rc = refine_locked(ctx, j->syms.data(), j->N, j->n, j->ed, j->k, j->km_iters, j->max_rounds, j->converge_n,
+                           j->seed, threads, ctx->graph_replay, j->order.data(), &cnt, nullptr);
+        j->order.resize(rc ? 0 : cnt);
+    }
+    if (rc) msg = last_error();
+    return rc;
 }
 
 // One batched build of the jobs on pipeline p's context (S->mu not held).
@@ -180,12 +215,12 @@ static void gpu_loop(igc_server *S, uint32_t p)
                 S->cv_done.notify_all();
                 continue;
             }
-            if (j->solo && !batch.empty()) break;
+            if ((j->solo || j->refine) && !batch.empty()) break;
             if (!slot.reserve(bytes + j->est)) break;
             S->queue.pop_front();
             bytes += j->est;
             batch.push_back(j);
-            if (j->solo) break;
+            if (j->solo || j->refine) break;
         }
         if (batch.empty()) {
             if (S->queue.empty()) continue;
@@ -216,6 +251,28 @@ static void gpu_loop(igc_server *S, uint32_t p)
         lk.unlock();
         std::vector<igc_csr> outs;
         std::string msg;
+        if (batch[0]->refine) {
+            // the whole clustered loop on this pipeline; the result is final
+            SJob *j = batch[0];
+            const int rc = run_refine(ctx, j, S->refine_threads, msg);
+            lk.lock();
+            S->busy[p] = 0;
+            const double tg = S->now();
+            S->gpu_busy_s += tg - ta;
+            j->t_gpu = j->t_greedy = tg;
+            j->syms.clear();
+            j->syms.shrink_to_fit();
+            finish_job(S, j, rc, msg.c_str());
+            if (!rc) S->jobs_done++;
+            if (slot.used > S->budget / S->ctx.size()) {
+                std::lock_guard<std::mutex> clk(ctx->mu);
+                ctx_trim_locked(ctx);
+            }
+            slot.settle();
+            S->cv_done.notify_all();
+            S->cv_queue.notify_all();
+            continue;
+        }
         int rc = run_batch(S, ctx, batch, outs, msg);
         lk.lock();
         S->busy[p] = 0;
@@ -359,6 +416,7 @@ IGC_API int igc_server_create(int device, uint64_t hbm_budget_bytes, uint32_t pi
     S->busy.assign(S->ctx.size(), 0);
     S->gpu_alive = (uint32_t)S->ctx.size();
     for (uint32_t p = 0; p < S->ctx.size(); ++p) S->gpu_threads.emplace_back(gpu_loop, S, p);
+    S->refine_threads = host_threads;
     for (uint32_t t = 0; t < host_threads; ++t) S->workers.emplace_back(greedy_loop, S);
     *out = S;
     return IGC_OK;
@@ -392,6 +450,47 @@ IGC_API int igc_server_submit(igc_server *S, const uint8_t *syms, uint32_t N, ui
     j->n = n;
     j->ed = edit_dist;
     j->est = job_bytes(N, n, edit_dist);
+    std::lock_guard<std::mutex> lk(S->mu);
+    j->id = S->next_id++;
+    j->t_submit = S->now();
+    S->jobs[j->id] = j;
+    S->queue.push_back(j);
+    *job_id = j->id;
+    S->cv_queue.notify_one();
+    return IGC_OK;
+}
+
+IGC_API int igc_server_submit_refine(igc_server *S, const uint8_t *syms, uint32_t N, uint32_t n, uint32_t edit_dist,
+                                     uint32_t k, uint32_t kmeans_iters, uint32_t max_rounds, uint32_t converge_n,
+                                     uint64_t seed, uint64_t *job_id)
+{
+    if (!S || (!syms && N) || !job_id) return set_error(IGC_ERR_INVALID, "null argument");
+    if (n < 1 || n > IGC_MAX_QUATERNARY_LEN)
+        return set_error(IGC_ERR_INVALID, "string length n=%u outside 1..%d (K-means runs on quaternary strings)", n,
+                         IGC_MAX_QUATERNARY_LEN);
+    if (k < 1) return set_error(IGC_ERR_INVALID, "k must be >= 1");
+    const size_t bytes = (size_t)N * n;
+    uint8_t any = 0;
+    for (size_t b = 0; b < bytes; ++b) any |= (uint8_t)(syms[b] > 3);
+    if (any) return set_error(IGC_ERR_INVALID, "symbol code above 3 in input");
+    SJob *j = new (std::nothrow) SJob();
+    if (!j) return set_error(IGC_ERR_OOM, "host allocation failed");
+    try {
+        j->syms.assign(syms, syms + bytes);
+    } catch (...) {
+        delete j;
+        return set_error(IGC_ERR_OOM, "host allocation failed");
+    }
+    j->N = N;
+    j->n = n;
+    j->ed = edit_dist;
+    j->refine = true;
+    j->k = k;
+    j->km_iters = kmeans_iters;
+    j->max_rounds = max_rounds;
+    j->converge_n = converge_n;
+    j->seed = seed;
+    j->est = refine_bytes(N, n, edit_dist, k);
     std::lock_guard<std::mutex> lk(S->mu);
     j->id = S->next_id++;
     j->t_submit = S->now();

@@ -165,3 +165,34 @@ def test_edge_buffer_regrow_and_replay(gpu_ctx):
     adj = gpu_ctx.graph_build(syms, 4)
     assert gpu_ctx.last_timing().retries == 0
     assert np.array_equal(adj.col, col)
+
+
+def test_refine_jobs_alone_and_in_the_mix(gpu_ctx):
+    """Refinement jobs (the clustered loop, state kept on the device across its
+    rounds) submitted among graph jobs: each refinement gives Context.refine's
+    codebook, each graph job its single build's."""
+    rng = np.random.default_rng(31)
+    ra = W.vt_refine(seed=6, total=40000)
+    rb = W.vt_refine(seed=7, total=25000)
+    kw = dict(kmeans_iters=5, max_rounds=4, converge_n=2, seed=3)
+    want_a, _ = gpu_ctx.refine(ra, 5, k=32, host_threads=4, **kw)
+    want_b, _ = gpu_ctx.refine(rb, 4, k=16, host_threads=4, **kw)
+    graphs = [(W.filtered_uniform(rng, int(rng.integers(300, 5000)), int(rng.integers(11, 17))),
+               int(rng.integers(3, 6))) for _ in range(30)]
+    srv = igc.Server(0, pipelines=2, host_threads=4)
+    try:
+        ids = [srv.submit(s, ed) for s, ed in graphs[:10]]
+        ia = srv.submit_refine(ra, 5, 32, **kw)
+        ids += [srv.submit(s, ed) for s, ed in graphs[10:20]]
+        ib = srv.submit_refine(rb, 4, 16, **kw)
+        ids += [srv.submit(s, ed) for s, ed in graphs[20:]]
+        res = [srv.wait(j) for j in ids]
+        got_a, got_b = srv.wait(ia), srv.wait(ib)
+    finally:
+        srv.close()
+    assert got_a["status"] == 0 and got_b["status"] == 0, (got_a["error"], got_b["error"])
+    assert np.array_equal(got_a["order"], want_a) and np.array_equal(got_b["order"], want_b)
+    assert got_a["batch_jobs"] == 1 and got_b["batch_jobs"] == 1
+    for (s, ed), r in zip(graphs, res):
+        assert r["status"] == 0
+        assert np.array_equal(r["order"], igc.greedy_order(gpu_ctx.graph_build(s, ed)))
