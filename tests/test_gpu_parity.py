@@ -567,42 +567,66 @@ def test_assign_fp32_matches_oracle(gpu_ctx):
     assert np.array_equal(gpu_ctx.assign_fp32(syms, C), a)
 
 
+def _forms(gpu_ctx, build):
+    out = {}
+    for form in (igc.IGC_CSR_WIDE_KEYS, igc.IGC_CSR_AUTO):
+        gpu_ctx.set_csr_form(form)
+        try:
+            out[form] = build()
+        finally:
+            gpu_ctx.set_csr_form(igc.IGC_CSR_AUTO)
+    return out
+
+
 def test_csr_forms_agree(gpu_ctx):
     """The segmented 32-bit CSR form (split pass on the row's high bits) and the
     64-bit-key form give the same CSR: single sets from 2 to 140,000 strings
     (no split up to 65,536 nodes, a 2-bit split at 70,000, 4-bit at 140,000),
-    dense and sparse, the upper-triangle form, and a batch whose largest set
-    has 100,000 strings (neighbours stored relative to their set's first node,
-    2-bit split)."""
+    dense and sparse, rows of a few entries up to all 17,499 other strings,
+    the upper-triangle form, and a batch whose largest set has 100,000
+    strings (neighbours stored relative to their set's first node, 2-bit
+    split)."""
     rng = np.random.default_rng(66)
     cases = [(2, 5, 3), (700, 9, 4), (6000, 13, 5), (20000, 10, 4), (70000, 11, 3), (140000, 12, 3)]
     for N, n, ed in cases:
         banner(f"csr forms N={N} n={n} ed={ed}")
         syms = rng.integers(0, 4, size=(N, n), dtype=np.uint8)
         _near_copies(rng, syms, N // 4)
-        out = {}
-        for form in (igc.IGC_CSR_WIDE_KEYS, igc.IGC_CSR_AUTO):
-            gpu_ctx.set_csr_form(form)
-            out[form] = (gpu_ctx.graph_build(syms, ed), gpu_ctx.graph_build(syms, ed, igc.IGC_UPPER_TRIANGLE))
-        gpu_ctx.set_csr_form(igc.IGC_CSR_AUTO)
-        (a, au), (b, bu) = out[igc.IGC_CSR_WIDE_KEYS], out[igc.IGC_CSR_AUTO]
-        assert np.array_equal(a.row_ptr, b.row_ptr) and np.array_equal(a.col, b.col), N
-        assert np.array_equal(au.row_ptr, bu.row_ptr) and np.array_equal(au.col, bu.col), N
+        out = _forms(gpu_ctx, lambda: (gpu_ctx.graph_build(syms, ed), gpu_ctx.graph_build(syms, ed,
+                                                                                          igc.IGC_UPPER_TRIANGLE)))
+        (a, au) = out[igc.IGC_CSR_WIDE_KEYS]
+        for form in (igc.IGC_CSR_AUTO,):
+            b, bu = out[form]
+            assert np.array_equal(a.row_ptr, b.row_ptr) and np.array_equal(a.col, b.col), (N, form)
+            assert np.array_equal(au.row_ptr, bu.row_ptr) and np.array_equal(au.col, bu.col), (N, form)
+        b = out[igc.IGC_CSR_AUTO][0]
         check_csr_wellformed(b)
         if N <= 6000:
             rp, col = O.graph_csr(syms, ed)
             assert np.array_equal(b.row_ptr, rp) and np.array_equal(b.col, col)
+    # long rows: clusters of near-identical strings (rows of a few thousand
+    # entries), and one cluster where every pair conflicts
+    for per, clusters in ((3000, 6), (17500, 1)):
+        base = rng.integers(0, 4, size=(clusters, 16), dtype=np.uint8)
+        syms = np.repeat(base, per, axis=0)
+        pos = rng.integers(0, 16, size=syms.shape[0])
+        syms[np.arange(syms.shape[0]), pos] = rng.integers(0, 4, size=syms.shape[0], dtype=np.uint8)
+        syms = syms[rng.permutation(syms.shape[0])]
+        banner(f"csr forms long rows: {clusters} x {per}")
+        out = _forms(gpu_ctx, lambda: gpu_ctx.graph_build(syms, 4))
+        a = out[igc.IGC_CSR_WIDE_KEYS]
+        assert np.diff(a.row_ptr.astype(np.int64)).max() >= per - 1
+        for form in (igc.IGC_CSR_AUTO,):
+            assert np.array_equal(a.row_ptr, out[form].row_ptr) and np.array_equal(a.col, out[form].col), form
     sets = W.batched(seed=5, sets=12, scale=0.1)
     big = rng.integers(0, 4, size=(100000, 12), dtype=np.uint8)
     _near_copies(rng, big, 20000)
     data = [(s.syms, s.edit_dist) for s in sets[:6]] + [(big, 4)] + [(s.syms, s.edit_dist) for s in sets[6:]]
     banner(f"csr forms batch of {len(data)}")
-    gpu_ctx.set_csr_form(igc.IGC_CSR_WIDE_KEYS)
-    wide = gpu_ctx.graph_build_batch(data)
-    gpu_ctx.set_csr_form(igc.IGC_CSR_AUTO)
-    seg = gpu_ctx.graph_build_batch(data)
-    for a, b in zip(wide, seg):
-        assert np.array_equal(a.row_ptr, b.row_ptr) and np.array_equal(a.col, b.col)
+    out = _forms(gpu_ctx, lambda: gpu_ctx.graph_build_batch(data))
+    for form in (igc.IGC_CSR_AUTO,):
+        for a, b in zip(out[igc.IGC_CSR_WIDE_KEYS], out[form]):
+            assert np.array_equal(a.row_ptr, b.row_ptr) and np.array_equal(a.col, b.col)
     with pytest.raises(igc.IGCError):
         gpu_ctx.set_csr_form(2)
 

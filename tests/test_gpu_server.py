@@ -90,9 +90,9 @@ def test_bad_job_refused_at_submission(gpu_ctx):
         assert np.array_equal(r["order"], igc.greedy_order(gpu_ctx.graph_build(g, 4)))
 
 
-def _dense_set(rng, bases=6, per=1000, n=16):
-    """Strings within two substitutions of a few bases: ~3M edges at editDist 5,
-    far above the density estimate of random length-16 strings."""
+def _dense_set(rng, bases=12, per=1000, n=16):
+    """Strings within two substitutions of a few bases: ~6M edges at editDist 5,
+    far above the density estimate of random length-16 strings (~1.1M)."""
     out = []
     for _ in range(bases):
         b = rng.integers(0, 4, size=n, dtype=np.uint8)
@@ -110,9 +110,10 @@ def test_underestimated_job_within_budget(gpu_ctx):
     small = W.filtered_uniform(rng, 500, 12)
     want = igc.greedy_order(gpu_ctx.graph_build(dense, 5))
     E = gpu_ctx.graph_count(dense, 5)
-    assert E > 2_000_000
-    # a budget the estimate fits but the real graph does not: a clean IGC_ERR_OOM
-    budget = 72 << 20
+    assert E > 5_000_000
+    # a budget the estimate (~52 MB) fits but the real graph (edge list and
+    # CSR, ~100 MB) does not: a clean IGC_ERR_OOM
+    budget = 64 << 20
     srv = igc.Server(0, hbm_budget_bytes=budget, pipelines=2, host_threads=2)
     try:
         r = srv.wait(srv.submit(dense, 5))
