@@ -64,10 +64,36 @@ __global__ __launch_bounds__(kKmThreads) void k_km_assign(const W *__restrict__ 
             for (uint32_t t = threadIdx.x; t < m * D; t += blockDim.x) sC[t] = C[(size_t)c0 * D + t];
             for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) sN[t] = cn2[c0 + t];
             __syncthreads();
-            for (uint32_t c = 0; c < m; ++c) {
+            // four centroids at a time, the symbol loop unrolled to the word's
+            // length: the LDS loads issue together (the loop is otherwise
+            // bound by their latency); each sum keeps its order, p ascending
+            uint32_t c = 0;
+            for (; c + 4 <= m; c += 4) {
+                const float *r0 = sC + (size_t)c * D;
+                float a[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (uint32_t p = 0; p < P; ++p) {
+                    if (p < n) {
+#pragma unroll
+                        for (uint32_t q = 0; q < 4; ++q) a[q] = a[q] + r0[q * D + off[p]];
+                    }
+                }
+#pragma unroll
+                for (uint32_t q = 0; q < 4; ++q) {
+                    const float twice = 2.0f * a[q];
+                    const float dist = sN[c + q] - twice;
+                    if (bi < 0 || dist < best) {
+                        best = dist;
+                        bi = (int32_t)(c0 + c + q);
+                    }
+                }
+            }
+            for (; c < m; ++c) {
                 const float *row = sC + (size_t)c * D;
                 float acc = 0.0f;
-                for (uint32_t p = 0; p < n; ++p) acc = acc + row[off[p]];
+#pragma unroll
+                for (uint32_t p = 0; p < P; ++p)
+                    if (p < n) acc = acc + row[off[p]];
                 const float twice = 2.0f * acc;
                 const float dist = sN[c] - twice;
                 if (bi < 0 || dist < best) {
