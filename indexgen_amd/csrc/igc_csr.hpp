@@ -21,6 +21,22 @@
 // src is >= r (binary search).  Every pass moves about 24 B per entry through
 // HBM (read for the count, read + write for the scatter) and needs no global
 // atomics; equal keys cannot occur (each edge is emitted once).
+//
+// Counting is done per group of lanes, not per entry: the entries of a tile
+// come from neighbouring pair tiles, so most of a wave's 64 entries share a
+// handful of digits, and one LDS atomic per entry would serialise on the same
+// counter.  The count kernels add each run of equal digits across neighbouring
+// lanes once (count_runs: a shuffle and a ballot).  The scatters need every
+// lane's rank among all lanes of its digit, for stability: the ballots of
+// wave_match give the group's size and the rank, the lowest lane of a group
+// adds the size, and the ranks are kept for the placement.  Every kernel
+// issues all of a thread's loads before using any of them.
+//
+// Scatters take their tile from xcd_tile: workgroups are dispatched to the 8
+// XCDs round-robin, and consecutive tiles, whose runs of one digit are
+// adjacent in the output, are given to the same XCD in order, so the partial
+// cache lines at the runs' ends meet in that XCD's L2 instead of being
+// written back from two.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -146,6 +162,113 @@ __device__ __forceinline__ uint32_t seg_of_tile(const SegMeta *__restrict__ meta
     return lo;
 }
 
+// Lanes of this wave holding the same digit as this lane (one ballot per digit
+// bit, all kMaxBits of them: bits above the pass's width are zero in every
+// lane and leave the set unchanged); lanes with valid == false belong to no
+// lane's set.
+template <uint32_t kMaxBits>
+__device__ __forceinline__ uint64_t wave_match(uint32_t digit, bool valid)
+{
+    uint64_t match = __ballot(valid);
+#pragma unroll
+    for (uint32_t bit = 0; bit < kMaxBits; ++bit) {
+        const uint32_t m32 = 0u - ((digit >> bit) & 1u); // all ones where the bit is set
+        const uint64_t bb = __builtin_amdgcn_ballot_w64(((digit >> bit) & 1u) != 0u);
+        match &= ~(bb ^ (((uint64_t)m32 << 32) | m32)); // bb where set, ~bb where clear
+    }
+    return match;
+}
+
+// Adds this round's entries to the histogram h, one atomic per run of equal
+// digits in neighbouring lanes.  The valid lanes are a prefix of the wave.
+__device__ __forceinline__ void count_runs(uint32_t *h, uint32_t digit, bool valid)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t prev = __shfl_up(digit, 1, 64);
+    const bool start = valid && (lane == 0 || prev != digit);
+    const uint64_t starts = __ballot(start);
+    const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));
+    if (start) {
+        const uint64_t later = starts & ~((2ull << lane) - 1ull); // starts above this lane (none for lane 63)
+        const uint32_t end = later ? (uint32_t)__builtin_ctzll(later) : nvalid;
+        atomicAdd(&h[digit], end - lane);
+    }
+}
+
+// Tile of workgroup b of G: XCD x (= b % 8) takes tiles x*q + min(x, G%8) ..
+// in order, q = G/8 (a bijection of 0..G-1).
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t G)
+{
+    const uint32_t x = b & 7u, i = b >> 3, q = G >> 3, rem = G & 7u;
+    return x * q + (x < rem ? x : rem) + i;
+}
+
+// Exclusive scan over the digits of a tile's per-wave counts wpos[w][d] (one
+// digit per thread, threads past kDigits own none).  Afterwards wpos[w][d] is
+// where wave w's first entry with digit d goes in the tile's digit-ordered
+// stage, and the return value is the stage position of digit d's run (for
+// the thread owning d).
+template <int kThreads, uint32_t kDigits>
+__device__ __forceinline__ uint32_t tile_digit_scan(uint32_t (*wpos)[kDigits], uint32_t *wsum)
+{
+    constexpr int kWaves = kThreads / 64;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t d = threadIdx.x;
+    uint32_t c = 0;
+    if (d < kDigits)
+        for (int w = 0; w < kWaves; ++w) c += wpos[w][d];
+    uint32_t incl = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o, 64);
+        if (lane >= (uint32_t)o) incl += y;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint32_t run = incl - c;
+    for (uint32_t w = 0; w < wave; ++w) run += wsum[w];
+    if (d < kDigits) {
+        uint32_t pos = run;
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t cw = wpos[w][d];
+            wpos[w][d] = pos;
+            pos += cw;
+        }
+    }
+    return run;
+}
+
+// A lane's result of the counting step, kept for the placement: rank in its
+// digit group (6 bits), the group's size (7 bits), the digit (up to 9 bits).
+// The empty asm makes the packed word the only value carried to the placement
+// (left alone, the compiler keeps the digit and the size in two registers per
+// round, 32 more registers for the 32-bit scatter's 16 rounds).
+__device__ __forceinline__ uint32_t rank_word(uint32_t rank, uint32_t cnt, uint32_t digit)
+{
+    uint32_t w = rank | (cnt << 6) | (digit << 13);
+    __asm__ volatile("" : "+v"(w));
+    return w;
+}
+
+// Places one round of a wave's entries (64 consecutive entries of the tile)
+// in the digit-ordered stage, from the counting step's rank word.  Every lane
+// reads its digit's position before the group's first lane advances it (one
+// wave: LDS operations complete in issue order).
+template <typename K>
+__device__ __forceinline__ void stage_round(K *stage, uint32_t *mypos, bool valid, uint32_t rw, K key)
+{
+    const uint32_t rank = rw & 63u, cnt = (rw >> 6) & 127u, digit = rw >> 13;
+    const uint32_t b0 = valid ? mypos[digit] : 0u;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (valid) {
+        stage[b0 + rank] = key;
+        if (rank == 0) mypos[digit] = b0 + cnt;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // counts[tfirst*nd + d*ntiles + k] = entries with digit d in tile k of the
 // segment (32-bit keys, digit = key >> shift, `bits` wide).  Workgroups past
 // the last tile zero the rest of the counts (up to gridDim.x * nd), which the
@@ -155,6 +278,7 @@ static __global__ __launch_bounds__(kSegThreads) void k_seg_count(const uint32_t
                                                                    const uint32_t *__restrict__ total, uint32_t shift,
                                                                    uint32_t bits, uint32_t *__restrict__ counts)
 {
+    constexpr uint32_t kPerWave = kSegTile / (kSegThreads / 64);
     __shared__ uint32_t h[kSegMaxDigits];
     const uint32_t t = blockIdx.x;
     const uint32_t nd = 1u << bits;
@@ -168,21 +292,33 @@ static __global__ __launch_bounds__(kSegThreads) void k_seg_count(const uint32_t
     const uint32_t k = t - m.tfirst;
     const uint64_t first = m.pstart + (uint64_t)k * kSegTile;
     const uint64_t left = m.size - (uint64_t)k * kSegTile;
-    const uint32_t n = (uint32_t)(left < kSegTile ? left : kSegTile);
+    const uint32_t tile_n = (uint32_t)(left < kSegTile ? left : kSegTile);
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t key[kSegRounds];
+#pragma unroll
+    for (int r = 0; r < kSegRounds; ++r) {
+        const uint32_t li = wave * kPerWave + (uint32_t)r * 64 + lane;
+        key[r] = li < tile_n ? keys[first + li] : 0u;
+    }
     for (uint32_t d = threadIdx.x; d < nd; d += blockDim.x) h[d] = 0;
     __syncthreads();
-    for (uint32_t li = threadIdx.x; li < n; li += kSegThreads)
-        atomicAdd(&h[(keys[first + li] >> shift) & (nd - 1u)], 1u);
+#pragma unroll
+    for (int r = 0; r < kSegRounds; ++r)
+        count_runs(h, (key[r] >> shift) & (nd - 1u), wave * kPerWave + (uint32_t)r * 64 + lane < tile_n);
     __syncthreads();
     for (uint32_t d = threadIdx.x; d < nd; d += blockDim.x)
         counts[(uint64_t)m.tfirst * nd + (uint64_t)d * m.ntiles + k] = h[d];
 }
 
-// Stable scatter of one segment tile of 32-bit keys (ranking as in
-// k_radix_scatter).  Not last: keys go to their padded slots.  Last: the entry
-// at final position g gets col[g] = neighbour and src[g] = row, where the key
-// holds (row low bits) << dbits | (neighbour - base[row]) and the row's high
-// bits are the segment.
+// Stable scatter of one segment tile of 32-bit keys.  Wave w owns the tile's
+// entries w*1024 .. w*1024+1023 (16 rounds of 64), so input order within the
+// tile is wave-major; each wave counts its digit groups into LDS, one scan
+// over the tile's digits gives every wave its running position per digit, the
+// waves place their entries in digit order in LDS with the ranks kept from the
+// counting, and each digit run is then written out contiguously.  Not last:
+// keys go to their padded slots.  Last: the entry at final position g gets
+// col[g] = neighbour and src[g] = row, where the key holds (row low bits) <<
+// dbits | (neighbour - base[row]) and the row's high bits are the segment.
 static __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(
     const uint32_t *__restrict__ keys, const SegMeta *__restrict__ meta, uint32_t nseg,
     const uint32_t *__restrict__ total, uint32_t shift, uint32_t bits, const uint64_t *__restrict__ offsets,
@@ -191,13 +327,12 @@ static __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(
 {
     constexpr int kWaves = kSegThreads / 64;
     constexpr uint32_t kPerWave = kSegTile / kWaves;
-    constexpr uint32_t kDigitsPerThreadS = (kSegMaxDigits + kSegThreads - 1) / kSegThreads; // 1
+    static_assert(kSegMaxDigits <= kSegThreads, "one digit per thread");
     __shared__ uint32_t stage[kSegTile];
-    __shared__ uint64_t gbase[kSegMaxDigits];
-    __shared__ uint32_t lstart[kSegMaxDigits];
+    __shared__ uint64_t dst[kSegMaxDigits]; // output index of stage position 0, per digit
     __shared__ uint32_t wpos[kWaves][kSegMaxDigits];
     __shared__ uint32_t wsum[kWaves];
-    const uint32_t t = blockIdx.x;
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= total[0]) return; // whole workgroup
     const uint32_t sg = seg_of_tile(meta, nseg, t);
     const SegMeta m = meta[sg];
@@ -207,97 +342,112 @@ static __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(
     const uint32_t tile_n = (uint32_t)(left < kSegTile ? left : kSegTile);
     const uint32_t nd = 1u << bits;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    // this thread's digit offset, loaded now so its latency overlaps the counting
     const uint64_t my_off =
         threadIdx.x < nd ? offsets[(uint64_t)m.tfirst * nd + (uint64_t)threadIdx.x * m.ntiles + k] : 0ull;
-    for (uint32_t d = threadIdx.x; d < kSegMaxDigits; d += kSegThreads)
-        for (int w = 0; w < kWaves; ++w) wpos[w][d] = 0;
-    __syncthreads();
     uint32_t key[kSegRounds];
 #pragma unroll
     for (int r = 0; r < kSegRounds; ++r) {
         const uint32_t li = wave * kPerWave + (uint32_t)r * 64 + lane;
         key[r] = li < tile_n ? keys[first + li] : 0u;
-        if (li < tile_n) atomicAdd(&wpos[wave][(key[r] >> shift) & (nd - 1u)], 1u);
+    }
+    for (uint32_t d = threadIdx.x; d < kSegMaxDigits; d += kSegThreads)
+        for (int w = 0; w < kWaves; ++w) wpos[w][d] = 0;
+    __syncthreads();
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t rw[kSegRounds];
+#pragma unroll
+    for (int r = 0; r < kSegRounds; ++r) {
+        const bool valid = wave * kPerWave + (uint32_t)r * 64 + lane < tile_n;
+        const uint32_t digit = (key[r] >> shift) & (nd - 1u);
+        const uint64_t match = wave_match<kSegMaxBits>(digit, valid);
+        const uint32_t rank = (uint32_t)__popcll(match & lt), cnt = (uint32_t)__popcll(match);
+        rw[r] = rank_word(rank, cnt, digit);
+        if (valid && rank == 0) atomicAdd(&wpos[wave][digit], cnt);
     }
     __syncthreads();
     {
-        // thread d owns digit d (threads past the digits hold zeros)
-        uint32_t c[kDigitsPerThreadS], tsum = 0;
-#pragma unroll
-        for (uint32_t q = 0; q < kDigitsPerThreadS; ++q) {
-            const uint32_t d = threadIdx.x * kDigitsPerThreadS + q;
-            c[q] = 0;
-            if (d < kSegMaxDigits) {
-                for (int w = 0; w < kWaves; ++w) c[q] += wpos[w][d];
-                gbase[d] = my_off; // d == threadIdx.x (one digit per thread)
-            }
-            tsum += c[q];
-        }
-        uint32_t incl = tsum;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(incl, o, 64);
-            if (lane >= (uint32_t)o) incl += y;
-        }
-        if (lane == 63) wsum[wave] = incl;
-        __syncthreads();
-        uint32_t run = incl - tsum;
-        for (uint32_t w = 0; w < wave; ++w) run += wsum[w];
-#pragma unroll
-        for (uint32_t q = 0; q < kDigitsPerThreadS; ++q) {
-            const uint32_t d = threadIdx.x * kDigitsPerThreadS + q;
-            if (d >= kSegMaxDigits) break;
-            lstart[d] = run;
-            uint32_t pos = run;
-            for (int w = 0; w < kWaves; ++w) {
-                const uint32_t cw = wpos[w][d];
-                wpos[w][d] = pos;
-                pos += cw;
-            }
-            run += c[q];
-        }
+        const uint32_t run = tile_digit_scan<kSegThreads, kSegMaxDigits>(wpos, wsum);
+        // final index = gbase + (stage position - run); padded slot when not last
+        if (threadIdx.x < kSegMaxDigits)
+            dst[threadIdx.x] = my_off - run + (last ? 0ull : m.pstart - m.ustart);
     }
     __syncthreads();
-    const uint64_t lt = (1ull << lane) - 1ull;
     uint32_t *mypos = wpos[wave];
 #pragma unroll
     for (int r = 0; r < kSegRounds; ++r) {
-        const uint32_t li = wave * kPerWave + (uint32_t)r * 64 + lane;
-        const bool valid = li < tile_n;
-        const uint32_t digit = (key[r] >> shift) & (nd - 1u);
-        uint64_t match = __ballot(valid);
-#pragma unroll
-        for (uint32_t bit = 0; bit < kSegMaxBits; ++bit) {
-            if (bit >= bits) break; // uniform
-            const bool set = (digit >> bit) & 1u;
-            const uint64_t bb = __ballot(set);
-            match &= set ? bb : ~bb;
-        }
-        const uint32_t rank = (uint32_t)__popcll(match & lt);
-        const uint32_t b0 = valid ? mypos[digit] : 0u;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        if (valid) {
-            stage[b0 + rank] = key[r];
-            if (rank == 0) mypos[digit] = b0 + (uint32_t)__popcll(match);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        __builtin_amdgcn_wave_barrier();
+        const bool valid = wave * kPerWave + (uint32_t)r * 64 + lane < tile_n;
+        stage_round<uint32_t>(stage, mypos, valid, rw[r], key[r]);
     }
     __syncthreads();
     const uint32_t dmask = dbits >= 32 ? 0xFFFFFFFFu : ((1u << dbits) - 1u);
-    for (uint32_t li = threadIdx.x; li < tile_n; li += kSegThreads) {
-        const uint32_t kk = stage[li];
-        const uint32_t digit = (kk >> shift) & (nd - 1u);
-        const uint64_t g = gbase[digit] + (li - lstart[digit]);
-        if (last) {
-            const uint32_t row = (sg << rowbits) | (dbits >= 32 ? 0u : (kk >> dbits));
-            col[g] = (kk & dmask) + (base ? base[row] : 0u);
-            src[g] = row;
-        } else {
-            out[seg_slot(m, g)] = kk;
+#pragma unroll 4
+    for (int r = 0; r < kSegRounds; ++r) {
+        const uint32_t li = (uint32_t)r * kSegThreads + threadIdx.x;
+        if (li < tile_n) {
+            const uint32_t kk = stage[li];
+            const uint64_t g = dst[(kk >> shift) & (nd - 1u)] + li;
+            if (last) {
+                const uint32_t row = (sg << rowbits) | (dbits >= 32 ? 0u : (kk >> dbits));
+                col[g] = (kk & dmask) + (base ? base[row] : 0u);
+                src[g] = row;
+            } else {
+                out[g] = kk;
+            }
         }
+    }
+}
+
+// Entry of the tile (0..kRadixTile-1) that lane `lane` of wave `wave` holds
+// in round r.  Passes over keys keep input order wave-major (the scatter is
+// stable).  A first pass from the edge list may take its tile's entries in any
+// order (its input order is arbitrary anyway): there round 2s holds the
+// direction i -> j of 64 consecutive edges and round 2s+1 the direction j -> i
+// of the same edges, so a round's rows come from one side of the pair tiles
+// that emitted them.  Either way the valid entries of a round (local index <
+// the tile's size) are a prefix of the wave.
+__device__ __forceinline__ uint32_t radix_local(const RadixPass &p, uint32_t wave, int r, uint32_t lane)
+{
+    constexpr uint32_t kPerWave = kRadixTile / (kRadixThreads / 64);
+    if (p.from_edges && !p.upper) return wave * kPerWave + ((uint32_t)r >> 1) * 128 + lane * 2 + ((uint32_t)r & 1);
+    return wave * kPerWave + (uint32_t)r * 64 + lane;
+}
+
+// This thread's keys for rounds r = 0..kRadixRounds-1 (entry start +
+// radix_local), every load issued before any is used.  full == false, for a
+// count whose digit lies in the row (the split pass): the neighbour field is
+// left zero and the set bases are not read.
+__device__ __forceinline__ void radix_load(const RadixPass &p, uint64_t start, uint32_t tile_n, bool full,
+                                           uint64_t (&key)[kRadixRounds])
+{
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (!p.from_edges) {
+#pragma unroll
+        for (int r = 0; r < kRadixRounds; ++r) {
+            const uint32_t li = radix_local(p, wave, r, lane);
+            key[r] = li < tile_n ? p.keys[start + li] : 0ull;
+        }
+        return;
+    }
+    uint64_t v[kRadixRounds];
+#pragma unroll
+    for (int r = 0; r < kRadixRounds; ++r) {
+        const uint32_t li = radix_local(p, wave, r, lane);
+        const uint64_t t = start + li;
+        v[r] = li < tile_n ? p.edges[p.upper ? t : t >> 1] : 0ull;
+    }
+    uint32_t b[kRadixRounds];
+#pragma unroll
+    for (int r = 0; r < kRadixRounds; ++r) {
+        const uint32_t li = radix_local(p, wave, r, lane);
+        b[r] = (full && p.base && li < tile_n) ? p.base[v[r] >> 32] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kRadixRounds; ++r) {
+        const uint64_t t = start + radix_local(p, wave, r, lane);
+        const uint64_t i = v[r] >> 32, j = (uint32_t)v[r];
+        const bool flip = p.upper ? j < i : (t & 1) != 0; // row = j
+        const uint64_t row = flip ? j : i, nbr = flip ? i : j;
+        key[r] = (row << p.dbits) | (full ? nbr - b[r] : 0ull);
     }
 }
 
@@ -308,27 +458,26 @@ static __global__ __launch_bounds__(kRadixThreads) void k_radix_count(RadixPass 
     __shared__ uint32_t h[kRadixMaxDigits];
     const uint32_t b = blockIdx.x;
     const uint32_t nd = 1u << p.bits;
-    for (uint32_t d = threadIdx.x; d < nd; d += blockDim.x) h[d] = 0;
-    __syncthreads();
     const uint64_t n = radix_items(p);
     const uint64_t start = (uint64_t)b * kRadixTile;
-    for (int r = 0; r < kRadixRounds; ++r) {
-        const uint64_t t = start + (uint64_t)r * kRadixThreads + threadIdx.x;
-        if (t < n) atomicAdd(&h[radix_digit(p, radix_key(p, t))], 1u);
-    }
+    const uint32_t tile_n = start >= n ? 0u : (uint32_t)((n - start) < kRadixTile ? (n - start) : kRadixTile);
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t key[kRadixRounds];
+    radix_load(p, start, tile_n, !(p.from_edges && p.shift >= p.dbits), key);
+    for (uint32_t d = threadIdx.x; d < nd; d += blockDim.x) h[d] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRadixRounds; ++r) count_runs(h, radix_digit(p, key[r]), radix_local(p, wave, r, lane) < tile_n);
     __syncthreads();
     for (uint32_t d = threadIdx.x; d < nd; d += blockDim.x) counts[(uint64_t)d * tiles + b] = h[d];
 }
 
 // Stable scatter of tile b; offsets[d * tiles + b] is where the tile's first
 // entry with digit d goes.  Wave w owns the tile's entries w*512 .. w*512+511
-// (8 rounds of 64), so input order within the tile is wave-major.  Each wave
-// counts its digits into LDS; one scan over the tile's digits gives every wave
-// its running position per digit; the waves then rank their entries round by
-// round with wave-level operations only (one ballot per digit bit gives each
-// lane the lanes holding its digit), placing the tile in digit order in LDS;
-// finally each digit run is written out contiguously by consecutive threads.
-// last: write col/src instead of keys.
+// (8 rounds of 64), so input order within the tile is wave-major; counting,
+// placement in LDS and the contiguous write-out as in k_seg_scatter.  seg: the
+// split pass of the segmented form (the low 32 bits of each key go to the
+// padded slot of its segment); else last: write col/src instead of keys.
 static __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(RadixPass p, uint32_t tiles,
                                                                          const uint64_t *__restrict__ offsets,
                                                                          uint32_t last, uint64_t *__restrict__ out,
@@ -338,111 +487,63 @@ static __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(RadixPas
                                                                          uint32_t *__restrict__ out32)
 {
     constexpr int kWaves = kRadixThreads / 64;
-    constexpr uint32_t kPerWave = kRadixTile / kWaves;
+    static_assert(kDigitsPerThread == 1, "one digit per thread");
     __shared__ uint64_t stage[kRadixTile];
-    __shared__ uint64_t gbase[kRadixMaxDigits];        // global position of the tile's first entry per digit
-    __shared__ uint32_t lstart[kRadixMaxDigits];       // local position of the same
-    __shared__ uint32_t wpos[kWaves][kRadixMaxDigits]; // per wave: digit count, then next local position
+    __shared__ uint64_t dst[kRadixMaxDigits];          // output index of stage position 0, per digit
+    __shared__ uint32_t wpos[kWaves][kRadixMaxDigits]; // per wave: digit count, then next stage position
     __shared__ uint32_t wsum[kWaves];
-    const uint32_t b = blockIdx.x;
+    const uint32_t b = xcd_tile(blockIdx.x, gridDim.x);
     const uint64_t n = radix_items(p);
     const uint64_t start = (uint64_t)b * kRadixTile;
     if (start >= n) return; // whole workgroup: no barrier is skipped by part of it
     const uint32_t tile_n = (uint32_t)((n - start) < kRadixTile ? (n - start) : kRadixTile);
     const uint32_t nd = 1u << p.bits;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    static_assert(kDigitsPerThread == 1, "one digit per thread");
-    // this thread's digit offset, loaded now so its latency overlaps the counting
     const uint64_t my_off = threadIdx.x < nd ? offsets[(uint64_t)threadIdx.x * tiles + b] : 0ull;
+    uint64_t slot_shift = 0; // segment slot - final position, for the thread's digit
+    if (seg && threadIdx.x < nd) slot_shift = seg[threadIdx.x].pstart - seg[threadIdx.x].ustart;
+    uint64_t key[kRadixRounds];
+    radix_load(p, start, tile_n, true, key);
     for (uint32_t d = threadIdx.x; d < kRadixMaxDigits; d += kRadixThreads)
         for (int w = 0; w < kWaves; ++w) wpos[w][d] = 0;
     __syncthreads();
-    uint64_t key[kRadixRounds];
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t rw[kRadixRounds];
 #pragma unroll
     for (int r = 0; r < kRadixRounds; ++r) {
-        const uint32_t li = wave * kPerWave + (uint32_t)r * 64 + lane;
-        key[r] = li < tile_n ? radix_key(p, start + li) : 0ull;
-        if (li < tile_n) atomicAdd(&wpos[wave][radix_digit(p, key[r])], 1u);
+        const bool valid = radix_local(p, wave, r, lane) < tile_n;
+        const uint32_t digit = radix_digit(p, key[r]);
+        const uint64_t match = wave_match<kRadixMaxBits>(digit, valid);
+        const uint32_t rank = (uint32_t)__popcll(match & lt), cnt = (uint32_t)__popcll(match);
+        rw[r] = rank_word(rank, cnt, digit);
+        if (valid && rank == 0) atomicAdd(&wpos[wave][digit], cnt);
     }
     __syncthreads();
     {
-        // tile counts, their exclusive scan over digits, and each wave's start;
-        // thread t owns digits t*kDigitsPerThread .. +kDigitsPerThread-1
-        uint32_t c[kDigitsPerThread], tsum = 0;
-#pragma unroll
-        for (uint32_t k = 0; k < kDigitsPerThread; ++k) {
-            const uint32_t d = threadIdx.x * kDigitsPerThread + k;
-            c[k] = 0;
-            for (int w = 0; w < kWaves; ++w) c[k] += wpos[w][d];
-            gbase[d] = my_off; // d == threadIdx.x
-            tsum += c[k];
-        }
-        uint32_t incl = tsum;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(incl, o, 64);
-            if (lane >= (uint32_t)o) incl += y;
-        }
-        if (lane == 63) wsum[wave] = incl;
-        __syncthreads();
-        uint32_t run = incl - tsum;
-        for (uint32_t w = 0; w < wave; ++w) run += wsum[w];
-#pragma unroll
-        for (uint32_t k = 0; k < kDigitsPerThread; ++k) {
-            const uint32_t d = threadIdx.x * kDigitsPerThread + k;
-            lstart[d] = run;
-            uint32_t pos = run;
-            for (int w = 0; w < kWaves; ++w) {
-                const uint32_t cw = wpos[w][d];
-                wpos[w][d] = pos;
-                pos += cw;
-            }
-            run += c[k];
-        }
+        const uint32_t run = tile_digit_scan<kRadixThreads, kRadixMaxDigits>(wpos, wsum);
+        if (threadIdx.x < kRadixMaxDigits) dst[threadIdx.x] = my_off - run + slot_shift;
     }
     __syncthreads();
-    const uint64_t lt = (1ull << lane) - 1ull; // lanes below this one
     uint32_t *mypos = wpos[wave];
 #pragma unroll
-    for (int r = 0; r < kRadixRounds; ++r) {
-        const uint32_t li = wave * kPerWave + (uint32_t)r * 64 + lane;
-        const bool valid = li < tile_n;
-        const uint32_t digit = radix_digit(p, key[r]);
-        uint64_t match = __ballot(valid);
-#pragma unroll
-        for (uint32_t bit = 0; bit < kRadixMaxBits; ++bit) {
-            if (bit >= p.bits) break; // uniform
-            const bool set = (digit >> bit) & 1u;
-            const uint64_t bb = __ballot(set);
-            match &= set ? bb : ~bb;
-        }
-        const uint32_t rank = (uint32_t)__popcll(match & lt);
-        // every lane reads its digit's position before the group's first lane
-        // advances it (one wave: LDS operations complete in issue order)
-        const uint32_t base = valid ? mypos[digit] : 0u;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        if (valid) {
-            stage[base + rank] = key[r];
-            if (rank == 0) mypos[digit] = base + (uint32_t)__popcll(match);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-    }
+    for (int r = 0; r < kRadixRounds; ++r)
+        stage_round<uint64_t>(stage, mypos, radix_local(p, wave, r, lane) < tile_n, rw[r], key[r]);
     __syncthreads();
-    // global position = gbase[d] + (local position - start of digit d's run)
     const uint64_t mask = (1ull << p.bsh) - 1ull;
-    for (uint32_t li = threadIdx.x; li < tile_n; li += kRadixThreads) {
-        const uint64_t k = stage[li];
-        const uint32_t digit = radix_digit(p, k);
-        const uint64_t g = gbase[digit] + (li - lstart[digit]);
-        if (seg) {
-            out32[seg_slot(seg[digit], g)] = (uint32_t)k; // the segment's padded slot
-        } else if (last) {
-            col[g] = (uint32_t)(k & mask);
-            src[g] = (uint32_t)(k >> p.bsh);
-        } else {
-            out[g] = k;
+#pragma unroll 4
+    for (int r = 0; r < kRadixRounds; ++r) {
+        const uint32_t li = (uint32_t)r * kRadixThreads + threadIdx.x;
+        if (li < tile_n) {
+            const uint64_t k = stage[li];
+            const uint64_t g = dst[radix_digit(p, k)] + li;
+            if (seg) {
+                out32[g] = (uint32_t)k; // the segment's padded slot
+            } else if (last) {
+                col[g] = (uint32_t)(k & mask);
+                src[g] = (uint32_t)(k >> p.bsh);
+            } else {
+                out[g] = k;
+            }
         }
     }
 }
