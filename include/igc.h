@@ -482,6 +482,12 @@ IGC_API int igc_server_destroy(igc_server *srv);
  * DeleteRow, src/SparseMat.cpp:81-251; Codebook loop :698-729).  Writes the
  * selected node ids in selection order to `order` (capacity n_nodes). */
 IGC_API int igc_greedy(const igc_csr *csr, uint32_t *order, uint32_t *count);
+/* The same with a thread count (0: automatic, one thread below 2^24 edges, else
+ * three; at most 17).  The calling thread makes every bucket move in the
+ * reference's order; threads - 1 helpers find, ahead of it, each ball member's
+ * neighbours still active, so the selection is the same for every count.  No
+ * counterpart in the reference, whose greedy is one thread. */
+IGC_API int igc_greedy_mt(const igc_csr *csr, uint32_t *order, uint32_t *count, uint32_t threads);
 
 /* Writes the graph as the reference's edge file: little-endian int32 pairs
  * (u, v), u < v, one per edge, ascending (format of src/gpu_graph_generator.py:57-77

@@ -219,6 +219,7 @@ def lib() -> ctypes.CDLL:
         "igc_csr_free": (i, [ctypes.POINTER(_Csr)]),
         "igc_csr_reorder": (i, [ctypes.POINTER(_Csr), u32]),
         "igc_greedy": (i, [ctypes.POINTER(_Csr), vp, ctypes.POINTER(u32)]),
+        "igc_greedy_mt": (i, [ctypes.POINTER(_Csr), vp, ctypes.POINTER(u32), u32]),
         "igc_write_pair_file": (i, [ctypes.POINTER(_Csr), ctypes.c_char_p]),
         "igc_write_codebook": (i, [ctypes.c_char_p, u8p, u32, vp, u32, ctypes.POINTER(_CodebookInfo),
                                    ctypes.c_char_p, ctypes.c_size_t]),
@@ -764,13 +765,16 @@ def reorder(adj: AdjList, flags: int) -> AdjList:
     return out
 
 
-def greedy_order(adj: AdjList) -> np.ndarray:
+def greedy_order(adj: AdjList, threads: int = 0) -> np.ndarray:
     """Node ids of the min-degree greedy independent set, in selection order
-    (AdjList::FindMinDel loop of Codebook, src/SparseMat.cpp:698-729)."""
+    (AdjList::FindMinDel loop of Codebook, src/SparseMat.cpp:698-729).
+    threads: 0 = automatic (one below 2^24 edges, else three: the moving
+    thread and two that scan ahead of it); any count gives the same
+    selection."""
     c = adj._as_c()
     order = np.zeros(max(adj.n_nodes, 1), dtype=np.uint32)
     cnt = ctypes.c_uint32()
-    _check(lib().igc_greedy(ctypes.byref(c), order.ctypes.data, ctypes.byref(cnt)))
+    _check(lib().igc_greedy_mt(ctypes.byref(c), order.ctypes.data, ctypes.byref(cnt), int(threads)))
     return order[: cnt.value].copy()
 
 

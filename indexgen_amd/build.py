@@ -22,7 +22,7 @@ CLI = os.path.join(HERE, "igc_cli")
 SOURCES = ["igc_runtime.hip", "igc_batch.hip", "igc_kmeans.hip", "igc_refine.hip", "igc_server.hip", "igc_gen.hip",
            "igc_noalloc.hip"]
 # plain host C++ (greedy, output writers): g++, no HIP
-HOST_SOURCES = ["igc_host.cpp"]
+HOST_SOURCES = ["igc_host.cpp", "igc_greedy.cpp"]
 
 HIPCC_FLAGS = [
     "-O3",
@@ -57,7 +57,7 @@ def build(verbose: bool = False) -> str:
     jobs = []
     for h in HOST_SOURCES:
         obj = os.path.join(CSRC, os.path.splitext(h)[0] + ".host.o")
-        jobs.append((obj, ["g++", "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden", *inc, "-c", "-o", obj,
+        jobs.append((obj, ["g++", "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-pthread", *inc, "-c", "-o", obj,
                            os.path.join(CSRC, h)]))
     for src in SOURCES:
         obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")

@@ -1,5 +1,6 @@
-// igc_host.cpp -- host side of libigc.so: the greedy independent set, neighbour
-// order policies and the reference's output formats.  Plain C++17, no device code.
+// igc_host.cpp -- host side of libigc.so: host memory for results, neighbour
+// order policies and the reference's output formats (the greedy independent
+// set is in igc_greedy.cpp).  Plain C++17, no device code.
 #include "igc_internal.hpp"
 
 #include <sys/mman.h>
@@ -124,179 +125,11 @@ void reorder_threads(igc_csr &csr, uint32_t T)
     }
 }
 
-// Min-degree greedy over a CSR, with the bucket mechanics of the reference's
-// AdjList (src/SparseMat.cpp:81-251):
-//   - bucket d lists the active nodes of current degree d; built in node order
-//     (RowsBySum, :92-120);
-//   - the pick is the LAST node of the lowest non-empty bucket (MinSumRow, :122-127);
-//   - DelBall (:229-251) deletes the pick's active neighbours in list order and
-//     then the pick; each deletion (DelRowCol, :207-227) moves every active
-//     neighbour u from bucket deg(u) to deg(u)-1 (DecreaseSum, :156-167): a
-//     swap-remove that fills the hole with the bucket's last node (DeleteRow,
-//     :139-154) and an append to the lower bucket.
-// Layout for speed, not for semantics: a node's degree and bucket position
-// share one 8-byte record, and a deleted node's degree field holds kDead, so
-// the most frequent test (deleted or degree 0) reads that record only; buckets
-// live in one flat array (bucket d never holds more nodes than there are nodes
-// of initial degree >= d, since degrees only fall), and the records of the
-// next neighbours are prefetched.  The
-// visiting order is exactly the reference's.
-class Greedy {
-  public:
-    Greedy(uint32_t N, const uint64_t *rp, const uint32_t *col) : N_(N), rp_(rp), col_(col) {}
-
-    int run(uint32_t *order, uint32_t *count)
-    {
-        uint32_t maxd = 0;
-        try {
-            st_.resize(N_);
-            dead_.assign(((size_t)N_ + 63) / 64, 0ull);
-            for (uint32_t v = 0; v < N_; ++v) {
-                st_[v].deg = (uint32_t)(rp_[v + 1] - rp_[v]);
-                maxd = std::max(maxd, st_[v].deg);
-            }
-            nb_ = maxd + 1;
-            std::vector<uint64_t> ge(nb_ + 1, 0);
-            for (uint32_t v = 0; v < N_; ++v) ge[st_[v].deg]++;
-            for (int64_t d = (int64_t)nb_ - 1; d >= 0; --d) ge[d] += ge[d + 1];
-            bk_.resize(nb_);
-            uint64_t off = 0;
-            for (uint32_t d = 0; d < nb_; ++d) {
-                bk_[d].off = off;
-                bk_[d].len = 0;
-                off += ge[d];
-            }
-            // capacity 2|E| + N, but only bucket tails are ever touched: leave it
-            // uninitialised so untouched pages are never faulted in
-            store_.reset(new uint32_t[off > 0 ? off : 1]);
-        } catch (const std::bad_alloc &) {
-            return set_error(IGC_ERR_OOM, "greedy: host allocation failed");
-        }
-        tracker_ = nb_;
-        for (uint32_t v = 0; v < N_; ++v) {
-            push(st_[v].deg, v);
-            if (st_[v].deg < tracker_) tracker_ = st_[v].deg;
-        }
-        uint32_t active = N_, picked = 0;
-        try {
-            ball_.resize((size_t)maxd + 2);
-            live_.resize((size_t)maxd + 1);
-        } catch (const std::bad_alloc &) {
-            return set_error(IGC_ERR_OOM, "greedy: host allocation failed");
-        }
-        while (active > 0) {
-            const Bucket &b = bk_[tracker_];
-            const uint32_t v = store_[b.off + b.len - 1];
-            order[picked++] = v;
-            // the ball: v's active neighbours in list order, then v (compacted
-            // without branches: about half the tests fail, unpredictably)
-            const uint64_t v0 = rp_[v], v1 = rp_[v + 1];
-            uint32_t nb = 0;
-            for (uint64_t q = v0; q < v1; ++q) {
-                const uint32_t u = col_[q];
-                ball_[nb] = u;
-                nb += 1u - dead(u);
-            }
-            ball_[nb++] = v;
-            for (uint32_t bi = 0; bi < nb; ++bi) {
-                const uint32_t x = ball_[bi];
-                const uint64_t x0 = rp_[x], x1 = rp_[x + 1];
-                // x's neighbours that move (active, degree > 0), compacted the same
-                // way; a move changes only the moved node's degree, so every test
-                // is already fixed before the first move.  An active neighbour of
-                // x has degree >= 1 while x is active (a node's degree is its
-                // number of active neighbours), so "active" alone decides, read
-                // from the bitmap of deleted nodes (N/8 bytes, cache-resident)
-                // instead of the node records
-                uint32_t m = 0;
-                for (uint64_t q = x0; q < x1; ++q) {
-                    const uint32_t u = col_[q];
-                    live_[m] = u;
-                    m += 1u - dead(u);
-                }
-                for (uint32_t e = 0; e < m; ++e) {
-                    // node records kAhead moves ahead, their bucket slots half as far
-                    if (e + kAhead < m) __builtin_prefetch(&st_[live_[e + kAhead]]);
-                    if (e + kAhead / 2 < m) {
-                        const Node nw = st_[live_[e + kAhead / 2]];
-                        if (nw.deg < nb_) __builtin_prefetch(&store_[bk_[nw.deg].off + nw.pos]);
-                    }
-                    const uint32_t u = live_[e];
-                    const uint32_t du = st_[u].deg;
-                    remove(du, u);
-                    st_[u].deg = du - 1;
-                    push(du - 1, u);
-                    if (du - 1 < tracker_) tracker_ = du - 1;
-                }
-                remove(st_[x].deg, x);
-                st_[x].deg = kDead;
-                dead_[x >> 6] |= 1ull << (x & 63);
-                --active;
-            }
-        }
-        *count = picked;
-        return IGC_OK;
-    }
-
-  private:
-    static constexpr uint64_t kAhead = 8;
-    static constexpr uint32_t kDead = 0xFFFFFFFFu; // degree field of a deleted node
-    std::vector<uint32_t> ball_, live_;            // the current ball; a node's moving neighbours
-    struct Node {
-        uint32_t deg;
-        uint32_t pos;
-    };
-    struct Bucket {
-        uint64_t off;
-        uint32_t len;
-    };
-    void push(uint32_t d, uint32_t v)
-    {
-        Bucket &b = bk_[d];
-        st_[v].pos = b.len;
-        store_[b.off + b.len++] = v;
-    }
-    void remove(uint32_t d, uint32_t v)
-    {
-        Bucket &b = bk_[d];
-        const uint32_t p = st_[v].pos;
-        const uint32_t last = store_[b.off + b.len - 1];
-        store_[b.off + p] = last;
-        st_[last].pos = p;
-        --b.len;
-        if (d == tracker_ && b.len == 0)
-            while (tracker_ < nb_ && bk_[tracker_].len == 0) ++tracker_;
-    }
-
-    uint32_t N_;
-    const uint64_t *rp_;
-    const uint32_t *col_;
-    uint32_t nb_ = 0, tracker_ = 0;
-    std::vector<Node> st_;
-    std::vector<uint64_t> dead_; // one bit per deleted node: the neighbour scans test only this
-    uint32_t dead(uint32_t u) const { return (uint32_t)(dead_[u >> 6] >> (u & 63)) & 1u; }
-    std::vector<Bucket> bk_;
-    std::unique_ptr<uint32_t[]> store_;
-};
-
 } // namespace igc
 
 using namespace igc;
 
 extern "C" {
-
-IGC_API int igc_greedy(const igc_csr *csr, uint32_t *order, uint32_t *count)
-{
-    if (!csr || !count || (!order && csr->n_nodes)) return set_error(IGC_ERR_INVALID, "null argument");
-    if (csr->location != IGC_ON_HOST) return set_error(IGC_ERR_INVALID, "igc_greedy needs a host CSR");
-    if (csr->n_nodes && csr->row_ptr[csr->n_nodes] != 2 * csr->n_edges)
-        return set_error(IGC_ERR_INVALID, "igc_greedy needs the symmetric CSR (not an upper-triangle one)");
-    *count = 0;
-    if (csr->n_nodes == 0) return IGC_OK;
-    if (!csr->row_ptr || (!csr->col && csr->n_edges)) return set_error(IGC_ERR_INVALID, "CSR has null arrays");
-    Greedy g(csr->n_nodes, csr->row_ptr, csr->col);
-    return g.run(order, count);
-}
 
 IGC_API int igc_csr_reorder(igc_csr *csr, uint32_t flags)
 {

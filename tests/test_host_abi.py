@@ -66,6 +66,49 @@ def test_greedy_matches_oracle_on_larger_graphs():
             assert np.array_equal(igc.greedy_order(igc.AdjList(rp, col)).astype(np.int32), O.greedy(rp, col))
 
 
+def _random_graph(rng, N: int, p: float, cliques: int = 0) -> "igc.AdjList":
+    """Symmetric CSR with ascending rows: G(N, p), plus `cliques` disjoint
+    cliques of 20 nodes (many equal degrees, so ties decide picks)."""
+    iu, ju = np.triu_indices(N, 1)
+    keep = rng.random(iu.size) < p
+    i, j = iu[keep], ju[keep]
+    if cliques:
+        members = rng.permutation(N)[: 20 * cliques].reshape(cliques, 20)
+        ci, cj = np.triu_indices(20, 1)
+        i = np.concatenate([i, np.minimum(members[:, ci], members[:, cj]).ravel()])
+        j = np.concatenate([j, np.maximum(members[:, ci], members[:, cj]).ravel()])
+        key = np.unique(i.astype(np.int64) * N + j)
+        i, j = key // N, key % N
+    src = np.concatenate([i, j])
+    dst = np.concatenate([j, i])
+    o = np.lexsort((dst, src))
+    rp = np.zeros(N + 1, np.uint64)
+    np.add.at(rp, src[o] + 1, 1)
+    return igc.AdjList(np.cumsum(rp).astype(np.uint64), dst[o].astype(np.uint32))
+
+
+def test_greedy_scanning_ahead_equals_one_thread():
+    """The greedy with helper threads (for picks that scan >= 2^15 neighbours,
+    the helpers find each ball member's active neighbours ahead of the moves)
+    selects exactly what one thread selects, also with rows in the reference's
+    T-thread order instead of ascending, and on a conflict graph against the
+    oracle's greedy."""
+    rng = np.random.default_rng(21)
+    graphs = [_random_graph(rng, 6000, 0.15), _random_graph(rng, 5000, 0.05, cliques=120),
+              _random_graph(rng, 3000, 0.6)]
+    graphs.append(igc.reorder(graphs[1], igc.order_threads(3)))
+    for adj in graphs:
+        one = igc.greedy_order(adj, threads=1)
+        for T in (2, 3, 8, 16):
+            assert np.array_equal(igc.greedy_order(adj, threads=T), one), T
+    syms = W.dedup_first(rng.integers(0, 4, size=(6000, 9), dtype=np.uint8))
+    rp, col = O.graph_csr(syms, 5)
+    assert rp[-1] > 2_000_000  # dense: most picks are split
+    want = O.greedy(rp, col)
+    for T in (1, 4, 7):
+        assert np.array_equal(igc.greedy_order(igc.AdjList(rp, col), threads=T).astype(np.int32), want)
+
+
 def test_pair_file_format(tmp_path):
     rp = np.array([0, 2, 3, 4, 5], np.uint64)
     col = np.array([1, 3, 0, 3, 0], np.uint32)[:5]
