@@ -32,11 +32,13 @@
 // adds the size, and the ranks are kept for the placement.  Every kernel
 // issues all of a thread's loads before using any of them.
 //
-// Scatters take their tile from xcd_tile: workgroups are dispatched to the 8
-// XCDs round-robin, and consecutive tiles, whose runs of one digit are
-// adjacent in the output, are given to the same XCD in order, so the partial
-// cache lines at the runs' ends meet in that XCD's L2 instead of being
-// written back from two.
+// Counts and scatters take their tile from xcd_tile: workgroups are dispatched
+// to the 8 XCDs round-robin, and consecutive tiles are given to the same XCD
+// in order.  A scatter's runs of one digit from consecutive tiles are adjacent
+// in the output, and a count kernel's counts of one digit for consecutive
+// tiles are adjacent words, so the partial cache lines meet in that XCD's L2
+// instead of being written back from two (the counts, written one word per
+// tile, otherwise cost a 32-byte sector each).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -280,7 +282,7 @@ static __global__ __launch_bounds__(kSegThreads) void k_seg_count(const uint32_t
 {
     constexpr uint32_t kPerWave = kSegTile / (kSegThreads / 64);
     __shared__ uint32_t h[kSegMaxDigits];
-    const uint32_t t = blockIdx.x;
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t nd = 1u << bits;
     const uint32_t tt = total[0];
     if (t >= tt) {
@@ -456,7 +458,7 @@ static __global__ __launch_bounds__(kRadixThreads) void k_radix_count(RadixPass 
                                                                        uint32_t *__restrict__ counts)
 {
     __shared__ uint32_t h[kRadixMaxDigits];
-    const uint32_t b = blockIdx.x;
+    const uint32_t b = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t nd = 1u << p.bits;
     const uint64_t n = radix_items(p);
     const uint64_t start = (uint64_t)b * kRadixTile;
