@@ -34,9 +34,16 @@ __device__ __forceinline__ uint32_t sym_at(W w, uint32_t p)
 }
 
 // assign[i] = argmin_c (cn2[c] - 2 * sum_p C[c][4p + s_p]), first index on ties.
-// The centroid table goes through LDS in chunks of kc centroids (<= 32 KiB).
-// dN (may be null) holds the point count in device memory instead of N, so a
-// launch captured for the largest round serves every round (grid-stride loops).
+// The centroid table goes through LDS in chunks of kc centroids, transposed:
+// row j = 4p + s holds C[c][j] for the chunk's centroids side by side (rows
+// km_stride(kc) floats apart, 4 more than the chunk rounded up to 4, so the four
+// rows one position can select start in different banks), and one 16-byte read
+// gives a string's term for four centroids.  Padding centroids have norm
+// +infinity and never win.  dN (may be null) holds the point count in device
+// memory instead of N, so a launch captured for the largest round serves every
+// round (grid-stride loops).
+__host__ __device__ inline uint32_t km_stride(uint32_t kc) { return ((kc + 3u) & ~3u) + 4u; }
+
 template <class W>
 __global__ __launch_bounds__(kKmThreads) void k_km_assign(const W *__restrict__ packed, uint32_t N,
                                                                   const uint32_t *__restrict__ dN, uint32_t n,
@@ -44,40 +51,47 @@ __global__ __launch_bounds__(kKmThreads) void k_km_assign(const W *__restrict__ 
                                                                   const float *__restrict__ cn2, uint32_t k,
                                                                   uint32_t kc, int32_t *__restrict__ assign)
 {
-    extern __shared__ float sh[];
+    extern __shared__ float4 sh4[];
+    float *sh = reinterpret_cast<float *>(sh4);
     if (dN) N = *dN;
     const uint32_t D = 4 * n;
-    float *sC = sh;
-    float *sN = sh + (size_t)kc * D;
+    const uint32_t S = km_stride(kc);
+    float *sT = sh;
+    float *sN = sh + (size_t)D * S;
     for (uint32_t base = blockIdx.x * blockDim.x; base < N; base += gridDim.x * blockDim.x) {
         const uint32_t i = base + threadIdx.x;
         const W w = i < N ? packed[i] : (W)0;
         constexpr uint32_t P = WordTraits<W>::half;
         uint32_t off[P];
 #pragma unroll
-        for (uint32_t p = 0; p < P; ++p) off[p] = 4 * p + sym_at(w, p);
+        for (uint32_t p = 0; p < P; ++p) off[p] = (4 * p + sym_at(w, p)) * S;
         float best = 0.0f;
         int32_t bi = -1;
         for (uint32_t c0 = 0; c0 < k; c0 += kc) {
             const uint32_t m = min(kc, k - c0);
+            const uint32_t m4 = (m + 3u) & ~3u;
             __syncthreads();
-            for (uint32_t t = threadIdx.x; t < m * D; t += blockDim.x) sC[t] = C[(size_t)c0 * D + t];
-            for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) sN[t] = cn2[c0 + t];
+            for (uint32_t t = threadIdx.x; t < m4 * D; t += blockDim.x) {
+                const uint32_t c = t / D, j = t - c * D;
+                sT[(size_t)j * S + c] = c < m ? C[(size_t)(c0 + c) * D + j] : 0.0f;
+            }
+            for (uint32_t t = threadIdx.x; t < m4; t += blockDim.x) sN[t] = t < m ? cn2[c0 + t] : INFINITY;
             __syncthreads();
-            // four centroids at a time, the symbol loop unrolled to the word's
-            // length: the LDS loads issue together (the loop is otherwise
-            // bound by their latency); each sum keeps its order, p ascending
-            uint32_t c = 0;
-            for (; c + 4 <= m; c += 4) {
-                const float *r0 = sC + (size_t)c * D;
-                float a[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            // four centroids per read, the symbol loop unrolled to the word's
+            // length; each sum keeps its order, p ascending
+            for (uint32_t c = 0; c < m4; c += 4) {
+                float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
 #pragma unroll
                 for (uint32_t p = 0; p < P; ++p) {
                     if (p < n) {
-#pragma unroll
-                        for (uint32_t q = 0; q < 4; ++q) a[q] = a[q] + r0[q * D + off[p]];
+                        const float4 v = *reinterpret_cast<const float4 *>(sT + off[p] + c);
+                        a0 = a0 + v.x;
+                        a1 = a1 + v.y;
+                        a2 = a2 + v.z;
+                        a3 = a3 + v.w;
                     }
                 }
+                const float a[4] = {a0, a1, a2, a3};
 #pragma unroll
                 for (uint32_t q = 0; q < 4; ++q) {
                     const float twice = 2.0f * a[q];
@@ -88,22 +102,15 @@ __global__ __launch_bounds__(kKmThreads) void k_km_assign(const W *__restrict__ 
                     }
                 }
             }
-            for (; c < m; ++c) {
-                const float *row = sC + (size_t)c * D;
-                float acc = 0.0f;
-#pragma unroll
-                for (uint32_t p = 0; p < P; ++p)
-                    if (p < n) acc = acc + row[off[p]];
-                const float twice = 2.0f * acc;
-                const float dist = sN[c] - twice;
-                if (bi < 0 || dist < best) {
-                    best = dist;
-                    bi = (int32_t)(c0 + c);
-                }
-            }
         }
         if (i < N) assign[i] = bi;
     }
+}
+
+// LDS bytes of k_km_assign for D = 4n dimensions and chunks of kc centroids.
+static size_t km_assign_lds(uint32_t D, uint32_t kc)
+{
+    return sizeof(float) * ((size_t)D * km_stride(kc) + ((kc + 3u) & ~3u));
 }
 
 // Integer histograms per cluster: cnt[c][4p+s] and size[c].  Each workgroup
@@ -250,7 +257,7 @@ static void lloyd_iteration(igc_ctx *ctx, KmBufs &b, uint32_t N, const uint32_t 
     hipStream_t s = ctx->stream;
     const uint32_t D = 4 * n;
     const uint32_t kc = km_chunk(D);
-    const size_t lds_assign = sizeof(float) * ((size_t)kc * D + kc);
+    const size_t lds_assign = km_assign_lds(D, kc);
     const size_t lds_accum = sizeof(uint32_t) * (((size_t)k * D + 1) / 2 + k);
     const uint32_t grid = km_grid(ctx, N);
     hipLaunchKernelGGL(k_km_assign<W>, dim3(grid), dim3(kKmThreads), lds_assign, s, b.packed.as<W>(), N, dN, n,
@@ -309,7 +316,7 @@ static void km_launch(igc_ctx *ctx, KmBufs &b, const uint8_t *d_syms, uint32_t N
     for (uint32_t t = 0; t < iters; ++t) lloyd_iteration<W>(ctx, b, N, dN, n, k);
     if (!final) return;
     const uint32_t kc = km_chunk(D);
-    hipLaunchKernelGGL(k_km_assign<W>, dim3(km_grid(ctx, N)), dim3(kKmThreads), sizeof(float) * ((size_t)kc * D + kc),
+    hipLaunchKernelGGL(k_km_assign<W>, dim3(km_grid(ctx, N)), dim3(kKmThreads), km_assign_lds(D, kc),
                        s, b.packed.as<W>(), N, dN, n, b.C.as<float>(), b.cn2.as<float>(), k, kc,
                        b.assign.as<int32_t>());
 }
@@ -355,8 +362,8 @@ static int kmeans_device_w(igc_ctx *ctx, const uint8_t *d_syms, uint32_t N, uint
         if (e != hipSuccess) return set_error(IGC_ERR_DEVICE, "graph launch failed: %s", hipGetErrorString(e));
         // final assignment against the last centroids
         const uint32_t D = 4 * n, kc = km_chunk(D);
-        hipLaunchKernelGGL(k_km_assign<W>, dim3(km_grid(ctx, N)), dim3(kKmThreads),
-                           sizeof(float) * ((size_t)kc * D + kc), s, b.packed.as<W>(), N, nullptr, n,
+        hipLaunchKernelGGL(k_km_assign<W>, dim3(km_grid(ctx, N)), dim3(kKmThreads), km_assign_lds(D, kc), s,
+                           b.packed.as<W>(), N, nullptr, n,
                            b.C.as<float>(), b.cn2.as<float>(), k, kc, b.assign.as<int32_t>());
         HIP_TRY(hipGetLastError());
     }
@@ -493,7 +500,7 @@ IGC_API int igc_assign_fp32(igc_ctx *ctx, const uint8_t *syms, uint32_t N, uint3
     if ((rc = launch_pack(s, ctx->syms.as<uint8_t>(), N, n, ctx->packed.p, misc + 2))) return rc;
     hipLaunchKernelGGL(k_km_norms, dim3((k + 255) / 256), dim3(256), 0, s, b.C.as<float>(), k, D, b.cn2.as<float>());
     const uint32_t kc = km_chunk(D);
-    const size_t lds = sizeof(float) * ((size_t)kc * D + kc);
+    const size_t lds = km_assign_lds(D, kc);
     if (wide)
         hipLaunchKernelGGL(k_km_assign<uint64_t>, dim3(km_grid(ctx, N)), dim3(kKmThreads), lds, s,
                            ctx->packed.as<uint64_t>(), N, nullptr, n, b.C.as<float>(), b.cn2.as<float>(), k, kc,
