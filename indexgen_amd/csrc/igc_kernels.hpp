@@ -373,14 +373,20 @@ __device__ __forceinline__ void wave_tile(const typename Pol::Word *__restrict__
             const uint32_t nl = r0 + 64u + lane;
             fnext = nl < rows ? flt.feat[row0 + nl] : make_uint2(0u, 0u);
         }
-        // the next row's word is loaded one row ahead (scalar load latency)
+        // the next row's word is loaded one row ahead (scalar load latency);
+        // the row index and mask stay in scalar registers: t is the next row's
+        // offset in the chunk (0 once none is left: row r0 exists, so the
+        // load stays in bounds and is not used)
         using Word = typename Pol::Word;
-        Word next = todo ? packed[row0 + r0 + (uint32_t)__builtin_ctzll(todo)] : (Word)0;
+        const Word *rowp = packed + row0 + r0;
+        uint32_t t = todo ? (uint32_t)__builtin_ctzll(todo) : 0u;
+        Word next = rowp[t];
         while (todo) {
-            const uint32_t i = row0 + r0 + (uint32_t)__builtin_ctzll(todo);
-            todo &= todo - 1ull;
+            const uint32_t i = row0 + r0 + t;
+            todo &= ~(1ull << t);
             const Word a = next;
-            if (todo) next = packed[row0 + r0 + (uint32_t)__builtin_ctzll(todo)];
+            t = todo ? (uint32_t)__builtin_ctzll(todo) : 0u;
+            next = rowp[t];
             typename Pol::Row row;
             Pol::row(row, a, n, k);
             uint64_t mk[SL];

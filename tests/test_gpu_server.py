@@ -142,6 +142,10 @@ def test_underestimated_job_within_budget(gpu_ctx):
 
 
 def test_edge_buffer_regrow_and_replay(gpu_ctx):
+    """A forced tiny first edge buffer (igc_ctx_set_edge_cap) overflows; the build
+    regrows it to the exact count and replays the pairs.  Single and batched
+    builds, quaternary n = 10..16, n = 20 (no density prior) and binary n = 40,
+    all equal to the oracle; with the cap cleared no retry happens."""
     rng = np.random.default_rng(21)
     syms = W.filtered_uniform(rng, 6000, 11)
     rp, col = O.graph_csr(syms, 4)
@@ -159,6 +163,20 @@ def test_edge_buffer_regrow_and_replay(gpu_ctx):
         outs = gpu_ctx.graph_build_batch(sets)
         assert gpu_ctx.last_timing().retries >= 1
         for (s, ed), a in zip(sets, outs):
+            r, c = O.graph_csr(s, ed)
+            assert np.array_equal(a.row_ptr, r) and np.array_equal(a.col, c)
+        # lengths without a density prior (n > 16: 64-bit words; n > 32: binary)
+        wide = [(W.filtered_uniform(rng, 2500, 20), 9),
+                (W.dedup_first(rng.integers(0, 2, size=(2500, 40), dtype=np.uint8)), 9)]
+        for s, ed in wide:
+            r, c = O.graph_csr(s, ed)
+            assert c.size // 2 > 1000
+            adj = gpu_ctx.graph_build(s, ed)
+            assert gpu_ctx.last_timing().retries >= 1
+            assert np.array_equal(adj.row_ptr, r) and np.array_equal(adj.col, c)
+        outs = gpu_ctx.graph_build_batch(wide)
+        assert gpu_ctx.last_timing().retries >= 1
+        for (s, ed), a in zip(wide, outs):
             r, c = O.graph_csr(s, ed)
             assert np.array_equal(a.row_ptr, r) and np.array_equal(a.col, c)
     finally:
