@@ -167,18 +167,26 @@ __device__ __forceinline__ uint32_t seg_of_tile(const SegMeta *__restrict__ meta
 // Lanes of this wave holding the same digit as this lane (one ballot per digit
 // bit, all kMaxBits of them: bits above the pass's width are zero in every
 // lane and leave the set unchanged); lanes with valid == false belong to no
-// lane's set.
+// lane's set.  The lanes that differ from this one in bit k are ballot_k XOR
+// m_k, m_k = this lane's bit k spread over the word (0 or ~0); their OR over
+// the bits, complemented, is the set.  Per bit: one sign-extending bit
+// extract, one compare (the ballot) and one three-input op per 32-bit half.
 template <uint32_t kMaxBits>
 __device__ __forceinline__ uint64_t wave_match(uint32_t digit, bool valid)
 {
-    uint64_t match = __ballot(valid);
+    const uint64_t vb = __ballot(valid);
+    uint32_t diff_lo = 0, diff_hi = 0;
 #pragma unroll
     for (uint32_t bit = 0; bit < kMaxBits; ++bit) {
-        const uint32_t m32 = 0u - ((digit >> bit) & 1u); // all ones where the bit is set
-        const uint64_t bb = __builtin_amdgcn_ballot_w64(((digit >> bit) & 1u) != 0u);
-        match &= ~(bb ^ (((uint64_t)m32 << 32) | m32)); // bb where set, ~bb where clear
+        uint32_t m = (uint32_t)((int32_t)(digit << (31 - bit)) >> 31); // 0 or ~0
+        // opaque, so the ballot compares m itself (not a second shift of digit)
+        __asm__ volatile("" : "+v"(m));
+        const uint64_t bb = __builtin_amdgcn_ballot_w64(m != 0u);
+        // diff |= m ^ ballot half: v_bitop3 with truth table 0xDE = src1 | (src0 ^ src2)
+        diff_lo = __builtin_amdgcn_bitop3_b32(m, diff_lo, (uint32_t)bb, 0xDE);
+        diff_hi = __builtin_amdgcn_bitop3_b32(m, diff_hi, (uint32_t)(bb >> 32), 0xDE);
     }
-    return match;
+    return vb & ~(((uint64_t)diff_hi << 32) | diff_lo);
 }
 
 // Adds this round's entries to the histogram h, one atomic per run of equal

@@ -606,7 +606,7 @@ def test_csr_forms_agree(gpu_ctx):
             assert np.array_equal(b.row_ptr, rp) and np.array_equal(b.col, col)
     # long rows: clusters of near-identical strings (rows of a few thousand
     # entries), and one cluster where every pair conflicts
-    for per, clusters in ((3000, 6), (17500, 1)):
+    for per, clusters in ((3000, 6), (6000, 2), (17500, 1)):
         base = rng.integers(0, 4, size=(clusters, 16), dtype=np.uint8)
         syms = np.repeat(base, per, axis=0)
         pos = rng.integers(0, 16, size=syms.shape[0])
