@@ -344,7 +344,7 @@ def greedy_profile(specs, res) -> dict:
     }
 
 
-def serving_block(jobs: int, pipelines: int = 2, host_threads: int = 16, seed: int = 5):
+def serving_block(jobs: int, pipelines: int = 3, host_threads: int = 16, seed: int = 5):
     """The codebook-request stream (Poisson 20 jobs/s, n in 11..16, editDist in
     3..5, sizes lognormal(ln 30000, 1.2) clipped to [500, 400000]) replayed in
     real time: each job is submitted at its arrival time; latency = completion
@@ -397,7 +397,7 @@ def serving_block(jobs: int, pipelines: int = 2, host_threads: int = 16, seed: i
     }
 
 
-def saturated_block(jobs: int, pipelines: int = 2, host_threads: int = 16, seed: int = 5):
+def saturated_block(jobs: int, pipelines: int = 3, host_threads: int = 16, seed: int = 5):
     """Server capacity on the same mixed jobs: every job submitted at once (the
     queue is never empty, so each pipeline takes up to 512 queued jobs that fit
     the HBM budget per batched build), against the plain form -- one host-API

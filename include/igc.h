@@ -441,7 +441,7 @@ typedef struct igc_server_stats {
 } igc_server_info;
 
 /* Starts a server on `device` with an HBM budget (bytes; 0 = 200 GiB) shared by
- * the launches in flight, `pipelines` GPU pipelines (1 or 2, each with its own
+ * the launches in flight, `pipelines` GPU pipelines (1 to 4, each with its own
  * stream) and `host_threads` greedy workers.  Jobs submitted while others run
  * are admitted FIFO as soon as a pipeline is free and the budget allows, and
  * all jobs admitted together run as one batched build. */

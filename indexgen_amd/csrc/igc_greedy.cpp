@@ -12,6 +12,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstdint>
+#include <cstdlib>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -32,9 +33,8 @@ namespace igc {
 //     :139-154) and an append to the lower bucket.
 // Layout for speed, not for semantics: a node's degree and bucket position
 // share one 8-byte record, and a deleted node's degree field holds kDead;
-// buckets live in one flat array (bucket d never holds more nodes than there
-// are nodes of initial degree >= d, since degrees only fall); the neighbour
-// scans test a bitmap of deleted nodes.
+// each bucket is its own compact array; the neighbour scans test a bitmap of
+// deleted nodes.
 //
 // A pick is one sequence of bucket operations: for each ball member x in
 // order, a move of each of x's neighbours still active at that point (in x's
@@ -61,19 +61,12 @@ class Greedy {
                 maxd = std::max(maxd, st_[v].deg);
             }
             nb_ = maxd + 1;
-            std::vector<uint64_t> ge(nb_ + 1, 0);
-            for (uint32_t v = 0; v < N_; ++v) ge[st_[v].deg]++;
-            for (int64_t d = (int64_t)nb_ - 1; d >= 0; --d) ge[d] += ge[d + 1];
-            bk_.resize(nb_);
-            uint64_t off = 0;
-            for (uint32_t d = 0; d < nb_; ++d) {
-                bk_[d].off = off;
-                bk_[d].len = 0;
-                off += ge[d];
-            }
-            // capacity 2|E| + N, but only bucket tails are ever touched: leave it
-            // uninitialised so untouched pages are never faulted in
-            store_.reset(new uint32_t[off > 0 ? off : 1]);
+            // each bucket starts with room for its nodes of that initial degree
+            std::vector<uint32_t> cnt(nb_, 0);
+            for (uint32_t v = 0; v < N_; ++v) cnt[st_[v].deg]++;
+            bk_.assign(nb_, Bucket{});
+            for (uint32_t d = 0; d < nb_; ++d)
+                if (cnt[d]) reserve(bk_[d], cnt[d]);
             ball_.resize((size_t)maxd + 2);
             live_.resize((size_t)maxd + 1);
             if (T_ > 1) ahead_setup();
@@ -81,14 +74,15 @@ class Greedy {
             return set_error(IGC_ERR_OOM, "greedy: host allocation failed");
         }
         tracker_ = nb_;
+        uint32_t active = N_, picked = 0;
+        try { // a bucket that grows past its room can fail to allocate
         for (uint32_t v = 0; v < N_; ++v) {
             push(st_[v].deg, v);
             if (st_[v].deg < tracker_) tracker_ = st_[v].deg;
         }
-        uint32_t active = N_, picked = 0;
         while (active > 0) {
             const Bucket &b = bk_[tracker_];
-            const uint32_t v = store_[b.off + b.len - 1];
+            const uint32_t v = b.p[b.len - 1];
             order[picked++] = v;
             // the ball: v's active neighbours in list order, then v (compacted
             // without branches: about half the tests fail, unpredictably)
@@ -108,6 +102,9 @@ class Greedy {
             else pick_one(nb);
             active -= nb;
         }
+        } catch (const std::bad_alloc &) {
+            return set_error(IGC_ERR_OOM, "greedy: host allocation failed");
+        }
         if (!helpers_.empty()) ahead_stop();
         *count = picked;
         return IGC_OK;
@@ -115,6 +112,7 @@ class Greedy {
     ~Greedy()
     {
         if (!helpers_.empty()) ahead_stop();
+        for (Bucket &b : bk_) std::free(b.p);
     }
 
   private:
@@ -125,22 +123,34 @@ class Greedy {
         uint32_t deg;
         uint32_t pos;
     };
+    // A bucket's nodes in their own array, grown by doubling: the buckets of a
+    // pick stay within a few hundred pages (a flat array with every bucket at
+    // its largest possible size put each bucket on pages of its own, and the
+    // moves of a 400,000-node graph missed the TLB on most of them).
     struct Bucket {
-        uint64_t off;
-        uint32_t len;
+        uint32_t *p = nullptr;
+        uint32_t len = 0, cap = 0;
     };
+    static void reserve(Bucket &b, uint32_t cap)
+    {
+        uint32_t *q = static_cast<uint32_t *>(std::realloc(b.p, sizeof(uint32_t) * (size_t)cap));
+        if (!q) throw std::bad_alloc();
+        b.p = q;
+        b.cap = cap;
+    }
     void push(uint32_t d, uint32_t v)
     {
         Bucket &b = bk_[d];
+        if (b.len == b.cap) reserve(b, b.cap ? 2 * b.cap : 16);
         st_[v].pos = b.len;
-        store_[b.off + b.len++] = v;
+        b.p[b.len++] = v;
     }
     void remove(uint32_t d, uint32_t v)
     {
         Bucket &b = bk_[d];
         const uint32_t p = st_[v].pos;
-        const uint32_t last = store_[b.off + b.len - 1];
-        store_[b.off + p] = last;
+        const uint32_t last = b.p[b.len - 1];
+        b.p[p] = last;
         st_[last].pos = p;
         --b.len;
         if (d == tracker_ && b.len == 0)
@@ -179,7 +189,7 @@ class Greedy {
             if (e + kAhead < m) __builtin_prefetch(&st_[live[e + kAhead]]);
             if (e + kAhead / 2 < m) {
                 const Node nw = st_[live[e + kAhead / 2]];
-                if (nw.deg < nb_) __builtin_prefetch(&store_[bk_[nw.deg].off + nw.pos]);
+                if (nw.deg < nb_) __builtin_prefetch(&bk_[nw.deg].p[nw.pos]);
             }
             const uint32_t u = live[e];
             const uint32_t du = st_[u].deg;
@@ -300,7 +310,6 @@ class Greedy {
     std::vector<Node> st_;
     std::vector<uint64_t> dead_; // one bit per deleted node: the neighbour scans test only this
     std::vector<Bucket> bk_;
-    std::unique_ptr<uint32_t[]> store_;
     std::vector<uint32_t> ball_, live_; // the current ball; a node's moving neighbours
     // scanning ahead
     static constexpr std::chrono::microseconds kSpin{300};

@@ -2,8 +2,10 @@
 //
 // Jobs (candidate set + editDist) are submitted from any host thread and
 // queued FIFO; their symbols are checked at submission.  Each GPU pipeline
-// (its own context and stream; up to two, so one batch's pair kernel overlaps
-// the previous batch's CSR assembly and copies) takes, as soon as it is free,
+// (its own context and stream; up to four, so one batch's pair kernel overlaps
+// another's CSR assembly and copies, and small jobs find a free pipeline while
+// others copy the multi-gigabyte CSRs of dense jobs to the host) takes, as
+// soon as it is free,
 // every queued job that fits the HBM budget and builds their conflict graphs
 // in one batched build (igc_batch.hip).  The per-job CSRs go to a pool of host
 // threads running the greedy independent set, while the pipeline already
@@ -395,7 +397,7 @@ IGC_API int igc_server_create(int device, uint64_t hbm_budget_bytes, uint32_t pi
     if (!out) return set_error(IGC_ERR_INVALID, "null out");
     *out = nullptr;
     if (pipelines < 1) pipelines = 1;
-    if (pipelines > 2) pipelines = 2;
+    if (pipelines > 4) pipelines = 4;
     if (host_threads < 1) host_threads = 1;
     igc_server *S = new (std::nothrow) igc_server();
     if (!S) return set_error(IGC_ERR_OOM, "host allocation failed");

@@ -32,11 +32,12 @@ def _draw_all(specs):
 
 
 def test_full_serving_stream(gpu_ctx):
+    """On four pipelines (the most a server runs)."""
     specs = W.serving(seed=5)
     assert len(specs) == 2000
     data = _draw_all(specs)
     print(f"[server] drew {sum(d.shape[0] for d in data)} strings for {len(specs)} jobs", flush=True)
-    srv = igc.Server(0, pipelines=2, host_threads=16)
+    srv = igc.Server(0, pipelines=4, host_threads=16)
     try:
         ids = [srv.submit(d, sp.edit_dist) for d, sp in zip(data, specs)]
         res = [srv.wait(j) for j in ids]
