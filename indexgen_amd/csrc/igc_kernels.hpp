@@ -290,6 +290,20 @@ __device__ __forceinline__ void flush_stage(const uint64_t *st, uint32_t staged,
     __builtin_amdgcn_wave_barrier();
 }
 
+// Lowest set bit of a wave-uniform mask, -1 when it is empty: s_ff1_i32_b64
+// itself (the builtins that define the empty case add a compare and a select).
+__device__ __forceinline__ int32_t first_set(uint64_t m)
+{
+    int32_t t;
+    __asm__("s_ff1_i32_b64 %0, %1" : "=s"(t) : "s"(m));
+    return t;
+}
+
+// Row of a chunk to load for first_set's result t (row 0 when none is left:
+// it exists, and the load is not used); the mask (a no-op on 0..63) shows the
+// compiler the range, so the scalar load takes its offset in a 32-bit register.
+__device__ __forceinline__ uint32_t row_slot(int32_t t) { return (uint32_t)max(t, 0) & 63u; }
+
 // One wave's share of tile (rb, cb) of a candidate set: rows rb*R .. rb*R+R-1,
 // columns colw .. colw+64*CPL*P-1 with colw = cb*W + wave*64*CPL*P (lane l,
 // slot s -> column colw + 64s + l; slots c*P .. c*P+P-1 share register group c).
@@ -375,18 +389,19 @@ __device__ __forceinline__ void wave_tile(const typename Pol::Word *__restrict__
         }
         // the next row's word is loaded one row ahead (scalar load latency);
         // the row index and mask stay in scalar registers: t is the next row's
-        // offset in the chunk (0 once none is left: row r0 exists, so the
-        // load stays in bounds and is not used)
+        // offset in the chunk, -1 once none is left (s_ff1's result for an
+        // empty mask; the loop tests its sign, and the look-ahead load then
+        // reads row r0, which exists, and is not used)
         using Word = typename Pol::Word;
         const Word *rowp = packed + row0 + r0;
-        uint32_t t = todo ? (uint32_t)__builtin_ctzll(todo) : 0u;
-        Word next = rowp[t];
-        while (todo) {
-            const uint32_t i = row0 + r0 + t;
+        int32_t t = first_set(todo);
+        Word next = rowp[row_slot(t)];
+        while (t >= 0) {
+            const uint32_t i = row0 + r0 + (uint32_t)t;
             todo &= ~(1ull << t);
             const Word a = next;
-            t = todo ? (uint32_t)__builtin_ctzll(todo) : 0u;
-            next = rowp[t];
+            t = first_set(todo);
+            next = rowp[row_slot(t)];
             typename Pol::Row row;
             Pol::row(row, a, n, k);
             uint64_t mk[SL];
