@@ -494,34 +494,74 @@ __device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t v, uint64_t *s
     return total;
 }
 
+// The kScanItems counts of a thread (base = its first index): four 16-byte
+// loads when all of them lie below N (the arrays are 16-byte aligned and base is
+// a multiple of kScanItems), else one load per count, 0 past N.
+__device__ __forceinline__ void scan_items_load(const uint32_t *__restrict__ a, uint64_t base, uint64_t N,
+                                                uint32_t (&v)[kScanItems])
+{
+    static_assert(kScanItems % 4 == 0, "whole 16-byte loads");
+    if (base + kScanItems <= N) {
+        const uint4 *p = reinterpret_cast<const uint4 *>(a + base);
+#pragma unroll
+        for (int q = 0; q < kScanItems / 4; ++q) {
+            const uint4 x = p[q];
+            v[4 * q] = x.x;
+            v[4 * q + 1] = x.y;
+            v[4 * q + 2] = x.z;
+            v[4 * q + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) v[k] = base + k < N ? a[base + k] : 0u;
+    }
+}
+
 static __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *__restrict__ deg, uint32_t N,
                                                                uint64_t *__restrict__ bsum)
 {
     __shared__ uint64_t sh[kScanThreads / 64 + 1];
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    scan_items_load(deg, base, N, v);
     uint64_t s = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k)
-        if (base + k < N) s += deg[base + k];
+    for (int k = 0; k < kScanItems; ++k) s += v[k];
     uint64_t excl;
     const uint64_t tot = block_exclusive_scan(s, sh, excl);
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
+// Exclusive scan of the nb block sums in place, kScanItems consecutive sums per
+// thread and kScanTile per step.
 static __global__ __launch_bounds__(kScanThreads) void k_scan_blocks(uint64_t *__restrict__ bsum, uint32_t nb)
 {
     __shared__ uint64_t sh[kScanThreads / 64 + 1];
     uint64_t carry = 0;
-    for (uint32_t b0 = 0; b0 < nb; b0 += kScanThreads) {
-        const uint32_t b = b0 + threadIdx.x;
-        const uint64_t v = b < nb ? bsum[b] : 0;
+    for (uint32_t b0 = 0; b0 < nb; b0 += kScanTile) {
+        const uint32_t b = b0 + threadIdx.x * kScanItems;
+        uint64_t v[kScanItems];
+        uint64_t s = 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            v[k] = b + k < nb ? bsum[b + k] : 0ull;
+            s += v[k];
+        }
         uint64_t excl;
-        const uint64_t tot = block_exclusive_scan(v, sh, excl);
-        if (b < nb) bsum[b] = carry + excl;
+        const uint64_t tot = block_exclusive_scan(s, sh, excl);
+        uint64_t run = carry + excl;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            if (b + k < nb) bsum[b + k] = run;
+            run += v[k];
+        }
         carry += tot;
     }
 }
 
+// offsets[i] = boff[block of i] + the counts before i in its block, and
+// offsets[N] = the total; a thread's 16 offsets go out as 16-byte stores when
+// all of them lie below N.
 static __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t *__restrict__ deg, uint32_t N,
                                                               const uint64_t *__restrict__ boff,
                                                               uint64_t *__restrict__ row_ptr)
@@ -529,21 +569,33 @@ static __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32
     __shared__ uint64_t sh[kScanThreads / 64 + 1];
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
     uint32_t v[kScanItems];
+    scan_items_load(deg, base, N, v);
     uint64_t s = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        v[k] = (base + k < N) ? deg[base + k] : 0u;
-        s += v[k];
-    }
+    for (int k = 0; k < kScanItems; ++k) s += v[k];
     uint64_t excl;
     block_exclusive_scan(s, sh, excl);
     uint64_t run = boff[blockIdx.x] + excl;
+    if (base + kScanItems <= N) {
+        ulonglong2 *p = reinterpret_cast<ulonglong2 *>(row_ptr + base);
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        if (base + k < N) {
-            row_ptr[base + k] = run;
-            run += v[k];
-            if (base + k == (uint64_t)N - 1) row_ptr[N] = run;
+        for (int q = 0; q < kScanItems / 2; ++q) {
+            ulonglong2 w;
+            w.x = run;
+            run += v[2 * q];
+            w.y = run;
+            run += v[2 * q + 1];
+            p[q] = w;
+        }
+        if (base + kScanItems == N) row_ptr[N] = run;
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            if (base + k < N) {
+                row_ptr[base + k] = run;
+                run += v[k];
+                if (base + k == (uint64_t)N - 1) row_ptr[N] = run;
+            }
         }
     }
 }
