@@ -35,7 +35,12 @@
 namespace igc {
 
 constexpr int kWaves = 4;          // waves per workgroup of the pair kernel
-constexpr int kStage = 256;        // edges staged per wave in LDS before a flush
+// Edges staged per wave in LDS before a flush.  Every flush takes its slots
+// from the one global edge counter, whose atomics serialise on dense graphs
+// (4% of pairs conflicting: 256-edge flushes held the pair kernel to a third of
+// its issue rate); 512 halves them, and at 16 KB per workgroup eight
+// workgroups still fit a CU's LDS.
+constexpr int kStage = 512;
 
 // ---------------------------------------------------------------------------
 // symbol packing
@@ -365,11 +370,20 @@ __device__ __forceinline__ void wave_tile(const typename Pol::Word *__restrict__
     // Each chunk's counts are loaded one chunk ahead, into registers that stay
     // live through the row loop (a load still in flight there would make the
     // row loop wait for every outstanding store as well).
+    // With a permutation, the caller ids of a chunk's rows (one per lane) are
+    // loaded a chunk ahead too; an emitting row reads its id from them.
     uint2 fnext = make_uint2(0u, 0u);
+    uint32_t pnext = 0u;
     if (flt.items && lane < rows) fnext = flt.feat[row0 + lane];
+    if (EMIT && flt.perm && lane < rows) pnext = flt.perm[row0 + lane];
     for (uint32_t r0 = 0; r0 < rows; r0 += 64) {
         const uint32_t left = rows - r0;
         uint64_t todo = left >= 64 ? ~0ull : ((1ull << left) - 1ull);
+        const uint32_t pids = pnext;
+        if (EMIT && flt.perm) {
+            const uint32_t nl = r0 + 64u + lane;
+            pnext = nl < rows ? flt.perm[row0 + nl] : 0u;
+        }
         if (flt.items) {
             const uint2 f = fnext;
             uint32_t best = 0xFFFFFFFFu;
@@ -416,9 +430,9 @@ __device__ __forceinline__ void wave_tile(const typename Pol::Word *__restrict__
                 }
             }
             if (any) {
-                // caller id of the row: a scalar load (uniform, read-only)
-                using CU32 = const __attribute__((address_space(4))) uint32_t;
-                const uint32_t oi = (EMIT && flt.perm) ? ((CU32 *)flt.perm)[i] : i;
+                // caller id of the row, from the chunk's ids
+                const uint32_t oi =
+                    (EMIT && flt.perm) ? (uint32_t)__builtin_amdgcn_readlane((int)pids, (int)(i - row0 - r0)) : i;
 #pragma unroll
                 for (int sl = 0; sl < SL; ++sl) {
                     if (!mk[sl]) continue;
