@@ -151,6 +151,27 @@ struct PolLV {
     }
 };
 
+// Landau-Vishkin on 32-bit words of n <= 16 - K/2: the row planes from the
+// whole shifted word (lvt_row_narrow), fewer scalar instructions per row.
+template <int K>
+struct PolLVN : PolLV<K, uint32_t> {
+    using Row = typename PolLV<K, uint32_t>::Row;
+    static __device__ __forceinline__ void row(Row &r, uint32_t a, uint32_t, int) { lvt_row_narrow<K>(a, r); }
+};
+
+// The narrow form of a policy where there is one (Landau-Vishkin on 32-bit
+// words, 1 <= K <= 5), for sets of at most max_n symbols.
+template <class Pol> struct NarrowOf {
+    static constexpr bool has = false;
+    using type = Pol;
+    static constexpr uint32_t max_n = 0;
+};
+template <int K> struct NarrowOf<PolLV<K, uint32_t>> {
+    static constexpr bool has = K >= 1 && K <= 5;
+    using type = PolLVN<K>;
+    static constexpr uint32_t max_n = (uint32_t)lv_narrow_max_n(K);
+};
+
 template <class W>
 struct PolMyers {
     using Word = W;

@@ -227,6 +227,42 @@ IGC_UNROLL
     for (int t = 0; t < LvShape<K>::WD; ++t) row_planes(a, t - C, r.lo[t], r.hi[t]);
 }
 
+// The same planes of a 32-bit word for n <= 16 - C, from the word shifted as a
+// whole: x = a >> s (or a << -s), lo plane = x's low half, hi plane = x's high
+// half, each copied into both halves.  A right shift by s brings the hi plane's
+// low s bits into positions 16-s..15 of the lo plane: past diagonal s's end
+// (positions >= n - s), where bits only ever mean "end reached".  A left shift
+// by u brings the lo plane's top u bits into positions 0..u-1 of the hi plane:
+// zero when n <= 16 - u.  One scalar shift and two packs per diagonal, where
+// the masked planes take a shift per plane besides.
+template <int K>
+IGC_HD inline void lvt_row_narrow(uint32_t a, LvtRow<K, u16x2> &r)
+{
+    constexpr int C = LvShape<K>::C;
+IGC_UNROLL
+    for (int t = 0; t < LvShape<K>::WD; ++t) {
+        const int s = t - C;
+        uint32_t x = s >= 0 ? a >> s : a << (-s);
+#if defined(__HIP_DEVICE_COMPILE__)
+        // opaque, so both packs read this one shifted word (left alone, the
+        // compiler shifts the word a second time for the high half)
+        if (s != 0) asm volatile("" : "+s"(x));
+#endif
+        const uint32_t l2 = (x << 16) | (x & 0xFFFFu);       // low half in both halves
+        const uint32_t h2 = (x & 0xFFFF0000u) | (x >> 16);   // high half in both halves
+        u16x2 lo, hi;
+        lo[0] = (uint16_t)l2;
+        lo[1] = (uint16_t)(l2 >> 16);
+        hi[0] = (uint16_t)h2;
+        hi[1] = (uint16_t)(h2 >> 16);
+        r.lo[t] = lo;
+        r.hi[t] = hi;
+    }
+}
+
+// largest n lvt_row_narrow serves at threshold K (0 <= K <= kLvMaxK)
+constexpr int lv_narrow_max_n(int K) { return 16 - K / 2; }
+
 // Edge lanes (bit h: the pair in lane h has Levenshtein <= K).
 template <int K, class V>
 IGC_HD inline uint32_t lvt_test(const LvtRow<K, V> &r, V blo, V bhi)

@@ -56,6 +56,39 @@ extern "C" int lv_pairs(const uint8_t *A, const uint8_t *B, int M, int n, int K,
     return -1;
 }
 
+namespace {
+struct LvNarrow {
+    template <int K>
+    static void run(const uint8_t *A, const uint8_t *B, int M, int n, uint8_t *out)
+    {
+        for (int r = 0; r < M; ++r) {
+            const uint8_t *a = A + (size_t)r * n, *b = B + (size_t)r * n;
+            const uint8_t *o = B + (size_t)((r + 1) % M) * n;
+            LvtRow<K, u16x2> row;
+            lvt_row_narrow<K>(pack_word(a, n), row);
+            u16x2 lo, hi;
+            col_planes(pack_word(b, n), pack_word(o, n), lo, hi);
+            const uint32_t e0 = lvt_test<K, u16x2>(row, lo, hi) & 1u;
+            col_planes(pack_word(o, n), pack_word(b, n), lo, hi);
+            const uint32_t e1 = (lvt_test<K, u16x2>(row, lo, hi) >> 1) & 1u;
+            out[r] = e0 == e1 ? (uint8_t)e0 : 2;
+        }
+    }
+};
+} // namespace
+
+// the same on 16-bit lanes with the row planes of lvt_row_narrow (n <= 16 - K/2)
+extern "C" int lv_pairs_narrow(const uint8_t *A, const uint8_t *B, int M, int n, int K, uint8_t *out)
+{
+    if (K < 0 || K > 15 || n > 16 - K / 2) return -1;
+    switch (K) {
+#define C_(k) case k: LvNarrow::run<k>(A, B, M, n, out); return 0;
+        C_(0) C_(1) C_(2) C_(3) C_(4) C_(5) C_(6) C_(7) C_(8) C_(9) C_(10) C_(11) C_(12) C_(13) C_(14) C_(15)
+#undef C_
+    }
+    return -1;
+}
+
 // the same by the Myers predicate (A is the text, B the pattern)
 extern "C" int myers_pairs(const uint8_t *A, const uint8_t *B, int M, int n, int K, int wide, uint8_t *out)
 {
