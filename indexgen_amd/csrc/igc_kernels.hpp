@@ -325,6 +325,14 @@ __device__ __forceinline__ int32_t first_set(uint64_t m)
     return t;
 }
 
+// m with bit t (0..63) cleared: s_bitset0_b64, one scalar instruction where
+// the shift and and-not take two.
+__device__ __forceinline__ uint64_t clear_bit(uint64_t m, int32_t t)
+{
+    __asm__("s_bitset0_b64 %0, %1" : "+s"(m) : "s"(t));
+    return m;
+}
+
 // Row of a chunk to load for first_set's result t (row 0 when none is left:
 // it exists, and the load is not used); the mask (a no-op on 0..63) shows the
 // compiler the range, so the scalar load takes its offset in a 32-bit register.
@@ -433,7 +441,7 @@ __device__ __forceinline__ void wave_tile(const typename Pol::Word *__restrict__
         Word next = rowp[row_slot(t)];
         while (t >= 0) {
             const uint32_t i = row0 + r0 + (uint32_t)t;
-            todo &= ~(1ull << t);
+            todo = clear_bit(todo, t);
             const Word a = next;
             t = first_set(todo);
             next = rowp[row_slot(t)];
