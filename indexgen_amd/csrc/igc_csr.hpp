@@ -50,7 +50,6 @@ constexpr int kRadixRounds = 8; // entries per thread and tile
 constexpr uint32_t kRadixTile = kRadixThreads * kRadixRounds;
 constexpr uint32_t kRadixMaxBits = 9; // digit width of a pass: 1..9 bits
 constexpr uint32_t kRadixMaxDigits = 1u << kRadixMaxBits;
-constexpr uint32_t kDigitsPerThread = kRadixMaxDigits / kRadixThreads;
 
 struct RadixPass {
     const uint64_t *keys;          // input keys (pass > 0)
@@ -488,6 +487,10 @@ static __global__ __launch_bounds__(kRadixThreads) void k_radix_count(RadixPass 
 // placement in LDS and the contiguous write-out as in k_seg_scatter.  seg: the
 // split pass of the segmented form (the low 32 bits of each key go to the
 // padded slot of its segment); else last: write col/src instead of keys.
+// kBits: the widest digit the instantiation takes (p.bits <= kBits); its LDS
+// tables hold 2^kBits digits, so the 8-bit one fits three workgroups per CU
+// (42 KB each) where the 9-bit one fits two (54 KB).
+template <uint32_t kBits>
 static __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(RadixPass p, uint32_t tiles,
                                                                          const uint64_t *__restrict__ offsets,
                                                                          uint32_t last, uint64_t *__restrict__ out,
@@ -497,10 +500,11 @@ static __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(RadixPas
                                                                          uint32_t *__restrict__ out32)
 {
     constexpr int kWaves = kRadixThreads / 64;
-    static_assert(kDigitsPerThread == 1, "one digit per thread");
+    constexpr uint32_t kDigits = 1u << kBits;
+    static_assert(kBits <= kRadixMaxBits && kDigits <= kRadixThreads, "one digit per thread");
     __shared__ uint64_t stage[kRadixTile];
-    __shared__ uint64_t dst[kRadixMaxDigits];          // output index of stage position 0, per digit
-    __shared__ uint32_t wpos[kWaves][kRadixMaxDigits]; // per wave: digit count, then next stage position
+    __shared__ uint64_t dst[kDigits];          // output index of stage position 0, per digit
+    __shared__ uint32_t wpos[kWaves][kDigits]; // per wave: digit count, then next stage position
     __shared__ uint32_t wsum[kWaves];
     const uint32_t b = xcd_tile(blockIdx.x, gridDim.x);
     const uint64_t n = radix_items(p);
@@ -514,7 +518,7 @@ static __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(RadixPas
     if (seg && threadIdx.x < nd) slot_shift = seg[threadIdx.x].pstart - seg[threadIdx.x].ustart;
     uint64_t key[kRadixRounds];
     radix_load(p, start, tile_n, true, key);
-    for (uint32_t d = threadIdx.x; d < kRadixMaxDigits; d += kRadixThreads)
+    for (uint32_t d = threadIdx.x; d < kDigits; d += kRadixThreads)
         for (int w = 0; w < kWaves; ++w) wpos[w][d] = 0;
     __syncthreads();
     const uint64_t lt = (1ull << lane) - 1ull;
@@ -523,15 +527,15 @@ static __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(RadixPas
     for (int r = 0; r < kRadixRounds; ++r) {
         const bool valid = radix_local(p, wave, r, lane) < tile_n;
         const uint32_t digit = radix_digit(p, key[r]);
-        const uint64_t match = wave_match<kRadixMaxBits>(digit, valid);
+        const uint64_t match = wave_match<kBits>(digit, valid);
         const uint32_t rank = (uint32_t)__popcll(match & lt), cnt = (uint32_t)__popcll(match);
         rw[r] = rank_word(rank, cnt, digit);
         if (valid && rank == 0) atomicAdd(&wpos[wave][digit], cnt);
     }
     __syncthreads();
     {
-        const uint32_t run = tile_digit_scan<kRadixThreads, kRadixMaxDigits>(wpos, wsum);
-        if (threadIdx.x < kRadixMaxDigits) dst[threadIdx.x] = my_off - run + slot_shift;
+        const uint32_t run = tile_digit_scan<kRadixThreads, kDigits>(wpos, wsum);
+        if (threadIdx.x < kDigits) dst[threadIdx.x] = my_off - run + slot_shift;
     }
     __syncthreads();
     uint32_t *mypos = wpos[wave];

@@ -348,8 +348,12 @@ void launch_csr_seg_into(hipStream_t s, const SegPlan &P, const SegBufs &B, cons
     hipLaunchKernelGGL(k_radix_count, dim3(t64), dim3(kRadixThreads), 0, s, rp, t64, B.rcount);
     scan_counts(s, B.rcount, L64, B.bsum, B.roffset);
     hipLaunchKernelGGL(k_seg_meta, dim3(1), dim3(512), 0, s, B.roffset, t64, P.nseg, B.segmeta, total);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(t64), dim3(kRadixThreads), 0, s, rp, t64, B.roffset, 0u, nullptr,
-                       nullptr, nullptr, B.segmeta, B.keys0);
+    if (P.hb <= 8)
+        hipLaunchKernelGGL(k_radix_scatter<8>, dim3(t64), dim3(kRadixThreads), 0, s, rp, t64, B.roffset, 0u, nullptr,
+                           nullptr, nullptr, B.segmeta, B.keys0);
+    else
+        hipLaunchKernelGGL(k_radix_scatter<kRadixMaxBits>, dim3(t64), dim3(kRadixThreads), 0, s, rp, t64, B.roffset,
+                           0u, nullptr, nullptr, nullptr, B.segmeta, B.keys0);
     *launches += 6;
     // 32-bit passes, least significant digit first; the last writes col and src
     for (uint32_t pass = 0; pass < P.passes; ++pass) {
@@ -449,10 +453,14 @@ int launch_csr(igc_ctx *ctx, uint32_t N, uint64_t cap, unsigned long long *count
         hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, s, ctx->bsum.as<uint64_t>(), nsb);
         hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(kScanThreads), 0, s, counts, L, ctx->bsum.as<uint64_t>(),
                            offsets);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kRadixThreads), 0, s, rp, tiles, offsets,
-                           (uint32_t)last, last ? nullptr : ctx->keys[pass & 1].as<uint64_t>(),
-                           ctx->col.as<uint32_t>(), ctx->keys[pass & 1].as<uint32_t>() /* src, last pass */,
-                           nullptr, nullptr);
+        uint64_t *kout = last ? nullptr : ctx->keys[pass & 1].as<uint64_t>();
+        uint32_t *ksrc = ctx->keys[pass & 1].as<uint32_t>(); // src, last pass
+        if (rp.bits <= 8)
+            hipLaunchKernelGGL(k_radix_scatter<8>, dim3(tiles), dim3(kRadixThreads), 0, s, rp, tiles, offsets,
+                               (uint32_t)last, kout, ctx->col.as<uint32_t>(), ksrc, nullptr, nullptr);
+        else
+            hipLaunchKernelGGL(k_radix_scatter<kRadixMaxBits>, dim3(tiles), dim3(kRadixThreads), 0, s, rp, tiles,
+                               offsets, (uint32_t)last, kout, ctx->col.as<uint32_t>(), ksrc, nullptr, nullptr);
         *launches += 5;
     }
     hipLaunchKernelGGL(k_csr_row_ptr, dim3(grid_for((uint64_t)N + 1, 256, 8192)), dim3(256), 0, s,
