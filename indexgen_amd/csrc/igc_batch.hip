@@ -129,7 +129,7 @@ static __global__ void k_pack_batch(const uint8_t *__restrict__ syms, const SetD
 
 // One group: workgroup tile0 + blockIdx.x is a tile of the set whose tile
 // range holds it; the set's predicate is the group's (Pol), one register of
-// columns per lane, with the narrow row planes for the sets that allow them.
+// columns per lane.
 template <class Pol>
 __global__ __launch_bounds__(256) void k_pairs_sets(const uint8_t *__restrict__ packed,
                                                     const uint8_t *__restrict__ packed_s, FilterBase fb,
@@ -151,13 +151,6 @@ __global__ __launch_bounds__(256) void k_pairs_sets(const uint8_t *__restrict__ 
                           fb.bias + (size_t)d.wbase * kItemCap, fb.count + d.wbase, fb.perm + d.node_off, d.window,
                           d.K);
     const auto *words = reinterpret_cast<const typename Pol::Word *>((d.filt ? packed_s : packed) + d.word_off);
-    if constexpr (NarrowOf<Pol>::has) { // the set's strings are short enough for the narrow row planes
-        if (d.n <= NarrowOf<Pol>::max_n) {
-            wave_tile<typename NarrowOf<Pol>::type, 1, true>(words, d.G, rb, cb, d.node_off, d.n, d.K, flt,
-                                                             stage[wave], lane, wave, edges, cap, counter);
-            return;
-        }
-    }
     wave_tile<Pol, 1, true>(words, d.G, rb, cb, d.node_off, d.n, d.K, flt, stage[wave], lane, wave, edges, cap,
                             counter);
 }
@@ -292,7 +285,7 @@ static void launch_sets_t(hipStream_t s, const uint8_t *packed, const uint8_t *p
 
 // Policies of one word class: two-plane words of type W, or binary words (BIN).
 template <class W, bool BIN> struct SetPols {
-    template <int K> using LV = PolLV<K, W>;
+    template <int K> using LV = typename LvPolOf<K, W, false>::type;
     using Myers = PolMyers<W>;
     using All = PolAll<W>;
 };

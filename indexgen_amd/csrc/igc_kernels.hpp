@@ -151,25 +151,32 @@ struct PolLV {
     }
 };
 
-// Landau-Vishkin on 32-bit words of n <= 16 - K/2: the row planes from the
-// whole shifted word (lvt_row_narrow), fewer scalar instructions per row.
+// Landau-Vishkin on 32-bit words (n <= 16, two pairs per register) in row
+// coordinates (igc_lv.hpp): the row's planes are the same for every diagonal,
+// two scalar packs per row, and each lane keeps its two columns shifted for
+// every diagonal, 2(2*floor(K/2)+1) registers loaded once per tile.
 template <int K>
-struct PolLVN : PolLV<K, uint32_t> {
-    using Row = typename PolLV<K, uint32_t>::Row;
-    static __device__ __forceinline__ void row(Row &r, uint32_t a, uint32_t, int) { lvt_row_narrow<K>(a, r); }
-};
-
-// The narrow form of a policy where there is one (Landau-Vishkin on 32-bit
-// words, 1 <= K <= 5), for sets of at most max_n symbols.
-template <class Pol> struct NarrowOf {
-    static constexpr bool has = false;
-    using type = Pol;
-    static constexpr uint32_t max_n = 0;
-};
-template <int K> struct NarrowOf<PolLV<K, uint32_t>> {
-    static constexpr bool has = K >= 1 && K <= 5;
-    using type = PolLVN<K>;
-    static constexpr uint32_t max_n = (uint32_t)lv_narrow_max_n(K);
+struct PolLVR {
+    using Word = uint32_t;
+    using V = u16x2;
+    static constexpr int P = 2;
+    struct Col {
+        LvtCol<K, u16x2> c;
+        __device__ __forceinline__ void load(const uint32_t *__restrict__ packed, uint32_t j0, uint32_t N)
+        {
+            ColPlanes<uint32_t> b;
+            b.load(packed, j0, N);
+            lvt_col<K>(b.lo, b.hi, c);
+        }
+    };
+    struct Row {
+        u16x2 lo, hi;
+    };
+    static __device__ __forceinline__ void row(Row &r, uint32_t a, uint32_t, int) { lvt_row_rc(a, r.lo, r.hi); }
+    static __device__ __forceinline__ uint32_t test(const Row &r, const Col &c, uint32_t, int)
+    {
+        return lvt_test_rc<K, u16x2>(r.lo, r.hi, c.c);
+    }
 };
 
 template <class W>
@@ -243,6 +250,19 @@ struct PolAllBin {
     struct Row {};
     static __device__ __forceinline__ void row(Row &, uint64_t, uint32_t, int) {}
     static __device__ __forceinline__ uint32_t test(const Row &, const Col &, uint32_t, int) { return 1u; }
+};
+
+// The Landau-Vishkin policy of a word class: row coordinates on 32-bit words,
+// column coordinates on 64-bit two-plane words (one pair per 32-bit lane, up to
+// 26 column registers at K = 13 in row coordinates) and on binary words.
+template <int K, class W, bool BIN> struct LvPolOf {
+    using type = PolLV<K, W>;
+};
+template <int K> struct LvPolOf<K, uint32_t, false> {
+    using type = PolLVR<K>;
+};
+template <int K> struct LvPolOf<K, uint64_t, true> {
+    using type = PolLVBin<K>;
 };
 
 // ---------------------------------------------------------------------------

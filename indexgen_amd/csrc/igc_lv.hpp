@@ -156,6 +156,22 @@ IGC_HD inline void col_planes(uint64_t w, uint32_t &lo, uint32_t &hi)
 // cells of neighbouring diagonals differ by at most one.  On diagonal 0 the
 // padding positions n.. match (both strings are zero-padded), so the final
 // residue m_0 & L_K is zero iff the last slide reaches the lane's end.
+//
+// Row coordinates (RC).  The same diagonals indexed by the row position
+// i = j + d instead: bit i of diagonal d's mask is A[i] != B[i-d], so A's planes
+// are the same for every diagonal and B's planes carry the shift (B << d, or
+// B >> -d).  A position q of diagonal d moves to q + d, so of the three terms
+// only the neighbours' change: FR[e-1][d-1] lands one bit further on diagonal
+// d (its mask << 1) and FR[e-1][d+1] + 1 lands on the same bit (its mask as it
+// is).  The diagonals' valid ranges shift with them: diagonal d > 0 starts at
+// bit d, and every mask it is given is at least FR[e-1][d-1] shifted there
+// (d - 1 is in range whenever d is), so its garbage bits below d (B's shifted-in
+// zeros) are never read; its end is bit n.  Diagonal d < 0 runs over bits
+// 0 .. n+d-1 and, as before, reads nothing below its start.  Every lane width
+// holds these ranges for any n the lane takes.  In the kernels the row is
+// uniform across a wave and the columns are fixed for a whole tile, so row
+// coordinates take every per-row shift off the scalar unit: a lane shifts its
+// columns once per tile.
 
 template <int K>
 struct LvShape {
@@ -163,9 +179,10 @@ struct LvShape {
     static constexpr int WD = 2 * C + 1;               // diagonals used
 };
 
-// m[t] is the mismatch mask of diagonal d = t - C; returns the residue
-// (a lane is an edge iff it is zero).
-template <int K, class V>
+// m[t] is the mismatch mask of diagonal d = t - C, in column coordinates, or
+// in row coordinates when RC; returns the residue (a lane is an edge iff it is
+// zero).
+template <int K, class V, bool RC = false>
 IGC_HD inline V lvt_residue(const V (&m)[LvShape<K>::WD])
 {
     constexpr int C = LvShape<K>::C;
@@ -195,8 +212,8 @@ IGC_UNROLL
                 if (d < -r || d > r) continue;
                 V x = m[d + C];
                 if (d >= -rp && d <= rp) x &= S[d + C];
-                if (d - 1 >= -rp && d - 1 <= rp) x &= T[d - 1 + C];
-                if (d + 1 >= -rp && d + 1 <= rp) x &= S[d + 1 + C];
+                if (d - 1 >= -rp && d - 1 <= rp) x &= RC ? S[d - 1 + C] : T[d - 1 + C];
+                if (d + 1 >= -rp && d + 1 <= rp) x &= RC ? T[d + 1 + C] : S[d + 1 + C];
                 NT[d + C] = ~(x - 1) | x;
             }
 IGC_UNROLL
@@ -206,7 +223,10 @@ IGC_UNROLL
             }
         }
         V L = S[C];
-        if constexpr (K >= 2) L = L & T[C - 1] & S[C + 1];
+        if constexpr (K >= 2) {
+            if constexpr (RC) L = L & S[C - 1] & T[C + 1];
+            else L = L & T[C - 1] & S[C + 1];
+        }
         return m[C] & L;
     }
 }
@@ -227,41 +247,54 @@ IGC_UNROLL
     for (int t = 0; t < LvShape<K>::WD; ++t) row_planes(a, t - C, r.lo[t], r.hi[t]);
 }
 
-// The same planes of a 32-bit word for n <= 16 - C, from the word shifted as a
-// whole: x = a >> s (or a << -s), lo plane = x's low half, hi plane = x's high
-// half, each copied into both halves.  A right shift by s brings the hi plane's
-// low s bits into positions 16-s..15 of the lo plane: past diagonal s's end
-// (positions >= n - s), where bits only ever mean "end reached".  A left shift
-// by u brings the lo plane's top u bits into positions 0..u-1 of the hi plane:
-// zero when n <= 16 - u.  One scalar shift and two packs per diagonal, where
-// the masked planes take a shift per plane besides.
-template <int K>
-IGC_HD inline void lvt_row_narrow(uint32_t a, LvtRow<K, u16x2> &r)
+// Row coordinates on 16-bit lanes: the column side, each lane's two columns
+// shifted for every diagonal (B << d, B >> -d within each 16-bit half; bits
+// shifted past bit 15 or in at either end lie outside the diagonal's range).
+template <int K, class V>
+struct LvtCol {
+    V lo[LvShape<K>::WD];
+    V hi[LvShape<K>::WD];
+};
+
+template <int K, class V>
+IGC_HD inline void lvt_col(V blo, V bhi, LvtCol<K, V> &c)
 {
     constexpr int C = LvShape<K>::C;
 IGC_UNROLL
     for (int t = 0; t < LvShape<K>::WD; ++t) {
-        const int s = t - C;
-        uint32_t x = s >= 0 ? a >> s : a << (-s);
-#if defined(__HIP_DEVICE_COMPILE__)
-        // opaque, so both packs read this one shifted word (left alone, the
-        // compiler shifts the word a second time for the high half)
-        if (s != 0) asm volatile("" : "+s"(x));
-#endif
-        const uint32_t l2 = (x << 16) | (x & 0xFFFFu);       // low half in both halves
-        const uint32_t h2 = (x & 0xFFFF0000u) | (x >> 16);   // high half in both halves
-        u16x2 lo, hi;
-        lo[0] = (uint16_t)l2;
-        lo[1] = (uint16_t)(l2 >> 16);
-        hi[0] = (uint16_t)h2;
-        hi[1] = (uint16_t)(h2 >> 16);
-        r.lo[t] = lo;
-        r.hi[t] = hi;
+        const int d = t - C;
+        c.lo[t] = d >= 0 ? (V)(blo << d) : (V)(blo >> (-d));
+        c.hi[t] = d >= 0 ? (V)(bhi << d) : (V)(bhi >> (-d));
     }
 }
 
-// largest n lvt_row_narrow serves at threshold K (0 <= K <= kLvMaxK)
-constexpr int lv_narrow_max_n(int K) { return 16 - K / 2; }
+// The row's planes, unshifted, each copied into both halves of a register:
+// lo = the word's low half twice, hi = its high half twice.
+IGC_HD inline void lvt_row_rc(uint32_t a, u16x2 &lo, u16x2 &hi)
+{
+    const uint32_t l2 = (a << 16) | (a & 0xFFFFu);
+    const uint32_t h2 = (a & 0xFFFF0000u) | (a >> 16);
+    lo[0] = (uint16_t)l2;
+    lo[1] = (uint16_t)(l2 >> 16);
+    hi[0] = (uint16_t)h2;
+    hi[1] = (uint16_t)(h2 >> 16);
+}
+
+// Edge lanes of the row (planes alo, ahi) against the columns c, in row
+// coordinates.
+template <int K, class V>
+IGC_HD inline uint32_t lvt_test_rc(V alo, V ahi, const LvtCol<K, V> &c)
+{
+    V m[LvShape<K>::WD];
+IGC_UNROLL
+    for (int t = 0; t < LvShape<K>::WD; ++t) {
+        m[t] = (alo ^ c.lo[t]) | (ahi ^ c.hi[t]);
+#if defined(__HIP_DEVICE_COMPILE__)
+        asm volatile("" : "+v"(m[t]));
+#endif
+    }
+    return zero_lanes(lvt_residue<K, V, true>(m));
+}
 
 // Edge lanes (bit h: the pair in lane h has Levenshtein <= K).
 template <int K, class V>

@@ -160,9 +160,6 @@ static void launch_pairs_t(hipStream_t s, uint64_t tiles, const void *packed, co
 
 // Landau-Vishkin policy of word type W (uint32_t, uint64_t: two planes) or of
 // binary words (BIN)
-template <int K, class W, bool BIN> struct LvPolOf { using type = PolLV<K, W>; };
-template <int K, class W> struct LvPolOf<K, W, true> { using type = PolLVBin<K>; };
-
 template <class W, int CPL, bool EMIT, bool BIN = false>
 static int launch_pairs_lv(hipStream_t s, uint64_t tiles, const void *packed, const TileGeom &G, uint32_t node_off,
                            uint32_t n, int K, const RowFilter &f, uint64_t *edges, uint64_t cap,
@@ -171,13 +168,8 @@ static int launch_pairs_lv(hipStream_t s, uint64_t tiles, const void *packed, co
     switch (K) {
 #define IGC_CASE(k)                                                                                           \
     case k:                                                                                                   \
-        if (NarrowOf<typename LvPolOf<k, W, BIN>::type>::has &&                                               \
-            n <= NarrowOf<typename LvPolOf<k, W, BIN>::type>::max_n)                                          \
-            launch_pairs_t<typename NarrowOf<typename LvPolOf<k, W, BIN>::type>::type, CPL, EMIT>(            \
-                s, tiles, packed, G, node_off, n, K, f, edges, cap, counter);                                 \
-        else                                                                                                  \
-            launch_pairs_t<typename LvPolOf<k, W, BIN>::type, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, \
-                                                                         f, edges, cap, counter);             \
+        launch_pairs_t<typename LvPolOf<k, W, BIN>::type, CPL, EMIT>(s, tiles, packed, G, node_off, n, K, f, edges, \
+                                                                     cap, counter);                           \
         return IGC_OK;
         IGC_CASE(0) IGC_CASE(1) IGC_CASE(2) IGC_CASE(3) IGC_CASE(4) IGC_CASE(5) IGC_CASE(6) IGC_CASE(7)
         IGC_CASE(8) IGC_CASE(9) IGC_CASE(10) IGC_CASE(11) IGC_CASE(12) IGC_CASE(13)

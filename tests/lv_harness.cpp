@@ -1,8 +1,9 @@
 // Host build of the GPU pair predicates (indexgen_amd/csrc/igc_lv.hpp) for the
 // CPU tests: the same template code the kernels run, compiled with g++.
 //
-// Each predicate runs on the lane type the kernels use: two pairs per call on
-// 16-bit lanes (n <= 16) or one pair on a 32-bit lane (wide, n <= 32).  For
+// Each predicate runs on the lane type and in the form the kernels use: two
+// pairs per call on 16-bit lanes (n <= 16; Landau-Vishkin in row coordinates)
+// or one pair on a 32-bit lane (wide, n <= 32; column coordinates).  For
 // 16-bit lanes every pair (A[r], B[r]) is run twice, once in each lane, with an
 // unrelated column string in the other lane; out[r] is 2 when the lanes
 // disagree.  Binary strings (n <= 64) run on 64-bit lanes.
@@ -29,13 +30,16 @@ struct LvPred {
                 col_planes(pack_word64(b, n), lo, hi);
                 out[r] = (uint8_t)lvt_test<K, uint32_t>(row, lo, hi);
             } else {
-                LvtRow<K, u16x2> row;
-                lvt_row<K>(pack_word(a, n), row);
-                u16x2 lo, hi;
+                // row coordinates, as the kernels run 16-bit lanes
+                u16x2 alo, ahi, lo, hi;
+                lvt_row_rc(pack_word(a, n), alo, ahi);
+                LvtCol<K, u16x2> col;
                 col_planes(pack_word(b, n), pack_word(o, n), lo, hi);
-                const uint32_t e0 = lvt_test<K, u16x2>(row, lo, hi) & 1u;
+                lvt_col<K>(lo, hi, col);
+                const uint32_t e0 = lvt_test_rc<K, u16x2>(alo, ahi, col) & 1u;
                 col_planes(pack_word(o, n), pack_word(b, n), lo, hi);
-                const uint32_t e1 = (lvt_test<K, u16x2>(row, lo, hi) >> 1) & 1u;
+                lvt_col<K>(lo, hi, col);
+                const uint32_t e1 = (lvt_test_rc<K, u16x2>(alo, ahi, col) >> 1) & 1u;
                 out[r] = e0 == e1 ? (uint8_t)e0 : 2;
             }
         }
@@ -50,39 +54,6 @@ extern "C" int lv_pairs(const uint8_t *A, const uint8_t *B, int M, int n, int K,
     if (wide ? n > 32 : n > 16) return -1;
     switch (K) {
 #define C_(k) case k: LvPred::run<k>(A, B, M, n, wide, out); return 0;
-        C_(0) C_(1) C_(2) C_(3) C_(4) C_(5) C_(6) C_(7) C_(8) C_(9) C_(10) C_(11) C_(12) C_(13) C_(14) C_(15)
-#undef C_
-    }
-    return -1;
-}
-
-namespace {
-struct LvNarrow {
-    template <int K>
-    static void run(const uint8_t *A, const uint8_t *B, int M, int n, uint8_t *out)
-    {
-        for (int r = 0; r < M; ++r) {
-            const uint8_t *a = A + (size_t)r * n, *b = B + (size_t)r * n;
-            const uint8_t *o = B + (size_t)((r + 1) % M) * n;
-            LvtRow<K, u16x2> row;
-            lvt_row_narrow<K>(pack_word(a, n), row);
-            u16x2 lo, hi;
-            col_planes(pack_word(b, n), pack_word(o, n), lo, hi);
-            const uint32_t e0 = lvt_test<K, u16x2>(row, lo, hi) & 1u;
-            col_planes(pack_word(o, n), pack_word(b, n), lo, hi);
-            const uint32_t e1 = (lvt_test<K, u16x2>(row, lo, hi) >> 1) & 1u;
-            out[r] = e0 == e1 ? (uint8_t)e0 : 2;
-        }
-    }
-};
-} // namespace
-
-// the same on 16-bit lanes with the row planes of lvt_row_narrow (n <= 16 - K/2)
-extern "C" int lv_pairs_narrow(const uint8_t *A, const uint8_t *B, int M, int n, int K, uint8_t *out)
-{
-    if (K < 0 || K > 15 || n > 16 - K / 2) return -1;
-    switch (K) {
-#define C_(k) case k: LvNarrow::run<k>(A, B, M, n, out); return 0;
         C_(0) C_(1) C_(2) C_(3) C_(4) C_(5) C_(6) C_(7) C_(8) C_(9) C_(10) C_(11) C_(12) C_(13) C_(14) C_(15)
 #undef C_
     }

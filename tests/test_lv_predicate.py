@@ -114,38 +114,6 @@ def test_random_and_near_pairs(lv, myers, n):
             assert np.array_equal(got, want), ("myers", n, K, wide, int((got != want).sum()))
 
 
-def _narrow(harness, A, B, K):
-    A = np.ascontiguousarray(A, np.uint8)
-    B = np.ascontiguousarray(B, np.uint8)
-    out = np.zeros(A.shape[0], np.uint8)
-    f = harness.lv_pairs_narrow
-    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
-    f.restype = ctypes.c_int
-    assert f(A.ctypes.data, B.ctypes.data, A.shape[0], A.shape[1], K, out.ctypes.data) == 0
-    return out
-
-
-@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 10, 11, 12, 13, 14, 15, 16])
-def test_narrow_row_planes(harness, n):
-    """Row planes taken from the whole shifted word (lvt_row_narrow, for
-    n <= 16 - K/2): exhaustive up to n = 5, random and near pairs above, every
-    K the form serves, against the oracle; longer strings are refused."""
-    if n <= 5:
-        S = all_strings(n)
-        A = np.repeat(S, S.shape[0], axis=0)
-        B = np.tile(S, (S.shape[0], 1))
-    else:
-        A, B = _pairs(n, M=12000, seed=7)
-    for K in range(0, 16):
-        if n > 16 - K // 2:
-            out = np.zeros(A.shape[0], np.uint8)
-            assert harness.lv_pairs_narrow(A.ctypes.data, B.ctypes.data, A.shape[0], n, K, out.ctypes.data) == -1
-            continue
-        want = O.conflict_pairs(A, B, K + 1)
-        got = _narrow(harness, A, B, K)
-        assert np.array_equal(got, want), ("narrow", n, K, int((got != want).sum()))
-
-
 @pytest.mark.parametrize("n", [17, 18, 20, 23, 24, 27, 31, 32])
 def test_random_and_near_pairs_wide(lv, myers, n):
     A, B = _pairs(n, M=12000)
