@@ -430,12 +430,14 @@ IGC_HD inline int word_class_of(uint32_t n) { return n <= 16 ? 0 : (n <= 32 ? 1 
 
 // Femtoseconds per pair of the count-only pair kernel at its fastest tile
 // shape on MI355X, N = 65536 random strings, each predicate forced in turn:
-// Landau-Vishkin per K for each word class (its cost does not depend on n),
-// Myers a least-squares line in n per word class (one step per symbol).
+// Landau-Vishkin per K for each word class (its cost does not depend on n) in
+// the form the kernels run (16-bit lanes: row coordinates up to K = 3, column
+// coordinates above), Myers a least-squares line in n per word class (one
+// step per symbol).
 IGC_HD inline uint32_t lv_fs(int wc, int K)
 {
     const uint32_t t[3][kLvLaunchMaxK + 1] = {
-        {116, 134, 291, 425, 726, 1005, 1484, 1971, 2556, 3107, 3771, 4369, 5283, 6028},
+        {92, 120, 276, 422, 722, 997, 1491, 2010, 2572, 3121, 3769, 4429, 5257, 6005},
         {150, 192, 493, 774, 1243, 1844, 2625, 3592, 4502, 5708, 7062, 8361, 9990, 11455},
         {152, 218, 672, 1150, 1986, 2997, 4239, 5559, 7189, 8921, 11123, 13123, 15748, 18382}};
     return t[wc][K];

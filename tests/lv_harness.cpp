@@ -1,12 +1,13 @@
 // Host build of the GPU pair predicates (indexgen_amd/csrc/igc_lv.hpp) for the
 // CPU tests: the same template code the kernels run, compiled with g++.
 //
-// Each predicate runs on the lane type and in the form the kernels use: two
-// pairs per call on 16-bit lanes (n <= 16; Landau-Vishkin in row coordinates)
-// or one pair on a 32-bit lane (wide, n <= 32; column coordinates).  For
-// 16-bit lanes every pair (A[r], B[r]) is run twice, once in each lane, with an
-// unrelated column string in the other lane; out[r] is 2 when the lanes
-// disagree.  Binary strings (n <= 64) run on 64-bit lanes.
+// Each predicate runs on the lane types and in the forms the kernels use: two
+// pairs per call on 16-bit lanes (n <= 16; Landau-Vishkin in row coordinates
+// and in column coordinates) or one pair on a 32-bit lane (wide, n <= 32;
+// column coordinates).  For 16-bit lanes every pair (A[r], B[r]) is run in
+// each lane, with an unrelated column string in the other lane, and by each
+// form; out[r] is 2 when any two of these disagree.  Binary strings (n <= 64)
+// run on 64-bit lanes.
 #include "../indexgen_amd/csrc/igc_lv.hpp"
 
 #include <stddef.h>
@@ -30,17 +31,22 @@ struct LvPred {
                 col_planes(pack_word64(b, n), lo, hi);
                 out[r] = (uint8_t)lvt_test<K, uint32_t>(row, lo, hi);
             } else {
-                // row coordinates, as the kernels run 16-bit lanes
+                // both forms the kernels run on 16-bit lanes, each pair in both lanes
                 u16x2 alo, ahi, lo, hi;
                 lvt_row_rc(pack_word(a, n), alo, ahi);
+                LvtRow<K, u16x2> row;
+                lvt_row<K>(pack_word(a, n), row);
                 LvtCol<K, u16x2> col;
+                uint32_t e[4];
                 col_planes(pack_word(b, n), pack_word(o, n), lo, hi);
                 lvt_col<K>(lo, hi, col);
-                const uint32_t e0 = lvt_test_rc<K, u16x2>(alo, ahi, col) & 1u;
+                e[0] = lvt_test_rc<K, u16x2>(alo, ahi, col) & 1u;
+                e[1] = lvt_test<K, u16x2>(row, lo, hi) & 1u;
                 col_planes(pack_word(o, n), pack_word(b, n), lo, hi);
                 lvt_col<K>(lo, hi, col);
-                const uint32_t e1 = (lvt_test_rc<K, u16x2>(alo, ahi, col) >> 1) & 1u;
-                out[r] = e0 == e1 ? (uint8_t)e0 : 2;
+                e[2] = (lvt_test_rc<K, u16x2>(alo, ahi, col) >> 1) & 1u;
+                e[3] = (lvt_test<K, u16x2>(row, lo, hi) >> 1) & 1u;
+                out[r] = (e[0] == e[1] && e[0] == e[2] && e[0] == e[3]) ? (uint8_t)e[0] : 2;
             }
         }
     }
