@@ -447,12 +447,23 @@ __device__ __forceinline__ void wave_tile(const typename Pol::Word *__restrict__
             // the window's items are wave-uniform and read-only: scalar loads
             using CU32 = const __attribute__((address_space(4))) uint32_t;
             CU32 *si = (CU32 *)items, *sb = (CU32 *)ibias;
-            for (uint32_t q = 0; q < nitems; ++q) {
-                uint32_t v = __builtin_amdgcn_sad_u8(f.x, si[4 * q], sb[q]);
-                v = __builtin_amdgcn_sad_u8(f.x, si[4 * q + 2], v);
-                v = __builtin_amdgcn_sad_u8(f.y, si[4 * q + 1], v);
-                v = __builtin_amdgcn_sad_u8(f.y, si[4 * q + 3], v);
-                best = min(best, v);
+            // two items a step (slots q0 and q0 + 1, adjacent: kItemCap is even),
+            // one min3; the slot is masked to the window's kItemCap (a no-op on
+            // q < nitems) so the loads take 32-bit scalar offsets, and a missing
+            // second item gets a bias no row can reach
+            for (uint32_t q = 0; q < nitems; q += 2) {
+                const uint32_t q0 = q & (kItemCap - 2u);
+                const uint32_t braw = sb[q0 + 1u]; // in the window's slots either way
+                const uint32_t b1 = q + 1u < nitems ? braw : 0x7FFFFFFFu;
+                uint32_t v0 = __builtin_amdgcn_sad_u8(f.x, si[4 * q0], sb[q0]);
+                uint32_t v1 = __builtin_amdgcn_sad_u8(f.x, si[4 * q0 + 4], b1);
+                v0 = __builtin_amdgcn_sad_u8(f.x, si[4 * q0 + 2], v0);
+                v1 = __builtin_amdgcn_sad_u8(f.x, si[4 * q0 + 6], v1);
+                v0 = __builtin_amdgcn_sad_u8(f.y, si[4 * q0 + 1], v0);
+                v1 = __builtin_amdgcn_sad_u8(f.y, si[4 * q0 + 5], v1);
+                v0 = __builtin_amdgcn_sad_u8(f.y, si[4 * q0 + 3], v0);
+                v1 = __builtin_amdgcn_sad_u8(f.y, si[4 * q0 + 7], v1);
+                best = min(best, min(v0, v1));
             }
             todo &= __ballot(best <= flt.limit);
             const uint32_t nl = r0 + 64u + lane;
